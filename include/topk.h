@@ -1,0 +1,209 @@
+/*
+ * topk.h -- C ABI of the MI355X (gfx950) top-k query engine.
+ *
+ * Drop-in boundary for the reference's query contract
+ * (avionicscode/TopK validation/bench_cpu.h:155-237 bench_tpac and
+ * :323-402 bench_pta): an object owning an N x d FP32 column-major
+ * attribute store, a separately timed init(), and per-query
+ *   findTopK(k, qq, weights, attr)
+ * calls (cpu/TPAc.h:19-22, cpu/PTA.h:71-76) whose results are k
+ * (id, score) tuples plus the k-th score (cpu/AA.h:71-73).
+ *
+ * Ranking contract.  score_i = fl(...fl(fl(0 + fl(x[a0][i]*w[a0]))
+ *   + fl(x[a1][i]*w[a1]))...) in the order of attr[], each product and
+ *   partial sum rounded to binary32 separately (no FMA contraction), with
+ *   weights indexed by attribute id (cpu/TPAc.h:129-142).  Objects rank by
+ *   (score desc, id asc); the reference breaks ties by heap order, this
+ *   engine by the lower object id.
+ *
+ * Output order.  out_ids/out_scores hold the top min(k, n) objects ranked
+ *   best first (the reference's res vector is ascending, res[0] = k-th,
+ *   cpu/TPAc.h:198-206; the C++ mirror in topk_amd.hpp restores that
+ *   order).  Slots past min(k, n) are set to id UINT64_MAX, score -inf.
+ *   *out_threshold receives the k-th (last valid) score.
+ *
+ * Errors.  Every entry point returns a topk_status code instead of the
+ *   reference's exit(1) (input/File.h:238-241, gpu/CudaHelper.h:175-177).
+ *
+ * Threading.  One HIP stream per context; a context is not thread-safe.
+ *   Different contexts may be used from different threads.
+ *
+ * Limits.  n < 2^32 (ids travel as 32 bits inside the ranking key),
+ *   1 <= d <= TOPK_MAX_DIMS, 1 <= qq <= d, 1 <= k <= TOPK_MAX_K.  Input
+ *   values and weights must be finite.
+ */
+#ifndef TOPK_H
+#define TOPK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOPK_ABI_VERSION 1
+#define TOPK_MAX_DIMS 32
+#define TOPK_MAX_K 1024
+
+typedef enum topk_status {
+    TOPK_OK = 0,
+    TOPK_ERR_INVALID = 1,      /* bad argument (k, qq, attr id, n, d, null pointer) */
+    TOPK_ERR_NOMEM = 2,        /* device or pinned-host allocation failed */
+    TOPK_ERR_HIP = 3,          /* a HIP runtime call failed */
+    TOPK_ERR_UNSUPPORTED = 4,  /* request outside this build's limits */
+    TOPK_ERR_NONFINITE = 5,    /* the store holds NaN or infinity */
+    TOPK_ERR_STATE = 6,        /* e.g. topk_pta before the partitions are built */
+    TOPK_ERR_NO_DEVICE = 7,    /* no usable GPU */
+    TOPK_ERR_OVERFLOW = 8      /* internal candidate buffer overflowed (retried on the exact path) */
+} topk_status;
+
+typedef struct topk_ctx topk_ctx;
+
+/* Data distributions of the synthetic generator. */
+typedef enum topk_distr {
+    TOPK_CORRELATED = 0,       /* u~U(0,1) per object, x = clip(u + N(0,0.05), 0, 1) */
+    TOPK_INDEPENDENT = 1,      /* x ~ U[0,1) i.i.d. */
+    TOPK_ANTICORRELATED = 2,   /* N(0,0.05) around the hyperplane sum = d/2, clipped to [0,1] */
+    TOPK_MIXED = 3             /* object-range thirds: correlated, independent, anticorrelated */
+} topk_distr;
+
+/* ---- context / column store (reference: AA(n,d) + get_cdata,
+ *      cpu/AA.h:66,77; File::load/gen, input/File.h:314-354) ---- */
+
+#define TOPK_ROWMAJOR 0x1u     /* ctx_create input is row-major (n x d); transposed on device */
+
+/* Copies an n x d FP32 table (column-major x[m*n+i] unless TOPK_ROWMAJOR)
+ * into a padded column store in HBM.  The caller keeps its buffer
+ * (the reference hands ownership over, cpu/AA.h:147-150). */
+int topk_ctx_create(topk_ctx **out, const float *data, uint64_t n, uint32_t d, uint32_t flags);
+
+/* Builds the column store directly in HBM with the synthetic generator
+ * (same bits as topk_generate).  Replaces File::gen (input/File.h:335-354). */
+int topk_ctx_create_generated(topk_ctx **out, uint64_t n, uint32_t d, int distr, uint64_t seed);
+
+int topk_ctx_destroy(topk_ctx *ctx);
+
+uint64_t topk_ctx_n(const topk_ctx *ctx);
+uint32_t topk_ctx_d(const topk_ctx *ctx);
+
+/* Copies the (possibly normalised / reordered) column store back to the
+ * host as a dense column-major n x d table. */
+int topk_ctx_read_columns(topk_ctx *ctx, float *out);
+
+/* ---- init (reference: TPAc::init cpu/TPAc.h:28-32, PTA::init
+ *      cpu/PTA.h:270-278, GPA::init gpu/GPA.h:62-94) ---- */
+
+#define TOPK_INIT_NORMALIZE 0x1u   /* per-column min-max normalisation, common/common.h:29-60 */
+#define TOPK_INIT_PTA       0x2u   /* build the PTA partition layout + per-partition bounds */
+
+/* Runs the requested preprocessing; its wall time is reported as tt_init. */
+int topk_init(topk_ctx *ctx, uint32_t flags);
+
+/* Partition size used by TOPK_INIT_PTA (default 4096 objects). Must be a
+ * multiple of 1024.  Call before topk_init. */
+int topk_set_partition_size(topk_ctx *ctx, uint32_t objects);
+
+/* reorder_attr: per object, sort its d values descending across the
+ * columns, in place (gpu/reorder_attr.h:13-236; invariant
+ * gpu/GPA.h:96-114).  Destroys attribute identity, as in the reference. */
+int topk_reorder_attr(topk_ctx *ctx);
+
+/* ---- queries ---- */
+
+/* Plain full-table evaluation: TPAc::findTopK* (cpu/TPAc.h:35-358);
+ * GPU analogue of the GFA stub (gpu/GFA.h:47-50). */
+int topk_fte(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+             uint64_t *out_ids, float *out_scores, float *out_threshold);
+
+/* Partition-pruned evaluation: PTA::findTopK* (cpu/PTA.h:281-671), GPA
+ * (gpu/GPA.h:161-227).  Requires topk_init(ctx, TOPK_INIT_PTA).  Same
+ * results as topk_fte. */
+int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+             uint64_t *out_ids, float *out_scores, float *out_threshold);
+
+/* Bitonic top-k aggregation: per 4096-object block top-k in LDS, then a
+ * merge (gpu/BTA.h:84-255).  Same results as topk_fte. */
+int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+             uint64_t *out_ids, float *out_scores, float *out_threshold);
+
+/* Iterative prune + compact driven by radix select (gpu/GPA.h:161-227,
+ * gpu/prune_tuples.h:90-118, gpu/radix_select.h:73-120), with bounds
+ * from per-column maxima so it is exact for any weights and attribute
+ * subset.  Same results as topk_fte. */
+int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+             uint64_t *out_ids, float *out_scores, float *out_threshold);
+
+/* One query of a batch (cpu/TPAc.h:209-254 findTopKsimdMQ arguments). */
+typedef struct topk_query {
+    uint32_t k;
+    uint32_t qq;
+    uint8_t attr[TOPK_MAX_DIMS];      /* first qq entries used, summation order */
+    float weights[TOPK_MAX_DIMS];     /* indexed by attribute id */
+} topk_query;
+
+/* Batched multi-query (validation/bench_cpu.h:198-235 IMP=3/4): every
+ * column tile is streamed once and scored for all nq queries.  Results of
+ * query q go to out_ids/out_scores[q*k_stride ...] and out_thresholds[q].
+ * Same per-query results as topk_fte. */
+int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
+               uint64_t *out_ids, float *out_scores, float *out_thresholds);
+
+/* ---- device-pointer variants (outputs stay in HBM; no host sync) ---- */
+
+/* Enqueues topk_fte on the context's stream; outputs are device pointers.
+ * Errors found on the device are reported by the next topk_sync. */
+int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+                   uint64_t *d_ids, float *d_scores);
+
+/* Waits for the context's stream; returns the first device-side error
+ * since the last sync (TOPK_OK if none) and re-runs nothing. */
+int topk_sync(topk_ctx *ctx);
+
+/* The context's hipStream_t, for callers that time or order work on it. */
+void *topk_stream(topk_ctx *ctx);
+
+/* ---- statistics (reference STATS_EFF counters, cpu/AA.h:194-223) ---- */
+
+typedef struct topk_stats {
+    uint64_t tuple_count;        /* objects scored by the last query (sum over a batch) */
+    uint64_t candidates;         /* keys handed to the final exact select */
+    uint64_t partitions_scored;  /* PTA: partitions whose objects were scored */
+    uint64_t partitions_total;
+    uint32_t fallback;           /* queries of the last call that took the exact fallback path */
+    uint32_t queries;            /* queries in the last call */
+    double ms_last;              /* host wall time of the last query call */
+    double tt_init;              /* host wall time of the last topk_init, ms */
+} topk_stats;
+
+int topk_get_stats(topk_ctx *ctx, topk_stats *out);
+
+/* Kernel timing with HIP events on the context's stream.  When enabled,
+ * every launch of the named kernel family is bracketed by events; the
+ * accumulated time is read by topk_timing_get (which synchronises). */
+int topk_timing_enable(topk_ctx *ctx, int on);
+int topk_timing_get(topk_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches);
+int topk_timing_reset(topk_ctx *ctx);
+
+/* ---- utilities ---- */
+
+/* Deterministic synthetic generator (replaces the time-seeded
+ * input/randdataset-1.1.0 generators, randdataset.c:325-533).  Writes
+ * objects [i0, i0+count) of the n-object dataset into out[m*ld + (i-i0)].
+ * Counter-based: any object range reproduces the same bits. */
+int topk_generate(float *out, uint64_t n, uint32_t d, uint64_t ld, int distr, uint64_t seed,
+                  uint64_t i0, uint64_t count, int threads);
+
+/* Radix select over device-resident 32-bit keys: the element of ascending
+ * rank r (0-based), as gpu/radix_select.h:73-120 radix_select_gpu_findK. */
+int topk_radix_select_u32(topk_ctx *ctx, const uint32_t *d_keys, uint64_t n, uint64_t r,
+                          uint32_t *out);
+
+const char *topk_strerror(int status);
+int topk_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPK_H */

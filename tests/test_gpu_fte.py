@@ -1,0 +1,186 @@
+"""GPU parity of the FTE path (topk_fte / topk_fte_async) against the CPU
+oracle: ids and scores must match bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+import topk_amd
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(99)
+
+
+def _check(eng, x, n, k, w, attr):
+    ids, sc, thr = eng.fte(k, w, attr)
+    rid, rsc, rthr = oracle.fte_topk(x, n, k, w, attr)
+    assert np.array_equal(ids, rid), (n, k, list(attr))
+    assert np.array_equal(sc.view(np.uint32), rsc.view(np.uint32)), (n, k, list(attr))
+    assert thr == rthr
+    return ids, sc
+
+
+@pytest.mark.parametrize("distr", [0, 1, 2, 3])
+def test_fte_parity_distributions(gpu, distr):
+    n, d = 300_007, 8
+    x = topk_amd.generate(n, d, distr, 11 + distr)
+    with topk_amd.Engine(x) as eng:
+        for m in range(1, d + 1):
+            attr = list(range(d - m, d))  # bench_cpu.h QM=0 rows: last m attributes
+            for k in (1, 16, 64, 128):
+                _check(eng, x, n, k, np.ones(d, np.float32), attr)
+        w = RNG.uniform(0.1, 1.0, d).astype(np.float32)
+        _check(eng, x, n, 64, w, [5, 1, 7, 0, 3])
+
+
+@pytest.mark.parametrize("n", [1, 3, 15, 16, 17, 63, 64, 65, 1000, 4095, 4096, 4097, 8193, 1_048_577])
+def test_fte_parity_ragged_sizes(gpu, n):
+    d = 4
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, n)
+    with topk_amd.Engine(x) as eng:
+        for k in (1, 16, 64):
+            _check(eng, x, n, k, np.ones(d, np.float32), [1, 2, 3])
+
+
+def test_fte_k_larger_than_n_sentinels(gpu):
+    n, d = 10, 2
+    x = topk_amd.generate(n, d, 1, 5)
+    with topk_amd.Engine(x) as eng:
+        ids = np.empty(16, np.uint64)
+        sc = np.empty(16, np.float32)
+        import ctypes
+        thr = np.zeros(1, np.float32)
+        w = np.ones(32, np.float32)
+        at = np.array([0, 1], np.uint8)
+        rc = topk_amd.lib().topk_fte(eng.handle, 16, 2, w.ctypes.data, at.ctypes.data, ids.ctypes.data,
+                                     sc.ctypes.data, thr.ctypes.data)
+        assert rc == 0
+        rid, rsc, _ = oracle.fte_topk(x, n, 16, w, [0, 1])
+        assert np.array_equal(ids[:10], rid) and np.all(ids[10:] == np.uint64(2 ** 64 - 1))
+        assert np.all(np.isneginf(sc[10:]))
+        assert thr[0] == rsc[-1]
+
+
+def test_fte_large_k(gpu):
+    n, d = 200_000, 8
+    x = topk_amd.generate(n, d, topk_amd.ANTICORRELATED, 3)
+    with topk_amd.Engine(x) as eng:
+        for k in (256, 512, 1000, 1024):
+            _check(eng, x, n, k, RNG.uniform(0.1, 1, d).astype(np.float32), list(range(d)))
+
+
+def test_fte_ties_all_equal(gpu):
+    # every score equal: the top-k are the k lowest ids
+    n, d = 100_000, 3
+    x = np.full((d, n), 0.5, np.float32)
+    with topk_amd.Engine(x) as eng:
+        ids, sc, thr = eng.fte(64, np.ones(d, np.float32), [0, 1, 2])
+        assert ids.tolist() == list(range(64)) and np.all(sc == 1.5)
+
+
+def test_fte_ties_quantised(gpu):
+    n, d = 500_000, 4
+    x = (RNG.integers(0, 3, size=(d, n)) / 2.0).astype(np.float32)
+    with topk_amd.Engine(x) as eng:
+        for k in (16, 128, 1024):
+            _check(eng, x, n, k, np.ones(d, np.float32), [0, 1, 2, 3])
+
+
+def test_fte_correlated_ties_at_max(gpu):
+    # correlated data clips to 1.0: many objects score exactly m
+    n, d = 2_000_000, 8
+    x = topk_amd.generate(n, d, topk_amd.CORRELATED, 0)
+    with topk_amd.Engine(x) as eng:
+        for m in (2, 4, 8):
+            ids, sc = _check(eng, x, n, 128, np.ones(d, np.float32), list(range(d - m, d)))
+            if m == 2:
+                assert np.all(sc == 2.0)
+
+
+def test_fte_negative_weights_and_repeats(gpu):
+    n, d = 123_457, 6
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 4)
+    with topk_amd.Engine(x) as eng:
+        w = np.array([0.5, -1.25, 3.0, -0.1, 0.0, 2.0], np.float32)
+        _check(eng, x, n, 32, w, [1, 3, 0])
+        _check(eng, x, n, 32, w, [2, 2, 5, 4])
+        _check(eng, x, n, 32, -np.ones(d, np.float32), [0, 1, 2, 3, 4, 5])
+
+
+def test_fte_wide_queries(gpu):
+    n, d = 70_001, 32
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 8)
+    with topk_amd.Engine(x) as eng:
+        w = RNG.uniform(0.1, 1, d).astype(np.float32)
+        for m in (9, 16, 17, 24, 32):
+            attr = RNG.permutation(d)[:m]
+            _check(eng, x, n, 64, w, attr)
+
+
+def test_fte_rowmajor_ingest(gpu):
+    n, d = 33_333, 5
+    x = topk_amd.generate(n, d, topk_amd.CORRELATED, 9)
+    with topk_amd.Engine(np.ascontiguousarray(x.T), rowmajor=True) as eng:
+        assert np.array_equal(eng.read_columns(), x)
+        _check(eng, x, n, 16, np.ones(d, np.float32), [0, 4])
+
+
+def test_generated_store_matches_host_generator(gpu):
+    for distr in range(4):
+        n, d = 77_777, 7
+        with topk_amd.Engine.generated(n, d, distr, 42) as eng:
+            assert np.array_equal(eng.read_columns(), topk_amd.generate(n, d, distr, 42))
+
+
+def test_nonfinite_rejected(gpu):
+    x = np.ones((2, 100), np.float32)
+    x[1, 50] = np.nan
+    with pytest.raises(topk_amd.TopkError) as e:
+        topk_amd.Engine(x)
+    assert e.value.status == 5
+
+
+def test_fte_overflow_falls_back_exactly(gpu):
+    # scores rise with the id, so the prefix threshold is weak and more than
+    # the candidate capacity (2^20) objects pass the stream filter
+    n = 3_000_000
+    x = (np.arange(n, dtype=np.float64) / n).astype(np.float32)[None, :]
+    with topk_amd.Engine(x) as eng:
+        _check(eng, x, n, 64, np.ones(1, np.float32), [0])
+        assert eng.stats()["fallback"] == 1
+        _check(eng, x, n, 64, -np.ones(1, np.float32), [0])
+        assert eng.stats()["fallback"] == 0
+
+
+def test_fte_async_matches_sync(gpu):
+    import torch
+    n, d = 1_000_000, 8
+    with topk_amd.Engine.generated(n, d, topk_amd.INDEPENDENT, 1) as eng:
+        k = 64
+        ids_d = torch.empty(4 * k, dtype=torch.int64, device="cuda")
+        sc_d = torch.empty(4 * k, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        qs = [(np.ones(d, np.float32), list(range(8 - m, 8))) for m in (2, 4, 6, 8)]
+        for i, (w, at) in enumerate(qs):
+            eng.fte_async(k, w, at, ids_d.data_ptr() + 8 * k * i, sc_d.data_ptr() + 4 * k * i)
+        eng.sync()
+        ids_h = ids_d.cpu().numpy().view(np.uint64)
+        sc_h = sc_d.cpu().numpy()
+        for i, (w, at) in enumerate(qs):
+            ids, sc, _ = eng.fte(k, w, at)
+            assert np.array_equal(ids_h[i * k:(i + 1) * k], ids)
+            assert np.array_equal(sc_h[i * k:(i + 1) * k], sc)
+
+
+def test_fte_full_size_parity(gpu):
+    # BASELINE headline shape: N = 2^26, d = 8, k = 64, all distributions
+    n, d = 1 << 26, 8
+    for distr, m in ((topk_amd.INDEPENDENT, 8), (topk_amd.CORRELATED, 2), (topk_amd.ANTICORRELATED, 5)):
+        x = topk_amd.generate(n, d, distr, distr, threads=16)
+        attr = list(range(d - m, d))
+        w = np.ones(d, np.float32)
+        with topk_amd.Engine.generated(n, d, distr, distr) as eng:
+            ids, sc, thr = eng.fte(64, w, attr)
+        rid, rsc, rthr = oracle.tpac_threads(x, n, 64, w, attr, 16)
+        assert np.array_equal(ids, rid) and np.array_equal(sc, rsc) and thr == rthr
+        del x

@@ -1,0 +1,261 @@
+"""Pins the CPU oracle (oracle/topk_oracle.c) without trusting it.
+
+Every check is a closed form, a hand-derived known answer, a brute-force
+computation by a different method, or an invariant -- chosen so that a
+plausible mistake (summation order, an FMA, a dropped term, a wrong
+tie-break, an off-by-one in a radix digit) fails one of them.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+RNG = np.random.default_rng(1234)
+
+
+def _dyadic_case(n, d, qq, rng):
+    """x = a/256, w = b/16 with small integers: every product and partial sum
+    is exact in binary32, so score == sum(a*b)/4096 exactly."""
+    a = rng.integers(0, 256, size=(d, n))
+    b = rng.integers(1, 17, size=d)
+    attr = rng.permutation(d)[:qq].astype(np.uint8)
+    x = (a / 256.0).astype(np.float32)
+    w = (b / 16.0).astype(np.float32)
+    exact = (a[attr, :] * b[attr, None]).sum(axis=0)  # integer numerator over 4096
+    return x, w, attr, exact
+
+
+def test_keys_order():
+    vals = np.array([-np.inf, -3e38, -1.5, -1e-30, 0.0, 1e-38, 1e-30, 0.5, 1.0, 7.0, 3e38, np.inf],
+                    dtype=np.float32)
+    ks = [oracle.okey(v) for v in vals]
+    assert ks == sorted(ks) and len(set(ks)) == len(ks)
+    # (score desc, id asc): higher score wins; on equal score the lower id wins
+    assert oracle.key(2.0, 10) > oracle.key(1.0, 0)
+    assert oracle.key(1.0, 3) > oracle.key(1.0, 4)
+    assert oracle.key(-np.inf, 0xFFFFFFFE) > 0
+
+
+def test_score_closed_form_dyadic():
+    for qq in (1, 2, 5, 8):
+        x, w, attr, exact = _dyadic_case(3000, 8, qq, RNG)
+        s = oracle.score(x, 3000, w, attr)
+        assert np.array_equal(s.astype(np.float64) * 4096.0, exact.astype(np.float64))
+
+
+def test_score_summation_order_known_answer():
+    # products 1, 2^-24, 2^-24: ((1 + 2^-24) + 2^-24) = 1 (ties to even)
+    # but ((2^-24 + 2^-24) + 1) = 1 + 2^-23.
+    x = np.array([[1.0], [2.0 ** -24], [2.0 ** -24]], dtype=np.float32)
+    w = np.ones(3, dtype=np.float32)
+    assert oracle.score(x, 1, w, [0, 1, 2])[0] == np.float32(1.0)
+    assert oracle.score(x, 1, w, [2, 1, 0])[0] == np.float32(1.0 + 2.0 ** -23)
+
+
+def test_score_no_fma_known_answer():
+    # attr order [1, 0]: acc = -1, then acc + fl((1+2^-12)^2).  Separately
+    # rounded: fl((1+2^-12)^2) = 1 + 2^-11 (the 2^-24 term is a tie to even),
+    # so the score is 2^-11; a fused multiply-add would give 2^-11 + 2^-24.
+    x = np.array([[1.0 + 2.0 ** -12], [1.0]], dtype=np.float32)
+    w = np.array([1.0 + 2.0 ** -12, -1.0], dtype=np.float32)
+    assert oracle.score(x, 1, w, [1, 0])[0] == np.float32(2.0 ** -11)
+
+
+def test_weights_indexed_by_attribute_id():
+    # cpu/TPAc.h:135: weight = weights[attr[m]], not weights[m]
+    x = np.array([[1.0], [10.0], [100.0]], dtype=np.float32)
+    w = np.array([1.0, 2.0, 3.0], dtype=np.float32)
+    assert oracle.score(x, 1, w, [2, 0])[0] == np.float32(301.0)
+
+
+@pytest.mark.parametrize("k", [1, 7, 16, 64])
+def test_fte_topk_bruteforce_with_ties(k):
+    # quantised dyadic data -> many exact score ties; the brute-force answer
+    # is a full lexicographic sort by (-score, id) of exact integer scores
+    n = 2000
+    a = RNG.integers(0, 4, size=(4, n))
+    x = (a / 4.0).astype(np.float32)
+    w = np.ones(4, dtype=np.float32)
+    exact = a[[1, 3], :].sum(axis=0)
+    order = np.lexsort((np.arange(n), -exact))[:k]
+    ids, sc, thr = oracle.fte_topk(x, n, k, w, [1, 3])
+    assert np.array_equal(ids, order.astype(np.uint64))
+    assert np.array_equal(sc * 4.0, exact[order].astype(np.float32))
+    assert thr == sc[-1]
+
+
+def test_fte_topk_n_smaller_than_k():
+    x = (RNG.integers(0, 100, size=(2, 5)) / 128.0).astype(np.float32)
+    ids, sc, thr = oracle.fte_topk(x, 5, 16, np.ones(2, np.float32), [0, 1])
+    assert len(ids) == 5 and sorted(ids.tolist()) == list(range(5))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_tpac_threads_matches_scalar(threads):
+    n = 50_001
+    x = RNG.random((8, n), dtype=np.float32)
+    w = RNG.uniform(0.1, 1.0, 8).astype(np.float32)
+    for attr, k in (([4, 5, 6, 7], 16), ([7, 0, 3], 100), (list(range(8)), 1)):
+        a = oracle.fte_topk(x, n, k, w, attr)
+        b = oracle.tpac_threads(x, n, k, w, attr, threads)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_mq_threads_matches_single():
+    n = 20_000
+    x = RNG.random((6, n), dtype=np.float32)
+    ks = np.array([5, 16, 33], dtype=np.uint32)
+    qqs = np.array([2, 6, 3], dtype=np.uint32)
+    attrs = np.array([[4, 5, 0, 0, 0, 0], [0, 1, 2, 3, 4, 5], [5, 1, 3, 0, 0, 0]], dtype=np.uint8)
+    ws = RNG.uniform(0.1, 1, (3, 6)).astype(np.float32)
+    res = oracle.mq_threads(x, n, ks, qqs, ws, attrs, 3)
+    for q in range(3):
+        ids, sc, _ = oracle.fte_topk(x, n, int(ks[q]), ws[q], attrs[q, :qqs[q]])
+        assert np.array_equal(res[q][0], ids) and np.array_equal(res[q][1], sc)
+
+
+def test_radix_select_matches_sort():
+    for n in (1, 2, 17, 5000):
+        keys = RNG.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+        keys[: n // 3] = keys[0]  # duplicates
+        s = np.sort(keys)
+        for r in {0, n // 2, n - 1}:
+            assert oracle.radix_select_u32(keys, r) == s[r]
+        k64 = RNG.integers(0, 2 ** 63, size=n, dtype=np.uint64) * 2 + 1
+        s64 = np.sort(k64)
+        for r in {0, n // 3, n - 1}:
+            assert oracle.radix_select_u64(k64, r) == s64[r]
+
+
+def test_radix_select_float_threshold():
+    # gpu/GPA.h:183-187 known-answer method: select(n-k) on float bit
+    # patterns of non-negative scores == k-th largest by sort
+    f = RNG.random(10_000, dtype=np.float32)
+    bits = f.view(np.uint32)
+    k = 64
+    got = np.uint32(oracle.radix_select_u32(bits, f.size - k)).view(np.float32)
+    assert got == np.sort(f)[-k]
+
+
+def test_normalize_closed_form():
+    x = np.array([[2.0, 4.0, 6.0], [-4.0, -2.0, -3.0]], dtype=np.float32)
+    y = oracle.normalize(x, 3)
+    assert np.array_equal(y[0], np.array([0.0, 0.5, 1.0], np.float32))
+    # max starts at 0 (common/common.h:36): an all-negative column keeps max 0
+    assert np.array_equal(y[1], np.array([0.0, 0.5, 0.25], np.float32))
+
+
+def test_reorder_attr_invariants():
+    n, d = 3000, 8
+    x = RNG.random((d, n), dtype=np.float32)
+    x[:, :10] = 0.5  # ties
+    y = oracle.reorder_attr(x, n)
+    assert oracle.check_order(y, n) == n
+    assert oracle.check_order(x, n) < n
+    assert np.array_equal(np.sort(x, axis=0), np.sort(y, axis=0))
+    assert np.array_equal(y[0], x.max(axis=0))
+    assert np.array_equal(y[-1], x.min(axis=0))
+
+
+def test_bta_blocks_match_sort():
+    n, blk, k = 10_000, 4096, 32
+    x = RNG.random((3, n), dtype=np.float32)
+    w = np.array([0.3, 1.0, 0.7], dtype=np.float32)
+    out = oracle.bta_blocks(x, n, w, [2, 0, 1], blk, k)
+    s = oracle.score(x, n, w, [2, 0, 1])
+    for b in range(out.shape[0]):
+        lo, hi = b * blk, min(n, (b + 1) * blk)
+        order = np.lexsort((np.arange(lo, hi), -s[lo:hi]))[:k] + lo
+        keys = [oracle.key(s[i], i) for i in order]
+        assert out[b].tolist() == keys
+
+
+def test_gpa_step_known_answer():
+    ids = np.array([0, 1, 2, 3], dtype=np.uint32)
+    score = np.array([1.0, 0.5, 0.25, 0.0], dtype=np.float32)
+    colC = np.array([0.5, 0.25, 0.5, 0.125], dtype=np.float32)
+    colN = np.array([1.0, 2.0, 3.0, 4.0], dtype=np.float32)
+    # upper bounds with suffix_len 2: 2.0, 1.0, 1.25, 0.25; keep >= 1.0
+    i2, s2 = oracle.gpa_step(ids, score, colC, colN, 1.0, 2.0)
+    assert i2.tolist() == [0, 1, 2] and s2.tolist() == [2.0, 2.5, 3.25]
+
+
+def _random_layout(x, n, psize, rng, cluster):
+    d = x.shape[0]
+    if cluster:
+        perm = np.lexsort(tuple((x[m] * 4).astype(int) for m in range(d)))
+    else:
+        perm = rng.permutation(n)
+    npart = (n + psize - 1) // psize
+    ld = npart * psize
+    pdata = np.zeros((d, ld), np.float32)
+    pdata[:, :n] = x[:, perm]
+    pid = np.zeros(ld, np.uint32)
+    pid[:n] = perm
+    pmax = np.zeros((npart, d), np.float32)
+    pmin = np.zeros((npart, d), np.float32)
+    for p in range(npart):
+        seg = pdata[:, p * psize:min(n, (p + 1) * psize)]
+        pmax[p] = seg.max(axis=1)
+        pmin[p] = seg.min(axis=1)
+    return pdata, pid, pmax, pmin
+
+
+@pytest.mark.parametrize("cluster", [False, True])
+def test_pta_equals_fte(cluster):
+    n, d, psize = 40_000, 4, 1024
+    x = RNG.random((d, n), dtype=np.float32)
+    pdata, pid, pmax, pmin = _random_layout(x, n, psize, RNG, cluster)
+    for w, attr, k in ((np.ones(d, np.float32), [0, 1, 2, 3], 16),
+                       (RNG.uniform(0.1, 1, d).astype(np.float32), [3, 1], 50),
+                       (np.array([1.0, -0.5, 0.25, 2.0], np.float32), [1, 0, 3], 8)):
+        ref = oracle.fte_topk(x, n, k, w, attr)
+        got = oracle.pta_topk(pdata, pid, n, psize, pmax, pmin, k, w, attr)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        got_t = oracle.pta_topk(pdata, pid, n, psize, pmax, pmin, k, w, attr, threads=4)
+        assert np.array_equal(got_t[0], ref[0]) and np.array_equal(got_t[1], ref[1])
+        if cluster:
+            assert got[3] < pmax.shape[0]  # some partitions were skipped
+
+
+def test_pta_bound_is_upper_bound():
+    n, d, psize = 8192, 3, 1024
+    x = RNG.random((d, n), dtype=np.float32)
+    pdata, pid, pmax, pmin = _random_layout(x, n, psize, RNG, True)
+    w = np.array([0.7, -1.3, 0.2], np.float32)
+    attr = [2, 1, 0]
+    s = oracle.score(pdata, n, w, attr)
+    for p in range(pmax.shape[0]):
+        b = oracle.pta_bound(pmax, pmin, d, p, w, attr)
+        assert s[p * psize:(p + 1) * psize].max() <= b
+
+
+def test_mlp_gradient_finite_differences():
+    nparam = oracle.mlp_nparams()
+    assert nparam == 96 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    p = RNG.normal(0, 0.2, nparam)
+    x = RNG.normal(0, 1, (5, 96))
+    y = RNG.normal(0, 1, 5)
+    loss, g = oracle.mlp_backward(p, x, y)
+    pred = oracle.mlp_forward(p, x)
+    assert abs(loss - np.mean((pred - y) ** 2)) < 1e-12
+    idx = RNG.choice(nparam, 60, replace=False).tolist() + [nparam - 1, 0, 96 * 64]
+    for i in idx:
+        e = 1e-6
+        pp = p.copy(); pp[i] += e
+        pm = p.copy(); pm[i] -= e
+        fd = (np.mean((oracle.mlp_forward(pp, x) - y) ** 2) - np.mean((oracle.mlp_forward(pm, x) - y) ** 2)) / (2 * e)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_mlp_forward_closed_form():
+    p = np.zeros(oracle.mlp_nparams())
+    p[-1] = 3.25  # b3 only -> constant output
+    assert np.all(oracle.mlp_forward(p, RNG.normal(0, 1, (4, 96))) == 3.25)
+    # one path: W1[0][5]=2, W2[0][0]=3, W3[0]=0.5 -> out = 0.5*relu(3*relu(2*x5))
+    p = np.zeros(oracle.mlp_nparams())
+    p[5] = 2.0
+    p[96 * 64 + 64] = 3.0
+    p[96 * 64 + 64 + 64 * 64 + 64] = 0.5
+    x = np.zeros((2, 96)); x[0, 5] = 1.5; x[1, 5] = -1.0
+    assert oracle.mlp_forward(p, x).tolist() == [4.5, 0.0]

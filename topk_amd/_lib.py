@@ -1,0 +1,109 @@
+"""ctypes binding of ``libtopk_amd.so`` (the C ABI declared in include/topk.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C topk_amd/csrc``).  There is no fallback: if the shared library is
+missing, :func:`lib` raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtopk_amd.so")
+
+TOPK_MAX_DIMS = 32
+TOPK_MAX_K = 1024
+
+TOPK_OK = 0
+TOPK_ERR_INVALID = 1
+TOPK_ERR_NOMEM = 2
+TOPK_ERR_HIP = 3
+TOPK_ERR_UNSUPPORTED = 4
+TOPK_ERR_NONFINITE = 5
+TOPK_ERR_STATE = 6
+TOPK_ERR_NO_DEVICE = 7
+TOPK_ERR_OVERFLOW = 8
+
+TOPK_ROWMAJOR = 0x1
+TOPK_INIT_NORMALIZE = 0x1
+TOPK_INIT_PTA = 0x2
+
+CORRELATED, INDEPENDENT, ANTICORRELATED, MIXED = 0, 1, 2, 3
+
+_c = ctypes
+_vp = _c.c_void_p
+
+
+class TopkStats(ctypes.Structure):
+    _fields_ = [("tuple_count", _c.c_uint64), ("candidates", _c.c_uint64),
+                ("partitions_scored", _c.c_uint64), ("partitions_total", _c.c_uint64),
+                ("fallback", _c.c_uint32), ("queries", _c.c_uint32),
+                ("ms_last", _c.c_double), ("tt_init", _c.c_double)]
+
+
+class TopkQuery(ctypes.Structure):
+    _fields_ = [("k", _c.c_uint32), ("qq", _c.c_uint32),
+                ("attr", _c.c_uint8 * TOPK_MAX_DIMS), ("weights", _c.c_float * TOPK_MAX_DIMS)]
+
+
+# name -> (restype, argtypes); mirrors include/topk.h one to one
+SIGNATURES = {
+    "topk_abi_version": (_c.c_int, []),
+    "topk_strerror": (_c.c_char_p, [_c.c_int]),
+    "topk_ctx_create": (_c.c_int, [_c.POINTER(_vp), _vp, _c.c_uint64, _c.c_uint32, _c.c_uint32]),
+    "topk_ctx_create_generated": (_c.c_int, [_c.POINTER(_vp), _c.c_uint64, _c.c_uint32, _c.c_int,
+                                             _c.c_uint64]),
+    "topk_ctx_destroy": (_c.c_int, [_vp]),
+    "topk_ctx_n": (_c.c_uint64, [_vp]),
+    "topk_ctx_d": (_c.c_uint32, [_vp]),
+    "topk_ctx_read_columns": (_c.c_int, [_vp, _vp]),
+    "topk_init": (_c.c_int, [_vp, _c.c_uint32]),
+    "topk_set_partition_size": (_c.c_int, [_vp, _c.c_uint32]),
+    "topk_reorder_attr": (_c.c_int, [_vp]),
+    "topk_fte": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
+    "topk_pta": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
+    "topk_bta": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
+    "topk_gpa": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
+    "topk_batch": (_c.c_int, [_vp, _c.c_uint32, _vp, _c.c_uint32, _vp, _vp, _vp]),
+    "topk_fte_async": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp]),
+    "topk_sync": (_c.c_int, [_vp]),
+    "topk_stream": (_vp, [_vp]),
+    "topk_get_stats": (_c.c_int, [_vp, _c.POINTER(TopkStats)]),
+    "topk_timing_enable": (_c.c_int, [_vp, _c.c_int]),
+    "topk_timing_get": (_c.c_int, [_vp, _c.c_char_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_uint64)]),
+    "topk_timing_reset": (_c.c_int, [_vp]),
+    "topk_generate": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint32, _c.c_uint64, _c.c_int, _c.c_uint64,
+                                 _c.c_uint64, _c.c_uint64, _c.c_int]),
+    "topk_radix_select_u32": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.c_uint64, _c.POINTER(_c.c_uint32)]),
+}
+
+_LIB = None
+
+
+class TopkError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().topk_strerror(status).decode()
+        super().__init__(f"{what}: topk status {status} ({msg})" if what else f"topk status {status} ({msg})")
+
+
+def lib():
+    """Load libtopk_amd.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "or `make -C topk_amd/csrc` (there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(status, what=""):
+    if status != TOPK_OK:
+        raise TopkError(status, what)
+    return status

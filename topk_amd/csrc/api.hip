@@ -1,0 +1,397 @@
+// C ABI of the top-k engine: context / column store, synchronous and
+// asynchronous FTE queries, statistics, kernel timing, generator.
+//
+// Reference interfaces replaced: AA<T,Z>(n,d) + get_cdata + ~AA
+// (cpu/AA.h:63-150), File::load/gen (input/File.h:314-354),
+// TPAc::findTopK* (cpu/TPAc.h:19-22), AA::benchmark counters
+// (cpu/AA.h:194-223).
+#include <string.h>
+
+#include <chrono>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace topk {
+
+int hip_status(hipError_t e) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return TOPK_ERR_NOMEM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return TOPK_ERR_NO_DEVICE;
+    return TOPK_ERR_HIP;
+}
+
+void timing_begin(topk_ctx *ctx, const char *, hipEvent_t &ev0) {
+    ev0 = nullptr;
+    if (!ctx->timing_on) return;
+    if (hipEventCreate(&ev0) != hipSuccess) { ev0 = nullptr; return; }
+    hipEventRecord(ev0, ctx->stream);
+}
+
+void timing_end(topk_ctx *ctx, const char *name, hipEvent_t ev0) {
+    if (!ev0) return;
+    hipEvent_t ev1;
+    if (hipEventCreate(&ev1) != hipSuccess) { hipEventDestroy(ev0); return; }
+    hipEventRecord(ev1, ctx->stream);
+    ctx->timing[name].pending.push_back({ev0, ev1});
+}
+
+static void timing_collect(topk_ctx *ctx) {
+    for (auto &kv : ctx->timing) {
+        for (auto &pr : kv.second.pending) {
+            float ms = 0;
+            hipEventSynchronize(pr.second);
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.total_ms += ms;
+            kv.second.launches++;
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+        kv.second.pending.clear();
+    }
+}
+
+int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq, const float *w,
+                const uint8_t *attr, QCols &q) {
+    if (!w || !attr) return TOPK_ERR_INVALID;
+    if (qq < 1 || qq > ctx->d) return TOPK_ERR_INVALID;
+    memset(&q, 0, sizeof(q));
+    for (uint32_t j = 0; j < qq; j++) {
+        if (attr[j] >= ctx->d) return TOPK_ERR_INVALID;
+        const float wj = w[attr[j]];
+        if (!std::isfinite(wj)) return TOPK_ERR_INVALID;
+        q.col[j] = cols + (uint64_t)attr[j] * ld;
+        q.w[j] = wj;
+    }
+    q.m = qq;
+    return TOPK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// store kernels
+// ---------------------------------------------------------------------------
+
+__global__ void transpose_rows_kernel(const float *__restrict__ rows, uint64_t n, uint32_t d, uint64_t ld,
+                                      float *__restrict__ cols) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = rows[i * d + m];
+}
+
+__global__ void generate_kernel(uint64_t n, uint32_t d, uint64_t ld, int distr, uint64_t seed,
+                                float *__restrict__ cols) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[TOPK_MAX_DIMS];
+    gen_object(v, n, d, distr, seed, i);
+    for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = v[m];
+}
+
+__global__ void nonfinite_kernel(const float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld,
+                                 unsigned int *__restrict__ flag) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    bool bad = false;
+    for (uint32_t m = 0; m < d; m++)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            bad |= !isfinite(cols[(uint64_t)m * ld + i]);
+    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+// Allocates the column store and per-query scratch of a fresh context.
+static int ctx_alloc(topk_ctx *ctx) {
+    ctx->ld = ((ctx->n + 4095) / 4096) * 4096;
+    TK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    TK_HIP(hipMalloc(&ctx->cols, sizeof(float) * ctx->ld * ctx->d));
+    TK_HIP(hipMemsetAsync(ctx->cols, 0, sizeof(float) * ctx->ld * ctx->d, ctx->stream));
+    TK_HIP(hipMalloc(&ctx->st, sizeof(FteState)));
+    TK_HIP(hipMemsetAsync(ctx->st, 0, sizeof(FteState), ctx->stream));
+    TK_HIP(hipMalloc(&ctx->blk, sizeof(uint64_t) * 256 * TOPK_MAX_K));
+    ctx->cand_cap = 1u << 20;
+    TK_HIP(hipMalloc(&ctx->cand, sizeof(uint64_t) * ctx->cand_cap));
+    ctx->res = ResultLayout::make(TOPK_MAX_K);
+    TK_HIP(hipMalloc(&ctx->d_res, ctx->res.bytes));
+    TK_HIP(hipHostMalloc(&ctx->h_res, ctx->res.bytes, hipHostMallocDefault));
+    return TOPK_OK;
+}
+
+static int check_finite(topk_ctx *ctx) {
+    unsigned int *flag = reinterpret_cast<unsigned int *>(ctx->d_res + ctx->res.status_off);
+    TK_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), ctx->stream));
+    hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, ctx->stream, ctx->cols, ctx->n, ctx->d, ctx->ld,
+                       flag);
+    TK_HIP(hipGetLastError());
+    unsigned int h = 0;
+    TK_HIP(hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return h ? TOPK_ERR_NONFINITE : TOPK_OK;
+}
+
+}  // namespace topk
+
+using namespace topk;
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int check_device() {
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt < 1) return TOPK_ERR_NO_DEVICE;
+    return TOPK_OK;
+}
+
+extern "C" {
+
+int topk_abi_version(void) { return TOPK_ABI_VERSION; }
+
+const char *topk_strerror(int s) {
+    switch (s) {
+        case TOPK_OK: return "ok";
+        case TOPK_ERR_INVALID: return "invalid argument";
+        case TOPK_ERR_NOMEM: return "out of memory";
+        case TOPK_ERR_HIP: return "HIP runtime error";
+        case TOPK_ERR_UNSUPPORTED: return "unsupported request";
+        case TOPK_ERR_NONFINITE: return "store holds NaN or infinity";
+        case TOPK_ERR_STATE: return "context not initialised for this call";
+        case TOPK_ERR_NO_DEVICE: return "no usable GPU";
+        case TOPK_ERR_OVERFLOW: return "candidate buffer overflow";
+        default: return "unknown status";
+    }
+}
+
+int topk_ctx_destroy(topk_ctx *ctx) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    timing_collect(ctx);
+    learned_free(ctx);
+    hipFree(ctx->cols);
+    hipFree(ctx->st);
+    hipFree(ctx->blk);
+    hipFree(ctx->cand);
+    hipFree(ctx->cand_big);
+    hipFree(ctx->d_res);
+    if (ctx->h_res) hipHostFree(ctx->h_res);
+    hipFree(ctx->batch_buf);
+    if (ctx->h_batch) hipHostFree(ctx->h_batch);
+    hipFree(ctx->pta.pcols);
+    hipFree(ctx->pta.pid);
+    hipFree(ctx->pta.pmax);
+    hipFree(ctx->pta.pmin);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TOPK_OK;
+}
+
+int topk_ctx_create(topk_ctx **out, const float *data, uint64_t n, uint32_t d, uint32_t flags) {
+    if (!out || !data || n < 1 || n >= 0xFFFFFFFFull || d < 1 || d > TOPK_MAX_DIMS) return TOPK_ERR_INVALID;
+    *out = nullptr;
+    int rc = check_device();
+    if (rc) return rc;
+    topk_ctx *ctx = new topk_ctx();
+    ctx->n = n;
+    ctx->d = d;
+    hipGetDevice(&ctx->device);
+    rc = ctx_alloc(ctx);
+    if (!rc) {
+        if (flags & TOPK_ROWMAJOR) {
+            float *tmp = nullptr;
+            if (hipMalloc(&tmp, sizeof(float) * n * d) != hipSuccess) rc = TOPK_ERR_NOMEM;
+            if (!rc && hipMemcpyAsync(tmp, data, sizeof(float) * n * d, hipMemcpyHostToDevice, ctx->stream))
+                rc = TOPK_ERR_HIP;
+            if (!rc) {
+                hipLaunchKernelGGL(transpose_rows_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, tmp, n,
+                                   d, ctx->ld, ctx->cols);
+                if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = TOPK_ERR_HIP;
+            }
+            hipFree(tmp);
+        } else {
+            for (uint32_t m = 0; m < d && !rc; m++)
+                if (hipMemcpyAsync(ctx->cols + (uint64_t)m * ctx->ld, data + (uint64_t)m * n, sizeof(float) * n,
+                                   hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                    rc = TOPK_ERR_HIP;
+        }
+    }
+    if (!rc) rc = check_finite(ctx);
+    if (rc) { topk_ctx_destroy(ctx); return rc; }
+    *out = ctx;
+    return TOPK_OK;
+}
+
+int topk_ctx_create_generated(topk_ctx **out, uint64_t n, uint32_t d, int distr, uint64_t seed) {
+    if (!out || n < 1 || n >= 0xFFFFFFFFull || d < 1 || d > TOPK_MAX_DIMS || distr < 0 || distr > 3)
+        return TOPK_ERR_INVALID;
+    *out = nullptr;
+    int rc = check_device();
+    if (rc) return rc;
+    topk_ctx *ctx = new topk_ctx();
+    ctx->n = n;
+    ctx->d = d;
+    hipGetDevice(&ctx->device);
+    rc = ctx_alloc(ctx);
+    if (!rc) {
+        hipLaunchKernelGGL(generate_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, d, ctx->ld, distr,
+                           seed, ctx->cols);
+        if (hipGetLastError() != hipSuccess) rc = TOPK_ERR_HIP;
+    }
+    if (!rc) rc = check_finite(ctx);
+    if (rc) { topk_ctx_destroy(ctx); return rc; }
+    *out = ctx;
+    return TOPK_OK;
+}
+
+uint64_t topk_ctx_n(const topk_ctx *ctx) { return ctx ? ctx->n : 0; }
+uint32_t topk_ctx_d(const topk_ctx *ctx) { return ctx ? ctx->d : 0; }
+void *topk_stream(topk_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int topk_ctx_read_columns(topk_ctx *ctx, float *out) {
+    if (!ctx || !out) return TOPK_ERR_INVALID;
+    for (uint32_t m = 0; m < ctx->d; m++)
+        TK_HIP(hipMemcpyAsync(out + (uint64_t)m * ctx->n, ctx->cols + (uint64_t)m * ctx->ld,
+                              sizeof(float) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TOPK_OK;
+}
+
+static int check_query_args(topk_ctx *ctx, uint32_t k) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    if (k < 1) return TOPK_ERR_INVALID;
+    if (k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    return TOPK_OK;
+}
+
+// Copies a finished result block to the caller (best first, sentinels past n).
+static void emit_result(topk_ctx *ctx, uint32_t k, uint64_t *out_ids, float *out_scores, float *out_thr) {
+    const uint64_t *ids = reinterpret_cast<const uint64_t *>(ctx->h_res + ctx->res.ids_off);
+    const float *sc = reinterpret_cast<const float *>(ctx->h_res + ctx->res.scores_off);
+    if (out_ids) memcpy(out_ids, ids, sizeof(uint64_t) * k);
+    if (out_scores) memcpy(out_scores, sc, sizeof(float) * k);
+    if (out_thr) *out_thr = *reinterpret_cast<const float *>(ctx->h_res + ctx->res.thr_off);
+}
+
+static int ensure_big(topk_ctx *ctx) {
+    if (ctx->cand_big) return TOPK_OK;
+    ctx->cand_big_cap = ctx->n + TOPK_MAX_K;
+    TK_HIP(hipMalloc(&ctx->cand_big, sizeof(uint64_t) * ctx->cand_big_cap));
+    return TOPK_OK;
+}
+
+// Runs one FTE query into the context's result block and copies it to the
+// pinned host mirror; on candidate overflow re-runs with an n-entry buffer.
+static int fte_run_sync(topk_ctx *ctx, uint32_t k, const QCols &q) {
+    uint64_t *d_ids = reinterpret_cast<uint64_t *>(ctx->d_res + ctx->res.ids_off);
+    float *d_sc = reinterpret_cast<float *>(ctx->d_res + ctx->res.scores_off);
+    float *d_thr = reinterpret_cast<float *>(ctx->d_res + ctx->res.thr_off);
+    uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
+    ctx->stats.fallback = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        const bool big = attempt == 1;
+        if (big) {
+            int rc = ensure_big(ctx);
+            if (rc) return rc;
+            TK_HIP(hipMemsetAsync(&ctx->st->status, 0, sizeof(uint32_t), ctx->stream));
+        }
+        int rc = fte_enqueue(ctx, k, q, d_ids, d_sc, d_thr, d_status, big);
+        if (rc) return rc;
+        TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        TK_HIP(hipStreamSynchronize(ctx->stream));
+        const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
+        if (!(status & STATUS_OVERFLOW)) return TOPK_OK;
+        ctx->stats.fallback = 1;
+    }
+    return TOPK_ERR_OVERFLOW;
+}
+
+int topk_fte(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *out_ids,
+             float *out_scores, float *out_threshold) {
+    int rc = check_query_args(ctx, k);
+    if (rc) return rc;
+    QCols q;
+    rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    rc = fte_run_sync(ctx, k, q);
+    if (rc) return rc;
+    emit_result(ctx, k, out_ids, out_scores, out_threshold);
+    ctx->stats.ms_last = now_ms() - t0;
+    ctx->stats.tuple_count = ctx->n;
+    ctx->stats.queries = 1;
+    return TOPK_OK;
+}
+
+int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+                   uint64_t *d_ids, float *d_scores) {
+    int rc = check_query_args(ctx, k);
+    if (rc) return rc;
+    if (!d_ids || !d_scores) return TOPK_ERR_INVALID;
+    QCols q;
+    rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
+    if (rc) return rc;
+    return fte_enqueue(ctx, k, q, d_ids, d_scores, nullptr, nullptr, false);
+}
+
+int topk_sync(topk_ctx *ctx) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    uint32_t status = 0;
+    TK_HIP(hipMemcpy(&status, &ctx->st->status, sizeof(status), hipMemcpyDeviceToHost));
+    if (status) {
+        TK_HIP(hipMemset(&ctx->st->status, 0, sizeof(uint32_t)));
+        return TOPK_ERR_OVERFLOW;
+    }
+    return TOPK_OK;
+}
+
+int topk_get_stats(topk_ctx *ctx, topk_stats *out) {
+    if (!ctx || !out) return TOPK_ERR_INVALID;
+    *out = ctx->stats;
+    return TOPK_OK;
+}
+
+int topk_timing_enable(topk_ctx *ctx, int on) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    ctx->timing_on = on != 0;
+    return TOPK_OK;
+}
+
+int topk_timing_get(topk_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches) {
+    if (!ctx || !kernel) return TOPK_ERR_INVALID;
+    timing_collect(ctx);
+    auto it = ctx->timing.find(kernel);
+    if (total_ms) *total_ms = it == ctx->timing.end() ? 0.0 : it->second.total_ms;
+    if (launches) *launches = it == ctx->timing.end() ? 0 : it->second.launches;
+    return TOPK_OK;
+}
+
+int topk_timing_reset(topk_ctx *ctx) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    timing_collect(ctx);
+    ctx->timing.clear();
+    return TOPK_OK;
+}
+
+int topk_generate(float *out, uint64_t n, uint32_t d, uint64_t ld, int distr, uint64_t seed, uint64_t i0,
+                  uint64_t count, int threads) {
+    if (!out || d < 1 || d > TOPK_MAX_DIMS || distr < 0 || distr > 3 || i0 + count > n || ld < count)
+        return TOPK_ERR_INVALID;
+    if (threads < 1) threads = (int)std::thread::hardware_concurrency();
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > count / 4096 + 1) threads = (int)(count / 4096 + 1);
+    auto work = [&](int t) {
+        const uint64_t lo = i0 + count * t / threads, hi = i0 + count * (t + 1) / threads;
+        float v[TOPK_MAX_DIMS];
+        for (uint64_t i = lo; i < hi; i++) {
+            gen_object(v, n, d, distr, seed, i);
+            for (uint32_t m = 0; m < d; m++) out[(uint64_t)m * ld + (i - i0)] = v[m];
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; t++) pool.emplace_back(work, t);
+    work(0);
+    for (auto &th : pool) th.join();
+    return TOPK_OK;
+}
+
+}  // extern "C"
