@@ -163,6 +163,17 @@ int topk_sync(topk_ctx *ctx);
 /* The context's hipStream_t, for callers that time or order work on it. */
 void *topk_stream(topk_ctx *ctx);
 
+/* ---- configuration (replaces the reference's compile-time knobs,
+ *      Makefile:7-54 / debug.sh:6-91, with run-time ones) ---- */
+
+/* Sets a launch-shape knob of the context:
+ *   "stream_grid"     blocks of the FTE stream pass (0 = default)
+ *   "stream_vpt"      float4 vectors per lane per column of a stream tile:
+ *                     1, 2, 4 or 8 (0 = chosen from the query width)
+ *   "fte_force_exact" 1 = seed FTE queries with the exact prefix pass
+ * Unknown names or values return TOPK_ERR_INVALID. */
+int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value);
+
 /* ---- statistics (reference STATS_EFF counters, cpu/AA.h:194-223) ---- */
 
 typedef struct topk_stats {

@@ -141,11 +141,15 @@ def test_nonfinite_rejected(gpu):
 
 
 def test_fte_overflow_falls_back_exactly(gpu):
-    # scores rise with the id, so the prefix threshold is weak and more than
-    # the candidate capacity (2^20) objects pass the stream filter
-    n = 3_000_000
+    # scores rise with the id, so the exact-prefix threshold is weak and more
+    # objects pass the stream filter than the candidate slots (2^21) and the
+    # spill region (2^21) hold together
+    n = 8_000_000
     x = (np.arange(n, dtype=np.float64) / n).astype(np.float32)[None, :]
     with topk_amd.Engine(x) as eng:
+        _check(eng, x, n, 64, np.ones(1, np.float32), [0])
+        assert eng.stats()["fallback"] == 0  # the spread sample copes
+        eng.set_tuning("fte_force_exact", 1)
         _check(eng, x, n, 64, np.ones(1, np.float32), [0])
         assert eng.stats()["fallback"] == 1
         _check(eng, x, n, 64, -np.ones(1, np.float32), [0])
@@ -184,3 +188,40 @@ def test_fte_full_size_parity(gpu):
         rid, rsc, rthr = oracle.tpac_threads(x, n, 64, w, attr, 16)
         assert np.array_equal(ids, rid) and np.array_equal(sc, rsc) and thr == rthr
         del x
+
+
+def test_fte_exact_seed_mode_parity(gpu):
+    n, d = 2_500_000, 8
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 21)
+    with topk_amd.Engine(x) as eng:
+        eng.set_tuning("fte_force_exact", 1)
+        for k in (1, 64, 256):
+            _check(eng, x, n, k, RNG.uniform(0.1, 1, d).astype(np.float32), [0, 2, 4, 6, 7])
+        eng.set_tuning("fte_force_exact", 0)
+        _check(eng, x, n, 64, np.ones(d, np.float32), [0, 2, 4, 6, 7])
+
+
+def test_fte_sample_guess_too_high_retries(gpu):
+    # the best objects sit inside the first sample chunk, so the sample's
+    # 4th-best key admits fewer than k candidates and the exact path reruns
+    n, d = 4_000_000, 2
+    x = (RNG.random((d, n), dtype=np.float32) * 0.5).astype(np.float32)
+    x[:, :200] = 0.9
+    x[0, :200] += np.arange(200, dtype=np.float32) * 1e-4
+    with topk_amd.Engine(x) as eng:
+        _check(eng, x, n, 64, np.ones(d, np.float32), [0, 1])
+        assert eng.stats()["fallback"] == 1
+
+
+@pytest.mark.parametrize("vpt", [1, 2, 4, 8])
+def test_fte_stream_shapes(gpu, vpt):
+    n, d = 3_000_001, 8
+    x = topk_amd.generate(n, d, topk_amd.ANTICORRELATED, 5)
+    with topk_amd.Engine(x) as eng:
+        eng.set_tuning("stream_vpt", vpt)
+        for grid in (1, 7, 300, 0):
+            eng.set_tuning("stream_grid", grid)
+            _check(eng, x, n, 32, np.ones(d, np.float32), list(range(8)))
+            eng.set_tuning("fte_force_exact", 1)
+            _check(eng, x, n, 32, np.ones(d, np.float32), [7, 3])
+            eng.set_tuning("fte_force_exact", 0)

@@ -67,6 +67,7 @@ SIGNATURES = {
     "topk_fte_async": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp]),
     "topk_sync": (_c.c_int, [_vp]),
     "topk_stream": (_vp, [_vp]),
+    "topk_set_tuning": (_c.c_int, [_vp, _c.c_char_p, _c.c_int64]),
     "topk_get_stats": (_c.c_int, [_vp, _c.POINTER(TopkStats)]),
     "topk_timing_enable": (_c.c_int, [_vp, _c.c_int]),
     "topk_timing_get": (_c.c_int, [_vp, _c.c_char_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_uint64)]),

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <string>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -102,18 +103,25 @@ static uint32_t grid_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 
 
 // Allocates the column store and per-query scratch of a fresh context.
 static int ctx_alloc(topk_ctx *ctx) {
-    ctx->ld = ((ctx->n + 4095) / 4096) * 4096;
+    ctx->ld = ((ctx->n + 8191) / 8192) * 8192;
     TK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    TK_HIP(hipMalloc(&ctx->cols, sizeof(float) * ctx->ld * ctx->d));
-    TK_HIP(hipMemsetAsync(ctx->cols, 0, sizeof(float) * ctx->ld * ctx->d, ctx->stream));
+    const size_t col_floats = ctx->ld * ctx->d + 8192;
+    TK_HIP(hipMalloc(&ctx->cols, sizeof(float) * col_floats));
+    TK_HIP(hipMemsetAsync(ctx->cols, 0, sizeof(float) * col_floats, ctx->stream));
     TK_HIP(hipMalloc(&ctx->st, sizeof(FteState)));
     TK_HIP(hipMemsetAsync(ctx->st, 0, sizeof(FteState), ctx->stream));
     TK_HIP(hipMalloc(&ctx->blk, sizeof(uint64_t) * 256 * TOPK_MAX_K));
-    ctx->cand_cap = 1u << 20;
+    TK_HIP(hipMalloc(&ctx->pre, sizeof(uint64_t) * TOPK_MAX_K));
+    TK_HIP(hipMalloc(&ctx->counts, sizeof(uint32_t) * MAX_STREAM_GRID));
+    TK_HIP(hipMalloc(&ctx->offsets, sizeof(uint32_t) * (MAX_STREAM_GRID + 1)));
+    ctx->cand_cap = 1u << 21;
     TK_HIP(hipMalloc(&ctx->cand, sizeof(uint64_t) * ctx->cand_cap));
+    ctx->spill_cap = 1u << 21;
+    TK_HIP(hipMalloc(&ctx->spill, sizeof(uint64_t) * ctx->spill_cap));
     ctx->res = ResultLayout::make(TOPK_MAX_K);
     TK_HIP(hipMalloc(&ctx->d_res, ctx->res.bytes));
     TK_HIP(hipHostMalloc(&ctx->h_res, ctx->res.bytes, hipHostMallocDefault));
+    TK_HIP(hipHostMalloc(&ctx->h_state, sizeof(FteState), hipHostMallocDefault));
     return TOPK_OK;
 }
 
@@ -170,10 +178,16 @@ int topk_ctx_destroy(topk_ctx *ctx) {
     hipFree(ctx->cols);
     hipFree(ctx->st);
     hipFree(ctx->blk);
+    hipFree(ctx->pre);
+    hipFree(ctx->counts);
+    hipFree(ctx->offsets);
     hipFree(ctx->cand);
+    hipFree(ctx->spill);
     hipFree(ctx->cand_big);
+    hipFree(ctx->flat);
     hipFree(ctx->d_res);
     if (ctx->h_res) hipHostFree(ctx->h_res);
+    if (ctx->h_state) hipHostFree(ctx->h_state);
     hipFree(ctx->batch_buf);
     if (ctx->h_batch) hipHostFree(ctx->h_batch);
     hipFree(ctx->pta.pcols);
@@ -271,35 +285,50 @@ static void emit_result(topk_ctx *ctx, uint32_t k, uint64_t *out_ids, float *out
     if (out_thr) *out_thr = *reinterpret_cast<const float *>(ctx->h_res + ctx->res.thr_off);
 }
 
+// Fallback buffers: slots large enough that no stream block can overflow
+// (every block's objects fit in its slot) and a flat gather buffer.
 static int ensure_big(topk_ctx *ctx) {
-    if (ctx->cand_big) return TOPK_OK;
-    ctx->cand_big_cap = ctx->n + TOPK_MAX_K;
-    TK_HIP(hipMalloc(&ctx->cand_big, sizeof(uint64_t) * ctx->cand_big_cap));
+    const uint64_t need = ctx->n + (uint64_t)MAX_STREAM_GRID * 8192 + TOPK_MAX_K;
+    if (ctx->cand_big && ctx->cand_big_cap >= need) return TOPK_OK;
+    hipFree(ctx->cand_big);
+    hipFree(ctx->flat);
+    ctx->cand_big = nullptr;
+    ctx->flat = nullptr;
+    TK_HIP(hipMalloc(&ctx->cand_big, sizeof(uint64_t) * need));
+    ctx->cand_big_cap = need;
+    TK_HIP(hipMalloc(&ctx->flat, sizeof(uint64_t) * (ctx->n + TOPK_MAX_K)));
+    ctx->flat_cap = ctx->n + TOPK_MAX_K;
     return TOPK_OK;
 }
 
 // Runs one FTE query into the context's result block and copies it to the
-// pinned host mirror; on candidate overflow re-runs with an n-entry buffer.
+// pinned host mirror.  The sample-seeded pipeline runs first; if its guess
+// left too few candidates (STATUS_RETRY) or too many for the buffer
+// (STATUS_OVERFLOW), the exact-prefix pipeline runs, with an n-entry
+// candidate buffer after an overflow.
 static int fte_run_sync(topk_ctx *ctx, uint32_t k, const QCols &q) {
     uint64_t *d_ids = reinterpret_cast<uint64_t *>(ctx->d_res + ctx->res.ids_off);
     float *d_sc = reinterpret_cast<float *>(ctx->d_res + ctx->res.scores_off);
     float *d_thr = reinterpret_cast<float *>(ctx->d_res + ctx->res.thr_off);
     uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
     ctx->stats.fallback = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
-        const bool big = attempt == 1;
-        if (big) {
+    int mode = ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (mode == FTE_EXACT_BIG) {
             int rc = ensure_big(ctx);
             if (rc) return rc;
-            TK_HIP(hipMemsetAsync(&ctx->st->status, 0, sizeof(uint32_t), ctx->stream));
         }
-        int rc = fte_enqueue(ctx, k, q, d_ids, d_sc, d_thr, d_status, big);
+        int rc = fte_enqueue(ctx, k, q, d_ids, d_sc, d_thr, d_status, mode);
         if (rc) return rc;
         TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        TK_HIP(hipMemcpyAsync(ctx->h_state, ctx->st, sizeof(FteState), hipMemcpyDeviceToHost, ctx->stream));
         TK_HIP(hipStreamSynchronize(ctx->stream));
         const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
-        if (!(status & STATUS_OVERFLOW)) return TOPK_OK;
+        ctx->stats.candidates = ctx->h_state->cand_last;
+        if (!status) return TOPK_OK;
+        TK_HIP(hipMemsetAsync(&ctx->st->status, 0, sizeof(uint32_t), ctx->stream));
         ctx->stats.fallback = 1;
+        mode = (status & STATUS_OVERFLOW) ? FTE_EXACT_BIG : FTE_EXACT;
     }
     return TOPK_ERR_OVERFLOW;
 }
@@ -329,7 +358,7 @@ int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, 
     QCols q;
     rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
     if (rc) return rc;
-    return fte_enqueue(ctx, k, q, d_ids, d_scores, nullptr, nullptr, false);
+    return fte_enqueue(ctx, k, q, d_ids, d_scores, nullptr, nullptr, ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE);
 }
 
 int topk_sync(topk_ctx *ctx) {
@@ -340,6 +369,23 @@ int topk_sync(topk_ctx *ctx) {
     if (status) {
         TK_HIP(hipMemset(&ctx->st->status, 0, sizeof(uint32_t)));
         return TOPK_ERR_OVERFLOW;
+    }
+    return TOPK_OK;
+}
+
+int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
+    if (!ctx || !name) return TOPK_ERR_INVALID;
+    const std::string key(name);
+    if (key == "stream_grid") {
+        if (value < 0 || value > (int64_t)MAX_STREAM_GRID) return TOPK_ERR_INVALID;
+        ctx->tune.stream_grid = (uint32_t)value;
+    } else if (key == "stream_vpt") {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return TOPK_ERR_INVALID;
+        ctx->tune.stream_vpt = (int)value;
+    } else if (key == "fte_force_exact") {
+        ctx->tune.fte_force_exact = value ? 1 : 0;
+    } else {
+        return TOPK_ERR_INVALID;
     }
     return TOPK_OK;
 }

@@ -1,5 +1,6 @@
-// Wave64 and workgroup primitives for gfx950: scans, an exact LDS radix
-// select over 64-bit ranking keys, and a bitonic sort in LDS.
+// Wave64 and workgroup primitives for gfx950: scans and reductions, an exact
+// workgroup radix select over 64-bit ranking keys, key gathering and a rank
+// sort in LDS.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,6 +34,15 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     return v;
 }
 
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -44,43 +54,111 @@ __device__ __forceinline__ f4 ld_stream(const float *p) {
     return __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
 }
 
-// LDS scratch of the block-wide selection.
+// LDS scratch of the workgroup selection.
 struct SelectScratch {
     uint32_t hist[256];
-    uint32_t digit, need, bcnt, flag;
-    unsigned long long kmin;
+    uint32_t digit, need, bcnt, counter;
+    unsigned long long kmin, kmax;
 };
 
-// Exact k-th largest of keys[0..cnt) (1 <= k <= cnt), computed by every
-// thread of the block (all threads must call; BS = blockDim.x, a multiple of
-// 64).  MSD radix select over 8-bit digits with LDS histograms; the digit
-// holding the k-th key is found by one wave with a lane prefix scan.  When
-// the bucket holding the k-th key lies entirely inside the top k, the
-// remaining digits are skipped and the bucket minimum is returned.  Keys may
-// repeat; the returned value is still the k-th largest.
-template <int BS, typename KeyPtr>
-__device__ uint64_t block_kth_largest(KeyPtr keys, uint32_t cnt, uint32_t k, SelectScratch &s) {
+// Key sources: the keys a thread owns, visited by for_each.  Key 0 marks an
+// empty slot and is never selected.
+template <int KPT>
+struct RegKeys {
+    uint64_t k[KPT];
+    template <class F>
+    __device__ __forceinline__ void for_each(F f) const {
+#pragma unroll
+        for (int i = 0; i < KPT; i++) f(k[i]);
+    }
+};
+
+template <int BS>
+struct MemKeys {
+    const uint64_t *p;
+    uint32_t cnt;
+    template <class F>
+    __device__ __forceinline__ void for_each(F f) const {
+        uint32_t i = threadIdx.x;
+        for (; i + 7 * BS < cnt; i += 8 * BS) {  // 8 independent loads in flight
+            uint64_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = p[i + j * BS];
+#pragma unroll
+            for (int j = 0; j < 8; j++) f(v[j]);
+        }
+        for (; i < cnt; i += BS) f(p[i]);
+    }
+};
+
+// Block-wide min / max of u64 (all threads call; result on every thread).
+template <int BS>
+__device__ __forceinline__ void block_minmax(uint64_t &lo, uint64_t &hi, SelectScratch &s) {
+    lo = wave_min_u64(lo);
+    hi = wave_max_u64(hi);
+    if (BS == 64) return;
+    if (threadIdx.x == 0) { s.kmin = ~0ull; s.kmax = 0; }
+    __syncthreads();
+    if (lane_id() == 0) {
+        atomicMin(&s.kmin, (unsigned long long)lo);
+        atomicMax(&s.kmax, (unsigned long long)hi);
+    }
+    __syncthreads();
+    lo = s.kmin;
+    hi = s.kmax;
+    __syncthreads();
+}
+
+// Exact k-th largest non-zero key of the block's key source, where setcnt
+// (>= k >= 1) is the number of non-zero keys.  Every thread of the block
+// (BS = blockDim.x, a multiple of 64) must call it and gets the result.
+//
+// MSD radix select that never spends a pass on bits all remaining
+// candidates share: each round takes the min / max of the candidate set,
+// starts its 8-bit digit just below their common prefix, builds the digit
+// histogram in LDS, and one wave finds the digit holding the k-th key with
+// a lane prefix scan.  Rounds stop when the candidate set is one value, when
+// the k-th is its maximum, or when the whole set is inside the top k (then
+// the k-th is its minimum).
+template <int BS, class Src>
+__device__ uint64_t block_kth_largest(const Src &src, uint32_t setcnt, uint32_t k, SelectScratch &s) {
     const uint32_t tid = threadIdx.x;
     uint64_t prefix = 0, mask = 0;
     uint32_t need = k;
-    for (int shift = 56; shift >= 0; shift -= 8) {
+    for (;;) {
+        uint64_t lo = ~0ull, hi = 0;
+        src.for_each([&](uint64_t key) {
+            if (key && (key & mask) == prefix) {
+                lo = key < lo ? key : lo;
+                hi = key > hi ? key : hi;
+            }
+        });
+        block_minmax<BS>(lo, hi, s);
+        if (need == setcnt) return lo;
+        if (need == 1 || lo == hi) return hi;
+        const int db = 63 - __clzll(lo ^ hi);          // highest differing bit
+        const uint64_t above = db == 63 ? 0ull : (~0ull << (db + 1));
+        prefix = hi & above;
+        mask = above;
+        const int width = db + 1 < 8 ? db + 1 : 8;
+        const int shift = db + 1 - width;
+        const uint32_t dmask = (1u << width) - 1u;
         for (uint32_t i = tid; i < 256; i += BS) s.hist[i] = 0;
         __syncthreads();
-        for (uint32_t i = tid; i < cnt; i += BS) {
-            uint64_t key = keys[i];
-            if ((key & mask) == prefix) atomicAdd(&s.hist[(uint32_t)(key >> shift) & 255u], 1u);
-        }
+        src.for_each([&](uint64_t key) {
+            if (key && (key & mask) == prefix) atomicAdd(&s.hist[(uint32_t)(key >> shift) & dmask], 1u);
+        });
         __syncthreads();
         if (tid < 64) {
             // lane l owns bins 255-4l .. 252-4l, highest first
-            uint32_t c0 = s.hist[255 - 4 * tid], c1 = s.hist[254 - 4 * tid];
-            uint32_t c2 = s.hist[253 - 4 * tid], c3 = s.hist[252 - 4 * tid];
-            uint32_t tot = c0 + c1 + c2 + c3;
-            uint32_t incl = wave_incl_scan(tot);
-            uint32_t excl = incl - tot;
+            const uint32_t c0 = s.hist[255 - 4 * tid], c1 = s.hist[254 - 4 * tid];
+            const uint32_t c2 = s.hist[253 - 4 * tid], c3 = s.hist[252 - 4 * tid];
+            const uint32_t tot = c0 + c1 + c2 + c3;
+            const uint32_t incl = wave_incl_scan(tot);
+            const uint32_t excl = incl - tot;
             if (excl < need && need <= incl) {
                 uint32_t r = need - excl, acc = 0;
-                uint32_t cs[4] = {c0, c1, c2, c3};
+                const uint32_t cs[4] = {c0, c1, c2, c3};
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if (r <= acc + cs[j]) {
@@ -94,57 +172,105 @@ __device__ uint64_t block_kth_largest(KeyPtr keys, uint32_t cnt, uint32_t k, Sel
             }
         }
         __syncthreads();
-        const uint32_t digit = s.digit, bcnt = s.bcnt;
+        const uint32_t digit = s.digit;
         need = s.need;
+        setcnt = s.bcnt;
+        __syncthreads();
         prefix |= (uint64_t)digit << shift;
-        mask |= (uint64_t)0xFF << shift;
-        if (bcnt == need) {
-            if (tid == 0) s.kmin = ~0ull;
-            __syncthreads();
-            uint64_t lm = ~0ull;
-            for (uint32_t i = tid; i < cnt; i += BS) {
-                uint64_t key = keys[i];
-                if ((key & mask) == prefix && key < lm) lm = key;
-            }
-            lm = wave_min_u64(lm);
-            if (lane_id() == 0) atomicMin(&s.kmin, (unsigned long long)lm);
-            __syncthreads();
-            uint64_t r = s.kmin;
-            __syncthreads();
-            return r;
-        }
+        mask |= (uint64_t)dmask << shift;
     }
-    __syncthreads();
-    return prefix;
 }
 
-// Appends every key >= thr from keys[0..cnt) to out[] (order unspecified)
-// using a wave-aggregated LDS counter; returns the number appended.  All
+// Writes every non-zero key >= thr of the source to out[] (order
+// unspecified) with a wave-aggregated LDS counter; returns the count.  All
 // threads of the block must call.
-template <int BS, typename KeyPtr, typename OutPtr>
-__device__ uint32_t block_gather_ge(KeyPtr keys, uint32_t cnt, uint64_t thr, OutPtr out,
-                                    uint32_t *lds_counter) {
-    const uint32_t tid = threadIdx.x;
-    if (tid == 0) *lds_counter = 0;
+template <int BS, class Src, class Out>
+__device__ uint32_t block_gather_ge(const Src &src, uint64_t thr, Out out, SelectScratch &s) {
+    if (threadIdx.x == 0) s.counter = 0;
     __syncthreads();
-    for (uint32_t i0 = 0; i0 < cnt; i0 += BS) {
-        uint32_t i = i0 + tid;
-        uint64_t key = i < cnt ? keys[i] : 0;
-        bool p = (i < cnt) && key >= thr;
-        uint64_t bal = __ballot(p);
+    src.for_each([&](uint64_t key) {
+        const bool p = key && key >= thr;
+        const uint64_t bal = __ballot(p);
         if (bal) {
-            uint32_t nw = __popcll(bal);
-            uint32_t rank = __popcll(bal & ((1ull << lane_id()) - 1ull));
+            const uint32_t rank = __popcll(bal & ((1ull << lane_id()) - 1ull));
             uint32_t base = 0;
-            if (lane_id() == 0) base = atomicAdd(lds_counter, nw);
+            if (lane_id() == 0) base = atomicAdd(&s.counter, (uint32_t)__popcll(bal));
             base = __shfl(base, 0, 64);
             if (p) out[base + rank] = key;
         }
-    }
+    });
     __syncthreads();
-    uint32_t r = *lds_counter;
+    const uint32_t r = s.counter;
     __syncthreads();
     return r;
+}
+
+// Memory key source for block_gather_ge: every lane of a wave must take
+// part in the ballot, so the trip count is uniform across the block.
+template <int BS>
+struct MemKeysUniform {
+    const uint64_t *p;
+    uint32_t cnt;
+    template <class F>
+    __device__ __forceinline__ void for_each(F f) const {
+        for (uint32_t i0 = 0; i0 < cnt; i0 += BS) {
+            const uint32_t i = i0 + threadIdx.x;
+            f(i < cnt ? p[i] : 0ull);
+        }
+    }
+};
+
+// Key source over a flat index space [0, cnt) resolved by a view's get(p);
+// Uniform=true keeps the trip count uniform across the block (for ballots).
+template <int BS, class View, bool Uniform>
+struct ViewKeys {
+    View v;
+    uint32_t cnt;
+    template <class F>
+    __device__ __forceinline__ void for_each(F f) const {
+        if constexpr (Uniform) {
+            for (uint32_t i0 = 0; i0 < cnt; i0 += BS) {
+                const uint32_t i = i0 + threadIdx.x;
+                f(i < cnt ? v.get(i) : 0ull);
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < cnt; i += BS) f(v.get(i));
+        }
+    }
+};
+
+// Block-wide exclusive prefix sum of one value per thread; returns the
+// thread's exclusive prefix and writes the block total to *total.
+template <int BS>
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *wave_sums, uint32_t *total) {
+    constexpr int NW = BS / 64;
+    const uint32_t w = threadIdx.x / 64;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane_id() == 63) wave_sums[w] = incl;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t ws = threadIdx.x < NW ? wave_sums[threadIdx.x] : 0u;
+        const uint32_t wi = wave_incl_scan(ws);
+        if (threadIdx.x < NW) wave_sums[threadIdx.x] = wi - ws;
+        if (threadIdx.x == NW - 1) *total = wi;
+    }
+    __syncthreads();
+    const uint32_t r = wave_sums[w] + incl - v;
+    __syncthreads();
+    return r;
+}
+
+// Sorts the got distinct keys in[0..got) descending into out[] by rank
+// counting (one pass, no barrier inside): out[#{j: in[j] > in[i]}] = in[i].
+template <int BS>
+__device__ void block_rank_sort_desc(const uint64_t *in, uint32_t got, uint64_t *out) {
+    for (uint32_t i = threadIdx.x; i < got; i += BS) {
+        const uint64_t x = in[i];
+        uint32_t r = 0;
+        for (uint32_t j = 0; j < got; j++) r += in[j] > x ? 1u : 0u;
+        out[r] = x;
+    }
+    __syncthreads();
 }
 
 // Sorts a[0..n2) descending in LDS (n2 a power of two); all threads call.
@@ -164,7 +290,7 @@ __device__ void bitonic_sort_desc(uint64_t *a, uint32_t n2) {
     }
 }
 
-__device__ __forceinline__ uint32_t next_pow2(uint32_t v) {
+__host__ __device__ __forceinline__ uint32_t next_pow2(uint32_t v) {
     uint32_t p = 1;
     while (p < v) p <<= 1;
     return p;

@@ -1,53 +1,60 @@
 // FTE (full-table evaluation) top-k on gfx950.
 //
 // Reference: cpu/TPAc.h:118-358 (scoring order, heap semantics) and the
-// empty GPU stub gpu/GFA.h:47-50.  The pipeline is three launches on the
-// context's stream, with no host round trip:
+// empty GPU stub gpu/GFA.h:47-50.  A query is three launches on the
+// context's stream with no host round trip:
 //
-//  1. prefix pass  (fte_prefix_kernel): the first P = G1*4096 objects are
-//     scored, each 4096-object block keeps its exact top-k (LDS radix
-//     select), and the last block to finish merges the G1*k block winners
-//     into the prefix's exact top-k.  Its k-th key tau is a safe lower bound
-//     of the global k-th key (k objects reach it).
-//  2. stream pass  (fte_stream_kernel): objects [P, n) are scored with
-//     128-bit column loads, scores stay in registers, and only keys > tau
-//     are appended (wave ballot + one atomic per wave) to a candidate list.
-//     No barrier and no LDS in the loop: a pure HBM stream.
-//  3. final select (fte_final_kernel): one workgroup selects the exact top-k
-//     of the candidates and bitonic-sorts them best first.
+//  1. seed pass    (fte_seed_kernel): a few 4096-object chunks are scored;
+//     each block keeps its top ks keys (register-resident radix select) and
+//     the last block to finish selects the ks-th largest of the block
+//     winners: tau.
+//       - sample mode: 16 chunks spread over the id range, small ks; tau is
+//         a guess whose safety is checked at the end (>= k candidates).
+//       - exact mode:  contiguous chunks [0, P), ks = k; the prefix's k-th
+//         key is a proven lower bound of the global k-th key, and the
+//         prefix's top k are written to the candidate list.
+//  2. stream pass  (fte_stream_kernel): each block streams whole tiles of
+//     contiguous column segments, scores stay in registers, and keys >= tau
+//     go to the block's own candidate slots (wave ballot + one LDS atomic
+//     per wave; no global atomic).  No barrier in the loop: a pure HBM
+//     stream.
+//  3. final select (fte_final_kernel): one workgroup scans the per-block
+//     counts, checks the candidate total, selects the exact top k and
+//     rank-sorts them.
 //
-// Exactness: every object outside the candidate list has key <= tau <=
-// global k-th key and is a different object from the k-th, so it is not in
-// the top k.  If the list overflows its capacity the final kernel flags
-// STATUS_OVERFLOW and the host re-runs with a buffer of n entries.
+// Exactness: with c >= k candidates of key >= tau, every other object has a
+// key < tau and is beaten by c >= k objects, so the top k are among the
+// candidates.  Too few candidates (a sample guess that was too high) raise
+// STATUS_RETRY, too many for the buffer STATUS_OVERFLOW; the host then runs
+// the exact mode, with an n-entry buffer if needed.
 #include "device.hpp"
 #include "internal.hpp"
 
 namespace topk {
 
-constexpr int PREFIX_BS = 256;
-constexpr uint32_t PREFIX_TILE = 4096;   // objects per prefix block
+constexpr int SEED_BS = 256;
+constexpr uint32_t SEED_TILE = 4096;     // objects per seed block (16 per thread)
 constexpr int STREAM_BS = 256;
-constexpr int STREAM_UNR = 2;            // float4 vectors per thread per step
 constexpr int FINAL_BS = 1024;
+constexpr int FINAL_KPT = 8;             // candidate keys cached per final-select thread
+constexpr uint32_t SAMPLE_CHUNKS = 16;   // sample mode: 16 x 4096 objects
 
 // Score of the 4 objects of float4 vector v: acc = fl(acc + fl(x*w)) in
 // attribute order, accumulator starting at +0 (cpu/TPAc.h:129-142).
 template <int M>
 __device__ __forceinline__ f4 score_vec(const QCols &q, uint64_t v) {
-    const uint32_t m = M ? M : q.m;
-    f4 x[M ? M : 1];
     f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (M) {
+    if constexpr (M > 0) {
+        f4 x[M];
 #pragma unroll
-        for (int j = 0; j < (M ? M : 1); j++) x[j] = ld_stream(q.col[j] + v * 4);
+        for (int j = 0; j < M; j++) x[j] = ld_stream(q.col[j] + v * 4);
 #pragma unroll
-        for (int j = 0; j < (M ? M : 1); j++) {
+        for (int j = 0; j < M; j++) {
             f4 p = x[j] * q.w[j];
             acc = acc + p;
         }
     } else {
-        for (uint32_t j = 0; j < m; j++) {
+        for (uint32_t j = 0; j < q.m; j++) {
             f4 xv = ld_stream(q.col[j] + v * 4);
             f4 p = xv * q.w[j];
             acc = acc + p;
@@ -57,42 +64,41 @@ __device__ __forceinline__ f4 score_vec(const QCols &q, uint64_t v) {
 }
 
 // ---------------------------------------------------------------------------
-// 1. prefix pass
+// 1. seed pass
 // ---------------------------------------------------------------------------
+// Block b scores objects [b*chunk_stride, +4096) of the first n_eff objects.
+// total_valid = number of non-zero keys the blocks deposit in blk.
 template <int M>
-__global__ __launch_bounds__(PREFIX_BS) void fte_prefix_kernel(QCols q, uint64_t n_eff, uint32_t k,
-                                                               uint64_t *__restrict__ blk,
-                                                               uint64_t *__restrict__ cand,
-                                                               FteState *__restrict__ st) {
-    __shared__ uint64_t keys[PREFIX_TILE];
+__global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_eff, uint64_t chunk_stride,
+                                                           uint32_t ks, uint32_t total_valid,
+                                                           uint64_t *__restrict__ blk,
+                                                           uint64_t *__restrict__ pre,
+                                                           FteState *__restrict__ st) {
     __shared__ SelectScratch ss;
-    __shared__ uint32_t counter;
     __shared__ uint32_t is_last;
     const uint32_t tid = threadIdx.x;
-    const uint64_t base = (uint64_t)blockIdx.x * PREFIX_TILE;
+    const uint64_t base = (uint64_t)blockIdx.x * chunk_stride;
 
+    RegKeys<SEED_TILE / SEED_BS> rk;
 #pragma unroll
-    for (int j = 0; j < (int)(PREFIX_TILE / (4 * PREFIX_BS)); j++) {
-        const uint32_t lv = j * PREFIX_BS + tid;       // local float4 index
-        const uint64_t v = base / 4 + lv;
-        f4 s = score_vec<M>(q, v);
+    for (int j = 0; j < (int)(SEED_TILE / (4 * SEED_BS)); j++) {
+        const uint64_t v = base / 4 + j * SEED_BS + tid;
+        const f4 s = score_vec<M>(q, v);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            uint64_t id = v * 4 + e;
-            keys[lv * 4 + e] = id < n_eff ? rkey(s[e], (uint32_t)id) : 0ull;
+            const uint64_t id = v * 4 + e;
+            rk.k[j * 4 + e] = id < n_eff ? rkey(s[e], (uint32_t)id) : 0ull;
         }
     }
-    __syncthreads();
-
-    const uint64_t valid64 = n_eff > base ? (n_eff - base < PREFIX_TILE ? n_eff - base : PREFIX_TILE) : 0;
-    const uint32_t valid = (uint32_t)valid64;
+    const uint64_t left = n_eff > base ? n_eff - base : 0;
+    const uint32_t valid = (uint32_t)(left < SEED_TILE ? left : SEED_TILE);
     uint64_t thr = 1;
-    if (valid > k) thr = block_kth_largest<PREFIX_BS>(keys, PREFIX_TILE, k, ss);
-    uint64_t *out = blk + (uint64_t)blockIdx.x * k;
-    uint32_t got = block_gather_ge<PREFIX_BS>(keys, PREFIX_TILE, thr, out, &counter);
-    for (uint32_t i = got + tid; i < k; i += PREFIX_BS) out[i] = 0;
+    if (valid > ks) thr = block_kth_largest<SEED_BS>(rk, valid, ks, ss);
+    uint64_t *out = blk + (uint64_t)blockIdx.x * ks;
+    const uint32_t got = block_gather_ge<SEED_BS>(rk, thr, out, ss);
+    for (uint32_t i = got + tid; i < ks; i += SEED_BS) out[i] = 0;
 
-    // last block to arrive merges the block winners
+    // the last block to arrive merges the block winners
     __threadfence();
     __syncthreads();
     if (tid == 0) is_last = (atomicAdd(&st->arrive, 1u) == gridDim.x - 1);
@@ -100,54 +106,148 @@ __global__ __launch_bounds__(PREFIX_BS) void fte_prefix_kernel(QCols q, uint64_t
     if (!is_last) return;
     __threadfence();
 
-    const uint32_t g = gridDim.x;
-    const uint64_t full = n_eff / PREFIX_TILE, rem = n_eff % PREFIX_TILE;
-    const uint64_t total = full * (k < PREFIX_TILE ? k : PREFIX_TILE) + (rem < k ? rem : k);
+    const uint32_t nk = gridDim.x * ks;
     uint64_t tau = 1;
-    if (total > k) tau = block_kth_largest<PREFIX_BS>(blk, g * k, k, ss);
-    uint32_t c = block_gather_ge<PREFIX_BS>(blk, g * k, tau, cand, &counter);
+    if (total_valid > ks) tau = block_kth_largest<SEED_BS>(MemKeys<SEED_BS>{blk, nk}, total_valid, ks, ss);
+    uint32_t c = 0;
+    if (pre) c = block_gather_ge<SEED_BS>(MemKeysUniform<SEED_BS>{blk, nk}, tau, pre, ss);
     if (tid == 0) {
         st->tau = tau;
-        st->count = c;
+        st->pre_count = c;
+        st->spill = 0;
         st->arrive = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// final select + sort
+// ---------------------------------------------------------------------------
+struct FinalOut {
+    uint64_t *ids;
+    float *scores;
+    float *thr;
+    uint32_t *status;
+};
+
+// Flat index -> key over pre[0..npre) followed by the blocks' slot regions;
+// off[] (LDS) holds the exclusive prefix of the per-block counts.
+struct SlotView {
+    const uint64_t *pre;
+    uint32_t npre;
+    const uint64_t *keys;
+    uint32_t slot;
+    const uint32_t *off;   // nblk + 2 entries; region nblk is the spill region
+    uint32_t nblk;
+    const uint64_t *spill;
+    __device__ __forceinline__ uint64_t get(uint32_t p) const {
+        if (p < npre) return pre[p];
+        const uint32_t q = p - npre;
+        if (q >= off[nblk]) return spill[q - off[nblk]];
+        uint32_t lo = 0, hi = nblk;  // largest b with off[b] <= q
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        return keys[(uint64_t)lo * slot + (q - off[lo])];
+    }
+};
+
+struct FlatView {
+    const uint64_t *p;
+    __device__ __forceinline__ uint64_t get(uint32_t i) const { return p[i]; }
+};
+
+// Selects the exact top k of the c candidates of view v and writes them best
+// first.  All threads of the block call it.
+template <int BS, int KPT, class View>
+__device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const FinalOut &o, uint64_t *top,
+                                uint64_t *sorted, SelectScratch &ss) {
+    const uint32_t tid = threadIdx.x;
+    uint32_t got;
+    if (c <= (uint32_t)BS * KPT) {
+        RegKeys<KPT> rk;
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            const uint32_t i = j * BS + tid;
+            rk.k[j] = i < c ? v.get(i) : 0ull;
+        }
+        const uint64_t thr = c > k ? block_kth_largest<BS>(rk, c, k, ss) : 1ull;
+        got = block_gather_ge<BS>(rk, thr, top, ss);
+    } else {
+        const uint64_t thr = block_kth_largest<BS>(ViewKeys<BS, View, false>{v, c}, c, k, ss);
+        got = block_gather_ge<BS>(ViewKeys<BS, View, true>{v, c}, thr, top, ss);
+    }
+    block_rank_sort_desc<BS>(top, got, sorted);
+    for (uint32_t i = tid; i < k; i += BS) {
+        if (i < got) {
+            o.ids[i] = rkey_id(sorted[i]);
+            o.scores[i] = rkey_score(sorted[i]);
+        } else {
+            o.ids[i] = ~0ull;
+            o.scores[i] = -INFINITY;
+        }
+    }
+    if (tid == 0) {
+        if (o.thr) *o.thr = got ? rkey_score(sorted[got - 1]) : -INFINITY;
+        if (o.status) *o.status = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // 2. stream pass
 // ---------------------------------------------------------------------------
-template <int M>
-__global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t n, uint64_t v_begin,
-                                                               uint64_t v_end,
-                                                               FteState *__restrict__ st,
-                                                               uint64_t *__restrict__ cand,
-                                                               uint32_t cap) {
+// Each block streams whole tiles of TILE = 1024*VPT objects: for every
+// queried column it reads one contiguous TILE*4-byte segment (VPT float4 per
+// lane).  Block-contiguous segments keep HBM near 88% of peak where a
+// thread-level grid stride over the columns stays near 72%.  The column
+// store is padded by one tile, so tile loads need no bounds check.
+template <int M, int VPT>
+__global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t n, uint64_t obj_begin,
+                                                               uint64_t ntiles, FteState *__restrict__ st,
+                                                               CandSet cs) {
+    constexpr uint32_t TILE = STREAM_BS * 4 * VPT;
+    __shared__ uint32_t lcount;
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
     const uint64_t tau = st->tau;
     const uint32_t tau_hi = (uint32_t)(tau >> 32);
-    const uint64_t stride = (uint64_t)gridDim.x * STREAM_BS;
     const uint32_t lane = lane_id();
-    // The trip count depends on blockIdx only, so every lane of a wave stays
-    // active through the ballot / scan below; lanes past v_end are masked.
-    for (uint64_t vb = v_begin + (uint64_t)blockIdx.x * STREAM_BS; vb < v_end; vb += stride * STREAM_UNR) {
-        const uint64_t v0 = vb + threadIdx.x;
-        f4 s[STREAM_UNR];
+    uint64_t *my = cs.keys + (uint64_t)blockIdx.x * cs.slot;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t v0 = (obj_begin + t * TILE) / 4 + threadIdx.x;
+        f4 s[VPT];
+        if constexpr (M > 0) {
+            f4 x[M][VPT];
 #pragma unroll
-        for (int u = 0; u < STREAM_UNR; u++) {
-            const uint64_t vv = v0 + u * stride;
-            if (vv < v_end) s[u] = score_vec<M>(q, vv);
-            else s[u] = f4{-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            for (int j = 0; j < M; j++)
+#pragma unroll
+                for (int u = 0; u < VPT; u++) x[j][u] = ld_stream(q.col[j] + (v0 + u * STREAM_BS) * 4);
+#pragma unroll
+            for (int u = 0; u < VPT; u++) {
+                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    f4 p = x[j][u] * q.w[j];
+                    acc = acc + p;
+                }
+                s[u] = acc;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < VPT; u++) s[u] = score_vec<0>(q, v0 + u * STREAM_BS);
         }
-        // filter: key > tau (score key first; ids only on ties / passes)
+        // filter: key >= tau (score key first; the id only matters on ties)
         uint32_t npass = 0;
-        uint64_t kk[STREAM_UNR * 4];
+        uint64_t kk[VPT * 4];
 #pragma unroll
-        for (int u = 0; u < STREAM_UNR; u++) {
+        for (int u = 0; u < VPT; u++) {
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const uint64_t id = (v0 + u * stride) * 4 + e;
+                const uint64_t id = (v0 + u * STREAM_BS) * 4 + e;
                 const uint32_t ok = okey(s[u][e]);
                 const uint64_t key = rkey_from_okey(ok, (uint32_t)id);
-                const bool p = ok >= tau_hi && key > tau && id < n;
+                const bool p = ok >= tau_hi && key >= tau && id < n;
                 kk[u * 4 + e] = p ? key : 0ull;
                 npass += p ? 1u : 0u;
             }
@@ -155,130 +255,252 @@ __global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t
         if (__ballot(npass != 0)) {
             const uint32_t incl = wave_incl_scan(npass);
             uint32_t base = 0;
-            if (lane == 63) base = atomicAdd(&st->count, incl);
+            if (lane == 63) base = atomicAdd(&lcount, incl);
             base = __shfl(base, 63, 64);
+            const uint32_t wtotal = __shfl(incl, 63, 64);
+            // positions >= slot spill to the shared region (one global atomic per wave)
+            const uint32_t first_spill = base > cs.slot ? base : cs.slot;
+            const uint32_t nspill = base + wtotal > first_spill ? base + wtotal - first_spill : 0u;
+            uint32_t sbase = 0;
+            if (nspill) {
+                if (lane == 63) sbase = atomicAdd(&st->spill, nspill);
+                sbase = __shfl(sbase, 63, 64);
+            }
             uint32_t pos = base + incl - npass;
 #pragma unroll
-            for (int j = 0; j < STREAM_UNR * 4; j++) {
+            for (int j = 0; j < VPT * 4; j++) {
                 if (kk[j]) {
-                    if (pos < cap) cand[pos] = kk[j];
+                    if (pos < cs.slot) {
+                        my[pos] = kk[j];
+                    } else {
+                        const uint32_t sp = sbase + (pos - first_spill);
+                        if (sp < cs.spill_cap) cs.spill[sp] = kk[j];
+                    }
                     pos++;
                 }
             }
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) cs.counts[blockIdx.x] = lcount < cs.slot ? lcount : cs.slot;
 }
 
-// ---------------------------------------------------------------------------
-// 3. final select + sort
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(FINAL_BS) void fte_final_kernel(FteState *__restrict__ st,
-                                                             const uint64_t *__restrict__ cand,
-                                                             uint32_t cap, uint32_t k,
-                                                             uint64_t *__restrict__ out_ids,
-                                                             float *__restrict__ out_scores,
-                                                             float *__restrict__ out_thr,
-                                                             uint32_t *__restrict__ out_status) {
+// 3. final select: one workgroup after the stream pass.  (A last-block merge
+// inside the stream pass would need an agent-scope release fence in every
+// one of its ~2000 blocks; each is an L2 write-back on gfx950 and together
+// they cost more than a launch.)
+__global__ __launch_bounds__(FINAL_BS) void fte_final_kernel(FteState *__restrict__ st, CandSet cs, uint32_t k,
+                                                             uint32_t kreq, FinalOut fo) {
+    __shared__ uint32_t off[MAX_STREAM_GRID + 2];
+    __shared__ uint32_t wsum[FINAL_BS / 64];
+    __shared__ uint32_t total, bad_lds;
     __shared__ uint64_t top[TOPK_MAX_K];
+    __shared__ uint64_t sorted[TOPK_MAX_K];
     __shared__ SelectScratch ss;
-    __shared__ uint32_t counter;
     const uint32_t tid = threadIdx.x;
-    const uint32_t c = st->count;
-    if (c > cap) {
+    if (tid == 0) bad_lds = 0;
+    // per-block counts -> exclusive offsets (thread t owns a contiguous run)
+    const uint32_t per = (cs.nblk + FINAL_BS - 1) / FINAL_BS;
+    const uint32_t b0 = tid * per, b1 = b0 + per < cs.nblk ? b0 + per : cs.nblk;
+    uint32_t mine = 0;
+    for (uint32_t b = b0; b < b1; b++) mine += cs.counts[b];
+    uint32_t run = block_excl_scan<FINAL_BS>(mine, wsum, &total);
+    for (uint32_t b = b0; b < b1; b++) {
+        off[b] = run;
+        run += cs.counts[b];
+    }
+    const uint32_t spilled = st->spill;
+    if (tid == 0) {
+        off[cs.nblk] = total;
+        off[cs.nblk + 1] = total + (spilled < cs.spill_cap ? spilled : cs.spill_cap);
+        if (spilled > cs.spill_cap) bad_lds = STATUS_OVERFLOW;
+    }
+    __syncthreads();
+    const uint32_t npre = cs.pre ? st->pre_count : 0u;
+    const uint32_t c = npre + off[cs.nblk + 1];
+    const uint32_t bad = bad_lds ? STATUS_OVERFLOW : (c < kreq ? STATUS_RETRY : 0u);
+    if (bad) {
         if (tid == 0) {
-            st->status |= STATUS_OVERFLOW;
-            if (out_status) *out_status = STATUS_OVERFLOW;
+            st->status |= bad;
+            if (fo.status) *fo.status = bad;
         }
         return;
     }
-    uint64_t thr = 1;
-    if (c > k) thr = block_kth_largest<FINAL_BS>(cand, c, k, ss);
-    const uint32_t got = block_gather_ge<FINAL_BS>(cand, c, thr, top, &counter);
-    const uint32_t n2 = next_pow2(k > 1 ? k : 2);
-    for (uint32_t i = got + tid; i < n2; i += FINAL_BS) top[i] = 0;
-    __syncthreads();
-    bitonic_sort_desc<FINAL_BS>(top, n2);
-    for (uint32_t i = tid; i < k; i += FINAL_BS) {
-        if (i < got) {
-            out_ids[i] = rkey_id(top[i]);
-            out_scores[i] = rkey_score(top[i]);
-        } else {
-            out_ids[i] = ~0ull;
-            out_scores[i] = -INFINITY;
+    if (tid == 0) st->cand_last = c;
+    const SlotView v{cs.pre, npre, cs.keys, cs.slot, off, cs.nblk, cs.spill};
+    select_and_emit<FINAL_BS, FINAL_KPT>(v, c, k, fo, top, sorted, ss);
+}
+
+// Fallback path for candidate sets too large for the regular slots: the
+// per-block counts are scanned into global offsets, every block copies its
+// slot region into one flat array, and the final select reads that array.
+__global__ __launch_bounds__(FINAL_BS) void cand_offsets_kernel(FteState *__restrict__ st, CandSet cs,
+                                                                uint32_t *__restrict__ offsets,
+                                                                uint64_t *__restrict__ flat) {
+    __shared__ uint32_t wsum[FINAL_BS / 64];
+    __shared__ uint32_t total;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (cs.nblk + FINAL_BS - 1) / FINAL_BS;
+    const uint32_t b0 = tid * per, b1 = b0 + per < cs.nblk ? b0 + per : cs.nblk;
+    uint32_t mine = 0;
+    for (uint32_t b = b0; b < b1; b++) mine += cs.counts[b];
+    const uint32_t npre = cs.pre ? st->pre_count : 0u;
+    uint32_t run = npre + block_excl_scan<FINAL_BS>(mine, wsum, &total);
+    for (uint32_t b = b0; b < b1; b++) {
+        offsets[b] = run;
+        run += cs.counts[b];
+    }
+    for (uint32_t i = tid; i < npre; i += FINAL_BS) flat[i] = cs.pre[i];
+    if (tid == 0) st->total = npre + total;
+}
+
+__global__ __launch_bounds__(256) void cand_compact_kernel(CandSet cs, const uint32_t *__restrict__ offsets,
+                                                           uint64_t *__restrict__ flat) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t c = cs.counts[b];
+    const uint64_t *src = cs.keys + (uint64_t)b * cs.slot;
+    uint64_t *dst = flat + offsets[b];
+    for (uint32_t i = threadIdx.x; i < c; i += 256) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(FINAL_BS) void fte_final_flat_kernel(FteState *__restrict__ st,
+                                                                  const uint64_t *__restrict__ flat, uint32_t k,
+                                                                  uint32_t kreq, FinalOut fo) {
+    __shared__ uint64_t top[TOPK_MAX_K];
+    __shared__ uint64_t sorted[TOPK_MAX_K];
+    __shared__ SelectScratch ss;
+    const uint32_t c = st->total;
+    if (c < kreq) {
+        if (threadIdx.x == 0) {
+            st->status |= STATUS_RETRY;
+            if (fo.status) *fo.status = STATUS_RETRY;
         }
+        return;
     }
-    if (tid == 0) {
-        if (out_thr) *out_thr = got ? rkey_score(top[got - 1]) : -INFINITY;
-        if (out_status) *out_status = 0;
-    }
+    if (threadIdx.x == 0) st->cand_last = c;
+    select_and_emit<FINAL_BS, FINAL_KPT>(FlatView{flat}, c, k, fo, top, sorted, ss);
 }
 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 
-// Launches KERN<M> for the query width m (M = 0: runtime-width fallback).
-#define TK_LAUNCH_M(m, KERN, grid, block, shmem, stream, ...)                                \
-    switch (m) {                                                                              \
-        case 1: hipLaunchKernelGGL(KERN<1>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 2: hipLaunchKernelGGL(KERN<2>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 3: hipLaunchKernelGGL(KERN<3>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 4: hipLaunchKernelGGL(KERN<4>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 5: hipLaunchKernelGGL(KERN<5>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 6: hipLaunchKernelGGL(KERN<6>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 7: hipLaunchKernelGGL(KERN<7>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 8: hipLaunchKernelGGL(KERN<8>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 9: hipLaunchKernelGGL(KERN<9>, grid, block, shmem, stream, __VA_ARGS__); break;   \
-        case 10: hipLaunchKernelGGL(KERN<10>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        case 11: hipLaunchKernelGGL(KERN<11>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        case 12: hipLaunchKernelGGL(KERN<12>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        case 13: hipLaunchKernelGGL(KERN<13>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        case 14: hipLaunchKernelGGL(KERN<14>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        case 15: hipLaunchKernelGGL(KERN<15>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        case 16: hipLaunchKernelGGL(KERN<16>, grid, block, shmem, stream, __VA_ARGS__); break; \
-        default: hipLaunchKernelGGL(KERN<0>, grid, block, shmem, stream, __VA_ARGS__); break;  \
-    }
-
-uint32_t fte_prefix_blocks(uint64_t n, uint32_t k) {
-    uint64_t need = (n + PREFIX_TILE - 1) / PREFIX_TILE;
+static uint32_t exact_prefix_blocks(uint64_t n, uint32_t k) {
+    const uint64_t need = (n + SEED_TILE - 1) / SEED_TILE;
     uint32_t g = 32768u / (k ? k : 1);
     if (g > 256) g = 256;
     if (g < 8) g = 8;
     return (uint32_t)(need < g ? need : g);
 }
 
-uint32_t stream_grid(uint64_t nvec) {
-    uint64_t blocks = (nvec + (uint64_t)STREAM_BS * STREAM_UNR - 1) / ((uint64_t)STREAM_BS * STREAM_UNR);
-    const uint64_t cap = 256 * 16;
-    if (blocks > cap) blocks = cap;
-    if (blocks < 1) blocks = 1;
-    return (uint32_t)blocks;
+static uint32_t stream_grid(const topk_ctx *ctx, uint64_t ntiles) {
+    const uint64_t cap = ctx->tune.stream_grid ? (uint64_t)ctx->tune.stream_grid : 2048;
+    const uint64_t g = ntiles < cap ? ntiles : cap;
+    return (uint32_t)(g < 1 ? 1 : g);
 }
 
-int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
-                float *d_thr, uint32_t *d_status, bool big) {
+// float4 vectors per lane per column of one stream tile: enough loads in
+// flight per lane (about 8-16) without exhausting registers.
+static int stream_vpt(uint32_t m) {
+    if (m <= 1) return 8;
+    if (m <= 4) return 4;
+    if (m <= 8) return 2;
+    return 1;
+}
+
+// Chooses the seed geometry for a query: sample mode when n is large enough
+// that a spread sample leaves many candidates, exact prefix mode otherwise.
+static void seed_plan(const topk_ctx *ctx, uint32_t k, bool exact, uint32_t &blocks, uint64_t &stride,
+                      uint64_t &n_eff, uint32_t &ks, uint32_t &total_valid) {
     const uint64_t n = ctx->n;
-    const uint32_t g1 = fte_prefix_blocks(n, k);
-    const uint64_t p = (uint64_t)g1 * PREFIX_TILE;
-    const uint64_t n_eff = n < p ? n : p;
-    uint64_t *cand = big ? ctx->cand_big : ctx->cand;
-    const uint32_t cap = big ? (uint32_t)(ctx->cand_big_cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : ctx->cand_big_cap)
-                             : ctx->cand_cap;
+    const uint64_t s = (uint64_t)SAMPLE_CHUNKS * SEED_TILE;
+    if (!exact && n > 4 * s) {
+        blocks = SAMPLE_CHUNKS;
+        stride = ((n / SAMPLE_CHUNKS) / SEED_TILE) * SEED_TILE;
+        n_eff = n;
+        // expected candidates ~ ks * n / s; aim for >= 8k
+        const uint64_t want = (8ull * k * s + n - 1) / n;
+        ks = (uint32_t)(want < 4 ? 4 : (want > k ? k : want));
+        total_valid = blocks * ks;
+    } else {
+        blocks = exact_prefix_blocks(n, k);
+        stride = SEED_TILE;
+        const uint64_t p = (uint64_t)blocks * SEED_TILE;
+        n_eff = n < p ? n : p;
+        ks = k;
+        const uint64_t full = n_eff / SEED_TILE, rem = n_eff % SEED_TILE;
+        total_valid = (uint32_t)(full * (k < SEED_TILE ? k : SEED_TILE) + (rem < k ? rem : k));
+    }
+}
+
+int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores, float *d_thr,
+                uint32_t *d_status, int mode) {
+    const uint64_t n = ctx->n;
+    const bool exact = mode != FTE_SAMPLE;
+    const bool big = mode == FTE_EXACT_BIG;
     hipStream_t s = ctx->stream;
 
-    TK_LAUNCH_M(q.m, fte_prefix_kernel, dim3(g1), dim3(PREFIX_BS), 0, s, q, n_eff, k, ctx->blk, cand,
-                ctx->st);
-    if (n > p) {
-        const uint64_t v_begin = p / 4, v_end = (n + 3) / 4;
-        const uint32_t g2 = stream_grid(v_end - v_begin);
+    uint32_t blocks, ks, total_valid;
+    uint64_t stride, n_eff;
+    seed_plan(ctx, k, exact, blocks, stride, n_eff, ks, total_valid);
+    uint64_t *pre = exact ? ctx->pre : nullptr;
+#define TK_SEED(MM)                                                                                        \
+    hipLaunchKernelGGL(fte_seed_kernel<MM>, dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, ks, \
+                       total_valid, ctx->blk, pre, ctx->st)
+    TK_SWITCH_M(q.m, TK_SEED);
+#undef TK_SEED
+
+    // exact mode streams only the objects after the prefix
+    const uint64_t first = exact ? n_eff : 0;
+    const uint32_t kreq = (uint32_t)(n < k ? n : k);
+    const FinalOut fo{d_ids, d_scores, d_thr, d_status};
+    const int vpt = ctx->tune.stream_vpt ? ctx->tune.stream_vpt : stream_vpt(q.m);
+    const uint64_t tile = (uint64_t)STREAM_BS * 4 * vpt;
+    const uint64_t ntiles = n > first ? (n - first + tile - 1) / tile : 0;
+    const uint32_t g = ntiles ? stream_grid(ctx, ntiles) : 0;
+    CandSet cs;
+    cs.counts = ctx->counts;
+    cs.nblk = g;
+    cs.pre = pre;
+    cs.spill = ctx->spill;
+    cs.spill_cap = ctx->spill_cap;
+    if (big) {
+        cs.keys = ctx->cand_big;
+        cs.slot = g ? (uint32_t)(((ntiles + g - 1) / g) * tile) : 0;  // a block's every object fits
+    } else {
+        cs.keys = ctx->cand;
+        cs.slot = g ? (uint32_t)(ctx->cand_cap / g) : 0;
+    }
+    if (g) {
         hipEvent_t ev0 = nullptr;
         timing_begin(ctx, "fte_stream", ev0);
-        TK_LAUNCH_M(q.m, fte_stream_kernel, dim3(g2), dim3(STREAM_BS), 0, s, q, n, v_begin, v_end, ctx->st,
-                    cand, cap);
+#define TK_STREAM(MM, VPT)                                                                                   \
+    hipLaunchKernelGGL((fte_stream_kernel<MM, VPT>), dim3(g), dim3(STREAM_BS), 0, s, q, n, first, ntiles, \
+                       ctx->st, cs)
+#define TK_STREAM1(MM) TK_STREAM(MM, 1)
+#define TK_STREAM2(MM) TK_STREAM(MM, 2)
+#define TK_STREAM4(MM) TK_STREAM(MM, 4)
+#define TK_STREAM8(MM) TK_STREAM(MM, 8)
+        switch (vpt) {
+            case 1: TK_SWITCH_M(q.m, TK_STREAM1); break;
+            case 2: TK_SWITCH_M(q.m, TK_STREAM2); break;
+            case 4: TK_SWITCH_M(q.m, TK_STREAM4); break;
+            default: TK_SWITCH_M(q.m, TK_STREAM8); break;
+        }
+#undef TK_STREAM
+#undef TK_STREAM1
+#undef TK_STREAM2
+#undef TK_STREAM4
+#undef TK_STREAM8
         timing_end(ctx, "fte_stream", ev0);
     }
-    hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, cand, cap, k, d_ids, d_scores,
-                       d_thr, d_status);
+    if (big) {
+        hipLaunchKernelGGL(cand_offsets_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, cs, ctx->offsets, ctx->flat);
+        if (g) hipLaunchKernelGGL(cand_compact_kernel, dim3(g), dim3(256), 0, s, cs, ctx->offsets, ctx->flat);
+        hipLaunchKernelGGL(fte_final_flat_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, ctx->flat, k, kreq, fo);
+    } else {
+        hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, cs, k, kreq, fo);
+    }
     TK_HIP(hipGetLastError());
     return TOPK_OK;
 }

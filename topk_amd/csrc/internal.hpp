@@ -23,16 +23,70 @@ struct QCols {
 
 // Device-side state of one FTE / PTA query pipeline.
 struct FteState {
-    unsigned long long tau;   // threshold key published by the prefix pass
-    unsigned int count;       // candidates appended (may exceed the capacity)
-    unsigned int arrive;      // arrival ticket of the prefix pass (last block merges)
+    unsigned long long tau;   // threshold key published by the seed pass
+    unsigned int pre_count;   // exact mode: prefix keys the seed pass stored in pre[]
+    unsigned int arrive;      // arrival ticket of the seed pass (last block merges)
     unsigned int status;      // sticky device error bits (STATUS_*)
     unsigned int scored;      // PTA: partitions scored
-    unsigned int arrive2;     // arrival ticket of a second last-block merge
+    unsigned int total;       // flat-gather mode: candidates gathered
+    unsigned int cand_last;   // candidates seen by the last final select
+    unsigned int spill;       // candidates that spilled past their block's slot region
     unsigned int pad;
 };
 
-enum : unsigned int { STATUS_OVERFLOW = 1u };
+// Candidate keys of one query.  Stream block b owns keys[b*slot, (b+1)*slot)
+// and publishes how many keys >= tau it found in counts[b]; keys past its
+// slot go to the shared spill region (spill_cap keys, counter st->spill).
+// pre[0..pre_count) holds the exact-mode prefix keys.  The common case needs
+// no global atomic: device-scope atomics on one counter serialise across the
+// 8 XCDs and cost ~15% of the stream's bandwidth at a few thousand
+// candidates; only clustered candidates reach the spill counter.
+struct CandSet {
+    uint64_t *keys;
+    uint32_t *counts;
+    uint32_t nblk;
+    uint32_t slot;
+    const uint64_t *pre;
+    uint64_t *spill;
+    uint32_t spill_cap;
+};
+
+enum : unsigned int { STATUS_OVERFLOW = 1u, STATUS_RETRY = 2u };
+
+// FTE pipeline modes: sample-seeded (fast path), exact-prefix-seeded with
+// the regular candidate buffer, and exact with an n-entry buffer.
+enum { FTE_SAMPLE = 0, FTE_EXACT = 1, FTE_EXACT_BIG = 2 };
+
+constexpr uint32_t MAX_STREAM_GRID = 4096;
+
+// Launch-shape knobs (set by topk_set_tuning / the autotuner).
+struct Tuning {
+    uint32_t stream_grid = 0;   // 0: default grid cap
+    int stream_vpt = 0;         // float4 vectors per lane per column in a stream tile (0: by width)
+    int fte_force_exact = 0;    // 1: skip the sample seed (testing / comparison)
+};
+
+// Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
+#define TK_SWITCH_M(m, CASE)          \
+    switch (m) {                      \
+        case 1: CASE(1); break;       \
+        case 2: CASE(2); break;       \
+        case 3: CASE(3); break;       \
+        case 4: CASE(4); break;       \
+        case 5: CASE(5); break;       \
+        case 6: CASE(6); break;       \
+        case 7: CASE(7); break;       \
+        case 8: CASE(8); break;       \
+        case 9: CASE(9); break;       \
+        case 10: CASE(10); break;     \
+        case 11: CASE(11); break;     \
+        case 12: CASE(12); break;     \
+        case 13: CASE(13); break;     \
+        case 14: CASE(14); break;     \
+        case 15: CASE(15); break;     \
+        case 16: CASE(16); break;     \
+        default: CASE(0); break;      \
+    }
 
 // Packed result block written by the final kernel, copied to pinned host
 // memory in one transfer: ids[kcap] u64 | scores[kcap] f32 | thr f32 | status u32.
@@ -74,15 +128,22 @@ struct topk_ctx {
     hipStream_t stream = nullptr;
     uint64_t n = 0;
     uint32_t d = 0;
-    uint64_t ld = 0;            // padded column stride (multiple of 4096 objects)
-    float *cols = nullptr;      // d * ld floats, zero padded
+    uint64_t ld = 0;            // padded column stride (multiple of 8192 objects)
+    float *cols = nullptr;      // d * ld + 8192 floats, zero padded (a stream tile never needs a bounds check)
     // FTE pipeline scratch
     topk::FteState *st = nullptr;
-    uint64_t *blk = nullptr;    // prefix-pass block candidates (256 * TOPK_MAX_K keys)
-    uint64_t *cand = nullptr;   // candidate keys
-    uint32_t cand_cap = 0;
-    uint64_t *cand_big = nullptr;  // fallback candidate buffer (lazily sized to n)
+    uint64_t *blk = nullptr;    // seed-pass block winners (256 * TOPK_MAX_K keys)
+    uint64_t *pre = nullptr;    // exact-mode prefix keys (TOPK_MAX_K)
+    uint32_t *counts = nullptr; // per-stream-block candidate counts (MAX_STREAM_GRID)
+    uint32_t *offsets = nullptr;  // flat-gather offsets (MAX_STREAM_GRID + 1)
+    uint64_t *cand = nullptr;   // candidate slots
+    uint64_t cand_cap = 0;
+    uint64_t *spill = nullptr;  // shared spill region of the candidate slots
+    uint32_t spill_cap = 0;
+    uint64_t *cand_big = nullptr;  // fallback slots, sized so no block can overflow
     uint64_t cand_big_cap = 0;
+    uint64_t *flat = nullptr;   // fallback: candidates gathered contiguously
+    uint64_t flat_cap = 0;
     // results
     unsigned char *d_res = nullptr;
     unsigned char *h_res = nullptr;  // pinned
@@ -93,9 +154,10 @@ struct topk_ctx {
     void *h_batch = nullptr;         // pinned staging for batch results
     size_t h_batch_bytes = 0;
     // misc
-    uint32_t sticky_status = 0;
+    topk::FteState *h_state = nullptr;  // pinned mirror of *st
     topk_stats stats{};
     bool timing_on = false;
+    topk::Tuning tune;
     std::map<std::string, topk::Timing> timing;
     topk::PtaLayout pta;
     // learned predictor (owned by learned.hip)
@@ -106,7 +168,7 @@ namespace topk {
 
 // Launch helpers implemented in the .hip translation units.
 int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
-                float *d_thr, uint32_t *d_status, bool big);
+                float *d_thr, uint32_t *d_status, int mode);
 int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq, const float *w,
                 const uint8_t *attr, QCols &q);
 

@@ -161,6 +161,9 @@ class Engine:
     def stream(self):
         return lib().topk_stream(self._h)
 
+    def set_tuning(self, name, value):
+        check(lib().topk_set_tuning(self._h, name.encode(), int(value)), "topk_set_tuning")
+
     # ---- stats / timing ----
     def stats(self):
         s = TopkStats()
