@@ -27,25 +27,34 @@ HBM_PEAK_GBS = 8000.0
 N_DEFAULT = 1 << 26
 D = 8
 K = 64
-PREFIX_TILE = 4096
-
-
-def prefix_objects(n, k):
-    # mirrors fte_prefix_blocks() in topk_amd/csrc/fte.hip
-    need = (n + PREFIX_TILE - 1) // PREFIX_TILE
-    g = min(256, max(8, 32768 // k))
-    return min(need, g) * PREFIX_TILE
 
 
 def load_profile_traffic():
     """HBM bytes per stream-pass launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE), if one is present."""
-    path = os.path.join(ROOT, "profiles", "fte_stream_traffic.json")
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), if one is present."""
+    path = os.path.join(ROOT, "profiles", "fte_stream_pmc.md")
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch ="):
+                    return float(line.split("=", 1)[1].strip())
     except (OSError, ValueError):
-        return None
+        pass
+    return None
+
+
+def calibrate_copy(nbytes):
+    """A plain device copy of nbytes (torch's copy kernel), the reference the
+    PMC byte counters are scaled by."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}), flush=True)
 
 
 def cpu_baseline(x, n, m, k, threads, budget_s, gpu_ids, gpu_scores):
@@ -78,7 +87,14 @@ def main():
     ap.add_argument("--k", type=int, default=K)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="pipeline lanes for back-to-back queries (1 = no overlap between queries)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain copy of BYTES bytes (PMC calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "the sharded design is described in DESIGN.md", file=sys.stderr)
@@ -91,6 +107,7 @@ def main():
     attr = list(range(D - m, D))
     w = np.ones(D, np.float32)
     eng = topk_amd.Engine.generated(n, D, topk_amd.INDEPENDENT, 0)
+    eng.set_tuning("lanes", args.lanes)
     ids_d = torch.empty(k, dtype=torch.int64, device="cuda")
     sc_d = torch.empty(k, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
@@ -108,7 +125,8 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     qps = args.steps / elapsed
 
-    # dominant kernel duration with HIP events on the engine's stream
+    # dominant kernel duration with HIP events on the engine's stream (timing
+    # mode runs every query on lane 0, so launches do not overlap)
     eng.timing(True)
     eng.timing_reset()
     for _ in range(args.steps):
@@ -117,12 +135,19 @@ def main():
     stream_ms, launches = eng.timing_get("fte_stream")
     eng.timing(False)
     avg_ms = stream_ms / max(1, launches)
-    streamed = n - prefix_objects(n, k)
+    # the sample-seeded pipeline streams every object (the 128 x 1024 sampled
+    # objects are scored again by the stream pass)
+    streamed = n
     alg_bytes = 4.0 * m * streamed
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = load_profile_traffic()
 
-    gpu_ids, gpu_sc, _ = eng.fte(k, w, attr)
+    # single-query latency through the synchronous C ABI (host buffers)
+    lat = []
+    for _ in range(10):
+        t1 = time.perf_counter()
+        gpu_ids, gpu_sc, _ = eng.fte(k, w, attr)
+        lat.append(time.perf_counter() - t1)
     st = eng.stats()
 
     out = {
@@ -147,6 +172,9 @@ def main():
                      "kernel": "fte_stream_kernel", "avg_launch_ms": avg_ms,
                      "alg_bytes_per_launch": alg_bytes},
         "fallback": st["fallback"],
+        "lanes": args.lanes,
+        "sync_query_latency_ms": float(np.median(lat)) * 1e3,
+        "candidates_per_query": st["candidates"],
     }
 
     if not args.no_cpu:

@@ -151,8 +151,11 @@ int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stri
 
 /* ---- device-pointer variants (outputs stay in HBM; no host sync) ---- */
 
-/* Enqueues topk_fte on the context's stream; outputs are device pointers.
- * Errors found on the device are reported by the next topk_sync. */
+/* Enqueues topk_fte; outputs are device pointers.  Consecutive calls
+ * alternate between the context's pipeline lanes (each a HIP stream with
+ * its own scratch), so they may finish out of order: read outputs only after
+ * topk_sync.  Errors found on the device are reported by the next
+ * topk_sync. */
 int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
                    uint64_t *d_ids, float *d_scores);
 
@@ -160,7 +163,7 @@ int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, 
  * since the last sync (TOPK_OK if none) and re-runs nothing. */
 int topk_sync(topk_ctx *ctx);
 
-/* The context's hipStream_t, for callers that time or order work on it. */
+/* The context's first hipStream_t (lane 0; synchronous calls run on it). */
 void *topk_stream(topk_ctx *ctx);
 
 /* ---- configuration (replaces the reference's compile-time knobs,
@@ -171,6 +174,9 @@ void *topk_stream(topk_ctx *ctx);
  *   "stream_vpt"      float4 vectors per lane per column of a stream tile:
  *                     1, 2, 4 or 8 (0 = chosen from the query width)
  *   "fte_force_exact" 1 = seed FTE queries with the exact prefix pass
+ *   "lanes"           1 or 2: pipelines that consecutive topk_fte_async
+ *                     calls alternate between (2 overlaps one query's seed
+ *                     and final passes with its neighbours' stream passes)
  * Unknown names or values return TOPK_ERR_INVALID. */
 int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value);
 

@@ -24,18 +24,18 @@ int hip_status(hipError_t e) {
     return TOPK_ERR_HIP;
 }
 
-void timing_begin(topk_ctx *ctx, const char *, hipEvent_t &ev0) {
+void timing_begin(topk_ctx *ctx, hipStream_t s, hipEvent_t &ev0) {
     ev0 = nullptr;
     if (!ctx->timing_on) return;
     if (hipEventCreate(&ev0) != hipSuccess) { ev0 = nullptr; return; }
-    hipEventRecord(ev0, ctx->stream);
+    hipEventRecord(ev0, s);
 }
 
-void timing_end(topk_ctx *ctx, const char *name, hipEvent_t ev0) {
+void timing_end(topk_ctx *ctx, hipStream_t s, const char *name, hipEvent_t ev0) {
     if (!ev0) return;
     hipEvent_t ev1;
     if (hipEventCreate(&ev1) != hipSuccess) { hipEventDestroy(ev0); return; }
-    hipEventRecord(ev1, ctx->stream);
+    hipEventRecord(ev1, s);
     ctx->timing[name].pending.push_back({ev0, ev1});
 }
 
@@ -108,16 +108,22 @@ static int ctx_alloc(topk_ctx *ctx) {
     const size_t col_floats = ctx->ld * ctx->d + 8192;
     TK_HIP(hipMalloc(&ctx->cols, sizeof(float) * col_floats));
     TK_HIP(hipMemsetAsync(ctx->cols, 0, sizeof(float) * col_floats, ctx->stream));
-    TK_HIP(hipMalloc(&ctx->st, sizeof(FteState)));
-    TK_HIP(hipMemsetAsync(ctx->st, 0, sizeof(FteState), ctx->stream));
-    TK_HIP(hipMalloc(&ctx->blk, sizeof(uint64_t) * 256 * TOPK_MAX_K));
-    TK_HIP(hipMalloc(&ctx->pre, sizeof(uint64_t) * TOPK_MAX_K));
-    TK_HIP(hipMalloc(&ctx->counts, sizeof(uint32_t) * MAX_STREAM_GRID));
-    TK_HIP(hipMalloc(&ctx->offsets, sizeof(uint32_t) * (MAX_STREAM_GRID + 1)));
     ctx->cand_cap = 1u << 21;
-    TK_HIP(hipMalloc(&ctx->cand, sizeof(uint64_t) * ctx->cand_cap));
     ctx->spill_cap = 1u << 21;
-    TK_HIP(hipMalloc(&ctx->spill, sizeof(uint64_t) * ctx->spill_cap));
+    ctx->lane[0].stream = ctx->stream;
+    for (int i = 0; i < NUM_LANES; i++) {
+        Lane &L = ctx->lane[i];
+        if (!L.stream) TK_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        TK_HIP(hipMalloc(&L.st, sizeof(FteState)));
+        TK_HIP(hipMemsetAsync(L.st, 0, sizeof(FteState), L.stream));
+        TK_HIP(hipMalloc(&L.blk, sizeof(uint64_t) * 256 * TOPK_MAX_K));
+        TK_HIP(hipMalloc(&L.pre, sizeof(uint64_t) * TOPK_MAX_K));
+        TK_HIP(hipMalloc(&L.counts, sizeof(uint32_t) * MAX_STREAM_GRID));
+        TK_HIP(hipMalloc(&L.cand, sizeof(uint64_t) * ctx->cand_cap));
+        TK_HIP(hipMalloc(&L.spill, sizeof(uint64_t) * ctx->spill_cap));
+        TK_HIP(hipStreamSynchronize(L.stream));
+    }
+    TK_HIP(hipMalloc(&ctx->offsets, sizeof(uint32_t) * (MAX_STREAM_GRID + 1)));
     ctx->res = ResultLayout::make(TOPK_MAX_K);
     TK_HIP(hipMalloc(&ctx->d_res, ctx->res.bytes));
     TK_HIP(hipHostMalloc(&ctx->h_res, ctx->res.bytes, hipHostMallocDefault));
@@ -172,17 +178,22 @@ const char *topk_strerror(int s) {
 
 int topk_ctx_destroy(topk_ctx *ctx) {
     if (!ctx) return TOPK_ERR_INVALID;
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < NUM_LANES; i++)
+        if (ctx->lane[i].stream) hipStreamSynchronize(ctx->lane[i].stream);
     timing_collect(ctx);
     learned_free(ctx);
     hipFree(ctx->cols);
-    hipFree(ctx->st);
-    hipFree(ctx->blk);
-    hipFree(ctx->pre);
-    hipFree(ctx->counts);
+    for (int i = 0; i < NUM_LANES; i++) {
+        Lane &L = ctx->lane[i];
+        hipFree(L.st);
+        hipFree(L.blk);
+        hipFree(L.pre);
+        hipFree(L.counts);
+        hipFree(L.cand);
+        hipFree(L.spill);
+        if (i > 0 && L.stream) hipStreamDestroy(L.stream);
+    }
     hipFree(ctx->offsets);
-    hipFree(ctx->cand);
-    hipFree(ctx->spill);
     hipFree(ctx->cand_big);
     hipFree(ctx->flat);
     hipFree(ctx->d_res);
@@ -312,21 +323,22 @@ static int fte_run_sync(topk_ctx *ctx, uint32_t k, const QCols &q) {
     float *d_thr = reinterpret_cast<float *>(ctx->d_res + ctx->res.thr_off);
     uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
     ctx->stats.fallback = 0;
+    Lane &L = ctx->lane[0];
     int mode = ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE;
     for (int attempt = 0; attempt < 3; attempt++) {
         if (mode == FTE_EXACT_BIG) {
             int rc = ensure_big(ctx);
             if (rc) return rc;
         }
-        int rc = fte_enqueue(ctx, k, q, d_ids, d_sc, d_thr, d_status, mode);
+        int rc = fte_enqueue(ctx, L, k, q, d_ids, d_sc, d_thr, d_status, mode);
         if (rc) return rc;
-        TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, ctx->stream));
-        TK_HIP(hipMemcpyAsync(ctx->h_state, ctx->st, sizeof(FteState), hipMemcpyDeviceToHost, ctx->stream));
-        TK_HIP(hipStreamSynchronize(ctx->stream));
+        TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
+        TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
+        TK_HIP(hipStreamSynchronize(L.stream));
         const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
         ctx->stats.candidates = ctx->h_state->cand_last;
         if (!status) return TOPK_OK;
-        TK_HIP(hipMemsetAsync(&ctx->st->status, 0, sizeof(uint32_t), ctx->stream));
+        TK_HIP(hipMemsetAsync(&L.st->status, 0, sizeof(uint32_t), L.stream));
         ctx->stats.fallback = 1;
         mode = (status & STATUS_OVERFLOW) ? FTE_EXACT_BIG : FTE_EXACT;
     }
@@ -358,19 +370,28 @@ int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, 
     QCols q;
     rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
     if (rc) return rc;
-    return fte_enqueue(ctx, k, q, d_ids, d_scores, nullptr, nullptr, ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE);
+    // alternate lanes so consecutive queries overlap (timing runs serialise on lane 0)
+    const uint32_t lanes = ctx->timing_on ? 1u : (uint32_t)ctx->tune.lanes;
+    Lane &L = ctx->lane[ctx->next_lane % lanes];
+    ctx->next_lane = (ctx->next_lane + 1) % lanes;
+    return fte_enqueue(ctx, L, k, q, d_ids, d_scores, nullptr, nullptr,
+                       ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE);
 }
 
 int topk_sync(topk_ctx *ctx) {
     if (!ctx) return TOPK_ERR_INVALID;
-    TK_HIP(hipStreamSynchronize(ctx->stream));
-    uint32_t status = 0;
-    TK_HIP(hipMemcpy(&status, &ctx->st->status, sizeof(status), hipMemcpyDeviceToHost));
-    if (status) {
-        TK_HIP(hipMemset(&ctx->st->status, 0, sizeof(uint32_t)));
-        return TOPK_ERR_OVERFLOW;
+    uint32_t any = 0;
+    for (int i = 0; i < NUM_LANES; i++) {
+        Lane &L = ctx->lane[i];
+        TK_HIP(hipStreamSynchronize(L.stream));
+        uint32_t status = 0;
+        TK_HIP(hipMemcpy(&status, &L.st->status, sizeof(status), hipMemcpyDeviceToHost));
+        if (status) {
+            TK_HIP(hipMemset(&L.st->status, 0, sizeof(uint32_t)));
+            any |= status;
+        }
     }
-    return TOPK_OK;
+    return any ? TOPK_ERR_OVERFLOW : TOPK_OK;
 }
 
 int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
@@ -382,6 +403,9 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
     } else if (key == "stream_vpt") {
         if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return TOPK_ERR_INVALID;
         ctx->tune.stream_vpt = (int)value;
+    } else if (key == "lanes") {
+        if (value < 1 || value > NUM_LANES) return TOPK_ERR_INVALID;
+        ctx->tune.lanes = (int)value;
     } else if (key == "fte_force_exact") {
         ctx->tune.fte_force_exact = value ? 1 : 0;
     } else {

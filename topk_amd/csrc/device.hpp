@@ -54,9 +54,9 @@ __device__ __forceinline__ f4 ld_stream(const float *p) {
     return __builtin_nontemporal_load(reinterpret_cast<const f4 *>(p));
 }
 
-// LDS scratch of the workgroup selection.
+// LDS scratch of the workgroup selection (histogram for up to 11-bit digits).
 struct SelectScratch {
-    uint32_t hist[256];
+    uint32_t hist[2048];
     uint32_t digit, need, bcnt, counter;
     unsigned long long kmin, kmax;
 };
@@ -120,8 +120,10 @@ __device__ __forceinline__ void block_minmax(uint64_t &lo, uint64_t &hi, SelectS
 // a lane prefix scan.  Rounds stop when the candidate set is one value, when
 // the k-th is its maximum, or when the whole set is inside the top k (then
 // the k-th is its minimum).
-template <int BS, class Src>
+template <int BS, class Src, int DB = 8>
 __device__ uint64_t block_kth_largest(const Src &src, uint32_t setcnt, uint32_t k, SelectScratch &s) {
+    static_assert(DB >= 6 && DB <= 11, "digit width");
+    constexpr uint32_t NB = 1u << DB, PER = NB / 64;
     const uint32_t tid = threadIdx.x;
     uint64_t prefix = 0, mask = 0;
     uint32_t need = k;
@@ -140,34 +142,33 @@ __device__ uint64_t block_kth_largest(const Src &src, uint32_t setcnt, uint32_t 
         const uint64_t above = db == 63 ? 0ull : (~0ull << (db + 1));
         prefix = hi & above;
         mask = above;
-        const int width = db + 1 < 8 ? db + 1 : 8;
+        const int width = db + 1 < DB ? db + 1 : DB;
         const int shift = db + 1 - width;
         const uint32_t dmask = (1u << width) - 1u;
-        for (uint32_t i = tid; i < 256; i += BS) s.hist[i] = 0;
+        for (uint32_t i = tid; i < NB; i += BS) s.hist[i] = 0;
         __syncthreads();
         src.for_each([&](uint64_t key) {
             if (key && (key & mask) == prefix) atomicAdd(&s.hist[(uint32_t)(key >> shift) & dmask], 1u);
         });
         __syncthreads();
         if (tid < 64) {
-            // lane l owns bins 255-4l .. 252-4l, highest first
-            const uint32_t c0 = s.hist[255 - 4 * tid], c1 = s.hist[254 - 4 * tid];
-            const uint32_t c2 = s.hist[253 - 4 * tid], c3 = s.hist[252 - 4 * tid];
-            const uint32_t tot = c0 + c1 + c2 + c3;
+            // lane l owns bins NB-1-PER*l .. NB-PER*(l+1), highest first
+            const uint32_t top = NB - 1 - PER * tid;
+            uint32_t tot = 0;
+            for (uint32_t j = 0; j < PER; j++) tot += s.hist[top - j];
             const uint32_t incl = wave_incl_scan(tot);
             const uint32_t excl = incl - tot;
             if (excl < need && need <= incl) {
                 uint32_t r = need - excl, acc = 0;
-                const uint32_t cs[4] = {c0, c1, c2, c3};
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (r <= acc + cs[j]) {
-                        s.digit = 255 - 4 * tid - j;
+                for (uint32_t j = 0; j < PER; j++) {
+                    const uint32_t cj = s.hist[top - j];
+                    if (r <= acc + cj) {
+                        s.digit = top - j;
                         s.need = r - acc;
-                        s.bcnt = cs[j];
+                        s.bcnt = cj;
                         break;
                     }
-                    acc += cs[j];
+                    acc += cj;
                 }
             }
         }
@@ -258,6 +259,23 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t *wave_sums, uint32_t *t
     const uint32_t r = wave_sums[w] + incl - v;
     __syncthreads();
     return r;
+}
+
+// The kth (1 <= kth <= 8) largest of the keys the wave's lanes hold (NPL
+// per lane; distinct except zeros), by kth rounds of a wave max.  Every lane
+// gets the result; keys below it stay in v, the chosen ones are zeroed.
+template <int NPL>
+__device__ __forceinline__ uint64_t wave_kth_largest(uint64_t (&v)[NPL], uint32_t kth) {
+    uint64_t mx = 0;
+    for (uint32_t r = 0; r < kth; r++) {
+        uint64_t lm = 0;
+#pragma unroll
+        for (int j = 0; j < NPL; j++) lm = v[j] > lm ? v[j] : lm;
+        mx = wave_max_u64(lm);
+#pragma unroll
+        for (int j = 0; j < NPL; j++) v[j] = v[j] == mx ? 0ull : v[j];
+    }
+    return mx;
 }
 
 // Sorts the got distinct keys in[0..got) descending into out[] by rank

@@ -33,11 +33,14 @@
 namespace topk {
 
 constexpr int SEED_BS = 256;
-constexpr uint32_t SEED_TILE = 4096;     // objects per seed block (16 per thread)
+constexpr uint32_t SEED_TILE = 4096;     // exact mode: objects per seed block (16 per thread)
+constexpr uint32_t SAMPLE_TILE = 1024;   // sample mode: objects per seed block (4 per thread)
+constexpr uint32_t SAMPLE_CHUNKS = 128;  // sample mode: 128 x 1024 objects spread over the ids
 constexpr int STREAM_BS = 256;
-constexpr int FINAL_BS = 1024;
-constexpr int FINAL_KPT = 8;             // candidate keys cached per final-select thread
-constexpr uint32_t SAMPLE_CHUNKS = 16;   // sample mode: 16 x 4096 objects
+constexpr int FINAL_BS = 512;            // regular final select
+constexpr int FINAL_KPT = 16;            // candidate keys per final-select thread (8192 in registers)
+constexpr uint32_t SEED_WAVE_MAX = 512;  // seed winners a stream wave can reduce to tau itself
+constexpr int FLAT_BS = 1024;            // fallback final select over a large flat list
 
 // Score of the 4 objects of float4 vector v: acc = fl(acc + fl(x*w)) in
 // attribute order, accumulator starting at +0 (cpu/TPAc.h:129-142).
@@ -66,9 +69,9 @@ __device__ __forceinline__ f4 score_vec(const QCols &q, uint64_t v) {
 // ---------------------------------------------------------------------------
 // 1. seed pass
 // ---------------------------------------------------------------------------
-// Block b scores objects [b*chunk_stride, +4096) of the first n_eff objects.
+// Block b scores objects [b*chunk_stride, +256*OPT) of the first n_eff objects.
 // total_valid = number of non-zero keys the blocks deposit in blk.
-template <int M>
+template <int M, int OPT, bool MERGE>
 __global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_eff, uint64_t chunk_stride,
                                                            uint32_t ks, uint32_t total_valid,
                                                            uint64_t *__restrict__ blk,
@@ -79,9 +82,10 @@ __global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_e
     const uint32_t tid = threadIdx.x;
     const uint64_t base = (uint64_t)blockIdx.x * chunk_stride;
 
-    RegKeys<SEED_TILE / SEED_BS> rk;
+    constexpr uint32_t TILE = SEED_BS * OPT;
+    RegKeys<OPT> rk;
 #pragma unroll
-    for (int j = 0; j < (int)(SEED_TILE / (4 * SEED_BS)); j++) {
+    for (int j = 0; j < OPT / 4; j++) {
         const uint64_t v = base / 4 + j * SEED_BS + tid;
         const f4 s = score_vec<M>(q, v);
 #pragma unroll
@@ -91,12 +95,13 @@ __global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_e
         }
     }
     const uint64_t left = n_eff > base ? n_eff - base : 0;
-    const uint32_t valid = (uint32_t)(left < SEED_TILE ? left : SEED_TILE);
+    const uint32_t valid = (uint32_t)(left < TILE ? left : TILE);
     uint64_t thr = 1;
     if (valid > ks) thr = block_kth_largest<SEED_BS>(rk, valid, ks, ss);
     uint64_t *out = blk + (uint64_t)blockIdx.x * ks;
     const uint32_t got = block_gather_ge<SEED_BS>(rk, thr, out, ss);
     for (uint32_t i = got + tid; i < ks; i += SEED_BS) out[i] = 0;
+    if constexpr (!MERGE) return;  // sample mode: each stream wave derives tau from blk itself
 
     // the last block to arrive merges the block winners
     __threadfence();
@@ -114,7 +119,6 @@ __global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_e
     if (tid == 0) {
         st->tau = tau;
         st->pre_count = c;
-        st->spill = 0;
         st->arrive = 0;
     }
 }
@@ -159,7 +163,8 @@ struct FlatView {
 };
 
 // Selects the exact top k of the c candidates of view v and writes them best
-// first.  All threads of the block call it.
+// first.  All threads of the block call it.  `sorted` may alias the view's
+// storage when c <= BS*KPT (the keys are read into registers first).
 template <int BS, int KPT, class View>
 __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const FinalOut &o, uint64_t *top,
                                 uint64_t *sorted, SelectScratch &ss) {
@@ -172,7 +177,8 @@ __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const Fin
             const uint32_t i = j * BS + tid;
             rk.k[j] = i < c ? v.get(i) : 0ull;
         }
-        const uint64_t thr = c > k ? block_kth_largest<BS>(rk, c, k, ss) : 1ull;
+        __syncthreads();  // the view's storage may be reused as `sorted` below
+        const uint64_t thr = c > k ? block_kth_largest<BS, RegKeys<KPT>, 11>(rk, c, k, ss) : 1ull;
         got = block_gather_ge<BS>(rk, thr, top, ss);
     } else {
         const uint64_t thr = block_kth_largest<BS>(ViewKeys<BS, View, false>{v, c}, c, k, ss);
@@ -194,6 +200,57 @@ __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const Fin
     }
 }
 
+// The ks-th largest of the nseed (<= 512) seed winners, computed by one wave.
+__device__ __forceinline__ uint64_t wave_seed_tau(const uint64_t *__restrict__ seed, uint32_t nseed,
+                                                  uint32_t ks) {
+    uint64_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint32_t i = j * 64 + lane_id();
+        v[j] = i < nseed ? seed[i] : 0ull;
+    }
+    return wave_kth_largest<8>(v, ks);
+}
+
+// Sample mode seed with few winners per block (ks <= 4): block b scores the
+// 1024 objects at b*chunk_stride, every wave keeps its top ks by wave-max
+// rounds, and wave 0 reduces the 4*ks wave winners to the block's top ks.
+// No radix select, one barrier.
+template <int M>
+__global__ __launch_bounds__(SEED_BS) void fte_sample_kernel(QCols q, uint64_t n, uint64_t chunk_stride,
+                                                             uint32_t ks, uint64_t *__restrict__ blk) {
+    __shared__ uint64_t wtop[SEED_BS / 64][4];
+    const uint32_t tid = threadIdx.x, w = tid / 64;
+    const uint64_t v = (uint64_t)blockIdx.x * chunk_stride / 4 + tid;
+    const f4 s = score_vec<M>(q, v);
+    uint64_t keys[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint64_t id = v * 4 + e;
+        keys[e] = id < n ? rkey(s[e], (uint32_t)id) : 0ull;
+    }
+    for (uint32_t r = 0; r < ks; r++) {
+        uint64_t lm = 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) lm = keys[e] > lm ? keys[e] : lm;
+        const uint64_t mx = wave_max_u64(lm);
+#pragma unroll
+        for (int e = 0; e < 4; e++) keys[e] = keys[e] == mx ? 0ull : keys[e];
+        if (lane_id() == 0) wtop[w][r] = mx;
+    }
+    __syncthreads();
+    if (w == 0) {
+        uint64_t c[1];
+        const uint32_t i = lane_id();
+        c[0] = i < (SEED_BS / 64) * ks ? wtop[i / ks][i % ks] : 0ull;
+        for (uint32_t r = 0; r < ks; r++) {
+            const uint64_t mx = wave_max_u64(c[0]);
+            c[0] = c[0] == mx ? 0ull : c[0];
+            if (i == 0) blk[(uint64_t)blockIdx.x * ks + r] = mx;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 2. stream pass
 // ---------------------------------------------------------------------------
@@ -202,15 +259,18 @@ __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const Fin
 // lane).  Block-contiguous segments keep HBM near 88% of peak where a
 // thread-level grid stride over the columns stays near 72%.  The column
 // store is padded by one tile, so tile loads need no bounds check.
+// tau comes from st->tau (published by a merging seed pass) or, when nseed
+// > 0, from the ks-th largest of the nseed seed winners in seed[].
 template <int M, int VPT>
 __global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t n, uint64_t obj_begin,
                                                                uint64_t ntiles, FteState *__restrict__ st,
-                                                               CandSet cs) {
+                                                               CandSet cs, const uint64_t *__restrict__ seed,
+                                                               uint32_t nseed, uint32_t ks) {
     constexpr uint32_t TILE = STREAM_BS * 4 * VPT;
     __shared__ uint32_t lcount;
     if (threadIdx.x == 0) lcount = 0;
     __syncthreads();
-    const uint64_t tau = st->tau;
+    const uint64_t tau = nseed ? wave_seed_tau(seed, nseed, ks) : st->tau;
     const uint32_t tau_hi = (uint32_t)(tau >> 32);
     const uint32_t lane = lane_id();
     uint64_t *my = cs.keys + (uint64_t)blockIdx.x * cs.slot;
@@ -292,33 +352,37 @@ __global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t
 __global__ __launch_bounds__(FINAL_BS) void fte_final_kernel(FteState *__restrict__ st, CandSet cs, uint32_t k,
                                                              uint32_t kreq, FinalOut fo) {
     __shared__ uint32_t off[MAX_STREAM_GRID + 2];
-    __shared__ uint32_t wsum[FINAL_BS / 64];
-    __shared__ uint32_t total, bad_lds;
     __shared__ uint64_t top[TOPK_MAX_K];
     __shared__ uint64_t sorted[TOPK_MAX_K];
+    __shared__ uint32_t wsum[FINAL_BS / 64];
+    __shared__ uint32_t total;
     __shared__ SelectScratch ss;
     const uint32_t tid = threadIdx.x;
-    if (tid == 0) bad_lds = 0;
-    // per-block counts -> exclusive offsets (thread t owns a contiguous run)
+    // per-block counts -> LDS (coalesced), then exclusive offsets in place
+    for (uint32_t b = tid; b < cs.nblk; b += FINAL_BS) off[b] = cs.counts[b];
+    __syncthreads();
     const uint32_t per = (cs.nblk + FINAL_BS - 1) / FINAL_BS;
-    const uint32_t b0 = tid * per, b1 = b0 + per < cs.nblk ? b0 + per : cs.nblk;
+    const uint32_t b0 = tid * per < cs.nblk ? tid * per : cs.nblk;
+    const uint32_t b1 = b0 + per < cs.nblk ? b0 + per : cs.nblk;
     uint32_t mine = 0;
-    for (uint32_t b = b0; b < b1; b++) mine += cs.counts[b];
+    for (uint32_t b = b0; b < b1; b++) mine += off[b];
     uint32_t run = block_excl_scan<FINAL_BS>(mine, wsum, &total);
     for (uint32_t b = b0; b < b1; b++) {
+        const uint32_t c = off[b];
         off[b] = run;
-        run += cs.counts[b];
+        run += c;
     }
     const uint32_t spilled = st->spill;
+    const uint32_t nspill = spilled < cs.spill_cap ? spilled : cs.spill_cap;
     if (tid == 0) {
         off[cs.nblk] = total;
-        off[cs.nblk + 1] = total + (spilled < cs.spill_cap ? spilled : cs.spill_cap);
-        if (spilled > cs.spill_cap) bad_lds = STATUS_OVERFLOW;
+        off[cs.nblk + 1] = total + nspill;
     }
     __syncthreads();
+    if (tid == 0) st->spill = 0;  // ready for the next query
     const uint32_t npre = cs.pre ? st->pre_count : 0u;
     const uint32_t c = npre + off[cs.nblk + 1];
-    const uint32_t bad = bad_lds ? STATUS_OVERFLOW : (c < kreq ? STATUS_RETRY : 0u);
+    const uint32_t bad = spilled > cs.spill_cap ? STATUS_OVERFLOW : (c < kreq ? STATUS_RETRY : 0u);
     if (bad) {
         if (tid == 0) {
             st->status |= bad;
@@ -334,24 +398,28 @@ __global__ __launch_bounds__(FINAL_BS) void fte_final_kernel(FteState *__restric
 // Fallback path for candidate sets too large for the regular slots: the
 // per-block counts are scanned into global offsets, every block copies its
 // slot region into one flat array, and the final select reads that array.
-__global__ __launch_bounds__(FINAL_BS) void cand_offsets_kernel(FteState *__restrict__ st, CandSet cs,
-                                                                uint32_t *__restrict__ offsets,
-                                                                uint64_t *__restrict__ flat) {
-    __shared__ uint32_t wsum[FINAL_BS / 64];
+__global__ __launch_bounds__(FLAT_BS) void cand_offsets_kernel(FteState *__restrict__ st, CandSet cs,
+                                                               uint32_t *__restrict__ offsets,
+                                                               uint64_t *__restrict__ flat) {
+    __shared__ uint32_t wsum[FLAT_BS / 64];
     __shared__ uint32_t total;
     const uint32_t tid = threadIdx.x;
-    const uint32_t per = (cs.nblk + FINAL_BS - 1) / FINAL_BS;
-    const uint32_t b0 = tid * per, b1 = b0 + per < cs.nblk ? b0 + per : cs.nblk;
+    const uint32_t per = (cs.nblk + FLAT_BS - 1) / FLAT_BS;
+    const uint32_t b0 = tid * per < cs.nblk ? tid * per : cs.nblk;
+    const uint32_t b1 = b0 + per < cs.nblk ? b0 + per : cs.nblk;
     uint32_t mine = 0;
     for (uint32_t b = b0; b < b1; b++) mine += cs.counts[b];
     const uint32_t npre = cs.pre ? st->pre_count : 0u;
-    uint32_t run = npre + block_excl_scan<FINAL_BS>(mine, wsum, &total);
+    uint32_t run = npre + block_excl_scan<FLAT_BS>(mine, wsum, &total);
     for (uint32_t b = b0; b < b1; b++) {
         offsets[b] = run;
         run += cs.counts[b];
     }
-    for (uint32_t i = tid; i < npre; i += FINAL_BS) flat[i] = cs.pre[i];
-    if (tid == 0) st->total = npre + total;
+    for (uint32_t i = tid; i < npre; i += FLAT_BS) flat[i] = cs.pre[i];
+    if (tid == 0) {
+        st->total = npre + total;
+        st->spill = 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void cand_compact_kernel(CandSet cs, const uint32_t *__restrict__ offsets,
@@ -363,7 +431,7 @@ __global__ __launch_bounds__(256) void cand_compact_kernel(CandSet cs, const uin
     for (uint32_t i = threadIdx.x; i < c; i += 256) dst[i] = src[i];
 }
 
-__global__ __launch_bounds__(FINAL_BS) void fte_final_flat_kernel(FteState *__restrict__ st,
+__global__ __launch_bounds__(FLAT_BS) void fte_final_flat_kernel(FteState *__restrict__ st,
                                                                   const uint64_t *__restrict__ flat, uint32_t k,
                                                                   uint32_t kreq, FinalOut fo) {
     __shared__ uint64_t top[TOPK_MAX_K];
@@ -378,7 +446,7 @@ __global__ __launch_bounds__(FINAL_BS) void fte_final_flat_kernel(FteState *__re
         return;
     }
     if (threadIdx.x == 0) st->cand_last = c;
-    select_and_emit<FINAL_BS, FINAL_KPT>(FlatView{flat}, c, k, fo, top, sorted, ss);
+    select_and_emit<FLAT_BS, 8>(FlatView{flat}, c, k, fo, top, sorted, ss);
 }
 
 // ---------------------------------------------------------------------------
@@ -410,18 +478,20 @@ static int stream_vpt(uint32_t m) {
 
 // Chooses the seed geometry for a query: sample mode when n is large enough
 // that a spread sample leaves many candidates, exact prefix mode otherwise.
-static void seed_plan(const topk_ctx *ctx, uint32_t k, bool exact, uint32_t &blocks, uint64_t &stride,
+// Returns true for the sample geometry.
+static bool seed_plan(const topk_ctx *ctx, uint32_t k, bool exact, uint32_t &blocks, uint64_t &stride,
                       uint64_t &n_eff, uint32_t &ks, uint32_t &total_valid) {
     const uint64_t n = ctx->n;
-    const uint64_t s = (uint64_t)SAMPLE_CHUNKS * SEED_TILE;
+    const uint64_t s = (uint64_t)SAMPLE_CHUNKS * SAMPLE_TILE;
     if (!exact && n > 4 * s) {
         blocks = SAMPLE_CHUNKS;
-        stride = ((n / SAMPLE_CHUNKS) / SEED_TILE) * SEED_TILE;
+        stride = ((n / SAMPLE_CHUNKS) / SAMPLE_TILE) * SAMPLE_TILE;
         n_eff = n;
         // expected candidates ~ ks * n / s; aim for >= 8k
         const uint64_t want = (8ull * k * s + n - 1) / n;
         ks = (uint32_t)(want < 4 ? 4 : (want > k ? k : want));
         total_valid = blocks * ks;
+        return true;
     } else {
         blocks = exact_prefix_blocks(n, k);
         stride = SEED_TILE;
@@ -430,25 +500,43 @@ static void seed_plan(const topk_ctx *ctx, uint32_t k, bool exact, uint32_t &blo
         ks = k;
         const uint64_t full = n_eff / SEED_TILE, rem = n_eff % SEED_TILE;
         total_valid = (uint32_t)(full * (k < SEED_TILE ? k : SEED_TILE) + (rem < k ? rem : k));
+        return false;
     }
 }
 
-int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores, float *d_thr,
-                uint32_t *d_status, int mode) {
+int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
+                float *d_thr, uint32_t *d_status, int mode) {
     const uint64_t n = ctx->n;
     const bool exact = mode != FTE_SAMPLE;
     const bool big = mode == FTE_EXACT_BIG;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = L.stream;
 
     uint32_t blocks, ks, total_valid;
     uint64_t stride, n_eff;
-    seed_plan(ctx, k, exact, blocks, stride, n_eff, ks, total_valid);
-    uint64_t *pre = exact ? ctx->pre : nullptr;
-#define TK_SEED(MM)                                                                                        \
-    hipLaunchKernelGGL(fte_seed_kernel<MM>, dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, ks, \
-                       total_valid, ctx->blk, pre, ctx->st)
-    TK_SWITCH_M(q.m, TK_SEED);
+    const bool sampled = seed_plan(ctx, k, exact, blocks, stride, n_eff, ks, total_valid);
+    uint64_t *pre = exact ? L.pre : nullptr;
+    // sample mode with few winners: no merge in the seed pass, every stream
+    // wave reduces the winners to tau itself
+    const bool wave_tau = sampled && ks <= 4 && blocks * ks <= SEED_WAVE_MAX;
+#define TK_SEED(MM, OPT, MERGE)                                                                           \
+    hipLaunchKernelGGL((fte_seed_kernel<MM, OPT, MERGE>), dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, \
+                       ks, total_valid, L.blk, pre, L.st)
+#define TK_SEED_SAMPLE(MM) TK_SEED(MM, SAMPLE_TILE / SEED_BS, true)
+#define TK_SEED_EXACT(MM) TK_SEED(MM, SEED_TILE / SEED_BS, true)
+#define TK_SAMPLE(MM) \
+    hipLaunchKernelGGL(fte_sample_kernel<MM>, dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, ks, L.blk)
+    if (wave_tau) {
+        TK_SWITCH_M(q.m, TK_SAMPLE);
+    } else if (sampled) {
+        TK_SWITCH_M(q.m, TK_SEED_SAMPLE);
+    } else {
+        TK_SWITCH_M(q.m, TK_SEED_EXACT);
+    }
 #undef TK_SEED
+#undef TK_SAMPLE
+#undef TK_SEED_SAMPLE
+#undef TK_SEED_EXACT
+    const uint32_t nseed = wave_tau ? blocks * ks : 0u;
 
     // exact mode streams only the objects after the prefix
     const uint64_t first = exact ? n_eff : 0;
@@ -459,24 +547,24 @@ int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, floa
     const uint64_t ntiles = n > first ? (n - first + tile - 1) / tile : 0;
     const uint32_t g = ntiles ? stream_grid(ctx, ntiles) : 0;
     CandSet cs;
-    cs.counts = ctx->counts;
+    cs.counts = L.counts;
     cs.nblk = g;
     cs.pre = pre;
-    cs.spill = ctx->spill;
+    cs.spill = L.spill;
     cs.spill_cap = ctx->spill_cap;
     if (big) {
         cs.keys = ctx->cand_big;
         cs.slot = g ? (uint32_t)(((ntiles + g - 1) / g) * tile) : 0;  // a block's every object fits
     } else {
-        cs.keys = ctx->cand;
+        cs.keys = L.cand;
         cs.slot = g ? (uint32_t)(ctx->cand_cap / g) : 0;
     }
     if (g) {
         hipEvent_t ev0 = nullptr;
-        timing_begin(ctx, "fte_stream", ev0);
+        timing_begin(ctx, s, ev0);
 #define TK_STREAM(MM, VPT)                                                                                   \
     hipLaunchKernelGGL((fte_stream_kernel<MM, VPT>), dim3(g), dim3(STREAM_BS), 0, s, q, n, first, ntiles, \
-                       ctx->st, cs)
+                       L.st, cs, L.blk, nseed, ks)
 #define TK_STREAM1(MM) TK_STREAM(MM, 1)
 #define TK_STREAM2(MM) TK_STREAM(MM, 2)
 #define TK_STREAM4(MM) TK_STREAM(MM, 4)
@@ -492,14 +580,14 @@ int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, floa
 #undef TK_STREAM2
 #undef TK_STREAM4
 #undef TK_STREAM8
-        timing_end(ctx, "fte_stream", ev0);
+        timing_end(ctx, s, "fte_stream", ev0);
     }
     if (big) {
-        hipLaunchKernelGGL(cand_offsets_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, cs, ctx->offsets, ctx->flat);
+        hipLaunchKernelGGL(cand_offsets_kernel, dim3(1), dim3(FLAT_BS), 0, s, L.st, cs, ctx->offsets, ctx->flat);
         if (g) hipLaunchKernelGGL(cand_compact_kernel, dim3(g), dim3(256), 0, s, cs, ctx->offsets, ctx->flat);
-        hipLaunchKernelGGL(fte_final_flat_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, ctx->flat, k, kreq, fo);
+        hipLaunchKernelGGL(fte_final_flat_kernel, dim3(1), dim3(FLAT_BS), 0, s, L.st, ctx->flat, k, kreq, fo);
     } else {
-        hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, ctx->st, cs, k, kreq, fo);
+        hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, L.st, cs, k, kreq, fo);
     }
     TK_HIP(hipGetLastError());
     return TOPK_OK;

@@ -64,6 +64,7 @@ struct Tuning {
     uint32_t stream_grid = 0;   // 0: default grid cap
     int stream_vpt = 0;         // float4 vectors per lane per column in a stream tile (0: by width)
     int fte_force_exact = 0;    // 1: skip the sample seed (testing / comparison)
+    int lanes = 2;              // pipeline lanes used by asynchronous queries
 };
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
@@ -103,6 +104,18 @@ struct ResultLayout {
     }
 };
 
+// One FTE pipeline: a stream plus the per-query scratch its kernels use.
+struct Lane {
+    hipStream_t stream = nullptr;
+    FteState *st = nullptr;
+    uint64_t *blk = nullptr;     // seed-pass block winners (256 * TOPK_MAX_K keys)
+    uint64_t *pre = nullptr;     // exact-mode prefix keys (TOPK_MAX_K)
+    uint32_t *counts = nullptr;  // per-stream-block candidate counts (MAX_STREAM_GRID)
+    uint64_t *cand = nullptr;    // candidate slots (cand_cap keys)
+    uint64_t *spill = nullptr;   // shared spill region (spill_cap keys)
+};
+constexpr int NUM_LANES = 2;
+
 struct Timing {
     double total_ms = 0;
     uint64_t launches = 0;
@@ -130,16 +143,13 @@ struct topk_ctx {
     uint32_t d = 0;
     uint64_t ld = 0;            // padded column stride (multiple of 8192 objects)
     float *cols = nullptr;      // d * ld + 8192 floats, zero padded (a stream tile never needs a bounds check)
-    // FTE pipeline scratch
-    topk::FteState *st = nullptr;
-    uint64_t *blk = nullptr;    // seed-pass block winners (256 * TOPK_MAX_K keys)
-    uint64_t *pre = nullptr;    // exact-mode prefix keys (TOPK_MAX_K)
-    uint32_t *counts = nullptr; // per-stream-block candidate counts (MAX_STREAM_GRID)
-    uint32_t *offsets = nullptr;  // flat-gather offsets (MAX_STREAM_GRID + 1)
-    uint64_t *cand = nullptr;   // candidate slots
-    uint64_t cand_cap = 0;
-    uint64_t *spill = nullptr;  // shared spill region of the candidate slots
-    uint32_t spill_cap = 0;
+    // FTE pipeline lanes: queries alternate between them so one query's seed
+    // and final passes overlap its neighbours' stream passes
+    topk::Lane lane[topk::NUM_LANES > 2 ? topk::NUM_LANES : 2];
+    uint32_t next_lane = 0;
+    uint64_t cand_cap = 0;      // slots per lane
+    uint32_t spill_cap = 0;     // spill keys per lane
+    uint32_t *offsets = nullptr;   // flat-gather offsets (MAX_STREAM_GRID + 1)
     uint64_t *cand_big = nullptr;  // fallback slots, sized so no block can overflow
     uint64_t cand_big_cap = 0;
     uint64_t *flat = nullptr;   // fallback: candidates gathered contiguously
@@ -167,7 +177,7 @@ struct topk_ctx {
 namespace topk {
 
 // Launch helpers implemented in the .hip translation units.
-int fte_enqueue(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
+int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
                 float *d_thr, uint32_t *d_status, int mode);
 int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq, const float *w,
                 const uint8_t *attr, QCols &q);
@@ -176,8 +186,8 @@ int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq,
 void learned_free(topk_ctx *ctx);
 
 // Event bracketing for topk_timing_*.
-void timing_begin(topk_ctx *ctx, const char *name, hipEvent_t &ev0);
-void timing_end(topk_ctx *ctx, const char *name, hipEvent_t ev0);
+void timing_begin(topk_ctx *ctx, hipStream_t s, hipEvent_t &ev0);
+void timing_end(topk_ctx *ctx, hipStream_t s, const char *name, hipEvent_t ev0);
 
 int hip_status(hipError_t e);
 
