@@ -104,6 +104,15 @@ int topk_init(topk_ctx *ctx, uint32_t flags);
  * multiple of 1024.  Call before topk_init. */
 int topk_set_partition_size(topk_ctx *ctx, uint32_t objects);
 
+/* Layout diagnostics (no reference counterpart beyond GPA::check_order,
+ * gpu/GPA.h:96-114, which reads the device store back to verify it): the
+ * number of partitions and of partition-order positions (incl. padding),
+ * and a copy of the layout: pid[npos] (object id per position, 0xFFFFFFFF
+ * for padding), poff[nparts+1], psz[nparts], pmax/pmin[nparts*d].  Null
+ * outputs are skipped.  TOPK_ERR_STATE before topk_init(TOPK_INIT_PTA). */
+int topk_pta_layout(topk_ctx *ctx, uint32_t *nparts, uint64_t *npos);
+int topk_pta_read_layout(topk_ctx *ctx, uint32_t *pid, uint32_t *poff, uint32_t *psz, float *pmax, float *pmin);
+
 /* reorder_attr: per object, sort its d values descending across the
  * columns, in place (gpu/reorder_attr.h:13-236; invariant
  * gpu/GPA.h:96-114).  Destroys attribute identity, as in the reference. */

@@ -54,12 +54,12 @@ def lib():
                                  _c.c_uint32, _c.c_uint32, _u64p]),
         "or_gpa_step": (_c.c_uint64, [_u32p, _f32p, _c.c_uint64, _f32p, _f32p, _c.c_float, _c.c_float]),
         "or_pta_bound": (_c.c_float, [_f32p, _f32p, _c.c_uint32, _c.c_uint64, _c.c_uint32, _f32p, _u8p]),
-        "or_pta_topk": (_c.c_uint32, [_f32p, _u32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _c.c_uint32,
+        "or_pta_topk": (_c.c_uint32, [_f32p, _u32p, _c.c_uint64, _c.c_uint32, _c.c_uint32, _u32p, _u32p,
                                       _f32p, _f32p, _c.c_uint32, _c.c_uint32, _f32p, _u8p,
                                       _u64p, _f32p, _f32p, _c.POINTER(_c.c_uint64)]),
-        "or_pta_threads": (_c.c_uint32, [_f32p, _u32p, _c.c_uint64, _c.c_uint64, _c.c_uint32,
-                                         _c.c_uint32, _f32p, _f32p, _c.c_uint32, _c.c_uint32, _f32p,
-                                         _u8p, _c.c_int, _u64p, _f32p, _f32p, _c.POINTER(_c.c_uint64)]),
+        "or_pta_threads": (_c.c_uint32, [_f32p, _u32p, _c.c_uint64, _c.c_uint32, _c.c_uint32, _u32p, _u32p,
+                                         _f32p, _f32p, _c.c_uint32, _c.c_uint32, _f32p, _u8p, _c.c_int,
+                                         _u64p, _f32p, _f32p, _c.POINTER(_c.c_uint64)]),
         "or_mlp_nparams": (_c.c_uint32, []),
         "or_mlp_forward": (None, [_f64p, _f64p, _c.c_uint32, _f64p]),
         "or_mlp_backward": (_c.c_double, [_f64p, _f64p, _f64p, _c.c_uint32, _f64p]),
@@ -204,13 +204,17 @@ def gpa_step(ids, score, colC, colN, thr, suffix_len):
     return ids[:c], sc[:c]
 
 
-def pta_topk(pdata, pid, n, psize, pmax, pmin, k, weights, attr, threads=0):
+def pta_topk(pdata, pid, poff, psz, pmax, pmin, k, weights, attr, threads=0):
     """PTA over an explicit partition layout (cpu/PTA.h:357-422, 569-671).
 
-    Returns (ids, scores, thr, scored_partitions)."""
+    pdata: (d, ld) partition-order columns; partition p is positions
+    [poff[p], poff[p]+psz[p]).  Returns (ids, scores, thr, scored_partitions)."""
     a, d, ld = _cols(pdata)
     w, at = _query(d, weights, attr)
     pid = np.ascontiguousarray(pid, dtype=np.uint32)
+    poff = np.ascontiguousarray(poff, dtype=np.uint32)
+    psz = np.ascontiguousarray(psz, dtype=np.uint32)
+    nparts = psz.size
     pmax = np.ascontiguousarray(pmax, dtype=np.float32).ravel()
     pmin = np.ascontiguousarray(pmin, dtype=np.float32).ravel()
     ids = np.zeros(k, dtype=np.uint64)
@@ -218,10 +222,10 @@ def pta_topk(pdata, pid, n, psize, pmax, pmin, k, weights, attr, threads=0):
     thr = np.zeros(1, dtype=np.float32)
     scored = _c.c_uint64(0)
     if threads:
-        c = lib().or_pta_threads(a.ravel(), pid, n, ld, d, psize, pmax, pmin, k, at.size, w, at,
+        c = lib().or_pta_threads(a.ravel(), pid, ld, d, nparts, poff, psz, pmax, pmin, k, at.size, w, at,
                                  threads, ids, sc, thr, _c.byref(scored))
     else:
-        c = lib().or_pta_topk(a.ravel(), pid, n, ld, d, psize, pmax, pmin, k, at.size, w, at,
+        c = lib().or_pta_topk(a.ravel(), pid, ld, d, nparts, poff, psz, pmax, pmin, k, at.size, w, at,
                               ids, sc, thr, _c.byref(scored))
     return ids[:c], sc[:c], float(thr[0]), int(scored.value)
 

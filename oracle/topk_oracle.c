@@ -378,16 +378,17 @@ uint64_t or_gpa_step(uint32_t *ids, float *score, uint64_t cnt, const float *col
 
 /* ------------------------------------------------------------------ */
 /* PTA: cpu/PTA.h:357-422 / 569-671 restated over an explicit partition */
-/* layout.  The layout (a permutation of object ids cut into partitions */
-/* of `psize` objects, stored column-major per partition: pdata[m*ld+j] */
-/* is attribute m of the object at position j, pid[j] its global id) is */
-/* an input, so CPU and GPU scan identical partitions.  A partition is  */
-/* skipped when its bound cannot beat the k-th kept key (strict: skip    */
-/* iff okey(bound) < okey(kth score) with a full heap).  The bound is    */
-/* the query's score evaluated on the partition's per-attribute maximum */
-/* (minimum where the weight is negative), which is FP-monotone.         */
-/* Partitions are visited in descending bound order.  Returns count and  */
-/* writes the number of scored partitions to *scored.                   */
+/* layout.  The layout is an input, so CPU and GPU scan identical        */
+/* partitions: partition p occupies positions [poff[p], poff[p]+psz[p])  */
+/* of the partition-order columns (pdata[m*ld + j] is attribute m of the */
+/* object at position j, pid[j] its global id).  A partition is skipped  */
+/* when its bound cannot beat the k-th kept key (strict: skip iff        */
+/* okey(bound) < okey(kth score) with a full heap); the reference stops  */
+/* on >= (cpu/PTA.h:340), which is unsafe once ties are broken by id.    */
+/* The bound is the query's score evaluated at the partition's           */
+/* per-attribute maximum (minimum where the weight is negative), which   */
+/* is FP-monotone.  Partitions are visited in descending bound order.    */
+/* Returns the count and writes the number of scored partitions.         */
 /* ------------------------------------------------------------------ */
 
 typedef struct { uint64_t key; uint32_t p; } pbound;
@@ -412,25 +413,38 @@ float or_pta_bound(const float *pmax, const float *pmin, uint32_t d, uint64_t p,
     return acc;
 }
 
-uint32_t or_pta_topk(const float *pdata, const uint32_t *pid, uint64_t n, uint64_t ld,
-                     uint32_t d, uint32_t psize, const float *pmax, const float *pmin,
+static pbound *pta_sorted_bounds(uint32_t nparts, const uint32_t *psz, const float *pmax,
+                                 const float *pmin, uint32_t d, uint32_t qq, const float *w,
+                                 const uint8_t *attr, uint32_t *nonempty)
+{
+    pbound *pb = (pbound *)malloc(sizeof(pbound) * (nparts ? nparts : 1));
+    uint32_t c = 0;
+    for (uint32_t p = 0; p < nparts; p++) {
+        if (!psz[p]) continue;
+        pb[c].key = (uint64_t)or_okey(or_pta_bound(pmax, pmin, d, p, qq, w, attr)) << 32;
+        pb[c].p = p;
+        c++;
+    }
+    qsort(pb, c, sizeof(pbound), cmp_pbound_desc);
+    *nonempty = c;
+    return pb;
+}
+
+uint32_t or_pta_topk(const float *pdata, const uint32_t *pid, uint64_t ld, uint32_t d,
+                     uint32_t nparts, const uint32_t *poff, const uint32_t *psz,
+                     const float *pmax, const float *pmin,
                      uint32_t k, uint32_t qq, const float *w, const uint8_t *attr,
                      uint64_t *ids, float *scores, float *thr, uint64_t *scored)
 {
-    uint64_t np = (n + psize - 1) / psize;
-    pbound *pb = (pbound *)malloc(sizeof(pbound) * (np ? np : 1));
-    for (uint64_t p = 0; p < np; p++) {
-        pb[p].key = (uint64_t)or_okey(or_pta_bound(pmax, pmin, d, p, qq, w, attr)) << 32;
-        pb[p].p = (uint32_t)p;
-    }
-    qsort(pb, np, sizeof(pbound), cmp_pbound_desc);
+    uint32_t np;
+    pbound *pb = pta_sorted_bounds(nparts, psz, pmax, pmin, d, qq, w, attr, &np);
     kheap h = { (uint64_t *)malloc(sizeof(uint64_t) * (k ? k : 1)), 0, k };
     uint64_t sc = 0;
-    for (uint64_t t = 0; t < np; t++) {
+    for (uint32_t t = 0; t < np; t++) {
         uint32_t bkey = (uint32_t)(pb[t].key >> 32);
         if (h.n == k && bkey < (uint32_t)(h.a[0] >> 32)) break;   /* strict skip */
-        uint64_t p = pb[t].p, lo = p * psize, hi = lo + psize < n ? lo + psize : n;
-        for (uint64_t j = lo; j < hi; j++)
+        uint32_t p = pb[t].p;
+        for (uint64_t j = poff[p]; j < (uint64_t)poff[p] + psz[p]; j++)
             heap_push(&h, or_key(score_one(pdata, ld, j, qq, w, attr), pid[j]));
         sc++;
     }
@@ -440,22 +454,18 @@ uint32_t or_pta_topk(const float *pdata, const uint32_t *pid, uint64_t n, uint64
     return c;
 }
 
-/* Threaded PTA baseline: partitions are split over threads after the    */
-/* bound sort (cpu/PTA.h:569-671 marches partitions per thread); each    */
+/* Threaded PTA baseline: partitions are dealt to threads in descending  */
+/* bound order (cpu/PTA.h:569-671 marches partitions per thread); each   */
 /* thread prunes against its own heap, then the heaps merge.             */
-uint32_t or_pta_threads(const float *pdata, const uint32_t *pid, uint64_t n, uint64_t ld,
-                        uint32_t d, uint32_t psize, const float *pmax, const float *pmin,
+uint32_t or_pta_threads(const float *pdata, const uint32_t *pid, uint64_t ld, uint32_t d,
+                        uint32_t nparts, const uint32_t *poff, const uint32_t *psz,
+                        const float *pmax, const float *pmin,
                         uint32_t k, uint32_t qq, const float *w, const uint8_t *attr,
                         int threads, uint64_t *ids, float *scores, float *thr, uint64_t *scored)
 {
     if (threads < 1) threads = 1;
-    uint64_t np = (n + psize - 1) / psize;
-    pbound *pb = (pbound *)malloc(sizeof(pbound) * (np ? np : 1));
-    for (uint64_t p = 0; p < np; p++) {
-        pb[p].key = (uint64_t)or_okey(or_pta_bound(pmax, pmin, d, p, qq, w, attr)) << 32;
-        pb[p].p = (uint32_t)p;
-    }
-    qsort(pb, np, sizeof(pbound), cmp_pbound_desc);
+    uint32_t np;
+    pbound *pb = pta_sorted_bounds(nparts, psz, pmax, pmin, d, qq, w, attr, &np);
     uint64_t *all = (uint64_t *)malloc(sizeof(uint64_t) * (uint64_t)k * threads);
     uint32_t *cnt = (uint32_t *)calloc(threads, sizeof(uint32_t));
     uint64_t *scnt = (uint64_t *)calloc(threads, sizeof(uint64_t));
@@ -467,11 +477,11 @@ uint32_t or_pta_threads(const float *pdata, const uint32_t *pid, uint64_t n, uin
         int t = 0;
 #endif
         kheap h = { all + (uint64_t)k * t, 0, k };
-        for (uint64_t r = t; r < np; r += threads) {
+        for (uint32_t r = t; r < np; r += threads) {
             uint32_t bkey = (uint32_t)(pb[r].key >> 32);
             if (h.n == k && bkey < (uint32_t)(h.a[0] >> 32)) break;
-            uint64_t p = pb[r].p, lo = p * psize, hi = lo + psize < n ? lo + psize : n;
-            for (uint64_t j = lo; j < hi; j++)
+            uint32_t p = pb[r].p;
+            for (uint64_t j = poff[p]; j < (uint64_t)poff[p] + psz[p]; j++)
                 heap_push(&h, or_key(score_one(pdata, ld, j, qq, w, attr), pid[j]));
             scnt[t]++;
         }

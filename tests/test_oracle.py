@@ -181,38 +181,43 @@ def test_gpa_step_known_answer():
 
 
 def _random_layout(x, n, psize, rng, cluster):
+    """A partition layout like the engine's: partitions of psize objects
+    (last one short), each padded to a multiple of 256 positions."""
     d = x.shape[0]
     if cluster:
         perm = np.lexsort(tuple((x[m] * 4).astype(int) for m in range(d)))
     else:
         perm = rng.permutation(n)
     npart = (n + psize - 1) // psize
-    ld = npart * psize
-    pdata = np.zeros((d, ld), np.float32)
-    pdata[:, :n] = x[:, perm]
-    pid = np.zeros(ld, np.uint32)
-    pid[:n] = perm
+    psz = np.array([min(psize, n - p * psize) for p in range(npart)], np.uint32)
+    pad = ((psz + 255) // 256) * 256
+    poff = np.concatenate([[0], np.cumsum(pad)]).astype(np.uint32)
+    npos = int(poff[-1])
+    pdata = np.zeros((d, npos), np.float32)
+    pid = np.full(npos, 0xFFFFFFFF, np.uint32)
     pmax = np.zeros((npart, d), np.float32)
     pmin = np.zeros((npart, d), np.float32)
     for p in range(npart):
-        seg = pdata[:, p * psize:min(n, (p + 1) * psize)]
-        pmax[p] = seg.max(axis=1)
-        pmin[p] = seg.min(axis=1)
-    return pdata, pid, pmax, pmin
+        ids = perm[p * psize:p * psize + psz[p]]
+        pdata[:, poff[p]:poff[p] + psz[p]] = x[:, ids]
+        pid[poff[p]:poff[p] + psz[p]] = ids
+        pmax[p] = x[:, ids].max(axis=1)
+        pmin[p] = x[:, ids].min(axis=1)
+    return pdata, pid, poff, psz, pmax, pmin
 
 
 @pytest.mark.parametrize("cluster", [False, True])
 def test_pta_equals_fte(cluster):
-    n, d, psize = 40_000, 4, 1024
+    n, d, psize = 40_000, 4, 1000
     x = RNG.random((d, n), dtype=np.float32)
-    pdata, pid, pmax, pmin = _random_layout(x, n, psize, RNG, cluster)
+    pdata, pid, poff, psz, pmax, pmin = _random_layout(x, n, psize, RNG, cluster)
     for w, attr, k in ((np.ones(d, np.float32), [0, 1, 2, 3], 16),
                        (RNG.uniform(0.1, 1, d).astype(np.float32), [3, 1], 50),
                        (np.array([1.0, -0.5, 0.25, 2.0], np.float32), [1, 0, 3], 8)):
         ref = oracle.fte_topk(x, n, k, w, attr)
-        got = oracle.pta_topk(pdata, pid, n, psize, pmax, pmin, k, w, attr)
+        got = oracle.pta_topk(pdata, pid, poff, psz, pmax, pmin, k, w, attr)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
-        got_t = oracle.pta_topk(pdata, pid, n, psize, pmax, pmin, k, w, attr, threads=4)
+        got_t = oracle.pta_topk(pdata, pid, poff, psz, pmax, pmin, k, w, attr, threads=4)
         assert np.array_equal(got_t[0], ref[0]) and np.array_equal(got_t[1], ref[1])
         if cluster:
             assert got[3] < pmax.shape[0]  # some partitions were skipped
@@ -221,13 +226,13 @@ def test_pta_equals_fte(cluster):
 def test_pta_bound_is_upper_bound():
     n, d, psize = 8192, 3, 1024
     x = RNG.random((d, n), dtype=np.float32)
-    pdata, pid, pmax, pmin = _random_layout(x, n, psize, RNG, True)
+    pdata, pid, poff, psz, pmax, pmin = _random_layout(x, n, psize, RNG, True)
     w = np.array([0.7, -1.3, 0.2], np.float32)
     attr = [2, 1, 0]
-    s = oracle.score(pdata, n, w, attr)
+    s = oracle.score(pdata, pdata.shape[1], w, attr)
     for p in range(pmax.shape[0]):
         b = oracle.pta_bound(pmax, pmin, d, p, w, attr)
-        assert s[p * psize:(p + 1) * psize].max() <= b
+        assert s[poff[p]:poff[p] + psz[p]].max() <= b
 
 
 def test_mlp_gradient_finite_differences():

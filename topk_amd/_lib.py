@@ -59,6 +59,8 @@ SIGNATURES = {
     "topk_init": (_c.c_int, [_vp, _c.c_uint32]),
     "topk_set_partition_size": (_c.c_int, [_vp, _c.c_uint32]),
     "topk_reorder_attr": (_c.c_int, [_vp]),
+    "topk_pta_layout": (_c.c_int, [_vp, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint64)]),
+    "topk_pta_read_layout": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "topk_fte": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
     "topk_pta": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
     "topk_bta": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),

@@ -18,6 +18,8 @@
 
 namespace topk {
 
+void pta_release(topk_ctx *ctx);
+
 int hip_status(hipError_t e) {
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return TOPK_ERR_NOMEM;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return TOPK_ERR_NO_DEVICE;
@@ -201,10 +203,7 @@ int topk_ctx_destroy(topk_ctx *ctx) {
     if (ctx->h_state) hipHostFree(ctx->h_state);
     hipFree(ctx->batch_buf);
     if (ctx->h_batch) hipHostFree(ctx->h_batch);
-    hipFree(ctx->pta.pcols);
-    hipFree(ctx->pta.pid);
-    hipFree(ctx->pta.pmax);
-    hipFree(ctx->pta.pmin);
+    pta_release(ctx);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return TOPK_OK;

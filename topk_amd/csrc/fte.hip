@@ -126,13 +126,6 @@ __global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_e
 // ---------------------------------------------------------------------------
 // final select + sort
 // ---------------------------------------------------------------------------
-struct FinalOut {
-    uint64_t *ids;
-    float *scores;
-    float *thr;
-    uint32_t *status;
-};
-
 // Flat index -> key over pre[0..npre) followed by the blocks' slot regions;
 // off[] (LDS) holds the exclusive prefix of the per-block counts.
 struct SlotView {
@@ -582,15 +575,22 @@ int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_
 #undef TK_STREAM8
         timing_end(ctx, s, "fte_stream", ev0);
     }
-    if (big) {
+    final_enqueue(ctx, L, cs, k, kreq, fo, big);
+    TK_HIP(hipGetLastError());
+    return TOPK_OK;
+}
+
+void final_enqueue(topk_ctx *ctx, Lane &L, const CandSet &cs, uint32_t k, uint32_t kreq, const FinalOut &fo,
+                   bool flat_fallback) {
+    hipStream_t s = L.stream;
+    if (flat_fallback) {
         hipLaunchKernelGGL(cand_offsets_kernel, dim3(1), dim3(FLAT_BS), 0, s, L.st, cs, ctx->offsets, ctx->flat);
-        if (g) hipLaunchKernelGGL(cand_compact_kernel, dim3(g), dim3(256), 0, s, cs, ctx->offsets, ctx->flat);
+        if (cs.nblk)
+            hipLaunchKernelGGL(cand_compact_kernel, dim3(cs.nblk), dim3(256), 0, s, cs, ctx->offsets, ctx->flat);
         hipLaunchKernelGGL(fte_final_flat_kernel, dim3(1), dim3(FLAT_BS), 0, s, L.st, ctx->flat, k, kreq, fo);
     } else {
         hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, L.st, cs, k, kreq, fo);
     }
-    TK_HIP(hipGetLastError());
-    return TOPK_OK;
 }
 
 }  // namespace topk

@@ -1,0 +1,221 @@
+// Store preprocessing and the PTA entry points of the C ABI.
+//
+//  * normalisation: common/common.h:29-60 (per column, max starting at 0 and
+//    min at FLT_MAX, x = (x - min) / (max - min) in binary32), called by
+//    TPAc::init / PTA::init (cpu/TPAc.h:28-32, cpu/PTA.h:271).
+//  * reorder_attr: gpu/reorder_attr.h:13-236 (per object, sort the d values
+//    descending across the columns, in place); check_order invariant
+//    gpu/GPA.h:96-114.
+//  * PTA: layout build (pta.hip) and the pruned query.
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace topk {
+
+int pta_build(topk_ctx *ctx);
+void pta_release(topk_ctx *ctx);
+int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *d_ids,
+              float *d_scores, float *d_thr, uint32_t *d_status, uint64_t *scored_out);
+
+// Per column max (starting at 0) and min (starting at FLT_MAX) as monotone
+// keys, so atomicMax / atomicMin on uint32 implement float max / min.
+__global__ __launch_bounds__(256) void colminmax_kernel(const float *__restrict__ cols, uint64_t n, uint64_t ld,
+                                                        uint32_t *__restrict__ kmax, uint32_t *__restrict__ kmin) {
+    const uint32_t m = blockIdx.y;
+    const float *c = cols + (uint64_t)m * ld;
+    float mx = 0.0f, mn = 3.40282347e+38f;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const float v = c[i];
+        mx = mx < v ? v : mx;   // std::max(mmax, value)
+        mn = v < mn ? v : mn;   // std::min(mmin, value)
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float a = __shfl_xor(mx, o, 64), b = __shfl_xor(mn, o, 64);
+        mx = mx < a ? a : mx;
+        mn = b < mn ? b : mn;
+    }
+    if (lane_id() == 0) {
+        atomicMax(&kmax[m], okey(mx));
+        atomicMin(&kmin[m], okey(mn));
+    }
+}
+
+__global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t ld, const uint32_t *__restrict__ kmax,
+                                 const uint32_t *__restrict__ kmin) {
+    const uint32_t m = blockIdx.y;
+    const float mx = okey_inv(kmax[m]), mn = okey_inv(kmin[m]);
+    const float mm = mx - mn;
+    float *c = cols + (uint64_t)m * ld;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        c[i] = (c[i] - mn) / mm;
+}
+
+__global__ void reorder_attr_kernel(float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v[TOPK_MAX_DIMS];
+    for (uint32_t m = 0; m < d; m++) v[m] = cols[(uint64_t)m * ld + i];
+    for (uint32_t a = 1; a < d; a++) {  // insertion sort, descending
+        const float x = v[a];
+        int b = (int)a - 1;
+        while (b >= 0 && v[b] < x) {
+            v[b + 1] = v[b];
+            b--;
+        }
+        v[b + 1] = x;
+    }
+    for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = v[m];
+}
+
+__global__ void nonfinite_check_kernel(const float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld,
+                                       unsigned int *__restrict__ flag) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    bool bad = false;
+    for (uint32_t m = 0; m < d; m++)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            bad |= !isfinite(cols[(uint64_t)m * ld + i]);
+    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
+}
+
+static int normalize_store(topk_ctx *ctx) {
+    hipStream_t s = ctx->stream;
+    uint32_t *k = nullptr;
+    TK_HIP(hipMalloc(&k, sizeof(uint32_t) * (2 * ctx->d + 1)));
+    std::vector<uint32_t> init(2 * ctx->d + 1);
+    for (uint32_t m = 0; m < ctx->d; m++) {
+        init[m] = okey(0.0f);
+        init[ctx->d + m] = okey(3.40282347e+38f);
+    }
+    init[2 * ctx->d] = 0;
+    hipError_t e = hipMemcpyAsync(k, init.data(), sizeof(uint32_t) * init.size(), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(colminmax_kernel, dim3(512, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
+                           k + ctx->d);
+        hipLaunchKernelGGL(normalize_kernel, dim3(2048, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
+                           k + ctx->d);
+        hipLaunchKernelGGL(nonfinite_check_kernel, dim3(1024), dim3(256), 0, s, ctx->cols, ctx->n, ctx->d, ctx->ld,
+                           k + 2 * ctx->d);
+        e = hipGetLastError();
+    }
+    uint32_t flag = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, k + 2 * ctx->d, sizeof(flag), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(k);
+    TK_HIP(e);
+    return flag ? TOPK_ERR_NONFINITE : TOPK_OK;
+}
+
+}  // namespace topk
+
+using namespace topk;
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+extern "C" {
+
+int topk_init(topk_ctx *ctx, uint32_t flags) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    if (flags & ~(TOPK_INIT_NORMALIZE | TOPK_INIT_PTA)) return TOPK_ERR_INVALID;
+    const double t0 = now_ms();
+    int rc = TOPK_OK;
+    if (flags & TOPK_INIT_NORMALIZE) {
+        pta_release(ctx);  // a layout built from the old values would be stale
+        rc = normalize_store(ctx);
+        if (rc) return rc;
+    }
+    if (flags & TOPK_INIT_PTA) {
+        rc = pta_build(ctx);
+        if (rc) return rc;
+    }
+    ctx->stats.tt_init = now_ms() - t0;
+    return TOPK_OK;
+}
+
+int topk_set_partition_size(topk_ctx *ctx, uint32_t objects) {
+    if (!ctx || objects < 1024 || objects > 65536 || objects % 1024) return TOPK_ERR_INVALID;
+    ctx->pta.psize = objects;
+    return TOPK_OK;
+}
+
+int topk_reorder_attr(topk_ctx *ctx) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    pta_release(ctx);
+    hipLaunchKernelGGL(reorder_attr_kernel, dim3((uint32_t)((ctx->n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->cols, ctx->n, ctx->d, ctx->ld);
+    TK_HIP(hipGetLastError());
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TOPK_OK;
+}
+
+int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *out_ids,
+             float *out_scores, float *out_threshold) {
+    if (!ctx || k < 1) return TOPK_ERR_INVALID;
+    if (k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    if (!ctx->pta.built) return TOPK_ERR_STATE;
+    const double t0 = now_ms();
+    uint64_t *d_ids = reinterpret_cast<uint64_t *>(ctx->d_res + ctx->res.ids_off);
+    float *d_sc = reinterpret_cast<float *>(ctx->d_res + ctx->res.scores_off);
+    float *d_thr = reinterpret_cast<float *>(ctx->d_res + ctx->res.thr_off);
+    uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
+    uint64_t scored = 0;
+    int rc = pta_query(ctx, k, qq, weights, attr, d_ids, d_sc, d_thr, d_status, &scored);
+    if (rc) return rc;
+    Lane &L = ctx->lane[0];
+    TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
+    TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
+    TK_HIP(hipStreamSynchronize(L.stream));
+    const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
+    if (status) {
+        // candidate storage overflowed (pathological data): the exact FTE path answers
+        TK_HIP(hipMemsetAsync(&L.st->status, 0, sizeof(uint32_t), L.stream));
+        rc = topk_fte(ctx, k, qq, weights, attr, out_ids, out_scores, out_threshold);
+        ctx->stats.fallback = 1;
+        ctx->stats.partitions_scored = ctx->pta.nparts;
+        ctx->stats.partitions_total = ctx->pta.nparts;
+        return rc;
+    }
+    const uint64_t *ids = reinterpret_cast<const uint64_t *>(ctx->h_res + ctx->res.ids_off);
+    const float *sc = reinterpret_cast<const float *>(ctx->h_res + ctx->res.scores_off);
+    if (out_ids) memcpy(out_ids, ids, sizeof(uint64_t) * k);
+    if (out_scores) memcpy(out_scores, sc, sizeof(float) * k);
+    if (out_threshold) *out_threshold = *reinterpret_cast<const float *>(ctx->h_res + ctx->res.thr_off);
+    ctx->stats.candidates = ctx->h_state->cand_last;
+    ctx->stats.partitions_scored = scored;
+    ctx->stats.partitions_total = ctx->pta.nparts;
+    ctx->stats.fallback = 0;
+    ctx->stats.queries = 1;
+    ctx->stats.ms_last = now_ms() - t0;
+    ctx->stats.tuple_count = 0;
+    return TOPK_OK;
+}
+
+int topk_pta_layout(topk_ctx *ctx, uint32_t *nparts, uint64_t *npos) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    if (!ctx->pta.built) return TOPK_ERR_STATE;
+    if (nparts) *nparts = ctx->pta.nparts;
+    if (npos) *npos = ctx->pta.npos;
+    return TOPK_OK;
+}
+
+int topk_pta_read_layout(topk_ctx *ctx, uint32_t *pid, uint32_t *poff, uint32_t *psz, float *pmax, float *pmin) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    const PtaLayout &P = ctx->pta;
+    if (!P.built) return TOPK_ERR_STATE;
+    hipStream_t s = ctx->stream;
+    if (pid) TK_HIP(hipMemcpyAsync(pid, P.pid, sizeof(uint32_t) * P.npos, hipMemcpyDeviceToHost, s));
+    if (poff) TK_HIP(hipMemcpyAsync(poff, P.poff, sizeof(uint32_t) * (P.nparts + 1), hipMemcpyDeviceToHost, s));
+    if (psz) TK_HIP(hipMemcpyAsync(psz, P.psz, sizeof(uint32_t) * P.nparts, hipMemcpyDeviceToHost, s));
+    if (pmax) TK_HIP(hipMemcpyAsync(pmax, P.pmax, sizeof(float) * P.nparts * ctx->d, hipMemcpyDeviceToHost, s));
+    if (pmin) TK_HIP(hipMemcpyAsync(pmin, P.pmin, sizeof(float) * P.nparts * ctx->d, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipStreamSynchronize(s));
+    return TOPK_OK;
+}
+
+}  // extern "C"

@@ -123,16 +123,29 @@ struct Timing {
 };
 
 // Partition layout of the PTA path (built by topk_init with TOPK_INIT_PTA).
+// Objects are grouped by a k-d tree into nparts partitions of about psize
+// objects; partition p occupies positions [poff[p], poff[p] + psz[p]) of the
+// partition-order columns, padded to a multiple of PART_ALIGN positions.
 struct PtaLayout {
     bool built = false;
-    uint32_t psize = 4096;
-    uint64_t nparts = 0;
-    uint64_t ld = 0;          // column stride of pcols (nparts * psize)
-    float *pcols = nullptr;   // d columns in partition order
-    uint32_t *pid = nullptr;  // global object id at each position
-    float *pmax = nullptr;    // nparts x d per-partition per-attribute maxima
-    float *pmin = nullptr;    // nparts x d minima
+    uint32_t psize = 4096;       // target objects per partition
+    uint32_t nparts = 0;
+    uint64_t npos = 0;           // positions incl. padding (poff[nparts])
+    uint64_t ld = 0;             // column stride of pcols
+    float *pcols = nullptr;      // d columns in partition order, + one stream tile of padding
+    uint32_t *pid = nullptr;     // global object id per position (0xFFFFFFFF: padding)
+    uint32_t *poff = nullptr;    // nparts + 1 start positions
+    uint32_t *psz = nullptr;     // real objects per partition
+    float *pmax = nullptr;       // nparts x d per-attribute maxima
+    float *pmin = nullptr;       // nparts x d per-attribute minima
+    float *psum = nullptr;       // nparts: max over objects of the sum of its d attributes (rounded up)
+    float *pminsum = nullptr;    // nparts: sum of the d minima (rounded down)
+    uint64_t *pbound = nullptr;  // query scratch: bound key per partition
+    uint64_t *bmax = nullptr;    // query scratch: max bound key per 256 partitions
+    uint32_t *pscored = nullptr; // query scratch: partitions scored per stream block
+    double tt_build_ms = 0;
 };
+constexpr uint32_t PART_ALIGN = 256;
 
 }  // namespace topk
 
@@ -176,7 +189,18 @@ struct topk_ctx {
 
 namespace topk {
 
+// Where a final select writes its results (device pointers; thr/status may be null).
+struct FinalOut {
+    uint64_t *ids;
+    float *scores;
+    float *thr;
+    uint32_t *status;
+};
+
 // Launch helpers implemented in the .hip translation units.
+// Enqueues the final select over a candidate set on lane L (fte.hip).
+void final_enqueue(topk_ctx *ctx, Lane &L, const CandSet &cs, uint32_t k, uint32_t kreq, const FinalOut &fo,
+                   bool flat_fallback);
 int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
                 float *d_thr, uint32_t *d_status, int mode);
 int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq, const float *w,

@@ -1,0 +1,680 @@
+// PTA: partitioned threshold top-k on gfx950.
+//
+// Reference: cpu/PTA.h:88-267 (partition build: polar angles, sorted lists,
+// per-block threshold vectors) and cpu/PTA.h:357-422, 569-671 (scan with a
+// per-block bound, stop once the heap top beats it).  This build keeps the
+// idea -- group similar objects, bound each group, skip groups that cannot
+// reach the running k-th score -- with its own partitioning and bound:
+//
+//  * layout (topk_init with TOPK_INIT_PTA): a k-d tree whose split planes
+//    come from a deterministic sample (median splits on the axis of largest
+//    spread, built on the host); every object is routed to its leaf on the
+//    GPU, objects are counting-sorted by leaf (ids ascending inside a leaf),
+//    the columns are gathered into partition order and each partition keeps
+//    per-attribute max/min, the max of its objects' attribute sums and the
+//    sum of its minima.
+//  * bound: the largest weighted sum any point of the partition's box can
+//    reach under the attribute-sum limit (a fractional-knapsack LP, solved
+//    greedily in FP64), plus a margin covering binary32 rounding of a score,
+//    rounded up to binary32.  It is never below an object's score.
+//  * query: bound keys for all partitions; the exact k-th key among the best
+//    partitions' objects (seed) is a proven lower bound tau of the global
+//    k-th key; the stream pass scores only partitions whose bound reaches
+//    tau's score (skip iff bound < tau strictly) and keeps keys >= tau; the
+//    FTE final select finishes.  Results equal topk_fte's.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace topk {
+
+constexpr int PTA_BS = 256;
+constexpr uint32_t PTA_TILE = 1024;          // objects per stream step (4 per lane)
+constexpr uint32_t PTA_SORT_MAX = 8192;      // partitions up to this size get ids sorted in LDS
+constexpr uint32_t PTA_SEED_OBJS = 4096;     // objects of a seed partition that are scored
+constexpr uint32_t PTA_MAX_SEED = 64;
+
+struct KdNode {
+    float split;
+    uint32_t axis;
+    int32_t left;   // >= 0: internal node, < 0: leaf -(leaf + 1)
+    int32_t right;
+};
+
+// ---------------------------------------------------------------------------
+// layout build kernels
+// ---------------------------------------------------------------------------
+__global__ void pta_sample_kernel(const float *__restrict__ cols, uint64_t ld, uint32_t d, uint64_t n, uint64_t s,
+                                  float *__restrict__ rows) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s) return;
+    const uint64_t i = j * n / s;
+    for (uint32_t m = 0; m < d; m++) rows[j * d + m] = cols[(uint64_t)m * ld + i];
+}
+
+__global__ void pta_assign_kernel(const float *__restrict__ cols, uint64_t ld, uint64_t n,
+                                  const KdNode *__restrict__ nodes, int32_t root, uint32_t *__restrict__ leaf,
+                                  uint32_t *__restrict__ cnt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t c = root;
+    while (c >= 0) {
+        const KdNode nd = nodes[c];
+        c = cols[(uint64_t)nd.axis * ld + i] < nd.split ? nd.left : nd.right;
+    }
+    const uint32_t l = (uint32_t)(-c - 1);
+    leaf[i] = l;
+    atomicAdd(&cnt[l], 1u);
+}
+
+__global__ void pta_scatter_kernel(uint64_t n, const uint32_t *__restrict__ leaf, const uint32_t *__restrict__ poff,
+                                   uint32_t *__restrict__ cursor, uint32_t *__restrict__ perm) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t l = leaf[i];
+    perm[poff[l] + atomicAdd(&cursor[l], 1u)] = (uint32_t)i;
+}
+
+// Sorts each partition's ids ascending (deterministic layout).
+__global__ __launch_bounds__(1024) void pta_sort_ids_kernel(const uint32_t *__restrict__ poff,
+                                                            const uint32_t *__restrict__ psz,
+                                                            uint32_t *__restrict__ perm) {
+    __shared__ uint32_t a[PTA_SORT_MAX];
+    const uint32_t p = blockIdx.x, s = psz[p];
+    if (s < 2 || s > PTA_SORT_MAX) return;
+    uint32_t *seg = perm + poff[p];
+    uint32_t n2 = 1;
+    while (n2 < s) n2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < n2; i += 1024) a[i] = i < s ? seg[i] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= n2; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = threadIdx.x; t < n2 / 2; t += 1024) {
+                const uint32_t lo = 2 * stride * (t / stride) + (t & (stride - 1)), hi = lo + stride;
+                const bool asc = (lo & size) == 0;
+                const uint32_t x = a[lo], y = a[hi];
+                if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < s; i += 1024) seg[i] = a[i];
+}
+
+__global__ void pta_gather_kernel(const float *__restrict__ cols, uint64_t ld, uint32_t d, uint64_t npos,
+                                  const uint32_t *__restrict__ perm, float *__restrict__ pcols, uint64_t ldp,
+                                  uint32_t *__restrict__ pid) {
+    const uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= npos) return;
+    const uint32_t id = perm[pos];
+    pid[pos] = id;
+    for (uint32_t m = 0; m < d; m++) pcols[(uint64_t)m * ldp + pos] = id != 0xFFFFFFFFu ? cols[(uint64_t)m * ld + id] : 0.0f;
+}
+
+__device__ __forceinline__ float f32_round_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+__device__ __forceinline__ float f32_round_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// One block per partition: per-attribute max/min, the max attribute sum
+// (FP64, rounded up) and the sum of the minima (FP64, rounded down).
+__global__ __launch_bounds__(PTA_BS) void pta_bounds_kernel(const float *__restrict__ pcols, uint64_t ldp, uint32_t d,
+                                                            const uint32_t *__restrict__ poff,
+                                                            const uint32_t *__restrict__ psz,
+                                                            float *__restrict__ pmax, float *__restrict__ pmin,
+                                                            float *__restrict__ psum, float *__restrict__ pminsum) {
+    __shared__ float smax[PTA_BS / 64], smin[PTA_BS / 64];
+    __shared__ double ssum[PTA_BS / 64];
+    __shared__ float colmin[TOPK_MAX_DIMS];
+    const uint32_t p = blockIdx.x, start = poff[p], s = psz[p];
+    const uint32_t w = threadIdx.x / 64;
+    for (uint32_t m = 0; m < d; m++) {
+        float mx = -INFINITY, mn = INFINITY;
+        for (uint32_t i = threadIdx.x; i < s; i += PTA_BS) {
+            const float v = pcols[(uint64_t)m * ldp + start + i];
+            mx = v > mx ? v : mx;
+            mn = v < mn ? v : mn;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+            mn = fminf(mn, __shfl_xor(mn, o, 64));
+        }
+        if (lane_id() == 0) { smax[w] = mx; smin[w] = mn; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int j = 1; j < PTA_BS / 64; j++) { mx = fmaxf(mx, smax[j]); mn = fminf(mn, smin[j]); }
+            pmax[(uint64_t)p * d + m] = s ? mx : 0.0f;
+            pmin[(uint64_t)p * d + m] = s ? mn : 0.0f;
+            colmin[m] = s ? mn : 0.0f;
+        }
+        __syncthreads();
+    }
+    double best = -INFINITY;
+    for (uint32_t i = threadIdx.x; i < s; i += PTA_BS) {
+        double t = 0;
+        for (uint32_t m = 0; m < d; m++) t += (double)pcols[(uint64_t)m * ldp + start + i];
+        best = t > best ? t : best;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(best, o, 64);
+        best = t > best ? t : best;
+    }
+    if (lane_id() == 0) ssum[w] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < PTA_BS / 64; j++) best = ssum[j] > best ? ssum[j] : best;
+        double ms = 0;
+        for (uint32_t m = 0; m < d; m++) ms += (double)colmin[m];
+        psum[p] = s ? f32_round_up(best) : -INFINITY;
+        pminsum[p] = s ? f32_round_down(ms) : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: k-d tree on the sample
+// ---------------------------------------------------------------------------
+namespace {
+
+struct KdBuilder {
+    const float *rows;
+    uint32_t d;
+    std::vector<uint32_t> idx;
+    std::vector<KdNode> nodes;  // internal node for the split between leaves m-1 and m is nodes[m-1]
+
+    // Builds the subtree holding leaves [leaf_lo, leaf_lo + nl) from sample
+    // rows idx[lo, hi); returns its node code.
+    int32_t build(size_t lo, size_t hi, uint32_t leaf_lo, uint32_t nl, int depth) {
+        if (nl == 1) return -(int32_t)leaf_lo - 1;
+        const uint32_t nleft = nl / 2;
+        const size_t mid = lo + (hi - lo) * nleft / nl;
+        uint32_t axis = 0;
+        float split = INFINITY;
+        if (hi > lo) {
+            float best = -1.0f;
+            for (uint32_t m = 0; m < d; m++) {
+                float mn = INFINITY, mx = -INFINITY;
+                for (size_t j = lo; j < hi; j++) {
+                    const float v = rows[(size_t)idx[j] * d + m];
+                    mn = v < mn ? v : mn;
+                    mx = v > mx ? v : mx;
+                }
+                if (mx - mn > best) { best = mx - mn; axis = m; }
+            }
+            if (mid < hi) {
+                auto less = [&](uint32_t a, uint32_t b) {
+                    const float va = rows[(size_t)a * d + axis], vb = rows[(size_t)b * d + axis];
+                    return va < vb || (va == vb && a < b);
+                };
+                std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, less);
+                split = rows[(size_t)idx[mid] * d + axis];
+            }
+        }
+        const int32_t me = (int32_t)(leaf_lo + nleft - 1);
+        int32_t l, r;
+        if (depth < 4 && hi - lo > 65536) {
+            std::thread th([&] { l = build(lo, mid, leaf_lo, nleft, depth + 1); });
+            r = build(mid, hi, leaf_lo + nleft, nl - nleft, depth + 1);
+            th.join();
+        } else {
+            l = build(lo, mid, leaf_lo, nleft, depth + 1);
+            r = build(mid, hi, leaf_lo + nleft, nl - nleft, depth + 1);
+        }
+        nodes[me] = KdNode{split, axis, l, r};
+        return me;
+    }
+};
+
+}  // namespace
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void pta_free(PtaLayout &P) {
+    hipFree(P.pcols);
+    hipFree(P.pid);
+    hipFree(P.poff);
+    hipFree(P.psz);
+    hipFree(P.pmax);
+    hipFree(P.pmin);
+    hipFree(P.psum);
+    hipFree(P.pminsum);
+    hipFree(P.pbound);
+    hipFree(P.bmax);
+    hipFree(P.pscored);
+    const uint32_t psize = P.psize;
+    P = PtaLayout();
+    P.psize = psize;
+}
+
+void pta_release(topk_ctx *ctx) { pta_free(ctx->pta); }
+
+int pta_build(topk_ctx *ctx) {
+    const double t0 = now_ms();
+    PtaLayout &P = ctx->pta;
+    pta_free(P);
+    const uint64_t n = ctx->n;
+    const uint32_t d = ctx->d;
+    hipStream_t s = ctx->stream;
+    const uint32_t L = (uint32_t)((n + P.psize - 1) / P.psize);
+
+    // 1. deterministic sample -> host k-d tree with L leaves
+    uint64_t ns = (uint64_t)L * 64;
+    if (ns < (1u << 20)) ns = 1u << 20;
+    if (ns > n) ns = n;
+    std::vector<float> rows((size_t)ns * d);
+    {
+        float *d_rows = nullptr;
+        TK_HIP(hipMalloc(&d_rows, sizeof(float) * ns * d));
+        hipLaunchKernelGGL(pta_sample_kernel, dim3((uint32_t)((ns + 255) / 256)), dim3(256), 0, s, ctx->cols,
+                           ctx->ld, d, n, ns, d_rows);
+        hipError_t e = hipMemcpyAsync(rows.data(), d_rows, sizeof(float) * ns * d, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        hipFree(d_rows);
+        TK_HIP(e);
+    }
+    KdBuilder kb{rows.data(), d, std::vector<uint32_t>(ns), std::vector<KdNode>(L > 1 ? L - 1 : 1)};
+    std::iota(kb.idx.begin(), kb.idx.end(), 0u);
+    const int32_t root = kb.build(0, ns, 0, L, 0);
+
+    // 2. route every object to its leaf and count
+    KdNode *d_nodes = nullptr;
+    uint32_t *leaf = nullptr, *cnt = nullptr, *cursor = nullptr, *perm = nullptr;
+    int rc = TOPK_OK;
+    auto cleanup = [&] {
+        hipFree(d_nodes);
+        hipFree(leaf);
+        hipFree(cnt);
+        hipFree(cursor);
+    };
+#define TK_TRY(call)                                              \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) { rc = hip_status(e_); goto fail; } \
+    } while (0)
+    {
+        TK_TRY(hipMalloc(&d_nodes, sizeof(KdNode) * kb.nodes.size()));
+        TK_TRY(hipMemcpyAsync(d_nodes, kb.nodes.data(), sizeof(KdNode) * kb.nodes.size(), hipMemcpyHostToDevice, s));
+        TK_TRY(hipMalloc(&leaf, sizeof(uint32_t) * n));
+        TK_TRY(hipMalloc(&cnt, sizeof(uint32_t) * L));
+        TK_TRY(hipMalloc(&cursor, sizeof(uint32_t) * L));
+        TK_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * L, s));
+        TK_TRY(hipMemsetAsync(cursor, 0, sizeof(uint32_t) * L, s));
+        hipLaunchKernelGGL(pta_assign_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, ctx->cols, ctx->ld, n,
+                           d_nodes, root, leaf, cnt);
+        std::vector<uint32_t> h_cnt(L), h_off(L + 1);
+        TK_TRY(hipMemcpyAsync(h_cnt.data(), cnt, sizeof(uint32_t) * L, hipMemcpyDeviceToHost, s));
+        TK_TRY(hipStreamSynchronize(s));
+        uint64_t acc = 0;
+        for (uint32_t p = 0; p < L; p++) {
+            h_off[p] = (uint32_t)acc;
+            acc += ((uint64_t)h_cnt[p] + PART_ALIGN - 1) / PART_ALIGN * PART_ALIGN;
+        }
+        h_off[L] = (uint32_t)acc;
+        if (acc >= 0xFFFFFFFFull) { rc = TOPK_ERR_UNSUPPORTED; goto fail; }
+        P.nparts = L;
+        P.npos = acc;
+        P.ld = (acc + 8191) / 8192 * 8192;
+        TK_TRY(hipMalloc(&P.poff, sizeof(uint32_t) * (L + 1)));
+        TK_TRY(hipMalloc(&P.psz, sizeof(uint32_t) * L));
+        TK_TRY(hipMemcpyAsync(P.poff, h_off.data(), sizeof(uint32_t) * (L + 1), hipMemcpyHostToDevice, s));
+        TK_TRY(hipMemcpyAsync(P.psz, h_cnt.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice, s));
+        TK_TRY(hipStreamSynchronize(s));  // host vectors go out of scope
+
+        // 3. counting sort by leaf, ids ascending inside each partition
+        TK_TRY(hipMalloc(&perm, sizeof(uint32_t) * P.ld));
+        TK_TRY(hipMemsetAsync(perm, 0xFF, sizeof(uint32_t) * P.ld, s));
+        hipLaunchKernelGGL(pta_scatter_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, n, leaf, P.poff,
+                           cursor, perm);
+        hipLaunchKernelGGL(pta_sort_ids_kernel, dim3(L), dim3(1024), 0, s, P.poff, P.psz, perm);
+
+        // 4. partition-order columns and bounds
+        const size_t pfl = P.ld * d + 8192;
+        TK_TRY(hipMalloc(&P.pcols, sizeof(float) * pfl));
+        TK_TRY(hipMemsetAsync(P.pcols, 0, sizeof(float) * pfl, s));
+        P.pid = perm;
+        perm = nullptr;
+        hipLaunchKernelGGL(pta_gather_kernel, dim3((uint32_t)((P.ld + 255) / 256)), dim3(256), 0, s, ctx->cols,
+                           ctx->ld, d, P.ld, P.pid, P.pcols, P.ld, P.pid);
+        TK_TRY(hipMalloc(&P.pmax, sizeof(float) * L * d));
+        TK_TRY(hipMalloc(&P.pmin, sizeof(float) * L * d));
+        TK_TRY(hipMalloc(&P.psum, sizeof(float) * L));
+        TK_TRY(hipMalloc(&P.pminsum, sizeof(float) * L));
+        hipLaunchKernelGGL(pta_bounds_kernel, dim3(L), dim3(PTA_BS), 0, s, P.pcols, P.ld, d, P.poff, P.psz, P.pmax,
+                           P.pmin, P.psum, P.pminsum);
+        TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
+        TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
+        TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
+        TK_TRY(hipGetLastError());
+        TK_TRY(hipStreamSynchronize(s));
+    }
+#undef TK_TRY
+    cleanup();
+    P.built = true;
+    P.tt_build_ms = now_ms() - t0;
+    return TOPK_OK;
+fail:
+    cleanup();
+    hipFree(perm);
+    pta_free(P);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// query kernels
+// ---------------------------------------------------------------------------
+// Query description for the bound: the distinct queried attributes sorted by
+// their aggregated weight (a weight listed twice counts twice), FP64.
+struct PtaBoundArgs {
+    uint32_t nd;
+    uint32_t d;
+    uint8_t attr[TOPK_MAX_DIMS];
+    double w[TOPK_MAX_DIMS];
+};
+
+__device__ __forceinline__ float pta_lp_bound(const PtaBoundArgs &b, const float *__restrict__ pmax,
+                                              const float *__restrict__ pmin, float psum, float pminsum) {
+    double base = 0, box = 0, absw = 0;
+    for (uint32_t i = 0; i < b.nd; i++) {
+        const double mx = pmax[b.attr[i]], mn = pmin[b.attr[i]];
+        base += b.w[i] * mn;
+        box += b.w[i] * (b.w[i] >= 0 ? mx : mn);
+        absw += fabs(b.w[i]) * fmax(fabs(mx), fabs(mn));
+    }
+    double budget = (double)psum - (double)pminsum;
+    for (uint32_t i = 0; i < b.nd; i++) {
+        if (b.w[i] <= 0 || budget <= 0) break;
+        const double room = fmin((double)pmax[b.attr[i]] - (double)pmin[b.attr[i]], budget);
+        base += b.w[i] * room;
+        budget -= room;
+    }
+    const double lp = base < box ? base : box;
+    return f32_round_up(lp + absw * 1.52587890625e-05 + 1e-30);  // margin 2^-16 * sum |w x|
+}
+
+__global__ __launch_bounds__(PTA_BS) void pta_bound_kernel(PtaBoundArgs b, uint32_t L,
+                                                           const uint32_t *__restrict__ psz,
+                                                           const float *__restrict__ pmax,
+                                                           const float *__restrict__ pmin,
+                                                           const float *__restrict__ psum,
+                                                           const float *__restrict__ pminsum,
+                                                           uint64_t *__restrict__ pbound, uint64_t *__restrict__ bmax) {
+    __shared__ uint64_t wm[PTA_BS / 64];
+    const uint32_t p = blockIdx.x * PTA_BS + threadIdx.x;
+    uint64_t key = 0;
+    if (p < L && psz[p]) {
+        const float ub = pta_lp_bound(b, pmax + (uint64_t)p * b.d, pmin + (uint64_t)p * b.d, psum[p], pminsum[p]);
+        key = rkey(ub, p);
+    }
+    if (p < L) pbound[p] = key;
+    const uint64_t m = wave_max_u64(key);
+    if (lane_id() == 0) wm[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t r = wm[0];
+        for (int j = 1; j < PTA_BS / 64; j++) r = wm[j] > r ? wm[j] : r;
+        bmax[blockIdx.x] = r;
+    }
+}
+
+// Block t scores (up to 4096 objects of) the partition holding the t-th
+// largest block-maximum bound and keeps its top k; the last block to finish
+// takes the k-th largest of all seed keys as tau.
+template <int M>
+__global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_t *__restrict__ bmax, uint32_t nb,
+                                                          const uint32_t *__restrict__ poff,
+                                                          const uint32_t *__restrict__ psz,
+                                                          const uint32_t *__restrict__ pid, uint32_t k,
+                                                          uint64_t *__restrict__ blk, FteState *__restrict__ st) {
+    __shared__ SelectScratch ss;
+    __shared__ uint64_t pick;
+    __shared__ uint32_t is_last, nvalid;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64) {
+        // (t+1)-th largest block maximum by wave-max rounds
+        uint64_t best = 0;
+        for (uint32_t r = 0; r <= blockIdx.x; r++) {
+            uint64_t lm = 0;
+            for (uint32_t i = lane_id(); i < nb; i += 64) {
+                const uint64_t v = bmax[i];
+                if (v > lm && (r == 0 || v < best)) lm = v;
+            }
+            best = wave_max_u64(lm);
+        }
+        if (tid == 0) pick = best;
+    }
+    __syncthreads();
+    const uint64_t pk = pick;
+    const uint32_t p = rkey_id(pk);
+    const uint32_t start = pk ? poff[p] : 0u;
+    const uint32_t size = pk ? (psz[p] < PTA_SEED_OBJS ? psz[p] : PTA_SEED_OBJS) : 0u;
+    RegKeys<PTA_SEED_OBJS / PTA_BS> rk;
+#pragma unroll
+    for (int j = 0; j < (int)(PTA_SEED_OBJS / (4 * PTA_BS)); j++) {
+        const uint32_t lo = j * 4 * PTA_BS + tid * 4;
+        f4 sc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (lo < size) {
+            const uint64_t v = (start + lo) / 4;
+            if constexpr (M > 0) {
+#pragma unroll
+                for (int c = 0; c < M; c++) {
+                    f4 pr = ld_stream(q.col[c] + v * 4) * q.w[c];
+                    sc = sc + pr;
+                }
+            } else {
+                for (uint32_t c = 0; c < q.m; c++) {
+                    f4 pr = ld_stream(q.col[c] + v * 4) * q.w[c];
+                    sc = sc + pr;
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) rk.k[j * 4 + e] = lo + e < size ? rkey(sc[e], pid[start + lo + e]) : 0ull;
+    }
+    uint64_t thr = 1;
+    if (size > k) thr = block_kth_largest<PTA_BS>(rk, size, k, ss);
+    uint64_t *out = blk + (uint64_t)blockIdx.x * k;
+    const uint32_t got = block_gather_ge<PTA_BS>(rk, thr, out, ss);
+    for (uint32_t i = got + tid; i < k; i += PTA_BS) out[i] = 0;
+
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) is_last = (atomicAdd(&st->arrive, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    const uint32_t nk = gridDim.x * k;
+    if (tid == 0) nvalid = 0;
+    __syncthreads();
+    uint32_t mine = 0;
+    for (uint32_t i = tid; i < nk; i += PTA_BS) mine += blk[i] != 0;
+    atomicAdd(&nvalid, mine);
+    __syncthreads();
+    uint64_t tau = 1;
+    if (nvalid > k) tau = block_kth_largest<PTA_BS>(MemKeys<PTA_BS>{blk, nk}, nvalid, k, ss);
+    if (tid == 0) {
+        st->tau = tau;
+        st->pre_count = 0;
+        st->arrive = 0;
+    }
+}
+
+// Grid-stride over partitions: skip a partition whose bound score is below
+// tau's score, stream the others in 1024-object steps and keep keys >= tau.
+template <int M>
+__global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L, const uint64_t *__restrict__ pbound,
+                                                            const uint32_t *__restrict__ poff,
+                                                            const uint32_t *__restrict__ psz,
+                                                            const uint32_t *__restrict__ pid,
+                                                            FteState *__restrict__ st, CandSet cs,
+                                                            uint32_t *__restrict__ pscored) {
+    __shared__ uint32_t lcount;
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
+    const uint64_t tau = st->tau;
+    const uint32_t tau_hi = (uint32_t)(tau >> 32);
+    const uint32_t lane = lane_id();
+    uint64_t *my = cs.keys + (uint64_t)blockIdx.x * cs.slot;
+    uint32_t scored = 0;
+    for (uint32_t p = blockIdx.x; p < L; p += gridDim.x) {
+        if ((uint32_t)(pbound[p] >> 32) < tau_hi) continue;
+        scored++;
+        const uint32_t start = poff[p], end = start + psz[p];
+        for (uint32_t t0 = start; t0 < end; t0 += PTA_TILE) {
+            const uint64_t v = t0 / 4 + threadIdx.x;
+            f4 sc = {0.0f, 0.0f, 0.0f, 0.0f};
+            if constexpr (M > 0) {
+                f4 x[M];
+#pragma unroll
+                for (int c = 0; c < M; c++) x[c] = ld_stream(q.col[c] + v * 4);
+#pragma unroll
+                for (int c = 0; c < M; c++) {
+                    f4 pr = x[c] * q.w[c];
+                    sc = sc + pr;
+                }
+            } else {
+                for (uint32_t c = 0; c < q.m; c++) {
+                    f4 pr = ld_stream(q.col[c] + v * 4) * q.w[c];
+                    sc = sc + pr;
+                }
+            }
+            uint32_t npass = 0;
+            uint64_t kk[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t pos = (uint32_t)(v * 4 + e);
+                const uint32_t ok = okey(sc[e]);
+                uint64_t key = 0;
+                if (pos < end && ok >= tau_hi) {
+                    key = rkey_from_okey(ok, pid[pos]);
+                    if (key < tau) key = 0;
+                }
+                kk[e] = key;
+                npass += key ? 1u : 0u;
+            }
+            if (__ballot(npass != 0)) {
+                const uint32_t incl = wave_incl_scan(npass);
+                uint32_t base = 0;
+                if (lane == 63) base = atomicAdd(&lcount, incl);
+                base = __shfl(base, 63, 64);
+                const uint32_t wtotal = __shfl(incl, 63, 64);
+                const uint32_t first_spill = base > cs.slot ? base : cs.slot;
+                const uint32_t nspill = base + wtotal > first_spill ? base + wtotal - first_spill : 0u;
+                uint32_t sbase = 0;
+                if (nspill) {
+                    if (lane == 63) sbase = atomicAdd(&st->spill, nspill);
+                    sbase = __shfl(sbase, 63, 64);
+                }
+                uint32_t pos = base + incl - npass;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (kk[e]) {
+                        if (pos < cs.slot) {
+                            my[pos] = kk[e];
+                        } else {
+                            const uint32_t sp = sbase + (pos - first_spill);
+                            if (sp < cs.spill_cap) cs.spill[sp] = kk[e];
+                        }
+                        pos++;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cs.counts[blockIdx.x] = lcount < cs.slot ? lcount : cs.slot;
+        pscored[blockIdx.x] = scored;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: query
+// ---------------------------------------------------------------------------
+int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *d_ids,
+              float *d_scores, float *d_thr, uint32_t *d_status, uint64_t *scored_out) {
+    PtaLayout &P = ctx->pta;
+    if (!P.built) return TOPK_ERR_STATE;
+    QCols q;
+    int rc = build_qcols(ctx, P.pcols, P.ld, qq, weights, attr, q);
+    if (rc) return rc;
+    // distinct attributes with aggregated weights, heaviest first
+    PtaBoundArgs b;
+    memset(&b, 0, sizeof(b));
+    b.d = ctx->d;
+    double agg[TOPK_MAX_DIMS] = {0};
+    bool used[TOPK_MAX_DIMS] = {false};
+    for (uint32_t j = 0; j < qq; j++) {
+        agg[attr[j]] += (double)weights[attr[j]];
+        used[attr[j]] = true;
+    }
+    uint32_t order[TOPK_MAX_DIMS], nd = 0;
+    for (uint32_t a = 0; a < ctx->d; a++)
+        if (used[a]) order[nd++] = a;
+    std::stable_sort(order, order + nd, [&](uint32_t x, uint32_t y) { return agg[x] > agg[y]; });
+    b.nd = nd;
+    for (uint32_t i = 0; i < nd; i++) {
+        b.attr[i] = (uint8_t)order[i];
+        b.w[i] = agg[order[i]];
+    }
+
+    Lane &Ln = ctx->lane[0];
+    hipStream_t s = Ln.stream;
+    const uint32_t L = P.nparts;
+    const uint32_t nb = (L + PTA_BS - 1) / PTA_BS;
+    hipLaunchKernelGGL(pta_bound_kernel, dim3(nb), dim3(PTA_BS), 0, s, b, L, P.psz, P.pmax, P.pmin, P.psum,
+                       P.pminsum, P.pbound, P.bmax);
+    uint32_t T = (4 * k + P.psize - 1) / P.psize;
+    if (T < 4) T = 4;
+    if (T > PTA_MAX_SEED) T = PTA_MAX_SEED;
+    if (T > nb) T = nb;
+#define TK_SEED(MM)                                                                                              \
+    hipLaunchKernelGGL(pta_seed_kernel<MM>, dim3(T), dim3(PTA_BS), 0, s, q, P.bmax, nb, P.poff, P.psz, P.pid, k, \
+                       Ln.blk, Ln.st)
+    TK_SWITCH_M(q.m, TK_SEED);
+#undef TK_SEED
+    const uint32_t g = L < MAX_STREAM_GRID / 2 ? L : MAX_STREAM_GRID / 2;
+    CandSet cs;
+    cs.keys = Ln.cand;
+    cs.counts = Ln.counts;
+    cs.nblk = g;
+    cs.slot = (uint32_t)(ctx->cand_cap / g);
+    cs.pre = nullptr;
+    cs.spill = Ln.spill;
+    cs.spill_cap = ctx->spill_cap;
+    hipEvent_t ev0 = nullptr;
+    timing_begin(ctx, s, ev0);
+#define TK_STREAM(MM)                                                                                        \
+    hipLaunchKernelGGL(pta_stream_kernel<MM>, dim3(g), dim3(PTA_BS), 0, s, q, L, P.pbound, P.poff, P.psz, \
+                       P.pid, Ln.st, cs, P.pscored)
+    TK_SWITCH_M(q.m, TK_STREAM);
+#undef TK_STREAM
+    timing_end(ctx, s, "pta_stream", ev0);
+    const uint32_t kreq = (uint32_t)(ctx->n < k ? ctx->n : k);
+    final_enqueue(ctx, Ln, cs, k, kreq, FinalOut{d_ids, d_scores, d_thr, d_status}, false);
+    TK_HIP(hipGetLastError());
+    if (scored_out) {
+        std::vector<uint32_t> sc(g);
+        TK_HIP(hipMemcpyAsync(sc.data(), P.pscored, sizeof(uint32_t) * g, hipMemcpyDeviceToHost, s));
+        TK_HIP(hipStreamSynchronize(s));
+        uint64_t tot = 0;
+        for (uint32_t v : sc) tot += v;
+        *scored_out = tot;
+    }
+    return TOPK_OK;
+}
+
+}  // namespace topk

@@ -7,12 +7,6 @@ void learned_free(topk_ctx *) {}
 }  // namespace topk
 
 extern "C" {
-int topk_init(topk_ctx *ctx, uint32_t) { return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID; }
-int topk_set_partition_size(topk_ctx *ctx, uint32_t) { return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID; }
-int topk_reorder_attr(topk_ctx *ctx) { return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID; }
-int topk_pta(topk_ctx *ctx, uint32_t, uint8_t, const float *, const uint8_t *, uint64_t *, float *, float *) {
-    return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID;
-}
 int topk_bta(topk_ctx *ctx, uint32_t, uint8_t, const float *, const uint8_t *, uint64_t *, float *, float *) {
     return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID;
 }

@@ -99,6 +99,22 @@ class Engine:
     def reorder_attr(self):
         check(lib().topk_reorder_attr(self._h), "topk_reorder_attr")
 
+    def pta_layout(self):
+        """The PTA partition layout as numpy arrays:
+        (pid[npos], poff[nparts+1], psz[nparts], pmax[nparts,d], pmin[nparts,d])."""
+        np_ = _c.c_uint32()
+        npos = _c.c_uint64()
+        check(lib().topk_pta_layout(self._h, _c.byref(np_), _c.byref(npos)), "topk_pta_layout")
+        P, N = np_.value, npos.value
+        pid = np.empty(N, np.uint32)
+        poff = np.empty(P + 1, np.uint32)
+        psz = np.empty(P, np.uint32)
+        pmax = np.empty((P, self.d), np.float32)
+        pmin = np.empty((P, self.d), np.float32)
+        check(lib().topk_pta_read_layout(self._h, _ptr(pid), _ptr(poff), _ptr(psz), _ptr(pmax), _ptr(pmin)),
+              "topk_pta_read_layout")
+        return pid, poff, psz, pmax, pmin
+
     # ---- queries ----
     def _single(self, fn, name, k, weights, attr):
         w, at = _query_arrays(self.d, weights, attr)
