@@ -151,12 +151,76 @@ typedef struct topk_query {
     float weights[TOPK_MAX_DIMS];     /* indexed by attribute id */
 } topk_query;
 
-/* Batched multi-query (validation/bench_cpu.h:198-235 IMP=3/4): every
- * column tile is streamed once and scored for all nq queries.  Results of
- * query q go to out_ids/out_scores[q*k_stride ...] and out_thresholds[q].
- * Same per-query results as topk_fte. */
+/* Batched multi-query (validation/bench_cpu.h:198-235 IMP=3/4, over
+ * TPAc::findTopKsimdMQ cpu/TPAc.h:253-358): up to 16384 queries, each with
+ * its own attribute list (any order, repeats allowed), weights and k
+ * (1 <= k <= k_stride <= TOPK_MAX_K).  Runs over the PTA partition layout
+ * (built on first use when topk_init(TOPK_INIT_PTA) was not called): every
+ * partition is loaded once per group of queries whose bound reaches the
+ * query's threshold and scored for all of them.  The threshold comes from a
+ * seed pass, or from the trained predictor once one is installed (see
+ * topk_learned_train); queries the device path cannot settle exactly are
+ * answered by topk_fte and counted in topk_stats.fallback.  Results of
+ * query q go to out_ids/out_scores[q*k_stride ...] (min(k, n) entries) and
+ * out_thresholds[q]; they equal topk_fte's for the same query. */
 int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
                uint64_t *out_ids, float *out_scores, float *out_thresholds);
+
+/* ---- learned threshold (no reference counterpart: it replaces PTA's
+ *      running threshold, cpu/PTA.h:357-422, for topk_batch) ----
+ *
+ * A 96 -> 64 -> 64 -> 1 ReLU MLP (FP32, 10433 parameters laid out
+ * W1[64][96], b1[64], W2[64][64], b2[64], W3[64], b3) predicts a query's
+ * k-th score / sum|W| from its weights, log2 k and a per-attribute quantile
+ * sketch of a 1% sample of the store.  topk_batch uses (prediction - margin)
+ * as the query's threshold once a predictor is installed; results stay
+ * exact (a too-high threshold leaves fewer than k survivors and the query
+ * is answered by topk_fte).  Stores with d <= 16 only. */
+typedef struct topk_learned_config {
+    uint32_t epochs;          /* passes over the training queries (default 60) */
+    uint32_t batch;           /* mini-batch, a multiple of 64 (default 512) */
+    float lr;                 /* Adam step size (default 2e-3, decayed x0.3 / x0.1 late) */
+    float target_fallback;    /* fraction of calibration queries allowed to fall back (0.01) */
+    float calib_fraction;     /* held-out fraction used to calibrate the margin (0.1) */
+    uint64_t seed;            /* split / init / shuffle seed (1) */
+} topk_learned_config;
+
+typedef struct topk_learned_report {
+    uint32_t n_train, n_calib;
+    float train_mse, calib_mse;  /* in (score / sum|W|)^2 */
+    float margin;                /* installed margin, score / sum|W| units */
+    float calib_fallback;        /* calibration queries whose threshold would exceed their k-th score */
+    double ms_label, ms_train;   /* wall time of labelling (exact batch path) and of training */
+} topk_learned_report;
+
+int topk_learned_default_config(topk_learned_config *cfg);
+
+/* Labels the nq (>= 16) queries with the exact batch path, trains the
+ * predictor on the GPU, calibrates the margin and installs it.  cfg may be
+ * NULL (defaults); rep may be NULL. */
+int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const topk_learned_config *cfg,
+                       topk_learned_report *rep);
+
+/* Removes the installed predictor (topk_batch returns to seed thresholds). */
+int topk_learned_clear(topk_ctx *ctx);
+
+/* Reads / installs predictor parameters (10433 floats) and margin. */
+int topk_learned_get(topk_ctx *ctx, float *params, float *margin);
+int topk_learned_set(topk_ctx *ctx, const float *params, float margin);
+
+/* The 96 features (x[q*96+i]) and sum|W| scales the device builds for the
+ * queries; either output may be NULL. */
+int topk_learned_features(topk_ctx *ctx, uint32_t nq, const topk_query *qs, float *x, float *scale);
+
+/* Installed predictor on the queries: pred[q] = predicted k-th score,
+ * tau[q] = threshold topk_batch would use.  Either output may be NULL. */
+int topk_learned_predict(topk_ctx *ctx, uint32_t nq, const topk_query *qs, float *pred, float *tau);
+
+/* The GPU forward / backward kernels on explicit rows: out[b] = MLP(x[b])
+ * for nb rows of 96 features; with y given, grad = d/dparams of
+ * mean_b (out[b] - y[b])^2 and *loss = that mean.  NULL outputs skipped. */
+int topk_mlp_eval(topk_ctx *ctx, const float *params, const float *x, const float *y, uint32_t nb,
+                  float *out, float *grad, float *loss);
 
 /* ---- device-pointer variants (outputs stay in HBM; no host sync) ---- */
 
@@ -186,6 +250,16 @@ void *topk_stream(topk_ctx *ctx);
  *   "lanes"           1 or 2: pipelines that consecutive topk_fte_async
  *                     calls alternate between (2 overlaps one query's seed
  *                     and final passes with its neighbours' stream passes)
+ *   "batch_vpt"       objects per thread of the batch scoring kernel
+ *                     (d <= 8: 4, 8, 16; d <= 16: 2, 4, 8; else 2, 4)
+ *   "batch_bs"        threads per batch scoring workgroup: 256 or 512
+ *   "batch_qpi"       queries per batch work item: 16, 32 or 64
+ *   "batch_digit"     radix digit width of the per-query select: 8 or 11
+ *   "batch_grid"      persistent batch scoring workgroups
+ *   "batch_graph"     1 = capture each batch shape in a hipGraph and replay
+ *   "batch_mode"      0 = learned threshold when trained, else seed;
+ *                     1 = seed threshold; 2 = learned threshold
+ * (0 for a batch knob = default or the autotuner's choice.)
  * Unknown names or values return TOPK_ERR_INVALID. */
 int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value);
 

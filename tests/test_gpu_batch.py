@@ -1,0 +1,169 @@
+"""GPU parity of the batched multi-query path (topk_batch) against the
+oracle's multi-query restatement (cpu/TPAc.h:209-254 findTopKsimdMQ)."""
+import numpy as np
+import pytest
+
+import oracle
+import topk_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def make_queries(rng, nq, d, ks, order="sorted", mmin=2, mmax=None, wlo=0.1, whi=1.0):
+    """Random queries: attribute subsets of mmin..mmax attributes, weights by
+    attribute id.  order: sorted / shuffled / repeats."""
+    mmax = d if mmax is None else mmax
+    out = []
+    for _ in range(nq):
+        m = int(rng.integers(mmin, mmax + 1))
+        if order == "repeats":
+            attr = rng.integers(0, d, size=m)
+        else:
+            attr = rng.choice(d, size=m, replace=False)
+            if order == "sorted":
+                attr = np.sort(attr)
+        w = rng.uniform(wlo, whi, size=d).astype(np.float32)
+        out.append((int(rng.choice(ks)), w, attr.astype(np.uint8)))
+    return out
+
+
+def oracle_batch(x, n, queries, threads=16):
+    d = x.shape[0]
+    nq = len(queries)
+    ks = np.array([q[0] for q in queries], np.uint32)
+    qqs = np.array([len(q[2]) for q in queries], np.uint32)
+    amax = int(qqs.max())
+    w = np.zeros((nq, d), np.float32)
+    at = np.zeros((nq, amax), np.uint8)
+    for i, (_, wi, ai) in enumerate(queries):
+        w[i] = wi
+        at[i, :ai.size] = ai
+    return oracle.mq_threads(x, n, ks, qqs, w, at, threads)
+
+
+def check_batch(eng, x, n, queries):
+    got = eng.batch(queries)
+    ref = oracle_batch(x, n, queries)
+    for i, ((gi, gs, gt), (ri, rs)) in enumerate(zip(got, ref)):
+        assert np.array_equal(gi, ri), (i, queries[i][0], list(queries[i][2]))
+        assert np.array_equal(gs.view(np.uint32), rs.view(np.uint32)), i
+        assert gt == rs[-1]
+    return eng.stats()
+
+
+@pytest.mark.parametrize("distr", [0, 1, 2, 3])
+def test_batch_parity_d8(gpu, distr):
+    n, d = 1_000_003, 8
+    rng = np.random.default_rng(100 + distr)
+    x = topk_amd.generate(n, d, distr, 50 + distr)
+    with topk_amd.Engine(x) as eng:
+        qs = make_queries(rng, 256, d, [1, 16, 32, 64, 128, 256, 1024])
+        st = check_batch(eng, x, n, qs)
+        assert st["queries"] == 256 and st["fallback"] == 0
+        assert st["partitions_scored"] < st["partitions_total"]
+
+
+def test_batch_parity_d16_mixed(gpu):
+    """Batched multi-query workload at reduced N: d=16 mixed distributions,
+    random subsets of 2-16 attributes, k from {16, 32, 64, 128, 256}."""
+    n, d = 1 << 20, 16
+    rng = np.random.default_rng(5)
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 11)
+    with topk_amd.Engine(x) as eng:
+        qs = make_queries(rng, 1024, d, [16, 32, 64, 128, 256])
+        st = check_batch(eng, x, n, qs)
+        assert st["fallback"] == 0
+
+
+@pytest.mark.parametrize("order", ["shuffled", "repeats"])
+def test_batch_term_order_and_repeats(gpu, order):
+    n, d = 300_000, 12
+    rng = np.random.default_rng(9)
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 12)
+    with topk_amd.Engine(x) as eng:
+        qs = make_queries(rng, 128, d, [8, 64, 300], order=order, mmin=1, mmax=20 if order == "repeats" else d,
+                          wlo=-1.0, whi=1.0)
+        check_batch(eng, x, n, qs)
+
+
+def test_batch_wide_store(gpu):
+    n, d = 200_000, 32
+    rng = np.random.default_rng(3)
+    x = topk_amd.generate(n, d, topk_amd.ANTICORRELATED, 4)
+    with topk_amd.Engine(x) as eng:
+        check_batch(eng, x, n, make_queries(rng, 64, d, [10, 100], order="shuffled"))
+
+
+@pytest.mark.parametrize("n", [1, 7, 1000, 5000])
+def test_batch_small_stores(gpu, n):
+    d = 4
+    rng = np.random.default_rng(n)
+    x = topk_amd.generate(n, d, topk_amd.CORRELATED, n)
+    with topk_amd.Engine(x) as eng:
+        qs = make_queries(rng, 40, d, [1, 5, 64, 1024], mmin=1)
+        got = eng.batch(qs)
+        ref = oracle_batch(x, n, qs)
+        for (gi, gs, _), (ri, rs) in zip(got, ref):
+            assert np.array_equal(gi, ri) and np.array_equal(gs, rs)
+
+
+def test_batch_variants_agree(gpu):
+    """Every scoring shape, query grouping and digit width gives the same bits."""
+    n, d = 400_000, 8
+    rng = np.random.default_rng(21)
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 21)
+    qs = make_queries(rng, 200, d, [16, 64, 256])
+    ref = oracle_batch(x, n, qs)
+    with topk_amd.Engine(x) as eng:
+        for vpt, bs, qpi, digit in [(4, 256, 64, 11), (8, 512, 32, 8), (16, 256, 16, 11), (8, 256, 64, 8)]:
+            for name, v in (("batch_vpt", vpt), ("batch_bs", bs), ("batch_qpi", qpi), ("batch_digit", digit),
+                            ("batch_grid", 300)):
+                eng.set_tuning(name, v)
+            got = eng.batch(qs)
+            for (gi, gs, _), (ri, rs) in zip(got, ref):
+                assert np.array_equal(gi, ri) and np.array_equal(gs, rs), (vpt, bs, qpi, digit)
+
+
+def test_batch_graph_replays(gpu):
+    """Captured-graph replays (with eager work between them) equal eager runs."""
+    n, d = 500_000, 8
+    rng = np.random.default_rng(8)
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 8)
+    with topk_amd.Engine(x) as eng:
+        qa = make_queries(rng, 100, d, [32, 64])
+        qb = make_queries(rng, 100, d, [32, 64])
+        eng.set_tuning("batch_graph", 0)
+        ea, eb = eng.batch(qa), eng.batch(qb)
+        eng.set_tuning("batch_graph", 1)
+        for _ in range(3):
+            for qs, ref in ((qa, ea), (qb, eb)):
+                got = eng.batch(qs)
+                eng.fte(8, np.ones(d, np.float32), [0, 1])  # eager work between replays
+                for (gi, gs, gt), (ri, rs, rt) in zip(got, ref):
+                    assert np.array_equal(gi, ri) and np.array_equal(gs, rs) and gt == rt
+
+
+def test_batch_overflow_falls_back(gpu):
+    """Heavy ties push a query's candidate list past its capacity; the query
+    is answered by the exact FTE path and counted as a fallback."""
+    n, d = 200_000, 2
+    x = np.zeros((d, n), np.float32)
+    x[:, ::2] = 1.0  # 100000 objects tie at the top score
+    with topk_amd.Engine(x) as eng:
+        qs = [(100, np.ones(d, np.float32), np.array([0, 1], np.uint8)),
+              (10, np.array([1.0, -1.0], np.float32), np.array([1, 0], np.uint8))]
+        st = check_batch(eng, x, n, qs)
+        assert st["fallback"] >= 1
+
+
+def test_batch_errors(gpu):
+    x = topk_amd.generate(10_000, 4, topk_amd.INDEPENDENT, 1)
+    with topk_amd.Engine(x) as eng:
+        with pytest.raises(topk_amd.TopkError) as e:
+            eng.batch([(8, np.ones(4, np.float32), np.array([0, 9], np.uint8))])
+        assert e.value.status == 1
+        with pytest.raises(topk_amd.TopkError) as e:
+            eng.batch([(2000, np.ones(4, np.float32), np.array([0], np.uint8))])
+        assert e.value.status == 4
+        with pytest.raises(topk_amd.TopkError):
+            eng.set_tuning("batch_qpi", 48)

@@ -1,0 +1,183 @@
+"""GPU tests of the learned k-th score predictor: the hand-written forward /
+backward kernels against the FP64 oracle, the device features, training and
+calibration, exactness of learned-threshold batches, and what the trained
+network gains over seed thresholds."""
+import numpy as np
+import pytest
+
+import oracle
+import topk_amd
+from topk_amd._lib import MLP_NPARAMS
+
+from test_gpu_batch import make_queries, oracle_batch
+
+pytestmark = pytest.mark.gpu
+
+
+def glorot_params(rng):
+    p = np.zeros(MLP_NPARAMS, np.float64)
+    o = 0
+    for rows, cols in ((64, 96), (64, 64), (1, 64)):
+        lim = np.sqrt(6.0 / (rows + cols))
+        p[o:o + rows * cols] = rng.uniform(-lim, lim, rows * cols)
+        o += rows * cols
+        p[o:o + rows] = rng.uniform(-0.1, 0.1, rows)
+        o += rows
+    assert o == MLP_NPARAMS
+    return p
+
+
+@pytest.mark.parametrize("nb", [1, 63, 64, 200])
+def test_mlp_forward_backward_match_oracle(gpu, nb):
+    rng = np.random.default_rng(nb)
+    p = glorot_params(rng).astype(np.float32)
+    x = rng.normal(0, 1, (nb, 96)).astype(np.float32)
+    y = rng.normal(0, 1, nb).astype(np.float32)
+    with topk_amd.Engine(np.zeros((1, 10), np.float32)) as eng:
+        out, grad, loss = eng.mlp_eval(p, x, y)
+        out_only, g2, l2 = eng.mlp_eval(p, x)
+    ref_out = oracle.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+    ref_loss, ref_grad = oracle.mlp_backward(p.astype(np.float64), x.astype(np.float64), y.astype(np.float64))
+    assert g2 is None and l2 is None
+    np.testing.assert_allclose(out, ref_out, rtol=1e-4, atol=1e-5)
+    np.testing.assert_array_equal(out, out_only)
+    assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss) + 1e-6
+    # FP32 sums of <= 96 terms against FP64: relative error ~1e-6 of the term scale
+    scale = np.abs(ref_grad).max()
+    np.testing.assert_allclose(grad, ref_grad, rtol=1e-3, atol=1e-5 * scale)
+
+
+def test_mlp_relu_gates_gradient(gpu):
+    """All-negative first-layer pre-activations: only b3 receives gradient."""
+    rng = np.random.default_rng(3)
+    p = glorot_params(rng)
+    p[:64 * 96] = 0.0
+    p[64 * 96:64 * 97] = -1.0  # b1 < 0 -> h1 = 0
+    x = rng.normal(0, 1, (20, 96)).astype(np.float32)
+    y = rng.normal(0, 1, 20).astype(np.float32)
+    with topk_amd.Engine(np.zeros((1, 10), np.float32)) as eng:
+        _, grad, _ = eng.mlp_eval(p.astype(np.float32), x, y)
+    _, ref = oracle.mlp_backward(p, x.astype(np.float64), y.astype(np.float64))
+    w2_end = 64 * 96 + 64 + 64 * 64
+    assert np.all(ref[:w2_end] == 0) and np.all(grad[:w2_end] == 0)  # W1, b1, W2 gated off
+    np.testing.assert_allclose(grad, ref, rtol=1e-4, atol=1e-6)
+
+
+def test_features_and_sketch(gpu):
+    n, d = 500_000, 8
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 2)
+    qs = [(64, np.array([0.5, 0, 0, 1.0, 0, 0, 0, 0.25], np.float32), np.array([7, 0, 3], np.uint8)),
+          (16, np.linspace(0.1, 0.8, d).astype(np.float32), np.array([2, 2, 5], np.uint8))]
+    with topk_amd.Engine(x) as eng:
+        f, sc = eng.learned_features(qs)
+    # sketch: quantiles of the objects j * n / s, s = n / 100
+    s = n // 100
+    samp = x[:, (np.arange(s, dtype=np.int64) * n) // s]
+    for a in range(d):
+        col = np.sort(samp[a])
+        for t, q in enumerate((0.5, 0.9, 0.99, 0.999)):
+            assert f[0, 17 + 4 * a + t] == col[int(q * (s - 1))]
+    for a in range(d, 16):
+        assert np.all(f[:, 17 + 4 * a:21 + 4 * a] == 0)
+    assert sc[0] == np.float32(1.75) and np.allclose(f[0, :8], np.array([0.5, 0, 0, 1, 0, 0, 0, 0.25]) / 1.75)
+    w2 = np.linspace(0.1, 0.8, d).astype(np.float32)
+    agg = np.float32(w2[2] + w2[2])
+    assert np.isclose(sc[1], agg + w2[5]) and np.isclose(f[1, 2], agg / sc[1])  # repeated attribute aggregates
+    assert f[0, 16] == np.float32(np.log2(64) * 0.1) and f[1, 16] == np.float32(np.log2(16) * 0.1)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """N=2^20, d=16 mixed store with a predictor trained on 6000 queries."""
+    if not __import__("torch").cuda.is_available():
+        pytest.skip("no GPU")
+    n, d = 1 << 20, 16
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 77)
+    eng = topk_amd.Engine(x)
+    rng = np.random.default_rng(1234)
+    train = make_queries(rng, 6000, d, [16, 32, 64, 128, 256])
+    rep = eng.learned_train(train, epochs=40)
+    test = make_queries(rng, 512, d, [16, 32, 64, 128, 256])
+    yield eng, x, n, rep, test
+    eng.close()
+
+
+def test_learned_training_report(gpu, trained):
+    eng, _, _, rep, _ = trained
+    assert rep["n_train"] + rep["n_calib"] == 6000
+    assert rep["margin"] > 0 and np.isfinite(rep["train_mse"]) and np.isfinite(rep["calib_mse"])
+    assert rep["calib_fallback"] <= 0.02
+
+
+def test_learned_batch_is_exact(gpu, trained):
+    eng, x, n, _, test = trained
+    eng.set_tuning("batch_mode", 2)
+    got = eng.batch(test)
+    st = eng.stats()
+    ref = oracle_batch(x, n, test)
+    for i, ((gi, gs, gt), (ri, rs)) in enumerate(zip(got, ref)):
+        assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)), i
+        assert gt == rs[-1]
+    assert st["fallback"] <= 0.05 * len(test)
+    eng.set_tuning("batch_mode", 0)
+
+
+def test_learned_gain_over_seed(gpu, trained):
+    """The trained thresholds prune more (partition, query) pairs and hand
+    fewer candidates to the final select than the seed thresholds."""
+    eng, _, _, _, test = trained
+    eng.set_tuning("batch_mode", 1)
+    seed_res = eng.batch(test)
+    s_seed = eng.stats()
+    eng.set_tuning("batch_mode", 2)
+    lrn_res = eng.batch(test)
+    s_lrn = eng.stats()
+    eng.set_tuning("batch_mode", 0)
+    for (a, b, c), (d, e, f) in zip(seed_res, lrn_res):
+        assert np.array_equal(a, d) and np.array_equal(b, e) and c == f
+    assert s_lrn["partitions_scored"] < s_seed["partitions_scored"]
+    assert s_lrn["candidates"] < s_seed["candidates"]
+
+
+def test_learned_thresholds_below_kth(gpu, trained):
+    eng, _, _, _, test = trained
+    pred, tau = eng.learned_predict(test)
+    eng.set_tuning("batch_mode", 1)
+    exact = eng.batch(test)
+    eng.set_tuning("batch_mode", 0)
+    kth = np.array([t for _, _, t in exact], np.float32)
+    assert np.all(tau <= pred)
+    assert np.mean(tau <= kth) >= 0.95  # calibrated for ~1% fallback
+    assert np.median(np.abs(pred - kth) / np.abs(kth)) < 0.05
+
+
+def test_forced_fallback_stays_exact(gpu, trained):
+    """A predictor whose thresholds sit above every k-th score: every query
+    falls back to the exact path and the answers do not change."""
+    eng, x, n, _, test = trained
+    params, margin = eng.learned_get()
+    sub = test[:40]
+    eng.learned_set(params, -1.0)  # tau = (pred + 1) * sum|W|: above any score
+    eng.set_tuning("batch_mode", 2)
+    got = eng.batch(sub)
+    st = eng.stats()
+    eng.learned_set(params, margin)
+    eng.set_tuning("batch_mode", 0)
+    assert st["fallback"] == len(sub)
+    for (gi, gs, _), (ri, rs) in zip(got, oracle_batch(x, n, sub)):
+        assert np.array_equal(gi, ri) and np.array_equal(gs, rs)
+
+
+def test_learned_state_errors(gpu):
+    x = topk_amd.generate(10_000, 20, topk_amd.INDEPENDENT, 1)
+    with topk_amd.Engine(x) as eng:
+        with pytest.raises(topk_amd.TopkError) as e:
+            eng.learned_get()
+        assert e.value.status == 6
+        eng.set_tuning("batch_mode", 2)
+        with pytest.raises(topk_amd.TopkError) as e:
+            eng.batch([(4, np.ones(20, np.float32), [0, 1])])
+        assert e.value.status == 6
+        with pytest.raises(topk_amd.TopkError) as e:
+            eng.learned_train([(4, np.ones(20, np.float32), [0, 1])] * 20)
+        assert e.value.status == 4  # d > 16

@@ -45,6 +45,20 @@ class TopkQuery(ctypes.Structure):
                 ("attr", _c.c_uint8 * TOPK_MAX_DIMS), ("weights", _c.c_float * TOPK_MAX_DIMS)]
 
 
+class TopkLearnedConfig(ctypes.Structure):
+    _fields_ = [("epochs", _c.c_uint32), ("batch", _c.c_uint32), ("lr", _c.c_float),
+                ("target_fallback", _c.c_float), ("calib_fraction", _c.c_float), ("seed", _c.c_uint64)]
+
+
+class TopkLearnedReport(ctypes.Structure):
+    _fields_ = [("n_train", _c.c_uint32), ("n_calib", _c.c_uint32), ("train_mse", _c.c_float),
+                ("calib_mse", _c.c_float), ("margin", _c.c_float), ("calib_fallback", _c.c_float),
+                ("ms_label", _c.c_double), ("ms_train", _c.c_double)]
+
+
+MLP_NPARAMS = 10433
+MLP_IN = 96
+
 # name -> (restype, argtypes); mirrors include/topk.h one to one
 SIGNATURES = {
     "topk_abi_version": (_c.c_int, []),
@@ -77,6 +91,15 @@ SIGNATURES = {
     "topk_generate": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint32, _c.c_uint64, _c.c_int, _c.c_uint64,
                                  _c.c_uint64, _c.c_uint64, _c.c_int]),
     "topk_radix_select_u32": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.c_uint64, _c.POINTER(_c.c_uint32)]),
+    "topk_learned_default_config": (_c.c_int, [_c.POINTER(TopkLearnedConfig)]),
+    "topk_learned_train": (_c.c_int, [_vp, _c.c_uint32, _vp, _c.POINTER(TopkLearnedConfig),
+                                      _c.POINTER(TopkLearnedReport)]),
+    "topk_learned_clear": (_c.c_int, [_vp]),
+    "topk_learned_get": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_float)]),
+    "topk_learned_set": (_c.c_int, [_vp, _vp, _c.c_float]),
+    "topk_learned_features": (_c.c_int, [_vp, _c.c_uint32, _vp, _vp, _vp]),
+    "topk_learned_predict": (_c.c_int, [_vp, _c.c_uint32, _vp, _vp, _vp]),
+    "topk_mlp_eval": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_uint32, _vp, _vp, _c.POINTER(_c.c_float)]),
 }
 
 _LIB = None

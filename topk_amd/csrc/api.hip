@@ -201,8 +201,7 @@ int topk_ctx_destroy(topk_ctx *ctx) {
     hipFree(ctx->d_res);
     if (ctx->h_res) hipHostFree(ctx->h_res);
     if (ctx->h_state) hipHostFree(ctx->h_state);
-    hipFree(ctx->batch_buf);
-    if (ctx->h_batch) hipHostFree(ctx->h_batch);
+    batch_free(ctx);
     pta_release(ctx);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -407,6 +406,26 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
         ctx->tune.lanes = (int)value;
     } else if (key == "fte_force_exact") {
         ctx->tune.fte_force_exact = value ? 1 : 0;
+    } else if (key == "batch_vpt") {
+        if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16) return TOPK_ERR_INVALID;
+        ctx->tune.batch_vpt = (int)value;
+    } else if (key == "batch_bs") {
+        if (value != 0 && value != 256 && value != 512) return TOPK_ERR_INVALID;
+        ctx->tune.batch_bs = (int)value;
+    } else if (key == "batch_qpi") {
+        if (value != 0 && value != 16 && value != 32 && value != 64) return TOPK_ERR_INVALID;
+        ctx->tune.batch_qpi = (int)value;
+    } else if (key == "batch_digit") {
+        if (value != 0 && value != 8 && value != 11) return TOPK_ERR_INVALID;
+        ctx->tune.batch_digit = (int)value;
+    } else if (key == "batch_grid") {
+        if (value < 0 || value > 65536) return TOPK_ERR_INVALID;
+        ctx->tune.batch_grid = (int)value;
+    } else if (key == "batch_graph") {
+        ctx->tune.batch_graph = value ? 1 : 0;
+    } else if (key == "batch_mode") {
+        if (value < 0 || value > 2) return TOPK_ERR_INVALID;
+        ctx->tune.batch_mode = (int)value;
     } else {
         return TOPK_ERR_INVALID;
     }

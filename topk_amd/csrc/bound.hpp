@@ -1,0 +1,87 @@
+// Upper bound of a query's score over one partition, shared by the PTA and
+// batch paths.
+//
+// A partition stores per-attribute max/min, the largest attribute sum of its
+// objects (rounded up) and the sum of its per-attribute minima (rounded
+// down).  The bound is the maximum of sum_a W_a x_a over the partition's box
+// under the attribute-sum limit -- a fractional knapsack: every queried
+// attribute starts at its minimum, then the positive weights, heaviest
+// first, take the room above their minimum until the sum budget is used --
+// capped by the plain box bound, in FP64.  W_a aggregates a weight listed
+// twice.  A binary32 score of m terms differs from the exact sum by at most
+// about 2m * 2^-24 * sum |w x|; the margin 2^-16 * sum |W| max(|max|,|min|)
+// covers that for m <= 32, and the result is rounded up to binary32, so the
+// bound is never below an object's computed score.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "../../include/topk.h"
+
+namespace topk {
+
+__device__ __forceinline__ float f32_round_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+__device__ __forceinline__ float f32_round_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// The distinct queried attributes sorted by aggregated weight, heaviest first.
+struct BoundArgs {
+    uint32_t nd;
+    uint32_t d;
+    uint8_t attr[TOPK_MAX_DIMS];
+    double w[TOPK_MAX_DIMS];
+};
+
+inline BoundArgs make_bound_args(uint32_t d, uint32_t qq, const float *weights, const uint8_t *attr) {
+    BoundArgs b{};
+    b.d = d;
+    double agg[TOPK_MAX_DIMS] = {0};
+    bool used[TOPK_MAX_DIMS] = {false};
+    for (uint32_t j = 0; j < qq; j++) {
+        agg[attr[j]] += (double)weights[attr[j]];
+        used[attr[j]] = true;
+    }
+    uint32_t order[TOPK_MAX_DIMS], nd = 0;
+    for (uint32_t a = 0; a < d; a++)
+        if (used[a]) order[nd++] = a;
+    std::stable_sort(order, order + nd, [&](uint32_t x, uint32_t y) { return agg[x] > agg[y]; });
+    b.nd = nd;
+    for (uint32_t i = 0; i < nd; i++) {
+        b.attr[i] = (uint8_t)order[i];
+        b.w[i] = agg[order[i]];
+    }
+    return b;
+}
+
+// pmax / pmin point at the partition's d per-attribute maxima / minima.
+__device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__restrict__ pmax,
+                                          const float *__restrict__ pmin, float psum, float pminsum) {
+    double base = 0, box = 0, absw = 0;
+    for (uint32_t i = 0; i < b.nd; i++) {
+        const double mx = pmax[b.attr[i]], mn = pmin[b.attr[i]];
+        base += b.w[i] * mn;
+        box += b.w[i] * (b.w[i] >= 0 ? mx : mn);
+        absw += fabs(b.w[i]) * fmax(fabs(mx), fabs(mn));
+    }
+    double budget = (double)psum - (double)pminsum;
+    for (uint32_t i = 0; i < b.nd; i++) {
+        if (b.w[i] <= 0 || budget <= 0) break;
+        const double room = fmin((double)pmax[b.attr[i]] - (double)pmin[b.attr[i]], budget);
+        base += b.w[i] * room;
+        budget -= room;
+    }
+    const double lp = base < box ? base : box;
+    return f32_round_up(lp + absw * 1.52587890625e-05 + 1e-30);  // margin 2^-16 * sum |W x|
+}
+
+}  // namespace topk

@@ -28,6 +28,7 @@
 // STATUS_RETRY, too many for the buffer STATUS_OVERFLOW; the host then runs
 // the exact mode, with an n-entry buffer if needed.
 #include "device.hpp"
+#include "emit.hpp"
 #include "internal.hpp"
 
 namespace topk {
@@ -150,48 +151,6 @@ struct SlotView {
     }
 };
 
-struct FlatView {
-    const uint64_t *p;
-    __device__ __forceinline__ uint64_t get(uint32_t i) const { return p[i]; }
-};
-
-// Selects the exact top k of the c candidates of view v and writes them best
-// first.  All threads of the block call it.  `sorted` may alias the view's
-// storage when c <= BS*KPT (the keys are read into registers first).
-template <int BS, int KPT, class View>
-__device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const FinalOut &o, uint64_t *top,
-                                uint64_t *sorted, SelectScratch &ss) {
-    const uint32_t tid = threadIdx.x;
-    uint32_t got;
-    if (c <= (uint32_t)BS * KPT) {
-        RegKeys<KPT> rk;
-#pragma unroll
-        for (int j = 0; j < KPT; j++) {
-            const uint32_t i = j * BS + tid;
-            rk.k[j] = i < c ? v.get(i) : 0ull;
-        }
-        __syncthreads();  // the view's storage may be reused as `sorted` below
-        const uint64_t thr = c > k ? block_kth_largest<BS, RegKeys<KPT>, 11>(rk, c, k, ss) : 1ull;
-        got = block_gather_ge<BS>(rk, thr, top, ss);
-    } else {
-        const uint64_t thr = block_kth_largest<BS>(ViewKeys<BS, View, false>{v, c}, c, k, ss);
-        got = block_gather_ge<BS>(ViewKeys<BS, View, true>{v, c}, thr, top, ss);
-    }
-    block_rank_sort_desc<BS>(top, got, sorted);
-    for (uint32_t i = tid; i < k; i += BS) {
-        if (i < got) {
-            o.ids[i] = rkey_id(sorted[i]);
-            o.scores[i] = rkey_score(sorted[i]);
-        } else {
-            o.ids[i] = ~0ull;
-            o.scores[i] = -INFINITY;
-        }
-    }
-    if (tid == 0) {
-        if (o.thr) *o.thr = got ? rkey_score(sorted[got - 1]) : -INFINITY;
-        if (o.status) *o.status = 0;
-    }
-}
 
 // The ks-th largest of the nseed (<= 512) seed winners, computed by one wave.
 __device__ __forceinline__ uint64_t wave_seed_tau(const uint64_t *__restrict__ seed, uint32_t nseed,

@@ -65,6 +65,25 @@ struct Tuning {
     int stream_vpt = 0;         // float4 vectors per lane per column in a stream tile (0: by width)
     int fte_force_exact = 0;    // 1: skip the sample seed (testing / comparison)
     int lanes = 2;              // pipeline lanes used by asynchronous queries
+    // batched multi-query (0 = let the autotuner / defaults decide)
+    int batch_vpt = 0;          // objects per thread of the scoring kernel
+    int batch_bs = 0;           // threads per scoring workgroup (256 / 512)
+    int batch_qpi = 0;          // queries per work item (16 / 32 / 64)
+    int batch_digit = 0;        // radix digit width of the per-query select (8 / 11)
+    int batch_grid = 0;         // persistent scoring workgroups
+    int batch_graph = 1;        // replay captured hipGraphs
+    int batch_mode = 0;         // 0 auto (learned when trained), 1 seed threshold, 2 learned threshold
+};
+
+// One query of a batch as the device kernels read it (prepared on the host;
+// tau is written on the device by the seed pass or the learned predictor).
+struct BatchQuery {
+    float tau;                      // objects scoring >= tau are candidates
+    uint32_t k;                     // min(k, n)
+    uint32_t m;                     // terms
+    uint32_t kuser;                 // k as requested
+    uint8_t attr[TOPK_MAX_DIMS];    // term j reads attribute attr[j]
+    float w[TOPK_MAX_DIMS];         // weight of term j
 };
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
@@ -130,6 +149,9 @@ struct PtaLayout {
     bool built = false;
     uint32_t psize = 4096;       // target objects per partition
     uint32_t nparts = 0;
+    uint32_t nonempty = 0;       // partitions holding at least one object
+    uint32_t maxsz = 0;          // largest partition
+    uint64_t generation = 0;     // bumped whenever the layout is freed (rebuilt or released)
     uint64_t npos = 0;           // positions incl. padding (poff[nparts])
     uint64_t ld = 0;             // column stride of pcols
     float *pcols = nullptr;      // d columns in partition order, + one stream tile of padding
@@ -171,11 +193,8 @@ struct topk_ctx {
     unsigned char *d_res = nullptr;
     unsigned char *h_res = nullptr;  // pinned
     topk::ResultLayout res;
-    // batch scratch (grown on demand)
-    void *batch_buf = nullptr;
-    size_t batch_bytes = 0;
-    void *h_batch = nullptr;         // pinned staging for batch results
-    size_t h_batch_bytes = 0;
+    // batched multi-query state (owned by batch.hip)
+    void *batch = nullptr;
     // misc
     topk::FteState *h_state = nullptr;  // pinned mirror of *st
     topk_stats stats{};
@@ -208,6 +227,18 @@ int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq,
 
 // Releases the learned predictor state of a context (learned.hip).
 void learned_free(topk_ctx *ctx);
+// Learned threshold (learned.hip): true when a trained predictor is
+// installed; learned_tau_enqueue writes q[i].tau for i < nq on stream s
+// (kernels only, so it can be captured in a graph).
+bool learned_ready(const topk_ctx *ctx);
+uint64_t learned_version(const topk_ctx *ctx);
+// Validates one query and converts it to the device form (tau left at 0).
+int make_batch_query(const topk_ctx *ctx, const topk_query &q, BatchQuery &out);
+int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq);
+// Releases the batch buffers and captured graphs of a context (batch.hip).
+void batch_free(topk_ctx *ctx);
+int pta_build(topk_ctx *ctx);
+void pta_release(topk_ctx *ctx);
 
 // Event bracketing for topk_timing_*.
 void timing_begin(topk_ctx *ctx, hipStream_t s, hipEvent_t &ev0);

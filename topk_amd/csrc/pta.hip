@@ -31,6 +31,7 @@
 #include <thread>
 #include <vector>
 
+#include "bound.hpp"
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -116,17 +117,6 @@ __global__ void pta_gather_kernel(const float *__restrict__ cols, uint64_t ld, u
     const uint32_t id = perm[pos];
     pid[pos] = id;
     for (uint32_t m = 0; m < d; m++) pcols[(uint64_t)m * ldp + pos] = id != 0xFFFFFFFFu ? cols[(uint64_t)m * ld + id] : 0.0f;
-}
-
-__device__ __forceinline__ float f32_round_up(double x) {
-    float f = (float)x;
-    if ((double)f < x) f = nextafterf(f, INFINITY);
-    return f;
-}
-__device__ __forceinline__ float f32_round_down(double x) {
-    float f = (float)x;
-    if ((double)f > x) f = nextafterf(f, -INFINITY);
-    return f;
 }
 
 // One block per partition: per-attribute max/min, the max attribute sum
@@ -256,8 +246,10 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.bmax);
     hipFree(P.pscored);
     const uint32_t psize = P.psize;
+    const uint64_t gen = P.generation;
     P = PtaLayout();
     P.psize = psize;
+    P.generation = gen + 1;
 }
 
 void pta_release(topk_ctx *ctx) { pta_free(ctx->pta); }
@@ -320,6 +312,8 @@ int pta_build(topk_ctx *ctx) {
         TK_TRY(hipStreamSynchronize(s));
         uint64_t acc = 0;
         for (uint32_t p = 0; p < L; p++) {
+            P.nonempty += h_cnt[p] ? 1u : 0u;
+            P.maxsz = h_cnt[p] > P.maxsz ? h_cnt[p] : P.maxsz;
             h_off[p] = (uint32_t)acc;
             acc += ((uint64_t)h_cnt[p] + PART_ALIGN - 1) / PART_ALIGN * PART_ALIGN;
         }
@@ -376,36 +370,7 @@ fail:
 // ---------------------------------------------------------------------------
 // query kernels
 // ---------------------------------------------------------------------------
-// Query description for the bound: the distinct queried attributes sorted by
-// their aggregated weight (a weight listed twice counts twice), FP64.
-struct PtaBoundArgs {
-    uint32_t nd;
-    uint32_t d;
-    uint8_t attr[TOPK_MAX_DIMS];
-    double w[TOPK_MAX_DIMS];
-};
-
-__device__ __forceinline__ float pta_lp_bound(const PtaBoundArgs &b, const float *__restrict__ pmax,
-                                              const float *__restrict__ pmin, float psum, float pminsum) {
-    double base = 0, box = 0, absw = 0;
-    for (uint32_t i = 0; i < b.nd; i++) {
-        const double mx = pmax[b.attr[i]], mn = pmin[b.attr[i]];
-        base += b.w[i] * mn;
-        box += b.w[i] * (b.w[i] >= 0 ? mx : mn);
-        absw += fabs(b.w[i]) * fmax(fabs(mx), fabs(mn));
-    }
-    double budget = (double)psum - (double)pminsum;
-    for (uint32_t i = 0; i < b.nd; i++) {
-        if (b.w[i] <= 0 || budget <= 0) break;
-        const double room = fmin((double)pmax[b.attr[i]] - (double)pmin[b.attr[i]], budget);
-        base += b.w[i] * room;
-        budget -= room;
-    }
-    const double lp = base < box ? base : box;
-    return f32_round_up(lp + absw * 1.52587890625e-05 + 1e-30);  // margin 2^-16 * sum |w x|
-}
-
-__global__ __launch_bounds__(PTA_BS) void pta_bound_kernel(PtaBoundArgs b, uint32_t L,
+__global__ __launch_bounds__(PTA_BS) void pta_bound_kernel(BoundArgs b, uint32_t L,
                                                            const uint32_t *__restrict__ psz,
                                                            const float *__restrict__ pmax,
                                                            const float *__restrict__ pmin,
@@ -416,7 +381,7 @@ __global__ __launch_bounds__(PTA_BS) void pta_bound_kernel(PtaBoundArgs b, uint3
     const uint32_t p = blockIdx.x * PTA_BS + threadIdx.x;
     uint64_t key = 0;
     if (p < L && psz[p]) {
-        const float ub = pta_lp_bound(b, pmax + (uint64_t)p * b.d, pmin + (uint64_t)p * b.d, psum[p], pminsum[p]);
+        const float ub = lp_bound(b, pmax + (uint64_t)p * b.d, pmin + (uint64_t)p * b.d, psum[p], pminsum[p]);
         key = rkey(ub, p);
     }
     if (p < L) pbound[p] = key;
@@ -611,25 +576,7 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     QCols q;
     int rc = build_qcols(ctx, P.pcols, P.ld, qq, weights, attr, q);
     if (rc) return rc;
-    // distinct attributes with aggregated weights, heaviest first
-    PtaBoundArgs b;
-    memset(&b, 0, sizeof(b));
-    b.d = ctx->d;
-    double agg[TOPK_MAX_DIMS] = {0};
-    bool used[TOPK_MAX_DIMS] = {false};
-    for (uint32_t j = 0; j < qq; j++) {
-        agg[attr[j]] += (double)weights[attr[j]];
-        used[attr[j]] = true;
-    }
-    uint32_t order[TOPK_MAX_DIMS], nd = 0;
-    for (uint32_t a = 0; a < ctx->d; a++)
-        if (used[a]) order[nd++] = a;
-    std::stable_sort(order, order + nd, [&](uint32_t x, uint32_t y) { return agg[x] > agg[y]; });
-    b.nd = nd;
-    for (uint32_t i = 0; i < nd; i++) {
-        b.attr[i] = (uint8_t)order[i];
-        b.w[i] = agg[order[i]];
-    }
+    const BoundArgs b = make_bound_args(ctx->d, qq, weights, attr);
 
     Lane &Ln = ctx->lane[0];
     hipStream_t s = Ln.stream;

@@ -3,7 +3,6 @@
 #include "internal.hpp"
 
 namespace topk {
-void learned_free(topk_ctx *) {}
 }  // namespace topk
 
 extern "C" {
@@ -11,9 +10,6 @@ int topk_bta(topk_ctx *ctx, uint32_t, uint8_t, const float *, const uint8_t *, u
     return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID;
 }
 int topk_gpa(topk_ctx *ctx, uint32_t, uint8_t, const float *, const uint8_t *, uint64_t *, float *, float *) {
-    return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID;
-}
-int topk_batch(topk_ctx *ctx, uint32_t, const topk_query *, uint32_t, uint64_t *, float *, float *) {
     return ctx ? TOPK_ERR_UNSUPPORTED : TOPK_ERR_INVALID;
 }
 int topk_radix_select_u32(topk_ctx *ctx, const uint32_t *, uint64_t, uint64_t, uint32_t *) {

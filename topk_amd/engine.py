@@ -8,7 +8,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkQuery, TopkStats, check, lib)
+from ._lib import (MLP_IN, MLP_NPARAMS, TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkLearnedConfig, TopkLearnedReport,
+                   TopkQuery, TopkStats, check, lib)
 
 _c = ctypes
 
@@ -35,6 +36,26 @@ def _query_arrays(d, weights, attr):
         raise ValueError("weights are indexed by attribute id and need d entries")
     w[:wa.size] = wa[:w.size]
     return w, attr
+
+
+def query_array(queries):
+    """ctypes array of topk_query from (k, weights_by_attr_id, attr) tuples."""
+    nq = len(queries)
+    arr = np.zeros(nq, dtype=np.dtype([("k", "<u4"), ("qq", "<u4"), ("attr", "u1", TOPK_MAX_DIMS),
+                                       ("weights", "<f4", TOPK_MAX_DIMS)]))
+    for i, (k, w, at) in enumerate(queries):
+        at = np.asarray(at, dtype=np.uint8).ravel()
+        wa = np.asarray(w, dtype=np.float32).ravel()
+        if at.size > TOPK_MAX_DIMS:
+            raise ValueError("at most 32 terms per query")
+        arr[i]["k"] = k
+        arr[i]["qq"] = at.size
+        arr[i]["attr"][:at.size] = at
+        n = min(wa.size, TOPK_MAX_DIMS)
+        arr[i]["weights"][:n] = wa[:n]
+    qs = (TopkQuery * nq)()
+    _c.memmove(qs, arr.ctypes.data, arr.nbytes)
+    return qs
 
 
 class Engine:
@@ -142,18 +163,8 @@ class Engine:
         """queries: list of (k, weights_by_attr_id, attr).  Returns a list of
         (ids, scores, threshold) per query."""
         nq = len(queries)
-        qs = (TopkQuery * nq)()
-        kmax = 1
-        for i, (k, w, at) in enumerate(queries):
-            at = np.asarray(at, dtype=np.uint8)
-            wa = np.asarray(w, dtype=np.float32).ravel()
-            qs[i].k = k
-            qs[i].qq = at.size
-            for j, a in enumerate(at):
-                qs[i].attr[j] = int(a)
-            for j in range(min(wa.size, TOPK_MAX_DIMS)):
-                qs[i].weights[j] = float(wa[j])
-            kmax = max(kmax, k)
+        qs = query_array(queries)
+        kmax = max(1, max(int(q[0]) for q in queries))
         ids = np.empty(nq * kmax, dtype=np.uint64)
         sc = np.empty(nq * kmax, dtype=np.float32)
         thr = np.empty(nq, dtype=np.float32)
@@ -164,6 +175,64 @@ class Engine:
             c = min(k, self.n)
             out.append((ids[i * kmax:i * kmax + c].copy(), sc[i * kmax:i * kmax + c].copy(), float(thr[i])))
         return out
+
+    # ---- learned threshold ----
+    def learned_train(self, queries, **cfg):
+        """Trains and installs the k-th score predictor; returns the report as a dict."""
+        c = TopkLearnedConfig()
+        check(lib().topk_learned_default_config(_c.byref(c)), "topk_learned_default_config")
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        r = TopkLearnedReport()
+        qs = query_array(queries)
+        check(lib().topk_learned_train(self._h, len(queries), _c.cast(qs, _c.c_void_p), _c.byref(c), _c.byref(r)),
+              "topk_learned_train")
+        return {f: getattr(r, f) for f, _ in TopkLearnedReport._fields_}
+
+    def learned_clear(self):
+        check(lib().topk_learned_clear(self._h), "topk_learned_clear")
+
+    def learned_get(self):
+        p = np.empty(MLP_NPARAMS, np.float32)
+        m = _c.c_float()
+        check(lib().topk_learned_get(self._h, _ptr(p), _c.byref(m)), "topk_learned_get")
+        return p, m.value
+
+    def learned_set(self, params, margin):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != MLP_NPARAMS:
+            raise ValueError("10433 parameters expected")
+        check(lib().topk_learned_set(self._h, _ptr(p), float(margin)), "topk_learned_set")
+
+    def learned_features(self, queries):
+        qs = query_array(queries)
+        x = np.empty((len(queries), MLP_IN), np.float32)
+        sc = np.empty(len(queries), np.float32)
+        check(lib().topk_learned_features(self._h, len(queries), _c.cast(qs, _c.c_void_p), _ptr(x), _ptr(sc)),
+              "topk_learned_features")
+        return x, sc
+
+    def learned_predict(self, queries):
+        qs = query_array(queries)
+        pred = np.empty(len(queries), np.float32)
+        tau = np.empty(len(queries), np.float32)
+        check(lib().topk_learned_predict(self._h, len(queries), _c.cast(qs, _c.c_void_p), _ptr(pred), _ptr(tau)),
+              "topk_learned_predict")
+        return pred, tau
+
+    def mlp_eval(self, params, x, y=None):
+        """GPU forward (and with y, backward): (out, grad, loss)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        xx = np.ascontiguousarray(x, dtype=np.float32)
+        nb = xx.shape[0]
+        out = np.empty(nb, np.float32)
+        grad = np.empty(MLP_NPARAMS, np.float32) if y is not None else None
+        loss = _c.c_float()
+        yy = np.ascontiguousarray(y, dtype=np.float32) if y is not None else None
+        check(lib().topk_mlp_eval(self._h, _ptr(p), _ptr(xx), _ptr(yy) if yy is not None else None, nb, _ptr(out),
+                                  _ptr(grad) if grad is not None else None,
+                                  _c.byref(loss) if y is not None else None), "topk_mlp_eval")
+        return out, grad, (loss.value if y is not None else None)
 
     # ---- device-pointer API (outputs in HBM; callers pass device addresses) ----
     def fte_async(self, k, weights, attr, d_ids, d_scores):
