@@ -1,16 +1,27 @@
 #!/usr/bin/env python3
-"""Headline benchmark: plain GPU FTE top-k queries/s on one MI355X.
+"""Headline benchmark on one MI355X: learned-threshold batched top-k
+queries/s against plain GPU FTE queries/s, N = 2^26, d = 8, k = 64.
 
-Workload (BASELINE "Plain GPU FTE + radix select"): N = 2^26 objects,
-d = 8 FP32 attributes, independent U[0,1) data (seed 0) generated in HBM,
-query on all m = 8 attributes with weights 1.0, k = 64.  One step = one
-query (prefix pass + stream pass + final select), outputs left in HBM.
+Store: N = 2^26 objects, d = 8 FP32 attributes, independent U[0,1) data
+(seed 0) generated in HBM.
 
-Prints ONE JSON line (rank 0).  Roofline: the dominant kernel is the stream
-pass; its algorithmic bytes per launch are 4*m bytes per streamed object
-times the objects it streams, divided by its average duration measured with
-HIP events on the engine's stream.  cpu_baseline: the oracle's SIMD+threads
-TPAc restatement on the same data on this host's cores.
+* plain FTE (the comparison point): back-to-back single queries on all 8
+  attributes (k = 64) through two pipeline lanes; its stream kernel is the
+  HBM-bound kernel (4*m bytes per object), timed with HIP events on the
+  engine's stream -> ``plain_fte.roofline``.
+* learned + batched (``value``): partition layout (``topk_init`` PTA), a
+  k-th score predictor trained on random-weight queries labelled by the
+  exact batch path, then timed batches of 1024 held-out queries (weights
+  U(0.1, 1) on all 8 attributes, k = 64).  One step = one batch.  Every
+  held-out answer is judged against the plain GPU FTE path (``mismatches``);
+  the fallback rate is reported.  The dominant kernel is the batch scoring
+  kernel, bound by FP32 VALU issue (2*m flops per scored (object, query)
+  pair) -> ``roofline``.
+* cpu_baseline: the oracle's SIMD+threads TPAc restatement on the same store
+  and held-out queries, on the host's cores, within a time budget.
+
+``--mode fte`` runs the plain FTE part alone (one step = one query); it is
+the command the FTE rocprofv3 summaries under profiles/ were taken with.
 """
 import argparse
 import json
@@ -24,16 +35,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3      # published MI355X FP32 vector peak (packed FMA)
+FP32_ISSUE_TFLOPS = 39.3      # 256 CUs x 64 lanes x 2.4 GHz: one non-packed mul or add per lane-cycle
 N_DEFAULT = 1 << 26
 D = 8
 K = 64
+METRIC = "queries/s + HBM GB/s, N=2^26 d=8 k=64: learned/batched vs plain GPU FTE"
 
 
-def load_profile_traffic():
-    """HBM bytes per stream-pass launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), if one is present."""
-    path = os.path.join(ROOT, "profiles", "fte_stream_pmc.md")
+def load_profile_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
@@ -45,8 +58,8 @@ def load_profile_traffic():
 
 
 def calibrate_copy(nbytes):
-    """A plain device copy of nbytes (torch's copy kernel), the reference the
-    PMC byte counters are scaled by."""
+    """A plain device copy of nbytes, the reference the PMC byte counters are
+    scaled by."""
     import torch
     x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
@@ -57,40 +70,174 @@ def calibrate_copy(nbytes):
     print(json.dumps({"calibration_copy_bytes": nbytes, "launches": 3}), flush=True)
 
 
-def cpu_baseline(x, n, m, k, threads, budget_s, gpu_ids, gpu_scores):
-    import oracle
-    w = np.ones(D, np.float32)
+def plain_fte(eng, n, m, k, steps, warmup, lanes):
+    import torch
     attr = list(range(D - m, D))
-    oracle.tpac_threads(x, n, k, w, attr, threads)  # warm-up (reference: one warm-up call)
-    times = []
-    ids = sc = None
+    w = np.ones(D, np.float32)
+    eng.set_tuning("lanes", lanes)
+    ids_d = torch.empty(k, dtype=torch.int64, device="cuda")
+    sc_d = torch.empty(k, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        eng.fte_async(k, w, attr, ids_d.data_ptr(), sc_d.data_ptr())
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.fte_async(k, w, attr, ids_d.data_ptr(), sc_d.data_ptr())
+    eng.sync()
+    elapsed = time.perf_counter() - t0
+    # stream-kernel duration with HIP events on the engine's stream (timing
+    # mode runs every query on lane 0, so launches do not overlap)
+    eng.timing(True)
+    eng.timing_reset()
+    for _ in range(max(steps // 2, 10)):
+        eng.fte_async(k, w, attr, ids_d.data_ptr(), sc_d.data_ptr())
+    eng.sync()
+    stream_ms, launches = eng.timing_get("fte_stream")
+    eng.timing(False)
+    avg_ms = stream_ms / max(1, launches)
+    alg_bytes = 4.0 * m * n  # the sample-seeded pipeline streams every object
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    lat = []
+    for _ in range(10):
+        t1 = time.perf_counter()
+        eng.fte(k, w, attr)
+        lat.append(time.perf_counter() - t1)
+    qps = steps / elapsed
+    return {
+        "qps": qps, "ms_per_query": elapsed * 1e3 / steps, "steps": steps, "lanes": lanes, "m": m, "k": k,
+        "query_hbm_gbs": 4.0 * m * n / (elapsed / steps) / 1e9,
+        "sync_query_latency_ms": float(np.median(lat)) * 1e3,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_profile_traffic("fte_stream_pmc.md"),
+                     "kernel": "fte_stream_kernel", "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes},
+    }
+
+
+def random_queries(rng, count, k):
+    return [(k, rng.uniform(0.1, 1.0, D).astype(np.float32), np.arange(D, dtype=np.uint8)) for _ in range(count)]
+
+
+def learned_batched(eng, n, k, args):
+    import topk_amd
+    rng = np.random.default_rng(2026)
+    nb = args.batch
+    t0 = time.perf_counter()
+    eng.init(topk_amd.TOPK_INIT_PTA)
+    tt_layout = time.perf_counter() - t0
+    train = random_queries(rng, args.train_queries, k)
+    rep = eng.learned_train(train)
+    held = [eng.prepare_batch(random_queries(rng, nb, k)) for _ in range(args.held_batches)]
+
+    def timed(steps, warmup, mode):
+        eng.set_tuning("batch_mode", mode)
+        for i in range(warmup):
+            eng.batch_run(held[i % len(held)])
+        fb = pairs = objs = cand = 0
+        t1 = time.perf_counter()
+        for i in range(steps):
+            eng.batch_run(held[i % len(held)])
+            st = eng.stats()
+            fb += st["fallback"]
+            pairs += st["partitions_scored"]
+            objs += st["tuple_count"]
+            cand += st["candidates"]
+        el = time.perf_counter() - t1
+        return el, fb, pairs, objs, cand
+
+    # seed thresholds (the method without the network), then learned
+    s_el, s_fb, s_pairs, s_objs, _ = timed(max(2, args.steps // 4), 1, 1)
+    el, fb, pairs, objs, cand = timed(args.steps, args.warmup, 2)
+    steps = args.steps
+    qps = steps * nb / el
+
+    # scoring-kernel duration with HIP events (eager launches, one stream)
+    eng.timing(True)
+    eng.timing_reset()
+    tsteps = min(steps, 10)
+    tobjs = 0
+    for i in range(tsteps):
+        eng.batch_run(held[i % len(held)])
+        tobjs += eng.stats()["tuple_count"]
+    score_ms, launches = eng.timing_get("batch_score")
+    eng.timing(False)
+    avg_ms = score_ms / max(1, launches)
+    flops = 2.0 * D * tobjs / max(1, launches)  # mul + add per term, m = 8 terms
+    tflops = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+
+    # every held-out answer judged by the plain GPU FTE path
+    mism = 0
+    eng.set_tuning("batch_mode", 2)
+    for pb in held:
+        eng.batch_run(pb)
+        for i, (kq, w, attr) in enumerate(pb.queries):
+            gi, gs, gt = pb.result(i, n)
+            ri, rs, rt = eng.fte(kq, w, attr)
+            mism += int(not (np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
+                             and gt == rt))
+    eng.set_tuning("batch_mode", 0)
+    total_pairs = n * nb
+    return {
+        "qps": qps, "ms_per_batch": el * 1e3 / steps, "steps": steps, "queries_per_batch": nb,
+        "seed_batched_qps": max(2, args.steps // 4) * nb / s_el,
+        "fallback_rate": fb / (steps * nb),
+        "mismatches_vs_plain_fte": mism, "judged_queries": len(held) * nb,
+        "scored_pair_frac": objs / (steps * total_pairs),
+        "scored_pair_frac_seed": s_objs / (max(2, args.steps // 4) * total_pairs),
+        "candidates_per_query": cand / (steps * nb),
+        "layout_s": tt_layout,
+        "training": {"queries": len(train), "n_train": rep["n_train"], "n_calib": rep["n_calib"],
+                     "train_mse": rep["train_mse"], "calib_mse": rep["calib_mse"], "margin": rep["margin"],
+                     "calib_fallback": rep["calib_fallback"], "label_s": rep["ms_label"] / 1e3,
+                     "train_s": rep["ms_train"] / 1e3},
+        "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / FP32_PEAK_TFLOPS, "traffic": load_profile_traffic("batch_score_pmc.md"),
+                     "kernel": "batch_score_kernel", "avg_launch_ms": avg_ms, "flops_per_launch": flops,
+                     "issue_peak": FP32_ISSUE_TFLOPS, "issue_frac": tflops / FP32_ISSUE_TFLOPS},
+        "held": held,
+    }
+
+
+def cpu_baseline(x, n, k, threads, budget_s, held, gpu_check):
+    """Oracle TPAc (SIMD + threads) on held-out queries within budget_s."""
+    import oracle
+    qs = held[0].queries
+    oracle.tpac_threads(x, n, k, qs[0][1], qs[0][2], threads)  # warm-up
+    times, mism = [], 0
     t_start = time.time()
-    while not times or (time.time() - t_start < budget_s and len(times) < 50):
+    i = 0
+    while not times or (time.time() - t_start < budget_s and i < len(qs)):
+        kq, w, attr = qs[i]
         t0 = time.perf_counter()
-        ids, sc, _ = oracle.tpac_threads(x, n, k, w, attr, threads)
+        ids, sc, thr = oracle.tpac_threads(x, n, kq, w, attr, threads)
         times.append(time.perf_counter() - t0)
-    mism = int(np.sum(ids != gpu_ids) + np.sum(sc.view(np.uint32) != gpu_scores.view(np.uint32)))
+        gi, gs, gt = gpu_check(i)
+        mism += int(not (np.array_equal(ids, gi) and np.array_equal(sc.view(np.uint32), gs.view(np.uint32))))
+        i += 1
     return {"value": 1.0 / float(np.mean(times)), "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"full workload (N={n}, d={D}, m={m}, k={k}, same data); "
-                      f"{len(times)} timed queries after 1 warm-up, mean; oracle or_tpac_threads "
-                      f"(AVX2 16-wide + {threads} threads, cpu/TPAc.h:256-358 restated)",
+            "sample": f"{len(times)} held-out queries of the benchmark (N={n}, d={D}, all 8 attributes, "
+                      f"k={k}, same store), 1 warm-up, mean; oracle or_tpac_threads (AVX2 16-wide + "
+                      f"{threads} threads, cpu/TPAc.h:256-358 restated)",
             "median_s": float(np.median(times))}, mism
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", choices=["learned", "fte"], default="learned")
     ap.add_argument("--n", type=int, default=N_DEFAULT)
-    ap.add_argument("--m", type=int, default=D)
+    ap.add_argument("--m", type=int, default=D, help="--mode fte: query width")
     ap.add_argument("--k", type=int, default=K)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--train-queries", type=int, default=16384)
+    ap.add_argument("--held-batches", type=int, default=4)
+    ap.add_argument("--fte-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--lanes", type=int, default=2,
-                    help="pipeline lanes for back-to-back queries (1 = no overlap between queries)")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
-                    help="only run a plain copy of BYTES bytes (PMC calibration) and exit")
+    ap.add_argument("--lanes", type=int, default=2)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -100,87 +247,38 @@ def main():
               "the sharded design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import topk_amd
 
-    n, m, k = args.n, args.m, args.k
-    attr = list(range(D - m, D))
-    w = np.ones(D, np.float32)
+    n, k = args.n, args.k
     eng = topk_amd.Engine.generated(n, D, topk_amd.INDEPENDENT, 0)
-    eng.set_tuning("lanes", args.lanes)
-    ids_d = torch.empty(k, dtype=torch.int64, device="cuda")
-    sc_d = torch.empty(k, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
+    base = {"metric": METRIC, "unit": "queries/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (independent U[0,1), seed 0, generated in HBM)"}
 
-    for _ in range(args.warmup):
-        eng.fte_async(k, w, attr, ids_d.data_ptr(), sc_d.data_ptr())
-    eng.sync()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.fte_async(k, w, attr, ids_d.data_ptr(), sc_d.data_ptr())
-    eng.sync()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-    qps = args.steps / elapsed
+    if args.mode == "fte":
+        fte = plain_fte(eng, n, args.m, k, args.steps, args.warmup, args.lanes)
+        out = dict(base, value=fte["qps"], steps=args.steps, warmup=args.warmup, ms_per_step=fte["ms_per_query"],
+                   config={"workload": "plain GPU FTE top-k", "n": n, "d": D, "m": args.m, "k": k,
+                           "weights": "1.0", "distribution": "independent"},
+                   roofline=fte.pop("roofline"), plain_fte=fte)
+        print(json.dumps(out), flush=True)
+        eng.close()
+        return
 
-    # dominant kernel duration with HIP events on the engine's stream (timing
-    # mode runs every query on lane 0, so launches do not overlap)
-    eng.timing(True)
-    eng.timing_reset()
-    for _ in range(args.steps):
-        eng.fte_async(k, w, attr, ids_d.data_ptr(), sc_d.data_ptr())
-    eng.sync()
-    stream_ms, launches = eng.timing_get("fte_stream")
-    eng.timing(False)
-    avg_ms = stream_ms / max(1, launches)
-    # the sample-seeded pipeline streams every object (the 128 x 1024 sampled
-    # objects are scored again by the stream pass)
-    streamed = n
-    alg_bytes = 4.0 * m * streamed
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = load_profile_traffic()
-
-    # single-query latency through the synchronous C ABI (host buffers)
-    lat = []
-    for _ in range(10):
-        t1 = time.perf_counter()
-        gpu_ids, gpu_sc, _ = eng.fte(k, w, attr)
-        lat.append(time.perf_counter() - t1)
-    st = eng.stats()
-
-    out = {
-        "metric": "queries/s + HBM GB/s, N=2^26 d=8 k=64: learned/batched vs plain GPU FTE",
-        "value": qps,
-        "unit": "queries/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (independent U[0,1), seed 0, generated in HBM)",
-        "config": {"workload": "plain GPU FTE top-k", "n": n, "d": D, "m": m, "k": k,
-                   "weights": "1.0", "distribution": "independent"},
-        "query_hbm_gbs": 4.0 * m * n / (ms_per_step * 1e-3) / 1e9,
-        "query_roofline_frac": 4.0 * m * n / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "fte_stream_kernel", "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": alg_bytes},
-        "fallback": st["fallback"],
-        "lanes": args.lanes,
-        "sync_query_latency_ms": float(np.median(lat)) * 1e3,
-        "candidates_per_query": st["candidates"],
-    }
-
+    fte = plain_fte(eng, n, D, k, args.fte_steps, 20, args.lanes)
+    lb = learned_batched(eng, n, k, args)
+    held = lb.pop("held")
+    out = dict(base, value=lb["qps"], steps=args.steps, warmup=args.warmup, ms_per_step=lb["ms_per_batch"],
+               config={"workload": "learned-threshold batched top-k (1024 queries per step)", "n": n, "d": D,
+                       "k": k, "queries_per_batch": args.batch, "weights": "U(0.1,1) on all 8 attributes",
+                       "distribution": "independent", "train_queries": args.train_queries},
+               roofline=lb.pop("roofline"), speedup_vs_plain_fte=lb["qps"] / fte["qps"], plain_fte=fte,
+               learned_batched=lb)
     if not args.no_cpu:
-        x = topk_amd.generate(n, D, topk_amd.INDEPENDENT, 0, threads=min(os.cpu_count() or 1, 16))
         threads = min(os.cpu_count() or 1, 16)
-        cpu, mism = cpu_baseline(x, n, m, k, threads, args.cpu_budget, gpu_ids, gpu_sc)
+        x = topk_amd.generate(n, D, topk_amd.INDEPENDENT, 0, threads=threads)
+        eng.set_tuning("batch_mode", 2)
+        eng.batch_run(held[0])
+        cpu, mism = cpu_baseline(x, n, k, threads, args.cpu_budget, held, lambda i: held[0].result(i, n))
         out["cpu_baseline"] = cpu
         out["mismatches_vs_oracle"] = mism
         del x

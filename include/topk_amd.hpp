@@ -1,0 +1,218 @@
+/*
+ * topk_amd.hpp -- C++ host mirror of the reference's algorithm objects over
+ * the C ABI (topk.h), so reference-style callers switch by changing a type.
+ *
+ * Mirrors (avionicscode/TopK):
+ *   tuple_<T,Z>                      cpu/AA.h:29-35   (tid, score)
+ *   AA<T,Z> get_cdata/get_res/get_thres/benchmark
+ *                                    cpu/AA.h:64-124, 194-223
+ *   TPAc<T,Z> init/findTopK*         cpu/TPAc.h:15-32
+ *   PTA<T,Z>  init/findTopK*         cpu/PTA.h:60-80, 270-278
+ *
+ * Same conventions as the reference: the data buffer obtained through
+ * get_cdata() is column-major (x[m*n+i]), allocated by the caller with
+ * malloc (or by the object on first use) and owned by the object, which
+ * frees it in its destructor (cpu/AA.h:147-150); init() normalises the store
+ * (cpu/TPAc.h:28-32, cpu/PTA.h:271) unless set_normalize(false); get_res()
+ * is ascending by score (res[0] is the k-th, cpu/TPAc.h:198-206).  The
+ * store is uploaded to the GPU in init(); afterwards the host buffer is no
+ * longer read.  Differences: ties rank by lower id; errors throw
+ * topk_amd::error (with the topk_status) instead of exit(1).
+ */
+#ifndef TOPK_AMD_HPP
+#define TOPK_AMD_HPP
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstdlib>
+#include <iomanip>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "topk.h"
+
+namespace topk_amd {
+
+template <class T, class Z>
+struct tuple_ {
+    tuple_() : tid(0), score(0) {}
+    tuple_(Z t, T s) : tid(t), score(s) {}
+    Z tid;
+    T score;
+};
+
+class error : public std::runtime_error {
+public:
+    error(int st, const std::string &what)
+        : std::runtime_error(what + ": " + topk_strerror(st)), status(st) {}
+    int status;
+};
+
+inline void check(int st, const char *what) {
+    if (st != TOPK_OK) throw error(st, what);
+}
+
+/* Base of the GPU algorithm objects (float scores, uint64_t ids). */
+class AA {
+public:
+    AA(uint64_t n, uint64_t d) : n_(n), d_(d) {}
+    virtual ~AA() {
+        if (ctx_) topk_ctx_destroy(ctx_);
+        std::free(cdata_);
+    }
+    AA(const AA &) = delete;
+    AA &operator=(const AA &) = delete;
+
+    float *&get_cdata() {
+        if (!cdata_) cdata_ = static_cast<float *>(std::malloc(sizeof(float) * n_ * d_));
+        return cdata_;
+    }
+    void set_cdata(float *c) { cdata_ = c; }
+    std::vector<tuple_<float, uint64_t>> get_res() const { return res_; }
+    float get_thres() const { return threshold_; }
+    std::string get_algo() const { return algo_; }
+    void set_normalize(bool on) { normalize_ = on; }
+    void set_iter(uint32_t iter) { iter_ = iter ? iter : 1; }
+    void reset_clocks() { tt_processing_ = 0; }
+    double tt_init() const { return tt_init_; }
+    double tt_processing() const { return tt_processing_; }
+    uint64_t tuple_count() const { return tuple_count_; }
+    topk_ctx *ctx() const { return ctx_; }
+
+    /* Same stdout keys as AA::benchmark (cpu/AA.h:194-223). */
+    void benchmark() const {
+        std::cout << std::fixed << std::setprecision(8);
+        std::cout << "< Benchmark for " << algo_ << " algorithm >" << std::endl;
+        std::cout << "tt_init: " << tt_init_ << std::endl;
+        std::cout << "tt_procesing: " << tt_processing_ / iter_ << std::endl;
+        std::cout << "tuple_count: " << tuple_count_ << std::endl;
+    }
+
+protected:
+    static double now_ms() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+            .count();
+    }
+    void upload_and_init(uint32_t flags) {
+        const double t0 = now_ms();
+        if (!ctx_) {
+            if (!cdata_) throw error(TOPK_ERR_INVALID, "init: no data (fill get_cdata() first)");
+            check(topk_ctx_create(&ctx_, cdata_, n_, (uint32_t)d_, 0), "topk_ctx_create");
+        }
+        if (normalize_) flags |= TOPK_INIT_NORMALIZE;
+        if (flags) check(topk_init(ctx_, flags), "topk_init");
+        tt_init_ = now_ms() - t0;
+    }
+    template <class F>
+    void run(uint64_t k, F &&call) {
+        if (!ctx_) throw error(TOPK_ERR_STATE, "findTopK before init()");
+        ids_.resize(k);
+        scores_.resize(k);
+        const double t0 = now_ms();
+        float thr = 0;
+        check(call(ids_.data(), scores_.data(), &thr), algo_.c_str());
+        tt_processing_ += now_ms() - t0;
+        const uint64_t c = std::min<uint64_t>(k, n_);
+        res_.clear();
+        for (uint64_t i = c; i-- > 0;) res_.emplace_back(ids_[i], scores_[i]);  // ascending
+        threshold_ = thr;
+        topk_stats s;
+        if (topk_get_stats(ctx_, &s) == TOPK_OK) tuple_count_ += s.tuple_count;
+    }
+
+    uint64_t n_, d_;
+    float *cdata_ = nullptr;
+    topk_ctx *ctx_ = nullptr;
+    bool normalize_ = true;
+    std::string algo_;
+    std::vector<tuple_<float, uint64_t>> res_;
+    std::vector<uint64_t> ids_;
+    std::vector<float> scores_;
+    float threshold_ = 0;
+    double tt_init_ = 0, tt_processing_ = 0;
+    uint64_t tuple_count_ = 0;
+    uint32_t iter_ = 1;
+};
+
+/* Full-table evaluation on the GPU (TPAc, cpu/TPAc.h:15-358). */
+class TPAc : public AA {
+public:
+    TPAc(uint64_t n, uint64_t d) : AA(n, d) { algo_ = "TPAc (gpu fte)"; }
+    void init() { upload_and_init(0); }
+    void findTopK(uint64_t k, uint8_t qq, const float *weights, const uint8_t *attr) {
+        run(k, [&](uint64_t *ids, float *sc, float *thr) {
+            return topk_fte(ctx_, (uint32_t)k, qq, weights, attr, ids, sc, thr);
+        });
+    }
+    void findTopKscalar(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
+    void findTopKsimd(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
+    void findTopKthreads(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
+};
+
+/* Partition-pruned evaluation on the GPU (PTA, cpu/PTA.h:60-671). */
+class PTA : public AA {
+public:
+    PTA(uint64_t n, uint64_t d) : AA(n, d) { algo_ = "PTA (gpu)"; }
+    void init() { upload_and_init(TOPK_INIT_PTA); }
+    void findTopK(uint64_t k, uint8_t qq, const float *weights, const uint8_t *attr) {
+        run(k, [&](uint64_t *ids, float *sc, float *thr) {
+            return topk_pta(ctx_, (uint32_t)k, qq, weights, attr, ids, sc, thr);
+        });
+    }
+    void findTopKsimd(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
+    void findTopKthreads(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
+    void findTopKthreads2(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
+};
+
+/* Many queries per call (findTopKsimdMQ, cpu/TPAc.h:209-254, driven by
+ * validation/bench_cpu.h:198-235): add queries, run them as one batch, read
+ * each query's ascending result. */
+class MultiQuery : public AA {
+public:
+    MultiQuery(uint64_t n, uint64_t d) : AA(n, d) { algo_ = "MQ (gpu batch)"; }
+    void init() { upload_and_init(TOPK_INIT_PTA); }
+    void add(uint64_t k, uint8_t qq, const float *weights, const uint8_t *attr) {
+        topk_query q{};
+        q.k = (uint32_t)k;
+        q.qq = qq;
+        for (uint32_t j = 0; j < qq && j < TOPK_MAX_DIMS; j++) q.attr[j] = attr[j];
+        for (uint32_t a = 0; a < d_ && a < TOPK_MAX_DIMS; a++) q.weights[a] = weights[a];
+        qs_.push_back(q);
+    }
+    void clear() { qs_.clear(); }
+    void run_all() {
+        if (!ctx_) throw error(TOPK_ERR_STATE, "run_all before init()");
+        uint32_t ks = 1;
+        for (const auto &q : qs_) ks = std::max(ks, q.k);
+        ks_ = ks;
+        ids_.resize((size_t)qs_.size() * ks);
+        scores_.resize((size_t)qs_.size() * ks);
+        thr_.resize(qs_.size());
+        const double t0 = now_ms();
+        check(topk_batch(ctx_, (uint32_t)qs_.size(), qs_.data(), ks, ids_.data(), scores_.data(), thr_.data()),
+              "topk_batch");
+        tt_processing_ += now_ms() - t0;
+        topk_stats s;
+        if (topk_get_stats(ctx_, &s) == TOPK_OK) tuple_count_ += s.tuple_count;
+    }
+    size_t size() const { return qs_.size(); }
+    std::vector<tuple_<float, uint64_t>> get_res(size_t q) const {
+        std::vector<tuple_<float, uint64_t>> r;
+        const uint64_t c = std::min<uint64_t>(qs_[q].k, n_);
+        for (uint64_t i = c; i-- > 0;) r.emplace_back(ids_[q * ks_ + i], scores_[q * ks_ + i]);
+        return r;
+    }
+    float get_thres(size_t q) const { return thr_[q]; }
+
+private:
+    std::vector<topk_query> qs_;
+    std::vector<float> thr_;
+    uint32_t ks_ = 1;
+};
+
+}  // namespace topk_amd
+
+#endif

@@ -202,6 +202,7 @@ int topk_ctx_destroy(topk_ctx *ctx) {
     if (ctx->h_res) hipHostFree(ctx->h_res);
     if (ctx->h_state) hipHostFree(ctx->h_state);
     batch_free(ctx);
+    aux_free(ctx);
     pta_release(ctx);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -57,6 +57,9 @@ constexpr uint32_t BSEED_T = 4;                          // seed partitions per 
 constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2 };
+// Device counters of one batch: [0] work items, [1] work ticket,
+// [2..3] scored (partition, query) pairs, [4..5] scored (object, query) pairs.
+constexpr uint32_t BCTR_WORDS = 8;
 
 struct BatchItem {
     uint32_t p;                 // partition
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(BAUX_BS) void batch_zero_kernel(unsigned long long 
     const uint64_t i0 = (uint64_t)blockIdx.x * BAUX_BS + threadIdx.x;
     for (uint64_t i = i0; i < nmask; i += stride) masks[i] = 0ull;
     for (uint64_t i = i0; i < nq; i += stride) ccount[i] = 0u;
-    if (i0 < 4) ctr[i0] = 0u;
+    if (i0 < BCTR_WORDS) ctr[i0] = 0u;
 }
 
 // bnd[q * L + p] = LP bound of query q over partition p (-inf when empty).
@@ -189,10 +192,10 @@ __global__ __launch_bounds__(BAUX_BS) void batch_mask_kernel(const BatchQuery *_
 }
 
 // Non-empty groups of qpi queries of every partition bitmap become items
-// (one global atomic per workgroup); ctr[0] counts items, ctr[2..3] the
-// scored (partition, query) pairs.
+// (one global atomic per workgroup), with the pair counters updated.
 __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned long long *__restrict__ masks,
                                                                 uint64_t nmask, uint32_t nw, uint32_t qpi,
+                                                                const uint32_t *__restrict__ psz,
                                                                 BatchItem *__restrict__ items,
                                                                 unsigned int *__restrict__ ctr) {
     __shared__ uint32_t wsum[BAUX_BS / 64];
@@ -210,9 +213,15 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned l
     }
     const uint32_t flag = sub ? 1u : 0u;
     const uint32_t pos = block_excl_scan<BAUX_BS>(flag, wsum, &total);
-    uint32_t pairs = wave_sum_u32((uint32_t)__popcll(sub));
+    const uint32_t np = (uint32_t)__popcll(sub);
+    const uint32_t pairs = wave_sum_u32(np);
+    unsigned long long objs = (unsigned long long)np * (sub ? psz[pw / nw] : 0u);
+    for (int o = 32; o > 0; o >>= 1) objs += __shfl_xor(objs, o, 64);
     if (threadIdx.x == 0) base = total ? atomicAdd(&ctr[0], total) : 0u;
-    if (lane_id() == 0 && pairs) atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), (unsigned long long)pairs);
+    if (lane_id() == 0 && pairs) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), (unsigned long long)pairs);
+        atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 4), objs);
+    }
     __syncthreads();
     if (flag) {
         BatchItem it;
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     __shared__ SelectScratch ss;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const uint32_t c = a.ccount[q], k = a.q[q].k;
-    if (q == 0 && tid < 4) a.ctr_out[tid] = a.ctr[tid];
+    if (q == 0 && tid < BCTR_WORDS) a.ctr_out[tid] = a.ctr[tid];
     if (tid == 0) a.cnt[q] = c;
     if (c > BATCH_CAP || c < k) {
         if (tid == 0) a.status[q] = c > BATCH_CAP ? STATUS_OVERFLOW : STATUS_RETRY;
@@ -417,7 +426,7 @@ struct OutLayout {
         l.st_off = l.thr_off + sizeof(float) * nq;
         l.cnt_off = l.st_off + sizeof(uint32_t) * nq;
         l.ctr_off = (l.cnt_off + sizeof(uint32_t) * nq + 15) & ~(size_t)15;
-        l.bytes = l.ctr_off + 16;
+        l.bytes = l.ctr_off + sizeof(uint32_t) * BCTR_WORDS;
         return l;
     }
 };
@@ -505,7 +514,7 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, bool n
     if (rc) return rc;
     if (!B.d_ctr) {
         changed = true;
-        TK_HIP(hipMalloc(&B.d_ctr, sizeof(unsigned int) * 4));
+        TK_HIP(hipMalloc(&B.d_ctr, sizeof(unsigned int) * BCTR_WORDS));
     }
     if (!B.cus) {
         TK_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -578,7 +587,7 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     }
     const uint64_t nthreads = nmask * (64 / v.qpi);
     hipLaunchKernelGGL(batch_compact_kernel, dim3((uint32_t)((nthreads + BAUX_BS - 1) / BAUX_BS)), dim3(BAUX_BS), 0,
-                       s, B.d_masks, nmask, nw, (uint32_t)v.qpi, B.d_items, B.d_ctr);
+                       s, B.d_masks, nmask, nw, (uint32_t)v.qpi, P.psz, B.d_items, B.d_ctr);
     ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.d_ctr, dq, B.d_cand, B.d_ccount};
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
@@ -729,7 +738,7 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
         if (out_thr) out_thr[i] = thr[i];
     }
     const uint64_t pairs = (uint64_t)ctr[2] | ((uint64_t)ctr[3] << 32);
-    const uint64_t psum_objs = pairs * (ctx->n / (ctx->pta.nonempty ? ctx->pta.nonempty : 1));
+    const uint64_t psum_objs = (uint64_t)ctr[4] | ((uint64_t)ctr[5] << 32);
     for (uint32_t i : redo) {
         const topk_query &Q = qs[i];
         float t = 0;

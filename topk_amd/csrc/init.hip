@@ -127,6 +127,7 @@ int topk_init(topk_ctx *ctx, uint32_t flags) {
     int rc = TOPK_OK;
     if (flags & TOPK_INIT_NORMALIZE) {
         pta_release(ctx);  // a layout built from the old values would be stale
+        ctx->store_gen++;
         rc = normalize_store(ctx);
         if (rc) return rc;
     }
@@ -147,6 +148,7 @@ int topk_set_partition_size(topk_ctx *ctx, uint32_t objects) {
 int topk_reorder_attr(topk_ctx *ctx) {
     if (!ctx) return TOPK_ERR_INVALID;
     pta_release(ctx);
+    ctx->store_gen++;
     hipLaunchKernelGGL(reorder_attr_kernel, dim3((uint32_t)((ctx->n + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->cols, ctx->n, ctx->d, ctx->ld);
     TK_HIP(hipGetLastError());

@@ -195,6 +195,9 @@ struct topk_ctx {
     topk::ResultLayout res;
     // batched multi-query state (owned by batch.hip)
     void *batch = nullptr;
+    // radix select / BTA / GPA scratch and column statistics (owned by bta_gpa.hip)
+    void *aux = nullptr;
+    uint64_t store_gen = 0;     // bumped whenever the column store's values change
     // misc
     topk::FteState *h_state = nullptr;  // pinned mirror of *st
     topk_stats stats{};
@@ -237,6 +240,8 @@ int make_batch_query(const topk_ctx *ctx, const topk_query &q, BatchQuery &out);
 int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq);
 // Releases the batch buffers and captured graphs of a context (batch.hip).
 void batch_free(topk_ctx *ctx);
+// Releases the radix select / BTA / GPA scratch (bta_gpa.hip).
+void aux_free(topk_ctx *ctx);
 int pta_build(topk_ctx *ctx);
 void pta_release(topk_ctx *ctx);
 

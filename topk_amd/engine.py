@@ -58,6 +58,24 @@ def query_array(queries):
     return qs
 
 
+class PreparedBatch:
+    """A query list staged once as topk_query structs plus output arrays."""
+
+    def __init__(self, queries):
+        self.queries = queries
+        self.nq = len(queries)
+        self.qs = query_array(queries)
+        self.kmax = max(1, max(int(q[0]) for q in queries))
+        self.ids = np.empty(self.nq * self.kmax, dtype=np.uint64)
+        self.scores = np.empty(self.nq * self.kmax, dtype=np.float32)
+        self.thr = np.empty(self.nq, dtype=np.float32)
+
+    def result(self, i, n):
+        c = min(int(self.queries[i][0]), n)
+        o = i * self.kmax
+        return self.ids[o:o + c], self.scores[o:o + c], float(self.thr[i])
+
+
 class Engine:
     """One top-k context: column store + stream on the current GPU."""
 
@@ -175,6 +193,15 @@ class Engine:
             c = min(k, self.n)
             out.append((ids[i * kmax:i * kmax + c].copy(), sc[i * kmax:i * kmax + c].copy(), float(thr[i])))
         return out
+
+    def prepare_batch(self, queries):
+        """Staged form of a query list for repeated batch_run calls."""
+        return PreparedBatch(queries)
+
+    def batch_run(self, pb):
+        """Runs a PreparedBatch; results land in pb.ids / pb.scores / pb.thr."""
+        check(lib().topk_batch(self._h, pb.nq, _c.cast(pb.qs, _c.c_void_p), pb.kmax, _ptr(pb.ids), _ptr(pb.scores),
+                               _ptr(pb.thr)), "topk_batch")
 
     # ---- learned threshold ----
     def learned_train(self, queries, **cfg):
