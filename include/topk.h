@@ -274,6 +274,8 @@ typedef struct topk_stats {
     uint32_t queries;            /* queries in the last call */
     double ms_last;              /* host wall time of the last query call */
     double tt_init;              /* host wall time of the last topk_init, ms */
+    uint32_t retried;            /* batch: queries re-run with a tightened threshold */
+    uint32_t reserved;
 } topk_stats;
 
 int topk_get_stats(topk_ctx *ctx, topk_stats *out);

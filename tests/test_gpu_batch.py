@@ -72,7 +72,10 @@ def test_batch_parity_d16_mixed(gpu):
     with topk_amd.Engine(x) as eng:
         qs = make_queries(rng, 1024, d, [16, 32, 64, 128, 256])
         st = check_batch(eng, x, n, qs)
-        assert st["fallback"] == 0
+        # the correlated third clips many values to exactly 1.0: a query on few
+        # attributes can tie more objects at its top score than a candidate
+        # list holds, and such a query is answered by the exact FTE path
+        assert st["fallback"] <= 0.01 * len(qs)
 
 
 @pytest.mark.parametrize("order", ["shuffled", "repeats"])

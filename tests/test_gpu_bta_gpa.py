@@ -22,6 +22,7 @@ def test_radix_select_matches_oracle(gpu, n):
     if n > 1000:
         keys[: n // 3] = keys[0]  # a heavy duplicate
     t = torch.from_numpy(keys.view(np.int32)).cuda()
+    torch.cuda.synchronize()  # the engine reads the buffer on its own stream
     with topk_amd.Engine(np.zeros((1, 10), np.float32)) as eng:
         for r in sorted({0, n // 2, n - 1, max(0, n - 64)}):
             got = eng.radix_select_u32(t.data_ptr(), n, r)
@@ -34,6 +35,7 @@ def test_radix_select_float_threshold(gpu):
     n, k = 2_000_000, 64
     s = RNG.random(n, dtype=np.float32)
     t = torch.from_numpy(s.view(np.int32)).cuda()
+    torch.cuda.synchronize()
     with topk_amd.Engine(np.zeros((1, 10), np.float32)) as eng:
         bits = eng.radix_select_u32(t.data_ptr(), n, n - k)
     assert np.uint32(bits).view(np.float32) == np.sort(s)[n - k]

@@ -37,7 +37,8 @@ class TopkStats(ctypes.Structure):
     _fields_ = [("tuple_count", _c.c_uint64), ("candidates", _c.c_uint64),
                 ("partitions_scored", _c.c_uint64), ("partitions_total", _c.c_uint64),
                 ("fallback", _c.c_uint32), ("queries", _c.c_uint32),
-                ("ms_last", _c.c_double), ("tt_init", _c.c_double)]
+                ("ms_last", _c.c_double), ("tt_init", _c.c_double),
+                ("retried", _c.c_uint32), ("reserved", _c.c_uint32)]
 
 
 class TopkQuery(ctypes.Structure):

@@ -49,14 +49,13 @@
 namespace topk {
 
 constexpr uint32_t BATCH_MAX_Q = 16384;   // queries per call
-constexpr uint32_t BATCH_CAP = 8192;      // candidate keys per query (final select keeps them in registers)
-constexpr int BFIN_BS = 512, BFIN_KPT = 16;
-static_assert(BATCH_CAP == BFIN_BS * BFIN_KPT, "final select holds the whole candidate list");
+constexpr uint32_t BATCH_CAP = 32768;     // candidate keys per query
+constexpr int BFIN_BS = 512, BFIN_KPT = 16;  // lists up to 8192 keys are selected in registers, longer ones in memory
 constexpr int BSEED_BS = 512, BSEED_KPT = 32;
 constexpr uint32_t BSEED_T = 4;                          // seed partitions per query
 constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
-enum { BMODE_SEED = 1, BMODE_LEARNED = 2 };
+enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 // Device counters of one batch: [0] work items, [1] work ticket,
 // [2..3] scored (partition, query) pairs, [4..5] scored (object, query) pairs.
 constexpr uint32_t BCTR_WORDS = 8;
@@ -359,6 +358,7 @@ struct FinalArgs {
     uint32_t *status;
     uint32_t *cnt;
     uint32_t *ctr_out;
+    float *tau_next;
 };
 
 template <int DB>
@@ -370,8 +370,20 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     const uint32_t c = a.ccount[q], k = a.q[q].k;
     if (q == 0 && tid < BCTR_WORDS) a.ctr_out[tid] = a.ctr[tid];
     if (tid == 0) a.cnt[q] = c;
-    if (c > BATCH_CAP || c < k) {
-        if (tid == 0) a.status[q] = c > BATCH_CAP ? STATUS_OVERFLOW : STATUS_RETRY;
+    if (c > BATCH_CAP) {
+        // The stored keys are a subset of the objects, so their k-th largest
+        // is a lower bound of the query's k-th key: a tighter threshold for a
+        // second pass.
+        const uint64_t kth = block_kth_largest<BFIN_BS>(MemKeys<BFIN_BS>{a.cand + (uint64_t)q * BATCH_CAP, BATCH_CAP},
+                                                        BATCH_CAP, k, ss);
+        if (tid == 0) {
+            a.status[q] = STATUS_OVERFLOW;
+            a.tau_next[q] = rkey_score(kth);
+        }
+        return;
+    }
+    if (c < k) {
+        if (tid == 0) a.status[q] = STATUS_RETRY;
         return;
     }
     const FinalOut o{a.ids + (uint64_t)q * a.kstride, a.scores + (uint64_t)q * a.kstride, a.thr + q, a.status + q};
@@ -417,7 +429,7 @@ struct InLayout {
 };
 
 struct OutLayout {
-    size_t ids_off, sc_off, thr_off, st_off, cnt_off, ctr_off, bytes;
+    size_t ids_off, sc_off, thr_off, st_off, cnt_off, tn_off, ctr_off, bytes;
     static OutLayout make(uint32_t nq, uint32_t ks) {
         OutLayout l;
         l.ids_off = 0;
@@ -425,7 +437,8 @@ struct OutLayout {
         l.thr_off = l.sc_off + ((sizeof(float) * nq * ks + 15) & ~(size_t)15);
         l.st_off = l.thr_off + sizeof(float) * nq;
         l.cnt_off = l.st_off + sizeof(uint32_t) * nq;
-        l.ctr_off = (l.cnt_off + sizeof(uint32_t) * nq + 15) & ~(size_t)15;
+        l.tn_off = l.cnt_off + sizeof(uint32_t) * nq;
+        l.ctr_off = (l.tn_off + sizeof(float) * nq + 15) & ~(size_t)15;
         l.bytes = l.ctr_off + sizeof(uint32_t) * BCTR_WORDS;
         return l;
     }
@@ -580,8 +593,10 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
         hipLaunchKernelGGL(batch_mask_kernel<true>, pq, dim3(BAUX_BS), 0, s, dq, db, B.d_bnd, L, nw, P.psz, P.pmax,
                            P.pmin, P.psum, P.pminsum, B.d_masks);
     } else {
-        int rc = learned_tau_enqueue(ctx, s, dq, nq);
-        if (rc) return rc;
+        if (mode == BMODE_LEARNED) {
+            const int rc = learned_tau_enqueue(ctx, s, dq, nq);
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(batch_mask_kernel<false>, pq, dim3(BAUX_BS), 0, s, dq, db, nullptr, L, nw, P.psz, P.pmax,
                            P.pmin, P.psum, P.pminsum, B.d_masks);
     }
@@ -603,7 +618,8 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
                  reinterpret_cast<float *>(B.d_out + ol.thr_off),
                  reinterpret_cast<uint32_t *>(B.d_out + ol.st_off),
                  reinterpret_cast<uint32_t *>(B.d_out + ol.cnt_off),
-                 reinterpret_cast<uint32_t *>(B.d_out + ol.ctr_off)};
+                 reinterpret_cast<uint32_t *>(B.d_out + ol.ctr_off),
+                 reinterpret_cast<float *>(B.d_out + ol.tn_off)};
     if (v.digit == 8)
         hipLaunchKernelGGL(batch_final_kernel<8>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
     else
@@ -683,8 +699,56 @@ static int resolve_mode(const topk_ctx *ctx) {
     return learned_ready(ctx) ? BMODE_LEARNED : BMODE_SEED;
 }
 
-// Runs one batch and copies results out; queries the device path could not
-// finish go through topk_fte.
+// Parsed outcome of one device pass.
+struct PassTally {
+    uint64_t pairs = 0, objs = 0, candidates = 0;
+};
+
+// Copies the finished queries of the last pass (local index i -> caller row
+// rows[i]) and collects the others: overflowed queries with their tightened
+// threshold, and queries with too few candidates.
+static void collect(const topk_ctx *ctx, const BatchState &B, uint32_t nq, uint32_t ks, const topk_query *qs,
+                    const uint32_t *rows, uint64_t *out_ids, float *out_scores, float *out_thr,
+                    std::vector<std::pair<uint32_t, float>> &reseed, std::vector<uint32_t> &fte, PassTally &t) {
+    const OutLayout ol = OutLayout::make(nq, ks);
+    const uint64_t *ids = reinterpret_cast<const uint64_t *>(B.h_out + ol.ids_off);
+    const float *sc = reinterpret_cast<const float *>(B.h_out + ol.sc_off);
+    const float *thr = reinterpret_cast<const float *>(B.h_out + ol.thr_off);
+    const uint32_t *st = reinterpret_cast<const uint32_t *>(B.h_out + ol.st_off);
+    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(B.h_out + ol.cnt_off);
+    const float *tn = reinterpret_cast<const float *>(B.h_out + ol.tn_off);
+    const uint32_t *ctr = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr_off);
+    t.pairs += (uint64_t)ctr[2] | ((uint64_t)ctr[3] << 32);
+    t.objs += (uint64_t)ctr[4] | ((uint64_t)ctr[5] << 32);
+    for (uint32_t i = 0; i < nq; i++) {
+        const uint32_t row = rows ? rows[i] : i;
+        t.candidates += cnt[i] < BATCH_CAP ? cnt[i] : BATCH_CAP;
+        if (st[i] == STATUS_OVERFLOW) {
+            reseed.push_back({row, tn[i]});
+            continue;
+        }
+        if (st[i]) {
+            fte.push_back(row);
+            continue;
+        }
+        const uint32_t kq = qs[row].k;
+        const uint32_t c = (uint32_t)(ctx->n < kq ? ctx->n : kq);
+        const uint64_t o = (uint64_t)row * ks, li = (uint64_t)i * ks;
+        if (out_ids) {
+            memcpy(out_ids + o, ids + li, sizeof(uint64_t) * c);
+            for (uint32_t j = c; j < kq; j++) out_ids[o + j] = ~0ull;
+        }
+        if (out_scores) {
+            memcpy(out_scores + o, sc + li, sizeof(float) * c);
+            for (uint32_t j = c; j < kq; j++) out_scores[o + j] = -INFINITY;
+        }
+        if (out_thr) out_thr[row] = thr[i];
+    }
+}
+
+// Runs one batch and copies results out.  Overflowed queries get a second
+// device pass with the threshold their first pass tightened; queries the
+// device path still cannot settle go through topk_fte.
 static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
                      float *out_scores, float *out_thr, const BatchVariant *force) {
     if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
@@ -710,36 +774,32 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     rc = batch_launch(ctx, B, s, nq, ks, mode, v);
     if (rc) return rc;
     TK_HIP(hipStreamSynchronize(s));
+    PassTally tally;
+    std::vector<std::pair<uint32_t, float>> reseed;
+    std::vector<uint32_t> fte;
+    collect(ctx, B, nq, ks, qs, nullptr, out_ids, out_scores, out_thr, reseed, fte, tally);
 
-    const OutLayout ol = OutLayout::make(nq, ks);
-    const uint64_t *ids = reinterpret_cast<const uint64_t *>(B.h_out + ol.ids_off);
-    const float *sc = reinterpret_cast<const float *>(B.h_out + ol.sc_off);
-    const float *thr = reinterpret_cast<const float *>(B.h_out + ol.thr_off);
-    const uint32_t *st = reinterpret_cast<const uint32_t *>(B.h_out + ol.st_off);
-    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(B.h_out + ol.cnt_off);
-    const uint32_t *ctr = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr_off);
-    uint64_t candidates = 0;
-    std::vector<uint32_t> redo;
-    for (uint32_t i = 0; i < nq; i++) {
-        candidates += cnt[i] < BATCH_CAP ? cnt[i] : BATCH_CAP;
-        if (st[i]) {
-            redo.push_back(i);
-            continue;
+    const uint32_t retried = (uint32_t)reseed.size();
+    if (!reseed.empty()) {
+        std::vector<topk_query> sub(reseed.size());
+        std::vector<uint32_t> rows(reseed.size());
+        for (size_t i = 0; i < reseed.size(); i++) {
+            rows[i] = reseed[i].first;
+            sub[i] = qs[reseed[i].first];
         }
-        const uint32_t c = (uint32_t)(ctx->n < qs[i].k ? ctx->n : qs[i].k);
-        if (out_ids) {
-            memcpy(out_ids + (uint64_t)i * ks, ids + (uint64_t)i * ks, sizeof(uint64_t) * c);
-            for (uint32_t j = c; j < qs[i].k; j++) out_ids[(uint64_t)i * ks + j] = ~0ull;
-        }
-        if (out_scores) {
-            memcpy(out_scores + (uint64_t)i * ks, sc + (uint64_t)i * ks, sizeof(float) * c);
-            for (uint32_t j = c; j < qs[i].k; j++) out_scores[(uint64_t)i * ks + j] = -INFINITY;
-        }
-        if (out_thr) out_thr[i] = thr[i];
+        const uint32_t n2 = (uint32_t)sub.size();
+        rc = stage_queries(ctx, B, n2, sub.data(), ks);
+        if (rc) return rc;
+        BatchQuery *hq = reinterpret_cast<BatchQuery *>(B.h_in + InLayout::make(n2).q_off);
+        for (uint32_t i = 0; i < n2; i++) hq[i].tau = reseed[i].second;
+        rc = batch_enqueue(ctx, B, s, n2, ks, BMODE_GIVEN, v);
+        if (rc) return rc;
+        TK_HIP(hipStreamSynchronize(s));
+        std::vector<std::pair<uint32_t, float>> again;
+        collect(ctx, B, n2, ks, qs, rows.data(), out_ids, out_scores, out_thr, again, fte, tally);
+        for (const auto &r : again) fte.push_back(r.first);
     }
-    const uint64_t pairs = (uint64_t)ctr[2] | ((uint64_t)ctr[3] << 32);
-    const uint64_t psum_objs = (uint64_t)ctr[4] | ((uint64_t)ctr[5] << 32);
-    for (uint32_t i : redo) {
+    for (uint32_t i : fte) {
         const topk_query &Q = qs[i];
         float t = 0;
         rc = topk_fte(ctx, Q.k, (uint8_t)Q.qq, Q.weights, Q.attr, out_ids ? out_ids + (uint64_t)i * ks : nullptr,
@@ -749,11 +809,12 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     }
     topk_stats &S = ctx->stats;
     S.queries = nq;
-    S.fallback = (uint32_t)redo.size();
-    S.candidates = candidates;
-    S.partitions_scored = pairs;
+    S.fallback = (uint32_t)fte.size();
+    S.retried = retried;
+    S.candidates = tally.candidates;
+    S.partitions_scored = tally.pairs;
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;
-    S.tuple_count = psum_objs;
+    S.tuple_count = tally.objs;
     S.ms_last = now_ms() - t0;
     return TOPK_OK;
 }
