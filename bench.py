@@ -160,10 +160,14 @@ def learned_batched(eng, n, k, args):
         eng.batch_run(held[i % len(held)])
         tobjs += eng.stats()["tuple_count"]
     score_ms, launches = eng.timing_get("batch_score")
+    seed_ms, _ = eng.timing_get("batch_tau_seed")
+    lrn_ms, _ = eng.timing_get("batch_tau_learned")
     eng.timing(False)
     avg_ms = score_ms / max(1, launches)
-    flops = 2.0 * D * tobjs / max(1, launches)  # mul + add per term, m = 8 terms
-    tflops = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    # mul + add per term (m = 8 terms) for every scored (object, query) pair,
+    # over all scoring launches (a batch's second pass included)
+    flops = 2.0 * D * tobjs / max(1, launches)
+    tflops = 2.0 * D * tobjs / (score_ms * 1e-3) / 1e12 if score_ms > 0 else 0.0
 
     # every held-out answer judged by the plain GPU FTE path
     mism = 0
@@ -185,6 +189,7 @@ def learned_batched(eng, n, k, args):
         "scored_pair_frac": objs / (steps * total_pairs),
         "scored_pair_frac_seed": s_objs / (max(2, args.steps // 4) * total_pairs),
         "candidates_per_query": cand / (steps * nb),
+        "threshold_stage_ms": {"learned_mlp": lrn_ms / max(1, tsteps), "seed_pass_in_second_passes": seed_ms / max(1, tsteps)},
         "layout_s": tt_layout,
         "training": {"queries": len(train), "n_train": rep["n_train"], "n_calib": rep["n_calib"],
                      "train_mse": rep["train_mse"], "calib_mse": rep["calib_mse"], "margin": rep["margin"],
@@ -247,6 +252,9 @@ def main():
               "the sharded design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
+    import torch
+
+    torch.cuda.init()  # torch's HIP runtime first; the engine shares it
     import topk_amd
 
     n, k = args.n, args.k

@@ -253,7 +253,9 @@ void *topk_stream(topk_ctx *ctx);
  *   "batch_vpt"       objects per thread of the batch scoring kernel
  *                     (d <= 8: 4, 8, 16; d <= 16: 2, 4, 8; else 2, 4)
  *   "batch_bs"        threads per batch scoring workgroup: 256 or 512
- *   "batch_qpi"       queries per batch work item: 16, 32 or 64
+ *   "batch_qpi"       query budget of a batch work item (16..1024): a
+ *                     partition is loaded once per item and scored for up
+ *                     to this many of the queries that reach it
  *   "batch_digit"     radix digit width of the per-query select: 8 or 11
  *   "batch_grid"      persistent batch scoring workgroups
  *   "batch_graph"     1 = capture each batch shape in a hipGraph and replay

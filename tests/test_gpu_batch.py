@@ -118,7 +118,8 @@ def test_batch_variants_agree(gpu):
     qs = make_queries(rng, 200, d, [16, 64, 256])
     ref = oracle_batch(x, n, qs)
     with topk_amd.Engine(x) as eng:
-        for vpt, bs, qpi, digit in [(4, 256, 64, 11), (8, 512, 32, 8), (16, 256, 16, 11), (8, 256, 64, 8)]:
+        for vpt, bs, qpi, digit in [(4, 256, 64, 11), (8, 512, 32, 8), (16, 256, 16, 11), (8, 256, 256, 8),
+                                    (8, 256, 1024, 11)]:
             for name, v in (("batch_vpt", vpt), ("batch_bs", bs), ("batch_qpi", qpi), ("batch_digit", digit),
                             ("batch_grid", 300)):
                 eng.set_tuning(name, v)
@@ -169,4 +170,4 @@ def test_batch_errors(gpu):
             eng.batch([(2000, np.ones(4, np.float32), np.array([0], np.uint8))])
         assert e.value.status == 4
         with pytest.raises(topk_amd.TopkError):
-            eng.set_tuning("batch_qpi", 48)
+            eng.set_tuning("batch_qpi", 8)

@@ -121,6 +121,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                           "or `make -C topk_amd/csrc` (there is no CPU fallback)")
+    # A process that also uses PyTorch must share its HIP runtime: load
+    # torch's first so the library binds to the same libamdhip64 (two
+    # runtimes in one process leave the second without a usable device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

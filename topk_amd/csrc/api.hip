@@ -415,7 +415,7 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
         if (value != 0 && value != 256 && value != 512) return TOPK_ERR_INVALID;
         ctx->tune.batch_bs = (int)value;
     } else if (key == "batch_qpi") {
-        if (value != 0 && value != 16 && value != 32 && value != 64) return TOPK_ERR_INVALID;
+        if (value != 0 && (value < 16 || value > 1024)) return TOPK_ERR_INVALID;
         ctx->tune.batch_qpi = (int)value;
     } else if (key == "batch_digit") {
         if (value != 0 && value != 8 && value != 11) return TOPK_ERR_INVALID;

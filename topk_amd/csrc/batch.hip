@@ -57,13 +57,16 @@ constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed score
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 // Device counters of one batch: [0] work items, [1] work ticket,
-// [2..3] scored (partition, query) pairs, [4..5] scored (object, query) pairs.
+// [2..3] scored (partition, query) pairs, [4..5] scored (object, query)
+// pairs, [6] (query, group) pairs of the mask's first level.
 constexpr uint32_t BCTR_WORDS = 8;
 
+// Work item: partition p scored for the queries whose bits are set in its
+// bitmap between query indices [qbeg, qend) (at most the item budget).
 struct BatchItem {
-    uint32_t p;                 // partition
-    uint32_t q0;                // query of bit 0
-    unsigned long long mask;    // queries q0 + b for the set bits b
+    uint32_t p;
+    uint32_t qbeg, qend;
+    uint32_t pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -79,37 +82,18 @@ __global__ __launch_bounds__(BAUX_BS) void batch_zero_kernel(unsigned long long 
     if (i0 < BCTR_WORDS) ctr[i0] = 0u;
 }
 
-// bnd[q * L + p] = LP bound of query q over partition p (-inf when empty).
-__global__ __launch_bounds__(BAUX_BS) void batch_bounds_kernel(const BoundArgs *__restrict__ qb, uint32_t L,
-                                                               const uint32_t *__restrict__ psz,
-                                                               const float *__restrict__ pmax,
-                                                               const float *__restrict__ pmin,
-                                                               const float *__restrict__ psum,
-                                                               const float *__restrict__ pminsum,
-                                                               float *__restrict__ bnd) {
-    const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x, q = blockIdx.y;
-    if (p >= L) return;
-    const BoundArgs &b = qb[q];
-    float ub = -INFINITY;
-    if (psz[p]) ub = lp_bound(b, pmax + (uint64_t)p * b.d, pmin + (uint64_t)p * b.d, psum[p], pminsum[p]);
-    bnd[(uint64_t)q * L + p] = ub;
-}
-
-// Bound keys of one query's partitions (0 for empty partitions), uniform
-// trip count so the gather's ballots see every lane.
-template <int BS>
-struct BoundKeys {
-    const float *b;
-    const uint32_t *psz;
-    uint32_t L;
-    template <class F>
-    __device__ __forceinline__ void for_each(F f) const {
-        for (uint32_t i0 = 0; i0 < L; i0 += BS) {
-            const uint32_t i = i0 + threadIdx.x;
-            f(i < L && psz[i] ? rkey(b[i], i) : 0ull);
-        }
-    }
+// Bound tree of the partition layout as the batch kernels read it.
+struct TreeArgs {
+    const float *tmax, *tmin, *tsum, *tminsum;
+    uint32_t p2, d;
 };
+
+// Upper bound of query b's score over tree node v (-inf for an empty node).
+__device__ __forceinline__ float node_bound(const BoundArgs &b, const TreeArgs &t, uint32_t v) {
+    const float ts = t.tsum[v];
+    if (ts == -INFINITY) return -INFINITY;
+    return lp_bound(b, t.tmax + (uint64_t)v * t.d, t.tmin + (uint64_t)v * t.d, ts, t.tminsum[v]);
+}
 
 // Score of the object at partition-order position pos, terms in query order.
 __device__ __forceinline__ float batch_score_one(const BatchQuery &Q, const float *__restrict__ pcols, uint64_t ld,
@@ -122,29 +106,66 @@ __device__ __forceinline__ float batch_score_one(const BatchQuery &Q, const floa
     return s;
 }
 
-// Seed pass, one workgroup per query: the 4 partitions with the largest
-// bound keys, the first objects of each (16384 in all), and the exact k-th
-// largest key among them; its score is tau.  Fewer than k objects: tau =
-// -inf (every object is a candidate).
-__global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__restrict__ qs, const float *__restrict__ bnd,
-                                                              uint32_t L, uint32_t nonempty,
-                                                              const float *__restrict__ pcols, uint64_t ld,
-                                                              const uint32_t *__restrict__ poff,
+// Seed pass, one workgroup per query.  Wave 0 walks the bound tree from the
+// root keeping the 4 children with the largest bounds at every level (a
+// beam search); the first objects of the 4 partitions it reaches (16384 in
+// all) are scored, and the exact k-th largest key among them is a lower
+// bound of the query's k-th key: its score is tau.  Fewer than k objects:
+// tau = -inf (every object is a candidate).
+__global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__restrict__ qs,
+                                                              const BoundArgs *__restrict__ qb, TreeArgs t,
+                                                              uint32_t L, const float *__restrict__ pcols,
+                                                              uint64_t ld, const uint32_t *__restrict__ poff,
                                                               const uint32_t *__restrict__ psz,
                                                               const uint32_t *__restrict__ pid) {
     __shared__ SelectScratch ss;
-    __shared__ uint64_t best[BSEED_T];
+    __shared__ uint32_t best[BSEED_T];
+    __shared__ uint32_t nbest;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     const BatchQuery &Q = qs[q];
-    const BoundKeys<BSEED_BS> src{bnd + (uint64_t)q * L, psz, L};
-    const uint32_t T = nonempty < BSEED_T ? nonempty : BSEED_T;
-    const uint64_t kth = block_kth_largest<BSEED_BS>(src, nonempty, T, ss);
-    block_gather_ge<BSEED_BS>(src, kth, best, ss);  // exactly T keys (keys are distinct)
-    const uint32_t per = BSEED_OBJS / T;
+    if (tid < 64) {
+        const BoundArgs &b = qb[q];
+        const uint32_t lane = tid;
+        uint32_t beam[BSEED_T] = {1u, 0u, 0u, 0u};
+        uint32_t nb = 1;
+        while (beam[0] < t.p2) {  // all beam nodes sit on one level
+            uint64_t key = 0;
+            if (lane < 2 * nb) {
+                uint32_t parent = beam[0];
+#pragma unroll
+                for (uint32_t i = 1; i < BSEED_T; i++) parent = lane / 2 == i ? beam[i] : parent;
+                const uint32_t c = 2 * parent + (lane & 1);
+                const float ub = node_bound(b, t, c);
+                if (ub > -INFINITY) key = ((uint64_t)okey(ub) << 32) | c;
+            }
+            uint32_t got = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < BSEED_T; r++) {
+                const uint64_t m = wave_max_u64(key);
+                if (m == 0) break;
+                beam[r] = (uint32_t)(m & 0xFFFFFFFFu);
+                key = key == m ? 0ull : key;
+                got++;
+            }
+            nb = got;
+            if (!nb) break;
+        }
+        if (lane == 0) {
+            uint32_t c = 0;
+            for (uint32_t i = 0; i < nb; i++) {
+                const uint32_t p = beam[i] - t.p2;
+                if (beam[i] >= t.p2 && p < L && psz[p]) best[c++] = p;
+            }
+            nbest = c;
+        }
+    }
+    __syncthreads();
+    const uint32_t T = nbest;
+    const uint32_t per = T ? BSEED_OBJS / T : 0u;
     uint32_t start[BSEED_T], lim[BSEED_T], total = 0;
 #pragma unroll
     for (uint32_t i = 0; i < BSEED_T; i++) {
-        const uint32_t p = i < T ? rkey_id(best[i]) : 0u;
+        const uint32_t p = i < T ? best[i] : 0u;
         start[i] = i < T ? poff[p] : 0u;
         lim[i] = i < T ? (psz[p] < per ? psz[p] : per) : 0u;
         total += lim[i];
@@ -153,7 +174,7 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 #pragma unroll
     for (int j = 0; j < BSEED_KPT; j++) {
         const uint32_t g = j * BSEED_BS + tid;
-        const uint32_t i = g / per, o = g - i * per;
+        const uint32_t i = per ? g / per : BSEED_T, o = per ? g - i * per : 0u;
         uint64_t key = 0;
         if (i < T && o < lim[i]) {
             const uint64_t pos = (uint64_t)start[i] + o;
@@ -166,69 +187,121 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
     if (tid == 0) qs[q].tau = tau;
 }
 
-// Sets bit q of masks[p * nw + q / 64] for every pair with bound >= tau_q.
-template <bool STORED>
-__global__ __launch_bounds__(BAUX_BS) void batch_mask_kernel(const BatchQuery *__restrict__ qs,
-                                                             const BoundArgs *__restrict__ qb,
-                                                             const float *__restrict__ bnd, uint32_t L, uint32_t nw,
-                                                             const uint32_t *__restrict__ psz,
-                                                             const float *__restrict__ pmax,
-                                                             const float *__restrict__ pmin,
-                                                             const float *__restrict__ psum,
-                                                             const float *__restrict__ pminsum,
-                                                             unsigned long long *__restrict__ masks) {
-    const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x, q = blockIdx.y;
-    if (p >= L || !psz[p]) return;
-    const float tau = qs[q].tau;
-    float ub;
-    if constexpr (STORED) {
-        ub = bnd[(uint64_t)q * L + p];
-    } else {
-        const BoundArgs &b = qb[q];
-        ub = lp_bound(b, pmax + (uint64_t)p * b.d, pmin + (uint64_t)p * b.d, psum[p], pminsum[p]);
+// Level 1 of the mask: tree nodes of height H (groups of 2^H partitions,
+// nodes G..2G-1) whose bound reaches tau_q become (query, group) pairs.
+__global__ __launch_bounds__(BAUX_BS) void batch_group_kernel(const BatchQuery *__restrict__ qs,
+                                                              const BoundArgs *__restrict__ qb, TreeArgs t,
+                                                              uint32_t G, uint2 *__restrict__ pairs,
+                                                              unsigned int *__restrict__ ctr) {
+    const uint32_t g = blockIdx.x * BAUX_BS + threadIdx.x, q = blockIdx.y;
+    bool pass = false;
+    if (g < G) {
+        const float ub = node_bound(qb[q], t, G + g);
+        pass = ub > -INFINITY && ub >= qs[q].tau;
     }
-    if (ub >= tau) atomicOr(&masks[(uint64_t)p * nw + q / 64], 1ull << (q % 64));
+    const uint64_t bal = __ballot(pass);
+    if (!bal) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(&ctr[6], (uint32_t)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (pass) pairs[base + __popcll(bal & ((1ull << lane_id()) - 1ull))] = make_uint2(q, g);
 }
 
-// Non-empty groups of qpi queries of every partition bitmap become items
-// (one global atomic per workgroup), with the pair counters updated.
+// Level 2: one wave per (query, group) pair, lane l tests partition
+// g * 2^H + l; a partition whose bound reaches tau_q sets bit q of
+// masks[p * nw + q / 64].
+__global__ __launch_bounds__(BAUX_BS) void batch_leaf_kernel(const BatchQuery *__restrict__ qs,
+                                                             const BoundArgs *__restrict__ qb, TreeArgs t,
+                                                             uint32_t H, uint32_t L, uint32_t nw,
+                                                             const uint2 *__restrict__ pairs,
+                                                             const unsigned int *__restrict__ ctr,
+                                                             unsigned long long *__restrict__ masks) {
+    const uint32_t npairs = ctr[6];
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (BAUX_BS / 64);
+    for (uint32_t i = blockIdx.x * (BAUX_BS / 64) + threadIdx.x / 64; i < npairs; i += nwaves) {
+        const uint2 pr = pairs[i];
+        const uint32_t q = pr.x;
+        const uint32_t p = (pr.y << H) + lane;
+        if (lane < (1u << H) && p < L) {
+            const float ub = node_bound(qb[q], t, t.p2 + p);
+            if (ub > -INFINITY && ub >= qs[q].tau) atomicOr(&masks[(uint64_t)p * nw + q / 64], 1ull << (q % 64));
+        }
+    }
+}
+
+// Position just past the t-th (1-based) set bit of m (t <= popcount(m)).
+__device__ __forceinline__ uint32_t after_nth_bit(unsigned long long m, uint32_t t) {
+    for (uint32_t i = 1; i < t; i++) m &= m - 1ull;
+    return (uint32_t)__builtin_ctzll(m) + 1u;
+}
+
+// One thread per partition: its bitmap is cut into items of at most qmax
+// queries (cuts may fall inside a 64-query word); one global atomic per
+// workgroup reserves the items, and the pair counters are updated.
 __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned long long *__restrict__ masks,
-                                                                uint64_t nmask, uint32_t nw, uint32_t qpi,
+                                                                uint32_t L, uint32_t nw, uint32_t nq, uint32_t qmax,
                                                                 const uint32_t *__restrict__ psz,
                                                                 BatchItem *__restrict__ items,
                                                                 unsigned int *__restrict__ ctr) {
     __shared__ uint32_t wsum[BAUX_BS / 64];
     __shared__ uint32_t total, base;
-    const uint32_t nch = 64 / qpi;
-    const uint64_t t = (uint64_t)blockIdx.x * BAUX_BS + threadIdx.x;
-    unsigned long long sub = 0;
-    uint64_t pw = 0;
-    uint32_t c = 0;
-    if (t < nmask * nch) {
-        pw = t / nch;
-        c = (uint32_t)(t - pw * nch);
-        const unsigned long long cm = qpi == 64 ? ~0ull : (((1ull << qpi) - 1ull) << (c * qpi));
-        sub = masks[pw] & cm;
+    const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x;
+    const unsigned long long *mp = masks + (uint64_t)p * nw;
+    uint32_t nitems = 0, pairs = 0;
+    if (p < L) {
+        uint32_t run = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+            uint32_t c = (uint32_t)__popcll(mp[w]);
+            pairs += c;
+            while (c) {
+                const uint32_t take = c < qmax - run ? c : qmax - run;
+                run += take;
+                c -= take;
+                if (run == qmax) {
+                    nitems++;
+                    run = 0;
+                }
+            }
+        }
+        nitems += run ? 1u : 0u;
     }
-    const uint32_t flag = sub ? 1u : 0u;
-    const uint32_t pos = block_excl_scan<BAUX_BS>(flag, wsum, &total);
-    const uint32_t np = (uint32_t)__popcll(sub);
-    const uint32_t pairs = wave_sum_u32(np);
-    unsigned long long objs = (unsigned long long)np * (sub ? psz[pw / nw] : 0u);
+    const uint32_t off = block_excl_scan<BAUX_BS>(nitems, wsum, &total);
+    const uint32_t wp = wave_sum_u32(pairs);
+    unsigned long long objs = (unsigned long long)pairs * (p < L && pairs ? psz[p] : 0u);
     for (int o = 32; o > 0; o >>= 1) objs += __shfl_xor(objs, o, 64);
     if (threadIdx.x == 0) base = total ? atomicAdd(&ctr[0], total) : 0u;
-    if (lane_id() == 0 && pairs) {
-        atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), (unsigned long long)pairs);
+    if (lane_id() == 0 && wp) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), (unsigned long long)wp);
         atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 4), objs);
     }
     __syncthreads();
-    if (flag) {
-        BatchItem it;
-        it.p = (uint32_t)(pw / nw);
-        it.q0 = (uint32_t)(pw % nw) * 64u;
-        it.mask = sub;
-        items[base + pos] = it;
+    if (!nitems) return;
+    BatchItem *out = items + base + off;
+    uint32_t run = 0, beg = 0, k = 0;
+    bool open = false;
+    for (uint32_t w = 0; w < nw; w++) {
+        unsigned long long m = mp[w];
+        while (m) {
+            if (!open) {
+                beg = w * 64 + (uint32_t)__builtin_ctzll(m);
+                open = true;
+            }
+            const uint32_t c = (uint32_t)__popcll(m);
+            if (run + c < qmax) {
+                run += c;
+                m = 0;
+            } else {
+                const uint32_t take = qmax - run;
+                const uint32_t endbit = after_nth_bit(m, take);
+                out[k++] = BatchItem{p, beg, w * 64 + endbit, 0};
+                m = endbit >= 64 ? 0ull : (m & (~0ull << endbit));
+                run = 0;
+                open = false;
+            }
+        }
     }
+    if (open) out[k++] = BatchItem{p, beg, nq, 0};
 }
 
 struct ScoreArgs {
@@ -237,6 +310,8 @@ struct ScoreArgs {
     uint32_t d;
     const uint32_t *poff, *psz, *pid;
     const BatchItem *items;
+    const unsigned long long *masks;
+    uint32_t nw;
     unsigned int *ctr;
     const BatchQuery *q;
     uint64_t *cand;
@@ -300,11 +375,14 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     for (int e = 0; e < VPT; e++) x[c][e] = 0.0f;
                 }
             }
-            unsigned long long mask = I.mask;
+            for (uint32_t wd = I.qbeg / 64; wd * 64 < I.qend; wd++) {
+            unsigned long long mask = a.masks[(uint64_t)I.p * a.nw + wd];
+            if (wd * 64 < I.qbeg) mask &= ~0ull << (I.qbeg - wd * 64);
+            if (I.qend < wd * 64 + 64) mask &= (1ull << (I.qend - wd * 64)) - 1ull;
             while (mask) {
                 const uint32_t bit = (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1ull;
-                const uint32_t q = I.q0 + bit;
+                const uint32_t q = wd * 64 + bit;
                 const BatchQuery &Q = a.q[q];
                 float s[VPT];
 #pragma unroll
@@ -340,6 +418,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                         }
                     }
                 }
+            }
             }
         }
     }
@@ -404,7 +483,7 @@ struct BatchState {
     uint64_t gen = ~0ull;   // layout generation the buffers and graphs belong to
     uint32_t nq_cap = 0;
     size_t in_cap = 0, out_cap = 0;
-    uint64_t mask_cap = 0, item_cap = 0, bnd_cap = 0;
+    uint64_t mask_cap = 0, item_cap = 0, pair_cap = 0;
     unsigned char *d_in = nullptr, *h_in = nullptr;    // BatchQuery[nq] | BoundArgs[nq]
     unsigned char *d_out = nullptr, *h_out = nullptr;  // ids | scores | thr | status | cnt | ctr
     unsigned long long *d_masks = nullptr;
@@ -412,7 +491,7 @@ struct BatchState {
     unsigned int *d_ctr = nullptr;
     unsigned int *d_ccount = nullptr;
     uint64_t *d_cand = nullptr;
-    float *d_bnd = nullptr;
+    uint2 *d_pairs = nullptr;   // (query, group) pairs of the mask's first level
     int cus = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
@@ -460,7 +539,7 @@ void batch_free(topk_ctx *ctx) {
     hipFree(B->d_ctr);
     hipFree(B->d_ccount);
     hipFree(B->d_cand);
-    hipFree(B->d_bnd);
+    hipFree(B->d_pairs);
     if (B->h_in) hipHostFree(B->h_in);
     if (B->h_out) hipHostFree(B->h_out);
     delete B;
@@ -479,7 +558,14 @@ static int grow_dev(T *&p, uint64_t &cap, uint64_t need, bool &changed) {
     return TOPK_OK;
 }
 
-static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, bool need_bnd) {
+// Tree height of the mask's first level: groups of up to 64 partitions.
+static uint32_t group_height(uint32_t p2) {
+    uint32_t h = 0;
+    while (h < 6 && (1u << (h + 1)) <= p2) h++;
+    return h;
+}
+
+static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks) {
     const PtaLayout &P = ctx->pta;
     bool changed = false;
     if (B.gen != P.generation) {
@@ -522,8 +608,8 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, bool n
         B.out_cap = out_need;
     }
     int rc = grow_dev(B.d_masks, B.mask_cap, (uint64_t)L * nw, changed);
-    if (!rc) rc = grow_dev(B.d_items, B.item_cap, (uint64_t)L * nw * 4, changed);  // qpi >= 16: 4 items per word
-    if (!rc && need_bnd) rc = grow_dev(B.d_bnd, B.bnd_cap, (uint64_t)L * nq, changed);
+    if (!rc) rc = grow_dev(B.d_items, B.item_cap, (uint64_t)L * (4 * nw + 1), changed);  // budget >= 16 queries
+    if (!rc) rc = grow_dev(B.d_pairs, B.pair_cap, (uint64_t)(P.p2 >> group_height(P.p2)) * nq, changed);
     if (rc) return rc;
     if (!B.d_ctr) {
         changed = true;
@@ -544,7 +630,7 @@ static BatchVariant pick_variant(const topk_ctx *ctx, const BatchState &B) {
     BatchVariant v;
     v.vpt = ctx->tune.batch_vpt ? ctx->tune.batch_vpt : (D == 8 ? 8 : (D == 16 ? 4 : 2));
     v.bs = ctx->tune.batch_bs ? ctx->tune.batch_bs : 256;
-    v.qpi = ctx->tune.batch_qpi ? ctx->tune.batch_qpi : 64;
+    v.qpi = ctx->tune.batch_qpi ? ctx->tune.batch_qpi : 128;
     v.digit = ctx->tune.batch_digit ? ctx->tune.batch_digit : 11;
     v.grid = ctx->tune.batch_grid ? ctx->tune.batch_grid : B.cus * (v.bs == 256 ? 4 : 2);
     return v;
@@ -584,26 +670,27 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     if (zg > 2048) zg = 2048;
     if (zg < 1) zg = 1;
     hipLaunchKernelGGL(batch_zero_kernel, dim3(zg), dim3(BAUX_BS), 0, s, B.d_masks, nmask, B.d_ccount, nq, B.d_ctr);
-    const dim3 pq((L + BAUX_BS - 1) / BAUX_BS, nq);
+    const TreeArgs ta{P.tmax, P.tmin, P.tsum, P.tminsum, P.p2, ctx->d};
+    hipEvent_t evt = nullptr;
+    timing_begin(ctx, s, evt);
     if (mode == BMODE_SEED) {
-        hipLaunchKernelGGL(batch_bounds_kernel, pq, dim3(BAUX_BS), 0, s, db, L, P.psz, P.pmax, P.pmin, P.psum,
-                           P.pminsum, B.d_bnd);
-        hipLaunchKernelGGL(batch_seed_kernel, dim3(nq), dim3(BSEED_BS), 0, s, dq, B.d_bnd, L, P.nonempty, P.pcols,
-                           P.ld, P.poff, P.psz, P.pid);
-        hipLaunchKernelGGL(batch_mask_kernel<true>, pq, dim3(BAUX_BS), 0, s, dq, db, B.d_bnd, L, nw, P.psz, P.pmax,
-                           P.pmin, P.psum, P.pminsum, B.d_masks);
-    } else {
-        if (mode == BMODE_LEARNED) {
-            const int rc = learned_tau_enqueue(ctx, s, dq, nq);
-            if (rc) return rc;
-        }
-        hipLaunchKernelGGL(batch_mask_kernel<false>, pq, dim3(BAUX_BS), 0, s, dq, db, nullptr, L, nw, P.psz, P.pmax,
-                           P.pmin, P.psum, P.pminsum, B.d_masks);
+        hipLaunchKernelGGL(batch_seed_kernel, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff,
+                           P.psz, P.pid);
+    } else if (mode == BMODE_LEARNED) {
+        const int rc = learned_tau_enqueue(ctx, s, dq, nq);
+        if (rc) return rc;
     }
-    const uint64_t nthreads = nmask * (64 / v.qpi);
-    hipLaunchKernelGGL(batch_compact_kernel, dim3((uint32_t)((nthreads + BAUX_BS - 1) / BAUX_BS)), dim3(BAUX_BS), 0,
-                       s, B.d_masks, nmask, nw, (uint32_t)v.qpi, P.psz, B.d_items, B.d_ctr);
-    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.d_ctr, dq, B.d_cand, B.d_ccount};
+    timing_end(ctx, s, mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given"),
+               evt);
+    const uint32_t H = group_height(P.p2), G = P.p2 >> H;
+    hipLaunchKernelGGL(batch_group_kernel, dim3((G + BAUX_BS - 1) / BAUX_BS, nq), dim3(BAUX_BS), 0, s, dq, db, ta, G,
+                       B.d_pairs, B.d_ctr);
+    hipLaunchKernelGGL(batch_leaf_kernel, dim3(2048), dim3(BAUX_BS), 0, s, dq, db, ta, H, L, nw, B.d_pairs, B.d_ctr,
+                       B.d_masks);
+    hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_masks, L,
+                       nw, nq, (uint32_t)v.qpi, P.psz, B.d_items, B.d_ctr);
+    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.d_masks, nw, B.d_ctr, dq, B.d_cand,
+                 B.d_ccount};
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
     launch_score(dims_class(ctx->d), v, s, sa);
@@ -706,10 +793,10 @@ struct PassTally {
 
 // Copies the finished queries of the last pass (local index i -> caller row
 // rows[i]) and collects the others: overflowed queries with their tightened
-// threshold, and queries with too few candidates.
+// threshold, and queries whose threshold left fewer than k candidates.
 static void collect(const topk_ctx *ctx, const BatchState &B, uint32_t nq, uint32_t ks, const topk_query *qs,
                     const uint32_t *rows, uint64_t *out_ids, float *out_scores, float *out_thr,
-                    std::vector<std::pair<uint32_t, float>> &reseed, std::vector<uint32_t> &fte, PassTally &t) {
+                    std::vector<std::pair<uint32_t, float>> &reseed, std::vector<uint32_t> &retry, PassTally &t) {
     const OutLayout ol = OutLayout::make(nq, ks);
     const uint64_t *ids = reinterpret_cast<const uint64_t *>(B.h_out + ol.ids_off);
     const float *sc = reinterpret_cast<const float *>(B.h_out + ol.sc_off);
@@ -728,7 +815,7 @@ static void collect(const topk_ctx *ctx, const BatchState &B, uint32_t nq, uint3
             continue;
         }
         if (st[i]) {
-            fte.push_back(row);
+            retry.push_back(row);
             continue;
         }
         const uint32_t kq = qs[row].k;
@@ -746,9 +833,32 @@ static void collect(const topk_ctx *ctx, const BatchState &B, uint32_t nq, uint3
     }
 }
 
-// Runs one batch and copies results out.  Overflowed queries get a second
-// device pass with the threshold their first pass tightened; queries the
-// device path still cannot settle go through topk_fte.
+// A second device pass over some queries of the call (caller rows `rows`):
+// seed thresholds, or the given ones when taus is non-null.  Eager launches.
+static int sub_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, const topk_query *qs, uint32_t ks,
+                    const std::vector<uint32_t> &rows, const std::vector<float> *taus, const BatchVariant &v,
+                    uint64_t *out_ids, float *out_scores, float *out_thr,
+                    std::vector<std::pair<uint32_t, float>> &reseed, std::vector<uint32_t> &retry, PassTally &t) {
+    const uint32_t n2 = (uint32_t)rows.size();
+    std::vector<topk_query> sub(n2);
+    for (uint32_t i = 0; i < n2; i++) sub[i] = qs[rows[i]];
+    int rc = stage_queries(ctx, B, n2, sub.data(), ks);
+    if (rc) return rc;
+    if (taus) {
+        BatchQuery *hq = reinterpret_cast<BatchQuery *>(B.h_in + InLayout::make(n2).q_off);
+        for (uint32_t i = 0; i < n2; i++) hq[i].tau = (*taus)[i];
+    }
+    rc = batch_enqueue(ctx, B, s, n2, ks, taus ? BMODE_GIVEN : BMODE_SEED, v);
+    if (rc) return rc;
+    TK_HIP(hipStreamSynchronize(s));
+    collect(ctx, B, n2, ks, qs, rows.data(), out_ids, out_scores, out_thr, reseed, retry, t);
+    return TOPK_OK;
+}
+
+// Runs one batch and copies results out.  A query whose threshold left
+// fewer than k candidates is re-run with seed thresholds; an overflowed
+// query with the threshold its pass tightened; what a second pass cannot
+// settle goes through topk_fte.
 static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
                      float *out_scores, float *out_thr, const BatchVariant *force) {
     if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
@@ -762,12 +872,12 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     BatchState &B = *static_cast<BatchState *>(ctx->batch);
     const int mode = resolve_mode(ctx);
     if (mode < 0) return TOPK_ERR_STATE;
-    int rc = ensure(ctx, B, nq, ks, mode == BMODE_SEED);
+    int rc = ensure(ctx, B, nq, ks);
     if (rc) return rc;
     rc = stage_queries(ctx, B, nq, qs, ks);
     if (rc) return rc;
     const BatchVariant v = force ? *force : pick_variant(ctx, B);
-    if (!variant_ok(dims_class(ctx->d), v.vpt, v.bs) || (v.qpi != 16 && v.qpi != 32 && v.qpi != 64) ||
+    if (!variant_ok(dims_class(ctx->d), v.vpt, v.bs) || v.qpi < 16 || v.qpi > 1024 ||
         (v.digit != 8 && v.digit != 11) || v.grid < 1)
         return TOPK_ERR_INVALID;
     hipStream_t s = ctx->lane[0].stream;
@@ -775,30 +885,27 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     if (rc) return rc;
     TK_HIP(hipStreamSynchronize(s));
     PassTally tally;
-    std::vector<std::pair<uint32_t, float>> reseed;
-    std::vector<uint32_t> fte;
-    collect(ctx, B, nq, ks, qs, nullptr, out_ids, out_scores, out_thr, reseed, fte, tally);
+    std::vector<std::pair<uint32_t, float>> reseed, reseed2;
+    std::vector<uint32_t> retry, retry2, fte;
+    collect(ctx, B, nq, ks, qs, nullptr, out_ids, out_scores, out_thr, reseed, retry, tally);
+    const uint32_t missed = (uint32_t)retry.size();
 
-    const uint32_t retried = (uint32_t)reseed.size();
+    if (!retry.empty()) {  // threshold too high (a learned guess): seed thresholds are exact lower bounds
+        rc = sub_pass(ctx, B, s, qs, ks, retry, nullptr, v, out_ids, out_scores, out_thr, reseed, retry2, tally);
+        if (rc) return rc;
+    }
     if (!reseed.empty()) {
-        std::vector<topk_query> sub(reseed.size());
         std::vector<uint32_t> rows(reseed.size());
+        std::vector<float> taus(reseed.size());
         for (size_t i = 0; i < reseed.size(); i++) {
             rows[i] = reseed[i].first;
-            sub[i] = qs[reseed[i].first];
+            taus[i] = reseed[i].second;
         }
-        const uint32_t n2 = (uint32_t)sub.size();
-        rc = stage_queries(ctx, B, n2, sub.data(), ks);
+        rc = sub_pass(ctx, B, s, qs, ks, rows, &taus, v, out_ids, out_scores, out_thr, reseed2, retry2, tally);
         if (rc) return rc;
-        BatchQuery *hq = reinterpret_cast<BatchQuery *>(B.h_in + InLayout::make(n2).q_off);
-        for (uint32_t i = 0; i < n2; i++) hq[i].tau = reseed[i].second;
-        rc = batch_enqueue(ctx, B, s, n2, ks, BMODE_GIVEN, v);
-        if (rc) return rc;
-        TK_HIP(hipStreamSynchronize(s));
-        std::vector<std::pair<uint32_t, float>> again;
-        collect(ctx, B, n2, ks, qs, rows.data(), out_ids, out_scores, out_thr, again, fte, tally);
-        for (const auto &r : again) fte.push_back(r.first);
     }
+    for (const auto &r : reseed2) fte.push_back(r.first);
+    for (uint32_t i : retry2) fte.push_back(i);
     for (uint32_t i : fte) {
         const topk_query &Q = qs[i];
         float t = 0;
@@ -809,8 +916,8 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     }
     topk_stats &S = ctx->stats;
     S.queries = nq;
-    S.fallback = (uint32_t)fte.size();
-    S.retried = retried;
+    S.fallback = missed + (uint32_t)fte.size();
+    S.retried = (uint32_t)reseed.size();
     S.candidates = tally.candidates;
     S.partitions_scored = tally.pairs;
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;

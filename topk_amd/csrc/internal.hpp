@@ -165,6 +165,18 @@ struct PtaLayout {
     uint64_t *pbound = nullptr;  // query scratch: bound key per partition
     uint64_t *bmax = nullptr;    // query scratch: max bound key per 256 partitions
     uint32_t *pscored = nullptr; // query scratch: partitions scored per stream block
+    // Bound tree: an implicit binary tree over partition ids padded to p2 (a
+    // power of two); node v has children 2v, 2v+1 and leaf p is node p2 + p.
+    // Leaves are numbered in k-d tree order, so a node is a contiguous,
+    // spatially grouped range of partitions (exactly a k-d subtree when the
+    // partition count is a power of two).  Per node: the union box, the
+    // largest attribute sum, and the sum of the box minima; empty nodes have
+    // tsum = -inf.
+    uint32_t p2 = 0;
+    float *tmax = nullptr;       // 2*p2 x d
+    float *tmin = nullptr;       // 2*p2 x d
+    float *tsum = nullptr;       // 2*p2
+    float *tminsum = nullptr;    // 2*p2
     double tt_build_ms = 0;
 };
 constexpr uint32_t PART_ALIGN = 256;

@@ -363,12 +363,13 @@ static int build_sketch(topk_ctx *ctx, Sketch &S) {
 
 // Device buffers of one training / evaluation run.
 struct MlpBuffers {
-    float *X = nullptr, *Y = nullptr, *scale = nullptr, *P = nullptr, *M = nullptr, *V = nullptr, *G = nullptr,
-          *lsum = nullptr, *out = nullptr;
+    float *X = nullptr, *Xs = nullptr, *Y = nullptr, *scale = nullptr, *P = nullptr, *M = nullptr, *V = nullptr,
+          *G = nullptr, *lsum = nullptr, *out = nullptr;
     uint32_t *rows = nullptr;
     BatchQuery *q = nullptr;
     ~MlpBuffers() {
         hipFree(X);
+        hipFree(Xs);
         hipFree(Y);
         hipFree(scale);
         hipFree(P);
@@ -530,7 +531,38 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
     if (ntr < 8) return TOPK_ERR_INVALID;
     std::vector<uint32_t> cal(usable.begin(), usable.begin() + ncal), tr(usable.begin() + ncal, usable.end());
 
-    // 3. parameters: uniform Glorot init, b3 = mean target
+    // 3. standardised copies of features and targets (training-set mean and
+    // deviation; constant features map to 0), folded back into the first and
+    // last layer after training so the installed network reads raw features
+    std::vector<float> hx((size_t)nq * MLP_IN);
+    TK_HIP(hipMemcpy(hx.data(), B.X, sizeof(float) * hx.size(), hipMemcpyDeviceToHost));
+    double mu[MLP_IN], sd[MLP_IN], my = 0, sy = 0;
+    for (int i = 0; i < MLP_IN; i++) {
+        double a = 0, a2 = 0;
+        for (uint32_t r : tr) {
+            const double v = hx[(size_t)r * MLP_IN + i];
+            a += v;
+            a2 += v * v;
+        }
+        mu[i] = a / tr.size();
+        const double var = a2 / tr.size() - mu[i] * mu[i];
+        sd[i] = var > 1e-12 * (mu[i] * mu[i] + 1.0) ? std::sqrt(var) : 0.0;
+    }
+    for (uint32_t r : tr) my += y[r];
+    my /= tr.size();
+    for (uint32_t r : tr) sy += (y[r] - my) * (y[r] - my);
+    sy = std::sqrt(sy / tr.size());
+    if (!(sy > 0)) sy = 1.0;
+    std::vector<float> hxs(hx.size()), ys(nq);
+    for (uint32_t r = 0; r < nq; r++) {
+        for (int i = 0; i < MLP_IN; i++)
+            hxs[(size_t)r * MLP_IN + i] = sd[i] > 0 ? (float)((hx[(size_t)r * MLP_IN + i] - mu[i]) / sd[i]) : 0.0f;
+        ys[r] = (float)((y[r] - my) / sy);
+    }
+    TK_HIP(hipMalloc(&B.Xs, sizeof(float) * hxs.size()));
+    TK_HIP(hipMemcpy(B.Xs, hxs.data(), sizeof(float) * hxs.size(), hipMemcpyHostToDevice));
+
+    // 4. parameters: uniform Glorot init
     std::vector<float> p(MLP_NP, 0.0f);
     {
         auto fill = [&](uint32_t off, uint32_t cnt, double lim) {
@@ -540,9 +572,6 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
         fill(OFF_W1, MLP_H * MLP_IN, std::sqrt(6.0 / (MLP_IN + MLP_H)));
         fill(OFF_W2, MLP_H * MLP_H, std::sqrt(6.0 / (2 * MLP_H)));
         fill(OFF_W3, MLP_H, std::sqrt(6.0 / (MLP_H + 1)));
-        double my = 0;
-        for (uint32_t i : tr) my += y[i];
-        p[OFF_B3] = (float)(my / tr.size());
     }
     const uint32_t gmax = cfg.batch / MLP_ROWS;
     TK_HIP(hipMalloc(&B.Y, sizeof(float) * nq));
@@ -553,14 +582,14 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
     TK_HIP(hipMalloc(&B.lsum, sizeof(float) * gmax));
     TK_HIP(hipMalloc(&B.rows, sizeof(uint32_t) * nq));
     TK_HIP(hipMalloc(&B.out, sizeof(float) * nq));
-    TK_HIP(hipMemcpy(B.Y, y.data(), sizeof(float) * nq, hipMemcpyHostToDevice));
+    TK_HIP(hipMemcpy(B.Y, ys.data(), sizeof(float) * nq, hipMemcpyHostToDevice));
     TK_HIP(hipMemcpy(B.P, p.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
     TK_HIP(hipMemset(B.M, 0, sizeof(float) * MLP_NP));
     TK_HIP(hipMemset(B.V, 0, sizeof(float) * MLP_NP));
     rc = set_pass_lds();
     if (rc) return rc;
 
-    // 4. mini-batch Adam
+    // 5. mini-batch Adam on the standardised data
     hipStream_t s = ctx->stream;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     double pb1 = 1.0, pb2 = 1.0;
@@ -573,7 +602,7 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
         for (uint32_t r0 = 0; r0 < ntr; r0 += cfg.batch) {
             const uint32_t nr = ntr - r0 < cfg.batch ? ntr - r0 : cfg.batch;
             if (nr < cfg.batch / 2 && r0 > 0) break;  // drop a short last batch
-            rc = mlp_pass(s, B.P, B.X, B.Y, B.rows, r0, nr, 1.0f / (float)nr, nullptr, B.G, B.lsum, true);
+            rc = mlp_pass(s, B.P, B.Xs, B.Y, B.rows, r0, nr, 1.0f / (float)nr, nullptr, B.G, B.lsum, true);
             if (rc) return rc;
             pb1 *= b1;
             pb2 *= b2;
@@ -586,7 +615,26 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
     TK_HIP(hipStreamSynchronize(s));
     const double t2 = now_ms();
 
-    // 5. calibration on held-out queries
+    // 6. fold the standardisation into W1, b1 (inputs) and W3, b3 (target)
+    TK_HIP(hipMemcpy(p.data(), B.P, sizeof(float) * MLP_NP, hipMemcpyDeviceToHost));
+    for (int j = 0; j < MLP_H; j++) {
+        double b = p[OFF_B1 + j];
+        for (int i = 0; i < MLP_IN; i++) {
+            const double w = p[OFF_W1 + j * MLP_IN + i];
+            if (sd[i] > 0) {
+                p[OFF_W1 + j * MLP_IN + i] = (float)(w / sd[i]);
+                b -= w * mu[i] / sd[i];
+            } else {
+                p[OFF_W1 + j * MLP_IN + i] = 0.0f;
+            }
+        }
+        p[OFF_B1 + j] = (float)b;
+    }
+    for (int j = 0; j < MLP_H; j++) p[OFF_W3 + j] = (float)(p[OFF_W3 + j] * sy);
+    p[OFF_B3] = (float)(p[OFF_B3] * sy + my);
+    TK_HIP(hipMemcpy(B.P, p.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice));
+
+    // 7. calibration of the installed network on held-out queries
     std::vector<uint32_t> all(tr);
     all.insert(all.end(), cal.begin(), cal.end());
     TK_HIP(hipMemcpy(B.rows, all.data(), sizeof(uint32_t) * all.size(), hipMemcpyHostToDevice));

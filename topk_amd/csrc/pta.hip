@@ -233,6 +233,60 @@ static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+static float round_down_f32(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+// Aggregates the per-partition bounds bottom-up into the bound tree.
+static int build_bound_tree(PtaLayout &P, uint32_t d, hipStream_t s) {
+    const uint32_t L = P.nparts;
+    uint32_t p2 = 1;
+    while (p2 < L) p2 <<= 1;
+    const size_t nn = 2 * (size_t)p2;
+    std::vector<float> mx(nn * d, -INFINITY), mn(nn * d, INFINITY), sm(nn, -INFINITY), ms(nn, 0.0f);
+    std::vector<uint32_t> sz(L);
+    TK_HIP(hipMemcpyAsync(mx.data() + (size_t)p2 * d, P.pmax, sizeof(float) * L * d, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipMemcpyAsync(mn.data() + (size_t)p2 * d, P.pmin, sizeof(float) * L * d, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipMemcpyAsync(sm.data() + p2, P.psum, sizeof(float) * L, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipMemcpyAsync(ms.data() + p2, P.pminsum, sizeof(float) * L, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipMemcpyAsync(sz.data(), P.psz, sizeof(uint32_t) * L, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipStreamSynchronize(s));
+    for (uint32_t p = 0; p < L; p++)
+        if (!sz[p]) {  // empty partition: identity box
+            for (uint32_t m = 0; m < d; m++) {
+                mx[(size_t)(p2 + p) * d + m] = -INFINITY;
+                mn[(size_t)(p2 + p) * d + m] = INFINITY;
+            }
+            sm[p2 + p] = -INFINITY;
+        }
+    for (size_t v = p2 - 1; v >= 1; v--) {
+        const size_t a = 2 * v, b = 2 * v + 1;
+        bool any = false;
+        for (uint32_t m = 0; m < d; m++) {
+            mx[v * d + m] = std::max(mx[a * d + m], mx[b * d + m]);
+            mn[v * d + m] = std::min(mn[a * d + m], mn[b * d + m]);
+        }
+        sm[v] = std::max(sm[a], sm[b]);
+        any = sm[v] > -INFINITY;
+        double t = 0;
+        for (uint32_t m = 0; m < d; m++) t += any ? (double)mn[v * d + m] : 0.0;
+        ms[v] = any ? round_down_f32(t) : 0.0f;
+    }
+    P.p2 = p2;
+    TK_HIP(hipMalloc(&P.tmax, sizeof(float) * nn * d));
+    TK_HIP(hipMalloc(&P.tmin, sizeof(float) * nn * d));
+    TK_HIP(hipMalloc(&P.tsum, sizeof(float) * nn));
+    TK_HIP(hipMalloc(&P.tminsum, sizeof(float) * nn));
+    TK_HIP(hipMemcpyAsync(P.tmax, mx.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemcpyAsync(P.tmin, mn.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemcpyAsync(P.tsum, sm.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemcpyAsync(P.tminsum, ms.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
+    TK_HIP(hipStreamSynchronize(s));
+    return TOPK_OK;
+}
+
 static void pta_free(PtaLayout &P) {
     hipFree(P.pcols);
     hipFree(P.pid);
@@ -245,6 +299,10 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.pbound);
     hipFree(P.bmax);
     hipFree(P.pscored);
+    hipFree(P.tmax);
+    hipFree(P.tmin);
+    hipFree(P.tsum);
+    hipFree(P.tminsum);
     const uint32_t psize = P.psize;
     const uint64_t gen = P.generation;
     P = PtaLayout();
@@ -354,6 +412,8 @@ int pta_build(topk_ctx *ctx) {
         TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
         TK_TRY(hipGetLastError());
         TK_TRY(hipStreamSynchronize(s));
+        rc = build_bound_tree(P, d, s);
+        if (rc) goto fail;
     }
 #undef TK_TRY
     cleanup();
