@@ -128,12 +128,14 @@ def learned_batched(eng, n, k, args):
     train = random_queries(rng, args.train_queries, k)
     rep = eng.learned_train(train)
     held = [eng.prepare_batch(random_queries(rng, nb, k)) for _ in range(args.held_batches)]
+    eng.set_tuning("batch_mode", 2)
+    tuning = eng.batch_autotune(held[0]) if not args.no_autotune else None
 
     def timed(steps, warmup, mode):
         eng.set_tuning("batch_mode", mode)
         for i in range(warmup):
             eng.batch_run(held[i % len(held)])
-        fb = pairs = objs = cand = 0
+        fb = pairs = objs = cand = items = 0
         t1 = time.perf_counter()
         for i in range(steps):
             eng.batch_run(held[i % len(held)])
@@ -142,12 +144,13 @@ def learned_batched(eng, n, k, args):
             pairs += st["partitions_scored"]
             objs += st["tuple_count"]
             cand += st["candidates"]
+            items += st["items"]
         el = time.perf_counter() - t1
-        return el, fb, pairs, objs, cand
+        return el, fb, pairs, objs, cand, items
 
     # seed thresholds (the method without the network), then learned
-    s_el, s_fb, s_pairs, s_objs, _ = timed(max(2, args.steps // 4), 1, 1)
-    el, fb, pairs, objs, cand = timed(args.steps, args.warmup, 2)
+    s_el, s_fb, s_pairs, s_objs, _, _ = timed(max(2, args.steps // 4), 1, 1)
+    el, fb, pairs, objs, cand, items = timed(args.steps, args.warmup, 2)
     steps = args.steps
     qps = steps * nb / el
 
@@ -189,6 +192,8 @@ def learned_batched(eng, n, k, args):
         "scored_pair_frac": objs / (steps * total_pairs),
         "scored_pair_frac_seed": s_objs / (max(2, args.steps // 4) * total_pairs),
         "candidates_per_query": cand / (steps * nb),
+        "work_items_per_batch": items / steps, "scored_partition_query_pairs_per_batch": pairs / steps,
+        "autotune": tuning,
         "threshold_stage_ms": {"learned_mlp": lrn_ms / max(1, tsteps), "seed_pass_in_second_passes": seed_ms / max(1, tsteps)},
         "layout_s": tt_layout,
         "training": {"queries": len(train), "n_train": rep["n_train"], "n_calib": rep["n_calib"],
@@ -238,6 +243,7 @@ def main():
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--train-queries", type=int, default=16384)
     ap.add_argument("--held-batches", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fte-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")

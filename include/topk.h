@@ -166,6 +166,21 @@ typedef struct topk_query {
 int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
                uint64_t *out_ids, float *out_scores, float *out_thresholds);
 
+/* Launch-shape autotuner for topk_batch (no reference counterpart; it
+ * replaces the reference's compile-time block sizes, Makefile:7-54): times
+ * the whole batch pipeline with HIP events on these queries for every
+ * scoring shape (objects per thread, workgroup size, persistent workgroups
+ * per CU), query budget per work item and select digit width, and installs
+ * the fastest in the context (as topk_set_tuning would). */
+typedef struct topk_batch_tuning {
+    uint32_t vpt, bs, qpi, digit, grid;
+    uint32_t variants;           /* shapes timed */
+    double ms_best, ms_default;  /* device time of one batch: chosen / defaults */
+} topk_batch_tuning;
+
+int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
+                        topk_batch_tuning *out);
+
 /* ---- learned threshold (no reference counterpart: it replaces PTA's
  *      running threshold, cpu/PTA.h:357-422, for topk_batch) ----
  *
@@ -180,8 +195,8 @@ typedef struct topk_learned_config {
     uint32_t epochs;          /* passes over the training queries (default 60) */
     uint32_t batch;           /* mini-batch, a multiple of 64 (default 512) */
     float lr;                 /* Adam step size (default 2e-3, decayed x0.3 / x0.1 late) */
-    float target_fallback;    /* fraction of calibration queries allowed to fall back (0.01) */
-    float calib_fraction;     /* held-out fraction used to calibrate the margin (0.1) */
+    float target_fallback;    /* fraction of calibration queries allowed to fall back (0.002) */
+    float calib_fraction;     /* held-out fraction used to calibrate the margin (0.2) */
     uint64_t seed;            /* split / init / shuffle seed (1) */
 } topk_learned_config;
 
@@ -277,7 +292,7 @@ typedef struct topk_stats {
     double ms_last;              /* host wall time of the last query call */
     double tt_init;              /* host wall time of the last topk_init, ms */
     uint32_t retried;            /* batch: queries re-run with a tightened threshold */
-    uint32_t reserved;
+    uint32_t items;              /* batch: work items (partition loads) of the first pass */
 } topk_stats;
 
 int topk_get_stats(topk_ctx *ctx, topk_stats *out);

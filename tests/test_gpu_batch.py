@@ -128,6 +128,21 @@ def test_batch_variants_agree(gpu):
                 assert np.array_equal(gi, ri) and np.array_equal(gs, rs), (vpt, bs, qpi, digit)
 
 
+def test_batch_autotune_keeps_results(gpu):
+    n, d = 600_000, 8
+    rng = np.random.default_rng(31)
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 31)
+    qs = make_queries(rng, 256, d, [32, 64, 128])
+    with topk_amd.Engine(x) as eng:
+        before = eng.batch(qs)
+        t = eng.batch_autotune(qs)
+        assert t["variants"] > 10 and 0 < t["ms_best"] <= t["ms_default"]
+        assert t["qpi"] in (64, 128, 256) and t["bs"] in (256, 512) and t["digit"] in (8, 11)
+        after = eng.batch(qs)
+        for (a, b, c), (e, f, g) in zip(before, after):
+            assert np.array_equal(a, e) and np.array_equal(b, f) and c == g
+
+
 def test_batch_graph_replays(gpu):
     """Captured-graph replays (with eager work between them) equal eager runs."""
     n, d = 500_000, 8

@@ -38,12 +38,18 @@ class TopkStats(ctypes.Structure):
                 ("partitions_scored", _c.c_uint64), ("partitions_total", _c.c_uint64),
                 ("fallback", _c.c_uint32), ("queries", _c.c_uint32),
                 ("ms_last", _c.c_double), ("tt_init", _c.c_double),
-                ("retried", _c.c_uint32), ("reserved", _c.c_uint32)]
+                ("retried", _c.c_uint32), ("items", _c.c_uint32)]
 
 
 class TopkQuery(ctypes.Structure):
     _fields_ = [("k", _c.c_uint32), ("qq", _c.c_uint32),
                 ("attr", _c.c_uint8 * TOPK_MAX_DIMS), ("weights", _c.c_float * TOPK_MAX_DIMS)]
+
+
+class TopkBatchTuning(ctypes.Structure):
+    _fields_ = [("vpt", _c.c_uint32), ("bs", _c.c_uint32), ("qpi", _c.c_uint32), ("digit", _c.c_uint32),
+                ("grid", _c.c_uint32), ("variants", _c.c_uint32), ("ms_best", _c.c_double),
+                ("ms_default", _c.c_double)]
 
 
 class TopkLearnedConfig(ctypes.Structure):
@@ -92,6 +98,7 @@ SIGNATURES = {
     "topk_generate": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint32, _c.c_uint64, _c.c_int, _c.c_uint64,
                                  _c.c_uint64, _c.c_uint64, _c.c_int]),
     "topk_radix_select_u32": (_c.c_int, [_vp, _vp, _c.c_uint64, _c.c_uint64, _c.POINTER(_c.c_uint32)]),
+    "topk_batch_autotune": (_c.c_int, [_vp, _c.c_uint32, _vp, _c.c_uint32, _c.POINTER(TopkBatchTuning)]),
     "topk_learned_default_config": (_c.c_int, [_c.POINTER(TopkLearnedConfig)]),
     "topk_learned_train": (_c.c_int, [_vp, _c.c_uint32, _vp, _c.POINTER(TopkLearnedConfig),
                                       _c.POINTER(TopkLearnedReport)]),

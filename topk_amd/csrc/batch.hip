@@ -387,15 +387,30 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 float s[VPT];
 #pragma unroll
                 for (int e = 0; e < VPT; e++) s[e] = 0.0f;
-                for (uint32_t j = 0; j < Q.m; j++) {
-                    const float w = Q.w[j];
-                    switch (Q.attr[j]) {
-                        TK_TERM(0) TK_TERM(1) TK_TERM(2) TK_TERM(3) TK_TERM(4) TK_TERM(5) TK_TERM(6) TK_TERM(7)
-                        TK_TERM(8) TK_TERM(9) TK_TERM(10) TK_TERM(11) TK_TERM(12) TK_TERM(13) TK_TERM(14)
-                        TK_TERM(15) TK_TERM(16) TK_TERM(17) TK_TERM(18) TK_TERM(19) TK_TERM(20) TK_TERM(21)
-                        TK_TERM(22) TK_TERM(23) TK_TERM(24) TK_TERM(25) TK_TERM(26) TK_TERM(27) TK_TERM(28)
-                        TK_TERM(29) TK_TERM(30) TK_TERM(31)
-                        default: break;
+                if (Q.ascending) {
+                    // static register order, zero weights skipped by a uniform branch
+#pragma unroll
+                    for (int c = 0; c < D; c++) {
+                        const float w = Q.wa[c];
+                        if (w != 0.0f) {
+#pragma unroll
+                            for (int e = 0; e < VPT; e++) {
+                                const float pr = x[c][e] * w;
+                                s[e] = s[e] + pr;
+                            }
+                        }
+                    }
+                } else {
+                    for (uint32_t j = 0; j < Q.m; j++) {
+                        const float w = Q.w[j];
+                        switch (Q.attr[j]) {
+                            TK_TERM(0) TK_TERM(1) TK_TERM(2) TK_TERM(3) TK_TERM(4) TK_TERM(5) TK_TERM(6)
+                            TK_TERM(7) TK_TERM(8) TK_TERM(9) TK_TERM(10) TK_TERM(11) TK_TERM(12) TK_TERM(13)
+                            TK_TERM(14) TK_TERM(15) TK_TERM(16) TK_TERM(17) TK_TERM(18) TK_TERM(19) TK_TERM(20)
+                            TK_TERM(21) TK_TERM(22) TK_TERM(23) TK_TERM(24) TK_TERM(25) TK_TERM(26) TK_TERM(27)
+                            TK_TERM(28) TK_TERM(29) TK_TERM(30) TK_TERM(31)
+                            default: break;
+                        }
                     }
                 }
                 const float tau = Q.tau;
@@ -755,11 +770,15 @@ int topk::make_batch_query(const topk_ctx *ctx, const topk_query &Q, BatchQuery 
     b.k = (uint32_t)(ctx->n < Q.k ? ctx->n : Q.k);
     b.kuser = Q.k;
     b.m = Q.qq;
+    b.ascending = 1;
     for (uint32_t j = 0; j < Q.qq; j++) {
         if (Q.attr[j] >= ctx->d) return TOPK_ERR_INVALID;
         b.attr[j] = Q.attr[j];
         b.w[j] = Q.weights[Q.attr[j]];
+        if (j && Q.attr[j] <= Q.attr[j - 1]) b.ascending = 0;
     }
+    if (b.ascending)
+        for (uint32_t j = 0; j < Q.qq; j++) b.wa[Q.attr[j]] = b.w[j];
     return TOPK_OK;
 }
 
@@ -885,6 +904,10 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     if (rc) return rc;
     TK_HIP(hipStreamSynchronize(s));
     PassTally tally;
+    {
+        const uint32_t *ctr = reinterpret_cast<const uint32_t *>(B.h_out + OutLayout::make(nq, ks).ctr_off);
+        ctx->stats.items = ctr[0];
+    }
     std::vector<std::pair<uint32_t, float>> reseed, reseed2;
     std::vector<uint32_t> retry, retry2, fte;
     collect(ctx, B, nq, ks, qs, nullptr, out_ids, out_scores, out_thr, reseed, retry, tally);
@@ -935,6 +958,93 @@ extern "C" {
 int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride, uint64_t *out_ids,
                float *out_scores, float *out_thresholds) {
     return batch_run(ctx, nq, qs, k_stride, out_ids, out_scores, out_thresholds, nullptr);
+}
+
+int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
+                        topk_batch_tuning *out) {
+    if (!ctx || !qs || nq == 0 || k_stride == 0) return TOPK_ERR_INVALID;
+    if (nq > BATCH_MAX_Q || k_stride > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    if (!ctx->pta.built) {
+        const int rc = pta_build(ctx);
+        if (rc) return rc;
+    }
+    if (!ctx->batch) ctx->batch = new BatchState();
+    BatchState &B = *static_cast<BatchState *>(ctx->batch);
+    const int mode = resolve_mode(ctx);
+    if (mode < 0) return TOPK_ERR_STATE;
+    int rc = ensure(ctx, B, nq, k_stride);
+    if (!rc) rc = stage_queries(ctx, B, nq, qs, k_stride);
+    if (rc) return rc;
+    const int D = dims_class(ctx->d);
+    const BatchVariant def = pick_variant(ctx, B);
+    std::vector<BatchVariant> cand;
+    static const int vpts[] = {2, 4, 8, 16}, qpis[] = {64, 128, 256}, per_cu[] = {2, 4, 8};
+    for (int vpt : vpts)
+        for (int bs : {256, 512}) {
+            if (!variant_ok(D, vpt, bs)) continue;
+            for (int qpi : qpis)
+                for (int pc : per_cu) cand.push_back(BatchVariant{vpt, bs, qpi, def.digit, B.cus * pc * 256 / bs});
+        }
+    hipStream_t s = ctx->lane[0].stream;
+    hipEvent_t e0, e1;
+    TK_HIP(hipEventCreate(&e0));
+    TK_HIP(hipEventCreate(&e1));
+    auto time_one = [&](const BatchVariant &v, double &ms) -> int {
+        ms = 1e30;
+        for (int rep = 0; rep < 3; rep++) {
+            TK_HIP(hipEventRecord(e0, s));
+            const int r = batch_enqueue(ctx, B, s, nq, k_stride, mode, v);
+            if (r) return r;
+            TK_HIP(hipEventRecord(e1, s));
+            TK_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            TK_HIP(hipEventElapsedTime(&t, e0, e1));
+            if (rep && t < ms) ms = t;  // first run warms the code and caches
+        }
+        return TOPK_OK;
+    };
+    double ms_def = 0, best_ms = 1e30;
+    BatchVariant best = def;
+    rc = time_one(def, ms_def);
+    best_ms = ms_def;
+    for (size_t i = 0; !rc && i < cand.size(); i++) {
+        double ms;
+        rc = time_one(cand[i], ms);
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best = cand[i];
+        }
+    }
+    for (int dg : {8, 11}) {  // the final select's digit width, with the best shape
+        if (rc || dg == best.digit) continue;
+        BatchVariant v = best;
+        v.digit = dg;
+        double ms;
+        rc = time_one(v, ms);
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    ctx->tune.batch_vpt = best.vpt;
+    ctx->tune.batch_bs = best.bs;
+    ctx->tune.batch_qpi = best.qpi;
+    ctx->tune.batch_digit = best.digit;
+    ctx->tune.batch_grid = best.grid;
+    if (out) {
+        out->vpt = (uint32_t)best.vpt;
+        out->bs = (uint32_t)best.bs;
+        out->qpi = (uint32_t)best.qpi;
+        out->digit = (uint32_t)best.digit;
+        out->grid = (uint32_t)best.grid;
+        out->variants = (uint32_t)cand.size() + 2;
+        out->ms_best = best_ms;
+        out->ms_default = ms_def;
+    }
+    return TOPK_OK;
 }
 
 }  // extern "C"

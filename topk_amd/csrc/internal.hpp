@@ -84,6 +84,12 @@ struct BatchQuery {
     uint32_t kuser;                 // k as requested
     uint8_t attr[TOPK_MAX_DIMS];    // term j reads attribute attr[j]
     float w[TOPK_MAX_DIMS];         // weight of term j
+    // Ascending form: when attr[] is strictly ascending, the score equals
+    // the sum over all attributes a in ascending order of x[a] * wa[a] with
+    // wa[a] = 0 for attributes not queried (x * 0 is +-0 and s + (+-0) == s
+    // because a score is never -0).
+    uint32_t ascending;
+    float wa[TOPK_MAX_DIMS];
 };
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.

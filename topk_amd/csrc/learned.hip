@@ -679,8 +679,8 @@ int topk_learned_default_config(topk_learned_config *cfg) {
     cfg->epochs = 60;
     cfg->batch = 512;
     cfg->lr = 2e-3f;
-    cfg->target_fallback = 0.01f;
-    cfg->calib_fraction = 0.1f;
+    cfg->target_fallback = 0.002f;
+    cfg->calib_fraction = 0.2f;
     cfg->seed = 1;
     return TOPK_OK;
 }

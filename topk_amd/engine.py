@@ -8,8 +8,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (MLP_IN, MLP_NPARAMS, TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkLearnedConfig, TopkLearnedReport,
-                   TopkQuery, TopkStats, check, lib)
+from ._lib import (MLP_IN, MLP_NPARAMS, TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkBatchTuning, TopkLearnedConfig,
+                   TopkLearnedReport, TopkQuery, TopkStats, check, lib)
 
 _c = ctypes
 
@@ -202,6 +202,15 @@ class Engine:
         """Runs a PreparedBatch; results land in pb.ids / pb.scores / pb.thr."""
         check(lib().topk_batch(self._h, pb.nq, _c.cast(pb.qs, _c.c_void_p), pb.kmax, _ptr(pb.ids), _ptr(pb.scores),
                                _ptr(pb.thr)), "topk_batch")
+
+    def batch_autotune(self, queries):
+        """Times every batch launch shape on these queries and installs the
+        fastest; returns the choice as a dict."""
+        pb = queries if isinstance(queries, PreparedBatch) else PreparedBatch(queries)
+        t = TopkBatchTuning()
+        check(lib().topk_batch_autotune(self._h, pb.nq, _c.cast(pb.qs, _c.c_void_p), pb.kmax, _c.byref(t)),
+              "topk_batch_autotune")
+        return {f: getattr(t, f) for f, _ in TopkBatchTuning._fields_}
 
     # ---- learned threshold ----
     def learned_train(self, queries, **cfg):
