@@ -56,6 +56,8 @@ constexpr uint32_t BSEED_T = 4;                          // seed partitions per 
 constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
+// Per-query flags the first pass leaves for the second (in-graph) pass.
+enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
 // Device counters of one batch: [0] work items, [1] work ticket,
 // [2..3] scored (partition, query) pairs, [4..5] scored (object, query)
 // pairs, [6] (query, group) pairs of the mask's first level.
@@ -72,6 +74,68 @@ struct BatchItem {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// Device form of query i (one thread per query): terms in query order, the
+// ascending form, and the bound arguments -- the distinct attributes with
+// aggregated FP64 weights, heaviest first (ties by attribute id).  taus
+// (optional) gives the thresholds of a second pass.  Written field by field
+// with statically indexed loops, so nothing lands in scratch memory.
+__global__ __launch_bounds__(128) void batch_prep_kernel(const topk_query *__restrict__ raw,
+                                                         const float *__restrict__ taus, uint32_t nq, uint64_t n,
+                                                         uint32_t d, BatchQuery *__restrict__ out,
+                                                         BoundArgs *__restrict__ outb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const topk_query &Q = raw[i];
+    BatchQuery &b = out[i];
+    BoundArgs &g = outb[i];
+    const uint32_t qq = Q.qq;
+    b.tau = taus ? taus[i] : 0.0f;
+    b.tau_retry = -INFINITY;
+    b.k = (uint32_t)(n < Q.k ? n : Q.k);
+    b.kuser = Q.k;
+    b.m = qq;
+    uint32_t asc = 1, prev = 0;
+#pragma unroll
+    for (int j = 0; j < TOPK_MAX_DIMS; j++) {
+        const uint32_t a = (uint32_t)j < qq ? Q.attr[j] : 0u;
+        b.attr[j] = (uint8_t)a;
+        b.w[j] = (uint32_t)j < qq ? Q.weights[a] : 0.0f;
+        if (j && (uint32_t)j < qq && a <= prev) asc = 0;
+        prev = a;
+    }
+    b.ascending = asc;
+    // aggregated weight per attribute id (static index, compare-select)
+    double agg[TOPK_MAX_DIMS];
+    uint32_t used = 0;
+#pragma unroll
+    for (int a = 0; a < TOPK_MAX_DIMS; a++) agg[a] = 0.0;
+    for (uint32_t j = 0; j < qq; j++) {
+        const uint32_t at = Q.attr[j];
+        const double w = (double)Q.weights[at];
+        used |= 1u << at;
+#pragma unroll
+        for (int a = 0; a < TOPK_MAX_DIMS; a++) agg[a] = (uint32_t)a == at ? agg[a] + w : agg[a];
+    }
+#pragma unroll
+    for (int a = 0; a < TOPK_MAX_DIMS; a++) b.wa[a] = asc && ((used >> a) & 1u) ? Q.weights[a] : 0.0f;
+    // rank of each used attribute: heavier first, ties by lower id
+    g.d = d;
+    g.nd = (uint32_t)__popc(used);
+#pragma unroll
+    for (int a = 0; a < TOPK_MAX_DIMS; a++) {
+        if ((uint32_t)a >= d) break;
+        if (!((used >> a) & 1u)) continue;
+        uint32_t r = 0;
+#pragma unroll
+        for (int c = 0; c < TOPK_MAX_DIMS; c++) {
+            if ((uint32_t)c >= d) break;
+            r += ((used >> c) & 1u) && (agg[c] > agg[a] || (agg[c] == agg[a] && c < a)) ? 1u : 0u;
+        }
+        g.attr[r] = (uint8_t)a;
+        g.w[r] = agg[a];
+    }
+}
+
 __global__ __launch_bounds__(BAUX_BS) void batch_zero_kernel(unsigned long long *__restrict__ masks, uint64_t nmask,
                                                              unsigned int *__restrict__ ccount, uint32_t nq,
                                                              unsigned int *__restrict__ ctr) {
@@ -117,11 +181,13 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
                                                               uint32_t L, const float *__restrict__ pcols,
                                                               uint64_t ld, const uint32_t *__restrict__ poff,
                                                               const uint32_t *__restrict__ psz,
-                                                              const uint32_t *__restrict__ pid) {
+                                                              const uint32_t *__restrict__ pid,
+                                                              const uint32_t *__restrict__ qflag) {
     __shared__ SelectScratch ss;
     __shared__ uint32_t best[BSEED_T];
     __shared__ uint32_t nbest;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    if (qflag && qflag[q] != QF_SEED) return;  // second pass: only queries that need a seed
     const BatchQuery &Q = qs[q];
     if (tid < 64) {
         const BoundArgs &b = qb[q];
@@ -170,18 +236,37 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
         lim[i] = i < T ? (psz[p] < per ? psz[p] : per) : 0u;
         total += lim[i];
     }
-    RegKeys<BSEED_KPT> rk;
+    // object slot j of this thread: position pos[j] (valid when ok[j]); the
+    // terms run outside the objects so each term's loads are in flight together
+    uint32_t pos[BSEED_KPT];
+    bool ok[BSEED_KPT];
+    float sc[BSEED_KPT];
 #pragma unroll
     for (int j = 0; j < BSEED_KPT; j++) {
         const uint32_t g = j * BSEED_BS + tid;
         const uint32_t i = per ? g / per : BSEED_T, o = per ? g - i * per : 0u;
-        uint64_t key = 0;
-        if (i < T && o < lim[i]) {
-            const uint64_t pos = (uint64_t)start[i] + o;
-            key = rkey(batch_score_one(Q, pcols, ld, pos), pid[pos]);
+        uint32_t st = 0, lm = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < BSEED_T; u++) {
+            st = i == u ? start[u] : st;
+            lm = i == u ? lim[u] : lm;
         }
-        rk.k[j] = key;
+        ok[j] = i < T && o < lm;
+        pos[j] = ok[j] ? st + o : 0u;
+        sc[j] = 0.0f;
     }
+    for (uint32_t t = 0; t < Q.m; t++) {
+        const float *col = pcols + (uint64_t)Q.attr[t] * ld;
+        const float w = Q.w[t];
+#pragma unroll
+        for (int j = 0; j < BSEED_KPT; j++) {
+            const float pr = col[pos[j]] * w;
+            sc[j] = sc[j] + pr;
+        }
+    }
+    RegKeys<BSEED_KPT> rk;
+#pragma unroll
+    for (int j = 0; j < BSEED_KPT; j++) rk.k[j] = ok[j] ? rkey(sc[j], pid[pos[j]]) : 0ull;
     float tau = -INFINITY;
     if (total >= Q.k) tau = rkey_score(block_kth_largest<BSEED_BS>(rk, total, Q.k, ss));
     if (tid == 0) qs[q].tau = tau;
@@ -192,8 +277,10 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 __global__ __launch_bounds__(BAUX_BS) void batch_group_kernel(const BatchQuery *__restrict__ qs,
                                                               const BoundArgs *__restrict__ qb, TreeArgs t,
                                                               uint32_t G, uint2 *__restrict__ pairs,
-                                                              unsigned int *__restrict__ ctr) {
+                                                              unsigned int *__restrict__ ctr,
+                                                              const uint32_t *__restrict__ qflag) {
     const uint32_t g = blockIdx.x * BAUX_BS + threadIdx.x, q = blockIdx.y;
+    if (qflag && qflag[q] == QF_DONE) return;  // second pass: settled queries take no part
     bool pass = false;
     if (g < G) {
         const float ub = node_bound(qb[q], t, G + g);
@@ -251,16 +338,22 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned l
     uint32_t nitems = 0, pairs = 0;
     if (p < L) {
         uint32_t run = 0;
-        for (uint32_t w = 0; w < nw; w++) {
-            uint32_t c = (uint32_t)__popcll(mp[w]);
-            pairs += c;
-            while (c) {
-                const uint32_t take = c < qmax - run ? c : qmax - run;
-                run += take;
-                c -= take;
-                if (run == qmax) {
-                    nitems++;
-                    run = 0;
+        for (uint32_t w0 = 0; w0 < nw; w0 += 8) {
+            unsigned long long mw[8];  // 8 independent loads in flight
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) mw[u] = w0 + u < nw ? mp[w0 + u] : 0ull;
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) {
+                uint32_t c = (uint32_t)__popcll(mw[u]);
+                pairs += c;
+                while (c) {
+                    const uint32_t take = c < qmax - run ? c : qmax - run;
+                    run += take;
+                    c -= take;
+                    if (run == qmax) {
+                        nitems++;
+                        run = 0;
+                    }
                 }
             }
         }
@@ -280,8 +373,15 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned l
     BatchItem *out = items + base + off;
     uint32_t run = 0, beg = 0, k = 0;
     bool open = false;
+    unsigned long long mw[8];
     for (uint32_t w = 0; w < nw; w++) {
-        unsigned long long m = mp[w];
+        if ((w & 7) == 0) {
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) mw[u] = w + u < nw ? mp[w + u] : 0ull;
+        }
+        unsigned long long m = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) m = (w & 7) == u ? mw[u] : m;
         while (m) {
             if (!open) {
                 beg = w * 64 + (uint32_t)__builtin_ctzll(m);
@@ -441,7 +541,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
 #undef TK_TERM
 
 struct FinalArgs {
-    const BatchQuery *q;
+    BatchQuery *q;
     const uint64_t *cand;
     const unsigned int *ccount;
     const unsigned int *ctr;
@@ -452,7 +552,8 @@ struct FinalArgs {
     uint32_t *status;
     uint32_t *cnt;
     uint32_t *ctr_out;
-    float *tau_next;
+    uint32_t *qflag;   // first pass: written; second pass: read (settled queries are skipped)
+    int second;
 };
 
 template <int DB>
@@ -461,25 +562,45 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     __shared__ uint64_t sorted[TOPK_MAX_K];
     __shared__ SelectScratch ss;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    const uint32_t c = a.ccount[q], k = a.q[q].k;
     if (q == 0 && tid < BCTR_WORDS) a.ctr_out[tid] = a.ctr[tid];
+    if (a.second && a.qflag[q] == QF_DONE) return;
+    const uint32_t c = a.ccount[q], k = a.q[q].k;
     if (tid == 0) a.cnt[q] = c;
     if (c > BATCH_CAP) {
+        if (a.second) {
+            if (tid == 0) a.status[q] = STATUS_OVERFLOW;
+            return;
+        }
         // The stored keys are a subset of the objects, so their k-th largest
-        // is a lower bound of the query's k-th key: a tighter threshold for a
-        // second pass.
+        // is a lower bound of the query's k-th key: the second pass reruns the
+        // query with that tighter threshold.
         const uint64_t kth = block_kth_largest<BFIN_BS>(MemKeys<BFIN_BS>{a.cand + (uint64_t)q * BATCH_CAP, BATCH_CAP},
                                                         BATCH_CAP, k, ss);
         if (tid == 0) {
             a.status[q] = STATUS_OVERFLOW;
-            a.tau_next[q] = rkey_score(kth);
+            a.q[q].tau = rkey_score(kth);
+            a.qflag[q] = QF_GIVEN;
         }
         return;
     }
     if (c < k) {
-        if (tid == 0) a.status[q] = STATUS_RETRY;
+        // a predicted threshold above the k-th score: the second pass reruns
+        // the query with its lower retry threshold, or seeds it
+        if (tid == 0) {
+            a.status[q] = STATUS_RETRY;
+            if (!a.second) {
+                const float tr = a.q[q].tau_retry;
+                if (tr < a.q[q].tau) {
+                    a.q[q].tau = tr;
+                    a.qflag[q] = QF_LOWER;
+                } else {
+                    a.qflag[q] = QF_SEED;
+                }
+            }
+        }
         return;
     }
+    if (tid == 0 && !a.second) a.qflag[q] = QF_DONE;
     const FinalOut o{a.ids + (uint64_t)q * a.kstride, a.scores + (uint64_t)q * a.kstride, a.thr + q, a.status + q};
     select_and_emit<BFIN_BS, BFIN_KPT, FlatView, DB>(FlatView{a.cand + (uint64_t)q * BATCH_CAP}, c, k, o, top, sorted,
                                                      ss);
@@ -511,19 +632,26 @@ struct BatchState {
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
+// Staged inputs: the caller's topk_query records and given thresholds are
+// copied from pinned memory (the first `host` bytes); the device builds the
+// BatchQuery / BoundArgs forms behind them.
 struct InLayout {
-    size_t q_off, b_off, bytes;
+    size_t raw_off, tau_off, host, q_off, b_off, bytes;
     static InLayout make(uint32_t nq) {
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
         InLayout l;
-        l.q_off = 0;
-        l.b_off = (sizeof(BatchQuery) * nq + 255) & ~(size_t)255;
+        l.raw_off = 0;
+        l.tau_off = al(sizeof(topk_query) * nq);
+        l.host = l.tau_off + sizeof(float) * nq;
+        l.q_off = al(l.host);
+        l.b_off = al(l.q_off + sizeof(BatchQuery) * nq);
         l.bytes = l.b_off + sizeof(BoundArgs) * nq;
         return l;
     }
 };
 
 struct OutLayout {
-    size_t ids_off, sc_off, thr_off, st_off, cnt_off, tn_off, ctr_off, bytes;
+    size_t ids_off, sc_off, thr_off, st_off, cnt_off, fl_off, ctr_off, ctr2_off, bytes;
     static OutLayout make(uint32_t nq, uint32_t ks) {
         OutLayout l;
         l.ids_off = 0;
@@ -531,9 +659,10 @@ struct OutLayout {
         l.thr_off = l.sc_off + ((sizeof(float) * nq * ks + 15) & ~(size_t)15);
         l.st_off = l.thr_off + sizeof(float) * nq;
         l.cnt_off = l.st_off + sizeof(uint32_t) * nq;
-        l.tn_off = l.cnt_off + sizeof(uint32_t) * nq;
-        l.ctr_off = (l.tn_off + sizeof(float) * nq + 15) & ~(size_t)15;
-        l.bytes = l.ctr_off + sizeof(uint32_t) * BCTR_WORDS;
+        l.fl_off = l.cnt_off + sizeof(uint32_t) * nq;
+        l.ctr_off = (l.fl_off + sizeof(uint32_t) * nq + 15) & ~(size_t)15;
+        l.ctr2_off = l.ctr_off + sizeof(uint32_t) * BCTR_WORDS;
+        l.bytes = l.ctr2_off + sizeof(uint32_t) * BCTR_WORDS;
         return l;
     }
 };
@@ -670,17 +799,20 @@ static void launch_score(int D, const BatchVariant &v, hipStream_t s, const Scor
 #undef TK_SC
 }
 
-// Enqueues the whole batch pipeline (staged inputs already in h_in) on s.
-static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
-                         const BatchVariant &v) {
+// Enqueues one scoring pass: zero, threshold stage, two-level mask,
+// compaction, scoring, final select.  The second pass works only on the
+// queries the first one flagged.
+static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
+                        const BatchVariant &v, bool second) {
     const PtaLayout &P = ctx->pta;
     const uint32_t L = P.nparts, nw = (nq + 63) / 64;
     const uint64_t nmask = (uint64_t)L * nw;
     const InLayout il = InLayout::make(nq);
     const OutLayout ol = OutLayout::make(nq, ks);
     BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
-    const BoundArgs *db = reinterpret_cast<const BoundArgs *>(B.d_in + il.b_off);
-    TK_HIP(hipMemcpyAsync(B.d_in, B.h_in, il.bytes, hipMemcpyHostToDevice, s));
+    BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
+    uint32_t *qflag = reinterpret_cast<uint32_t *>(B.d_out + ol.fl_off);
+    const uint32_t *pass_flags = second ? qflag : nullptr;
     uint32_t zg = (uint32_t)((nmask + BAUX_BS - 1) / BAUX_BS);
     if (zg > 2048) zg = 2048;
     if (zg < 1) zg = 1;
@@ -688,18 +820,19 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     const TreeArgs ta{P.tmax, P.tmin, P.tsum, P.tminsum, P.p2, ctx->d};
     hipEvent_t evt = nullptr;
     timing_begin(ctx, s, evt);
-    if (mode == BMODE_SEED) {
+    if (mode == BMODE_SEED || second) {
         hipLaunchKernelGGL(batch_seed_kernel, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff,
-                           P.psz, P.pid);
+                           P.psz, P.pid, pass_flags);
     } else if (mode == BMODE_LEARNED) {
         const int rc = learned_tau_enqueue(ctx, s, dq, nq);
         if (rc) return rc;
     }
-    timing_end(ctx, s, mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given"),
+    timing_end(ctx, s,
+               second ? "batch_tau_second" : (mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given")),
                evt);
     const uint32_t H = group_height(P.p2), G = P.p2 >> H;
     hipLaunchKernelGGL(batch_group_kernel, dim3((G + BAUX_BS - 1) / BAUX_BS, nq), dim3(BAUX_BS), 0, s, dq, db, ta, G,
-                       B.d_pairs, B.d_ctr);
+                       B.d_pairs, B.d_ctr, pass_flags);
     hipLaunchKernelGGL(batch_leaf_kernel, dim3(2048), dim3(BAUX_BS), 0, s, dq, db, ta, H, L, nw, B.d_pairs, B.d_ctr,
                        B.d_masks);
     hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_masks, L,
@@ -709,7 +842,7 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
     launch_score(dims_class(ctx->d), v, s, sa);
-    timing_end(ctx, s, "batch_score", ev0);
+    timing_end(ctx, s, second ? "batch_score_second" : "batch_score", ev0);
     FinalArgs fa{dq,
                  B.d_cand,
                  B.d_ccount,
@@ -720,13 +853,33 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
                  reinterpret_cast<float *>(B.d_out + ol.thr_off),
                  reinterpret_cast<uint32_t *>(B.d_out + ol.st_off),
                  reinterpret_cast<uint32_t *>(B.d_out + ol.cnt_off),
-                 reinterpret_cast<uint32_t *>(B.d_out + ol.ctr_off),
-                 reinterpret_cast<float *>(B.d_out + ol.tn_off)};
+                 reinterpret_cast<uint32_t *>(B.d_out + (second ? ol.ctr2_off : ol.ctr_off)),
+                 qflag,
+                 second ? 1 : 0};
     if (v.digit == 8)
         hipLaunchKernelGGL(batch_final_kernel<8>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
     else
         hipLaunchKernelGGL(batch_final_kernel<11>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
     TK_HIP(hipGetLastError());
+    return TOPK_OK;
+}
+
+// Enqueues a whole batch on s: staged inputs to the device, the first pass,
+// the second pass for flagged queries, results to the pinned mirror.
+static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
+                         const BatchVariant &v) {
+    const InLayout il = InLayout::make(nq);
+    const OutLayout ol = OutLayout::make(nq, ks);
+    BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
+    BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
+    TK_HIP(hipMemcpyAsync(B.d_in, B.h_in, il.host, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(batch_prep_kernel, dim3((nq + 127) / 128), dim3(128), 0, s,
+                       reinterpret_cast<const topk_query *>(B.d_in + il.raw_off),
+                       mode == BMODE_GIVEN ? reinterpret_cast<const float *>(B.d_in + il.tau_off) : nullptr, nq, ctx->n,
+                       ctx->d, dq, db);
+    int rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, false);
+    if (!rc) rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, true);
+    if (rc) return rc;
     TK_HIP(hipMemcpyAsync(B.h_out, B.d_out, ol.bytes, hipMemcpyDeviceToHost, s));
     return TOPK_OK;
 }
@@ -784,18 +937,16 @@ int topk::make_batch_query(const topk_ctx *ctx, const topk_query &Q, BatchQuery 
 
 namespace topk {
 
-// Validates and stages the queries into h_in.
+// Validates the queries and copies them into the pinned staging area.
 static int stage_queries(topk_ctx *ctx, BatchState &B, uint32_t nq, const topk_query *qs, uint32_t ks) {
-    const InLayout il = InLayout::make(nq);
-    BatchQuery *hq = reinterpret_cast<BatchQuery *>(B.h_in + il.q_off);
-    BoundArgs *hb = reinterpret_cast<BoundArgs *>(B.h_in + il.b_off);
     for (uint32_t i = 0; i < nq; i++) {
         const topk_query &Q = qs[i];
-        const int rc = make_batch_query(ctx, Q, hq[i]);
-        if (rc) return rc;
-        if (Q.k > ks) return TOPK_ERR_INVALID;
-        hb[i] = make_bound_args(ctx->d, Q.qq, Q.weights, Q.attr);
+        if (Q.k < 1 || Q.k > ks || Q.qq < 1 || Q.qq > TOPK_MAX_DIMS) return TOPK_ERR_INVALID;
+        if (Q.k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+        for (uint32_t j = 0; j < Q.qq; j++)
+            if (Q.attr[j] >= ctx->d) return TOPK_ERR_INVALID;
     }
+    memcpy(B.h_in + InLayout::make(nq).raw_off, qs, sizeof(topk_query) * nq);
     return TOPK_OK;
 }
 
@@ -805,79 +956,8 @@ static int resolve_mode(const topk_ctx *ctx) {
     return learned_ready(ctx) ? BMODE_LEARNED : BMODE_SEED;
 }
 
-// Parsed outcome of one device pass.
-struct PassTally {
-    uint64_t pairs = 0, objs = 0, candidates = 0;
-};
-
-// Copies the finished queries of the last pass (local index i -> caller row
-// rows[i]) and collects the others: overflowed queries with their tightened
-// threshold, and queries whose threshold left fewer than k candidates.
-static void collect(const topk_ctx *ctx, const BatchState &B, uint32_t nq, uint32_t ks, const topk_query *qs,
-                    const uint32_t *rows, uint64_t *out_ids, float *out_scores, float *out_thr,
-                    std::vector<std::pair<uint32_t, float>> &reseed, std::vector<uint32_t> &retry, PassTally &t) {
-    const OutLayout ol = OutLayout::make(nq, ks);
-    const uint64_t *ids = reinterpret_cast<const uint64_t *>(B.h_out + ol.ids_off);
-    const float *sc = reinterpret_cast<const float *>(B.h_out + ol.sc_off);
-    const float *thr = reinterpret_cast<const float *>(B.h_out + ol.thr_off);
-    const uint32_t *st = reinterpret_cast<const uint32_t *>(B.h_out + ol.st_off);
-    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(B.h_out + ol.cnt_off);
-    const float *tn = reinterpret_cast<const float *>(B.h_out + ol.tn_off);
-    const uint32_t *ctr = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr_off);
-    t.pairs += (uint64_t)ctr[2] | ((uint64_t)ctr[3] << 32);
-    t.objs += (uint64_t)ctr[4] | ((uint64_t)ctr[5] << 32);
-    for (uint32_t i = 0; i < nq; i++) {
-        const uint32_t row = rows ? rows[i] : i;
-        t.candidates += cnt[i] < BATCH_CAP ? cnt[i] : BATCH_CAP;
-        if (st[i] == STATUS_OVERFLOW) {
-            reseed.push_back({row, tn[i]});
-            continue;
-        }
-        if (st[i]) {
-            retry.push_back(row);
-            continue;
-        }
-        const uint32_t kq = qs[row].k;
-        const uint32_t c = (uint32_t)(ctx->n < kq ? ctx->n : kq);
-        const uint64_t o = (uint64_t)row * ks, li = (uint64_t)i * ks;
-        if (out_ids) {
-            memcpy(out_ids + o, ids + li, sizeof(uint64_t) * c);
-            for (uint32_t j = c; j < kq; j++) out_ids[o + j] = ~0ull;
-        }
-        if (out_scores) {
-            memcpy(out_scores + o, sc + li, sizeof(float) * c);
-            for (uint32_t j = c; j < kq; j++) out_scores[o + j] = -INFINITY;
-        }
-        if (out_thr) out_thr[row] = thr[i];
-    }
-}
-
-// A second device pass over some queries of the call (caller rows `rows`):
-// seed thresholds, or the given ones when taus is non-null.  Eager launches.
-static int sub_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, const topk_query *qs, uint32_t ks,
-                    const std::vector<uint32_t> &rows, const std::vector<float> *taus, const BatchVariant &v,
-                    uint64_t *out_ids, float *out_scores, float *out_thr,
-                    std::vector<std::pair<uint32_t, float>> &reseed, std::vector<uint32_t> &retry, PassTally &t) {
-    const uint32_t n2 = (uint32_t)rows.size();
-    std::vector<topk_query> sub(n2);
-    for (uint32_t i = 0; i < n2; i++) sub[i] = qs[rows[i]];
-    int rc = stage_queries(ctx, B, n2, sub.data(), ks);
-    if (rc) return rc;
-    if (taus) {
-        BatchQuery *hq = reinterpret_cast<BatchQuery *>(B.h_in + InLayout::make(n2).q_off);
-        for (uint32_t i = 0; i < n2; i++) hq[i].tau = (*taus)[i];
-    }
-    rc = batch_enqueue(ctx, B, s, n2, ks, taus ? BMODE_GIVEN : BMODE_SEED, v);
-    if (rc) return rc;
-    TK_HIP(hipStreamSynchronize(s));
-    collect(ctx, B, n2, ks, qs, rows.data(), out_ids, out_scores, out_thr, reseed, retry, t);
-    return TOPK_OK;
-}
-
-// Runs one batch and copies results out.  A query whose threshold left
-// fewer than k candidates is re-run with seed thresholds; an overflowed
-// query with the threshold its pass tightened; what a second pass cannot
-// settle goes through topk_fte.
+// Runs one batch (both passes in one launch) and copies results out; a
+// query the second pass could not settle either goes through topk_fte.
 static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
                      float *out_scores, float *out_thr, const BatchVariant *force) {
     if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
@@ -903,32 +983,42 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     rc = batch_launch(ctx, B, s, nq, ks, mode, v);
     if (rc) return rc;
     TK_HIP(hipStreamSynchronize(s));
-    PassTally tally;
-    {
-        const uint32_t *ctr = reinterpret_cast<const uint32_t *>(B.h_out + OutLayout::make(nq, ks).ctr_off);
-        ctx->stats.items = ctr[0];
-    }
-    std::vector<std::pair<uint32_t, float>> reseed, reseed2;
-    std::vector<uint32_t> retry, retry2, fte;
-    collect(ctx, B, nq, ks, qs, nullptr, out_ids, out_scores, out_thr, reseed, retry, tally);
-    const uint32_t missed = (uint32_t)retry.size();
 
-    if (!retry.empty()) {  // threshold too high (a learned guess): seed thresholds are exact lower bounds
-        rc = sub_pass(ctx, B, s, qs, ks, retry, nullptr, v, out_ids, out_scores, out_thr, reseed, retry2, tally);
-        if (rc) return rc;
-    }
-    if (!reseed.empty()) {
-        std::vector<uint32_t> rows(reseed.size());
-        std::vector<float> taus(reseed.size());
-        for (size_t i = 0; i < reseed.size(); i++) {
-            rows[i] = reseed[i].first;
-            taus[i] = reseed[i].second;
+    const OutLayout ol = OutLayout::make(nq, ks);
+    const uint64_t *ids = reinterpret_cast<const uint64_t *>(B.h_out + ol.ids_off);
+    const float *sc = reinterpret_cast<const float *>(B.h_out + ol.sc_off);
+    const float *thr = reinterpret_cast<const float *>(B.h_out + ol.thr_off);
+    const uint32_t *st = reinterpret_cast<const uint32_t *>(B.h_out + ol.st_off);
+    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(B.h_out + ol.cnt_off);
+    const uint32_t *fl = reinterpret_cast<const uint32_t *>(B.h_out + ol.fl_off);
+    const uint32_t *c1 = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr_off);
+    const uint32_t *c2 = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr2_off);
+    uint64_t candidates = 0;
+    uint32_t missed = 0, retried = 0;
+    std::vector<uint32_t> fte;
+    for (uint32_t i = 0; i < nq; i++) {
+        candidates += cnt[i] < BATCH_CAP ? cnt[i] : BATCH_CAP;
+        // a query counts once as a fallback: its first threshold missed, or
+        // the device passes could not settle it
+        missed += fl[i] == QF_SEED || fl[i] == QF_LOWER || st[i] ? 1u : 0u;
+        retried += fl[i] == QF_GIVEN ? 1u : 0u;
+        if (st[i]) {
+            fte.push_back(i);
+            continue;
         }
-        rc = sub_pass(ctx, B, s, qs, ks, rows, &taus, v, out_ids, out_scores, out_thr, reseed2, retry2, tally);
-        if (rc) return rc;
+        const uint32_t kq = qs[i].k;
+        const uint32_t c = (uint32_t)(ctx->n < kq ? ctx->n : kq);
+        const uint64_t o = (uint64_t)i * ks;
+        if (out_ids) {
+            memcpy(out_ids + o, ids + o, sizeof(uint64_t) * c);
+            for (uint32_t j = c; j < kq; j++) out_ids[o + j] = ~0ull;
+        }
+        if (out_scores) {
+            memcpy(out_scores + o, sc + o, sizeof(float) * c);
+            for (uint32_t j = c; j < kq; j++) out_scores[o + j] = -INFINITY;
+        }
+        if (out_thr) out_thr[i] = thr[i];
     }
-    for (const auto &r : reseed2) fte.push_back(r.first);
-    for (uint32_t i : retry2) fte.push_back(i);
     for (uint32_t i : fte) {
         const topk_query &Q = qs[i];
         float t = 0;
@@ -939,12 +1029,13 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     }
     topk_stats &S = ctx->stats;
     S.queries = nq;
-    S.fallback = missed + (uint32_t)fte.size();
-    S.retried = (uint32_t)reseed.size();
-    S.candidates = tally.candidates;
-    S.partitions_scored = tally.pairs;
+    S.fallback = missed;
+    S.retried = retried;
+    S.items = c1[0];
+    S.candidates = candidates;
+    S.partitions_scored = ((uint64_t)c1[2] | ((uint64_t)c1[3] << 32)) + ((uint64_t)c2[2] | ((uint64_t)c2[3] << 32));
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;
-    S.tuple_count = tally.objs;
+    S.tuple_count = ((uint64_t)c1[4] | ((uint64_t)c1[5] << 32)) + ((uint64_t)c2[4] | ((uint64_t)c2[5] << 32));
     S.ms_last = now_ms() - t0;
     return TOPK_OK;
 }

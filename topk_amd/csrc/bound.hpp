@@ -64,19 +64,32 @@ inline BoundArgs make_bound_args(uint32_t d, uint32_t qq, const float *weights, 
 }
 
 // pmax / pmin point at the partition's d per-attribute maxima / minima.
+// NMAX bounds b.nd at compile time: the queried maxima / minima are read
+// with independent loads into registers before any arithmetic.
+template <int NMAX = TOPK_MAX_DIMS>
 __device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__restrict__ pmax,
                                           const float *__restrict__ pmin, float psum, float pminsum) {
+    float mxv[NMAX], mnv[NMAX];
+#pragma unroll
+    for (int i = 0; i < NMAX; i++) {
+        const bool on = (uint32_t)i < b.nd;
+        mxv[i] = on ? pmax[b.attr[i]] : 0.0f;
+        mnv[i] = on ? pmin[b.attr[i]] : 0.0f;
+    }
     double base = 0, box = 0, absw = 0;
-    for (uint32_t i = 0; i < b.nd; i++) {
-        const double mx = pmax[b.attr[i]], mn = pmin[b.attr[i]];
-        base += b.w[i] * mn;
-        box += b.w[i] * (b.w[i] >= 0 ? mx : mn);
-        absw += fabs(b.w[i]) * fmax(fabs(mx), fabs(mn));
+#pragma unroll
+    for (int i = 0; i < NMAX; i++) {
+        if ((uint32_t)i >= b.nd) break;
+        const double mx = mxv[i], mn = mnv[i], w = b.w[i];
+        base += w * mn;
+        box += w * (w >= 0 ? mx : mn);
+        absw += fabs(w) * fmax(fabs(mx), fabs(mn));
     }
     double budget = (double)psum - (double)pminsum;
-    for (uint32_t i = 0; i < b.nd; i++) {
-        if (b.w[i] <= 0 || budget <= 0) break;
-        const double room = fmin((double)pmax[b.attr[i]] - (double)pmin[b.attr[i]], budget);
+#pragma unroll
+    for (int i = 0; i < NMAX; i++) {
+        if ((uint32_t)i >= b.nd || b.w[i] <= 0 || budget <= 0) break;
+        const double room = fmin((double)mxv[i] - (double)mnv[i], budget);
         base += b.w[i] * room;
         budget -= room;
     }

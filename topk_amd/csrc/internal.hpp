@@ -90,6 +90,9 @@ struct BatchQuery {
     // because a score is never -0).
     uint32_t ascending;
     float wa[TOPK_MAX_DIMS];
+    // Lower threshold for a rerun when tau left fewer than k candidates
+    // (written with a predicted tau; -inf: rerun with a seed threshold).
+    float tau_retry;
 };
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.

@@ -310,6 +310,8 @@ __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict
         if (lane == 0) {
             const float y = o + b3;
             qs[q].tau = sc > 0.0f ? (y - margin) * sc : -INFINITY;
+            const float wide = 9.0f * fabsf(margin);  // a miss reruns far below the prediction
+            qs[q].tau_retry = sc > 0.0f ? (y - wide) * sc : -INFINITY;
             if (pred) pred[q] = y * sc;
         }
     }
