@@ -129,7 +129,14 @@ def learned_batched(eng, n, k, args):
     rep = eng.learned_train(train)
     held = [eng.prepare_batch(random_queries(rng, nb, k)) for _ in range(args.held_batches)]
     eng.set_tuning("batch_mode", 2)
-    tuning = eng.batch_autotune(held[0]) if not args.no_autotune else None
+    tuning = None
+    if args.tune:
+        for name, val in zip(("batch_vpt", "batch_bs", "batch_qpi", "batch_digit", "batch_grid"),
+                             (int(t) for t in args.tune.split(","))):
+            eng.set_tuning(name, val)
+        tuning = {"fixed": args.tune}
+    elif not args.no_autotune:
+        tuning = eng.batch_autotune(held[0])
 
     def timed(steps, warmup, mode):
         eng.set_tuning("batch_mode", mode)
@@ -244,6 +251,8 @@ def main():
     ap.add_argument("--train-queries", type=int, default=16384)
     ap.add_argument("--held-batches", type=int, default=4)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune", default="", metavar="VPT,BS,QPI,DIGIT,GRID",
+                    help="fixed batch launch shape instead of the autotuner's choice")
     ap.add_argument("--fte-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
