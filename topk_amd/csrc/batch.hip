@@ -55,20 +55,23 @@ constexpr int BSEED_BS = 512, BSEED_KPT = 32;
 constexpr uint32_t BSEED_T = 4;                          // seed partitions per query
 constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
+constexpr uint32_t ITEM_MIN_OBJECTS = 2048;              // smallest object chunk of a work item
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 // Per-query flags the first pass leaves for the second (in-graph) pass.
 enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
-// Device counters of one batch: [0] work items, [1] work ticket,
-// [2..3] scored (partition, query) pairs, [4..5] scored (object, query)
-// pairs, [6] (query, group) pairs of the mask's first level.
+// Device counters of one batch: [0] heavy work items (front of the list),
+// [1] work ticket, [2..3] scored (partition, query) pairs, [4..5] scored
+// (object, query) pairs, [6] (query, group) pairs of the mask's first
+// level, [7] light work items (back of the list).
 constexpr uint32_t BCTR_WORDS = 8;
 
-// Work item: partition p scored for the queries whose bits are set in its
-// bitmap between query indices [qbeg, qend) (at most the item budget).
+// Work item: objects [chunk * ochunk, (chunk + 1) * ochunk) of partition p,
+// scored for the queries whose bits are set in its bitmap between query
+// indices [qbeg, qend) (at most the item budget).
 struct BatchItem {
     uint32_t p;
     uint32_t qbeg, qend;
-    uint32_t pad;
+    uint32_t chunk;
 };
 
 // ---------------------------------------------------------------------------
@@ -323,19 +326,27 @@ __device__ __forceinline__ uint32_t after_nth_bit(unsigned long long m, uint32_t
     return (uint32_t)__builtin_ctzll(m) + 1u;
 }
 
-// One thread per partition: its bitmap is cut into items of at most qmax
-// queries (cuts may fall inside a 64-query word); one global atomic per
-// workgroup reserves the items, and the pair counters are updated.
+// One thread per partition: its bitmap is cut into query ranges of at most
+// qmax queries (cuts may fall inside a 64-query word), and every range is
+// split into object chunks of ochunk.  Ranges holding at least qmax / 2
+// queries are "heavy" and fill the item list from the front, the others
+// from the back (index cap - 1 - i), so the persistent scoring kernel starts
+// the long items first.  One pair of global atomics per workgroup reserves
+// the items; the pair counters are updated.
 __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned long long *__restrict__ masks,
                                                                 uint32_t L, uint32_t nw, uint32_t nq, uint32_t qmax,
+                                                                uint32_t ochunk, uint64_t cap,
                                                                 const uint32_t *__restrict__ psz,
                                                                 BatchItem *__restrict__ items,
                                                                 unsigned int *__restrict__ ctr) {
     __shared__ uint32_t wsum[BAUX_BS / 64];
-    __shared__ uint32_t total, base;
+    __shared__ uint32_t tot_h, tot_l, base_h, base_l;
     const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x;
     const unsigned long long *mp = masks + (uint64_t)p * nw;
-    uint32_t nitems = 0, pairs = 0;
+    const uint32_t sz = p < L ? psz[p] : 0u;
+    const uint32_t nch = (sz + ochunk - 1) / ochunk;
+    const uint32_t heavy_min = (qmax + 1) / 2;
+    uint32_t nh = 0, nl = 0, pairs = 0;
     if (p < L) {
         uint32_t run = 0;
         for (uint32_t w0 = 0; w0 < nw; w0 += 8) {
@@ -351,27 +362,43 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned l
                     run += take;
                     c -= take;
                     if (run == qmax) {
-                        nitems++;
+                        nh++;
                         run = 0;
                     }
                 }
             }
         }
-        nitems += run ? 1u : 0u;
+        if (run) {
+            if (run >= heavy_min) nh++;
+            else nl++;
+        }
+        nh *= nch;
+        nl *= nch;
     }
-    const uint32_t off = block_excl_scan<BAUX_BS>(nitems, wsum, &total);
+    const uint32_t offh = block_excl_scan<BAUX_BS>(nh, wsum, &tot_h);
+    const uint32_t offl = block_excl_scan<BAUX_BS>(nl, wsum, &tot_l);
     const uint32_t wp = wave_sum_u32(pairs);
-    unsigned long long objs = (unsigned long long)pairs * (p < L && pairs ? psz[p] : 0u);
+    unsigned long long objs = (unsigned long long)pairs * sz;
     for (int o = 32; o > 0; o >>= 1) objs += __shfl_xor(objs, o, 64);
-    if (threadIdx.x == 0) base = total ? atomicAdd(&ctr[0], total) : 0u;
+    if (threadIdx.x == 0) {
+        base_h = tot_h ? atomicAdd(&ctr[0], tot_h) : 0u;
+        base_l = tot_l ? atomicAdd(&ctr[7], tot_l) : 0u;
+    }
     if (lane_id() == 0 && wp) {
         atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), (unsigned long long)wp);
         atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 4), objs);
     }
     __syncthreads();
-    if (!nitems) return;
-    BatchItem *out = items + base + off;
-    uint32_t run = 0, beg = 0, k = 0;
+    if (!nh && !nl) return;
+    uint32_t kh = base_h + offh, kl = base_l + offl;
+    auto emit = [&](uint32_t beg, uint32_t end, uint32_t cnt) {
+        for (uint32_t c = 0; c < nch; c++) {
+            const BatchItem it{p, beg, end, c};
+            if (cnt >= heavy_min) items[kh++] = it;
+            else items[cap - 1 - kl++] = it;
+        }
+    };
+    uint32_t run = 0, beg = 0;
     bool open = false;
     unsigned long long mw[8];
     for (uint32_t w = 0; w < nw; w++) {
@@ -394,14 +421,14 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned l
             } else {
                 const uint32_t take = qmax - run;
                 const uint32_t endbit = after_nth_bit(m, take);
-                out[k++] = BatchItem{p, beg, w * 64 + endbit, 0};
+                emit(beg, w * 64 + endbit, qmax);
                 m = endbit >= 64 ? 0ull : (m & (~0ull << endbit));
                 run = 0;
                 open = false;
             }
         }
     }
-    if (open) out[k++] = BatchItem{p, beg, nq, 0};
+    if (open) emit(beg, nq, run);
 }
 
 struct ScoreArgs {
@@ -410,6 +437,8 @@ struct ScoreArgs {
     uint32_t d;
     const uint32_t *poff, *psz, *pid;
     const BatchItem *items;
+    uint64_t cap;       // item list capacity (light items sit at its end)
+    uint32_t ochunk;    // objects per item
     const unsigned long long *masks;
     uint32_t nw;
     unsigned int *ctr;
@@ -453,15 +482,18 @@ template <int D, int VPT, int BS>
 __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
     __shared__ uint32_t s_it;
     const uint32_t tid = threadIdx.x, lane = lane_id();
-    const uint32_t nitems = __builtin_amdgcn_readfirstlane(a.ctr[0]);
+    const uint32_t nheavy = __builtin_amdgcn_readfirstlane(a.ctr[0]);
+    const uint32_t nlight = __builtin_amdgcn_readfirstlane(a.ctr[7]);
     for (;;) {
         if (tid == 0) s_it = atomicAdd(&a.ctr[1], 1u);
         __syncthreads();
         const uint32_t it = __builtin_amdgcn_readfirstlane(s_it);
         __syncthreads();
-        if (it >= nitems) break;
-        const BatchItem I = a.items[it];
-        const uint32_t start = a.poff[I.p], size = a.psz[I.p];
+        if (it >= nheavy + nlight) break;
+        const BatchItem I = it < nheavy ? a.items[it] : a.items[a.cap - 1 - (it - nheavy)];
+        const uint32_t first = I.chunk * a.ochunk, psz_ = a.psz[I.p];
+        const uint32_t start = a.poff[I.p] + first;
+        const uint32_t size = psz_ - first < a.ochunk ? psz_ - first : a.ochunk;
         for (uint32_t b0 = 0; b0 < size; b0 += BS * VPT) {
             const uint32_t i0 = b0 + tid * VPT;
             const uint32_t nvalid = i0 < size ? (size - i0 < (uint32_t)VPT ? size - i0 : (uint32_t)VPT) : 0u;
@@ -752,7 +784,10 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks) {
         B.out_cap = out_need;
     }
     int rc = grow_dev(B.d_masks, B.mask_cap, (uint64_t)L * nw, changed);
-    if (!rc) rc = grow_dev(B.d_items, B.item_cap, (uint64_t)L * (4 * nw + 1), changed);  // budget >= 16 queries
+    // budget >= 16 queries, chunks >= ITEM_MIN_OBJECTS objects
+    if (!rc)
+        rc = grow_dev(B.d_items, B.item_cap,
+                      (uint64_t)L * (4 * nw + 1) * ((P.maxsz + ITEM_MIN_OBJECTS - 1) / ITEM_MIN_OBJECTS + 1), changed);
     if (!rc) rc = grow_dev(B.d_pairs, B.pair_cap, (uint64_t)(P.p2 >> group_height(P.p2)) * nq, changed);
     if (rc) return rc;
     if (!B.d_ctr) {
@@ -768,6 +803,13 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks) {
 }
 
 static int dims_class(uint32_t d) { return d <= 8 ? 8 : (d <= 16 ? 16 : 32); }
+
+// Objects of one work item: one scoring pass of the shape, at least
+// ITEM_MIN_OBJECTS (bounds the item list).
+static uint32_t item_objects(const BatchVariant &v) {
+    const uint32_t pass = (uint32_t)(v.bs * v.vpt);
+    return pass > ITEM_MIN_OBJECTS ? pass : ITEM_MIN_OBJECTS;
+}
 
 static BatchVariant pick_variant(const topk_ctx *ctx, const BatchState &B) {
     const int D = dims_class(ctx->d);
@@ -835,10 +877,11 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                        B.d_pairs, B.d_ctr, pass_flags);
     hipLaunchKernelGGL(batch_leaf_kernel, dim3(2048), dim3(BAUX_BS), 0, s, dq, db, ta, H, L, nw, B.d_pairs, B.d_ctr,
                        B.d_masks);
+    const uint32_t ochunk = item_objects(v);
     hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_masks, L,
-                       nw, nq, (uint32_t)v.qpi, P.psz, B.d_items, B.d_ctr);
-    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.d_masks, nw, B.d_ctr, dq, B.d_cand,
-                 B.d_ccount};
+                       nw, nq, (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
+    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.item_cap, ochunk, B.d_masks, nw, B.d_ctr,
+                 dq, B.d_cand, B.d_ccount};
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
     launch_score(dims_class(ctx->d), v, s, sa);
@@ -1031,7 +1074,7 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     S.queries = nq;
     S.fallback = missed;
     S.retried = retried;
-    S.items = c1[0];
+    S.items = c1[0] + c1[7];
     S.candidates = candidates;
     S.partitions_scored = ((uint64_t)c1[2] | ((uint64_t)c1[3] << 32)) + ((uint64_t)c2[2] | ((uint64_t)c2[3] << 32));
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;
@@ -1069,7 +1112,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     const int D = dims_class(ctx->d);
     const BatchVariant def = pick_variant(ctx, B);
     std::vector<BatchVariant> cand;
-    static const int vpts[] = {2, 4, 8, 16}, qpis[] = {64, 128, 256}, per_cu[] = {2, 4, 8};
+    static const int vpts[] = {2, 4, 8, 16}, qpis[] = {16, 32, 64, 128}, per_cu[] = {2, 4, 8};
     for (int vpt : vpts)
         for (int bs : {256, 512}) {
             if (!variant_ok(D, vpt, bs)) continue;
