@@ -56,6 +56,8 @@ constexpr uint32_t BSEED_T = 4;                          // seed partitions per 
 constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
 constexpr uint32_t ITEM_MIN_OBJECTS = 2048;              // smallest object chunk of a work item
+constexpr int BMASK_BS = 256;                            // mask: one workgroup per query
+constexpr uint32_t BMASK_MAX_G = 4096;                   // groups of the mask's first level (LDS list)
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 // Per-query flags the first pass leaves for the second (in-graph) pass.
 enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
@@ -66,85 +68,88 @@ enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
 constexpr uint32_t BCTR_WORDS = 8;
 
 // Work item: objects [chunk * ochunk, (chunk + 1) * ochunk) of partition p,
-// scored for the queries whose bits are set in its bitmap between query
-// indices [qbeg, qend) (at most the item budget).
+// scored for the qcnt queries at plist[p * nq + qbeg ...].
 struct BatchItem {
     uint32_t p;
-    uint32_t qbeg, qend;
+    uint32_t qbeg, qcnt;
     uint32_t chunk;
 };
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// Device form of query i (one thread per query): terms in query order, the
-// ascending form, and the bound arguments -- the distinct attributes with
-// aggregated FP64 weights, heaviest first (ties by attribute id).  taus
-// (optional) gives the thresholds of a second pass.  Written field by field
-// with statically indexed loops, so nothing lands in scratch memory.
-__global__ __launch_bounds__(128) void batch_prep_kernel(const topk_query *__restrict__ raw,
-                                                         const float *__restrict__ taus, uint32_t nq, uint64_t n,
-                                                         uint32_t d, BatchQuery *__restrict__ out,
-                                                         BoundArgs *__restrict__ outb) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const topk_query &Q = raw[i];
-    BatchQuery &b = out[i];
-    BoundArgs &g = outb[i];
-    const uint32_t qq = Q.qq;
-    b.tau = taus ? taus[i] : 0.0f;
-    b.tau_retry = -INFINITY;
-    b.k = (uint32_t)(n < Q.k ? n : Q.k);
-    b.kuser = Q.k;
-    b.m = qq;
-    uint32_t asc = 1, prev = 0;
-#pragma unroll
-    for (int j = 0; j < TOPK_MAX_DIMS; j++) {
-        const uint32_t a = (uint32_t)j < qq ? Q.attr[j] : 0u;
-        b.attr[j] = (uint8_t)a;
-        b.w[j] = (uint32_t)j < qq ? Q.weights[a] : 0.0f;
-        if (j && (uint32_t)j < qq && a <= prev) asc = 0;
-        prev = a;
+// Device form of query q, one wave per query (lane j: term j / attribute
+// j): terms in query order, the ascending form, and the bound arguments --
+// the distinct attributes with aggregated FP64 weights, heaviest first (ties
+// by attribute id).  taus (optional) gives the thresholds of a second pass.
+// Every thread also clears its share of the per-batch counters.
+__global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__restrict__ raw,
+                                                        const float *__restrict__ taus, uint32_t nq, uint64_t n,
+                                                        uint32_t d, BatchQuery *__restrict__ out,
+                                                        BoundArgs *__restrict__ outb, unsigned int *__restrict__ pcnt,
+                                                        uint32_t L, unsigned int *__restrict__ ccount,
+                                                        unsigned int *__restrict__ ctr) {
+    const uint32_t lane = lane_id();
+    const uint64_t stride = (uint64_t)gridDim.x * 64;
+    for (uint64_t i = (uint64_t)blockIdx.x * 64 + lane; i < L; i += stride) pcnt[i] = 0u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 64 + lane; i < nq; i += stride) ccount[i] = 0u;
+    if (blockIdx.x == 0 && lane < BCTR_WORDS) ctr[lane] = 0u;
+    const uint32_t q = blockIdx.x;
+    if (q >= nq) return;
+    const topk_query &Q = raw[q];
+    BatchQuery &b = out[q];
+    BoundArgs &g = outb[q];
+    const uint32_t qq = Q.qq, j = lane;
+    const bool term = j < qq;
+    const uint32_t aj = term ? Q.attr[j] : 0u;
+    const float wj = term ? Q.weights[aj] : 0.0f;
+    const uint32_t aprev = __shfl(aj, j ? j - 1 : 0, 64);
+    const bool asc = __ballot(term && j > 0 && aj <= aprev) == 0ull;
+    if (j < TOPK_MAX_DIMS) {
+        b.attr[j] = (uint8_t)aj;
+        b.w[j] = wj;
     }
-    b.ascending = asc;
-    // aggregated weight per attribute id (static index, compare-select)
-    double agg[TOPK_MAX_DIMS];
-    uint32_t used = 0;
-#pragma unroll
-    for (int a = 0; a < TOPK_MAX_DIMS; a++) agg[a] = 0.0;
-    for (uint32_t j = 0; j < qq; j++) {
-        const uint32_t at = Q.attr[j];
-        const double w = (double)Q.weights[at];
-        used |= 1u << at;
-#pragma unroll
-        for (int a = 0; a < TOPK_MAX_DIMS; a++) agg[a] = (uint32_t)a == at ? agg[a] + w : agg[a];
-    }
-#pragma unroll
-    for (int a = 0; a < TOPK_MAX_DIMS; a++) b.wa[a] = asc && ((used >> a) & 1u) ? Q.weights[a] : 0.0f;
-    // rank of each used attribute: heavier first, ties by lower id
-    g.d = d;
-    g.nd = (uint32_t)__popc(used);
-#pragma unroll
-    for (int a = 0; a < TOPK_MAX_DIMS; a++) {
-        if ((uint32_t)a >= d) break;
-        if (!((used >> a) & 1u)) continue;
-        uint32_t r = 0;
-#pragma unroll
-        for (int c = 0; c < TOPK_MAX_DIMS; c++) {
-            if ((uint32_t)c >= d) break;
-            r += ((used >> c) & 1u) && (agg[c] > agg[a] || (agg[c] == agg[a] && c < a)) ? 1u : 0u;
+    // lane a: aggregated weight of attribute a
+    double agg = 0.0;
+    bool used = false;
+    for (uint32_t t = 0; t < qq; t++) {
+        const uint32_t at = __shfl(aj, t, 64);
+        const float wt = __shfl(wj, t, 64);
+        if (at == lane) {
+            agg += (double)wt;
+            used = true;
         }
-        g.attr[r] = (uint8_t)a;
-        g.w[r] = agg[a];
+    }
+    if (lane < TOPK_MAX_DIMS) b.wa[lane] = asc && used ? Q.weights[lane] : 0.0f;
+    uint32_t r = 0;  // rank: heavier first, ties by lower attribute id
+    for (uint32_t c = 0; c < d; c++) {
+        const double ac = __shfl(agg, c, 64);
+        const bool uc = __shfl((int)used, c, 64) != 0;
+        r += uc && (ac > agg || (ac == agg && c < lane)) ? 1u : 0u;
+    }
+    if (used && lane < d) {
+        g.attr[r] = (uint8_t)lane;
+        g.w[r] = agg;
+    }
+    const uint32_t nd = (uint32_t)__popcll(__ballot(used && lane < d));
+    if (lane == 0) {
+        b.tau = taus ? taus[q] : 0.0f;
+        b.tau_retry = -INFINITY;
+        b.k = (uint32_t)(n < Q.k ? n : Q.k);
+        b.kuser = Q.k;
+        b.m = qq;
+        b.ascending = asc ? 1u : 0u;
+        g.d = d;
+        g.nd = nd;
     }
 }
 
-__global__ __launch_bounds__(BAUX_BS) void batch_zero_kernel(unsigned long long *__restrict__ masks, uint64_t nmask,
+__global__ __launch_bounds__(BAUX_BS) void batch_zero_kernel(unsigned int *__restrict__ pcnt, uint32_t L,
                                                              unsigned int *__restrict__ ccount, uint32_t nq,
                                                              unsigned int *__restrict__ ctr) {
     const uint64_t stride = (uint64_t)gridDim.x * BAUX_BS;
     const uint64_t i0 = (uint64_t)blockIdx.x * BAUX_BS + threadIdx.x;
-    for (uint64_t i = i0; i < nmask; i += stride) masks[i] = 0ull;
+    for (uint64_t i = i0; i < L; i += stride) pcnt[i] = 0u;
     for (uint64_t i = i0; i < nq; i += stride) ccount[i] = 0u;
     if (i0 < BCTR_WORDS) ctr[i0] = 0u;
 }
@@ -275,110 +280,65 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
     if (tid == 0) qs[q].tau = tau;
 }
 
-// Level 1 of the mask: tree nodes of height H (groups of 2^H partitions,
-// nodes G..2G-1) whose bound reaches tau_q become (query, group) pairs.
-__global__ __launch_bounds__(BAUX_BS) void batch_group_kernel(const BatchQuery *__restrict__ qs,
+// One workgroup per query walks the bound tree two levels deep: tree nodes
+// of height H (groups of 2^H partitions, nodes G..2G-1) whose bound reaches
+// tau_q go to an LDS list; every partition of a listed group whose own
+// bound reaches tau_q gets q appended to its query list plist[p * nq ...]
+// (one atomic per (partition, query) pair on the partition's counter).
+__global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *__restrict__ qs,
                                                               const BoundArgs *__restrict__ qb, TreeArgs t,
-                                                              uint32_t G, uint2 *__restrict__ pairs,
-                                                              unsigned int *__restrict__ ctr,
+                                                              uint32_t G, uint32_t H, uint32_t L, uint32_t nq,
+                                                              unsigned int *__restrict__ pcnt,
+                                                              uint16_t *__restrict__ plist,
                                                               const uint32_t *__restrict__ qflag) {
-    const uint32_t g = blockIdx.x * BAUX_BS + threadIdx.x, q = blockIdx.y;
+    __shared__ uint32_t list[BMASK_MAX_G];
+    __shared__ uint32_t nlist;
+    const uint32_t q = blockIdx.x, tid = threadIdx.x;
     if (qflag && qflag[q] == QF_DONE) return;  // second pass: settled queries take no part
-    bool pass = false;
-    if (g < G) {
-        const float ub = node_bound(qb[q], t, G + g);
-        pass = ub > -INFINITY && ub >= qs[q].tau;
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    const BoundArgs &b = qb[q];
+    const float tau = qs[q].tau;
+    for (uint32_t g = tid; g < G; g += BMASK_BS) {
+        const float ub = node_bound(b, t, G + g);
+        if (ub > -INFINITY && ub >= tau) list[atomicAdd(&nlist, 1u)] = g;
     }
-    const uint64_t bal = __ballot(pass);
-    if (!bal) return;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(&ctr[6], (uint32_t)__popcll(bal));
-    base = __shfl(base, 0, 64);
-    if (pass) pairs[base + __popcll(bal & ((1ull << lane_id()) - 1ull))] = make_uint2(q, g);
-}
-
-// Level 2: one wave per (query, group) pair, lane l tests partition
-// g * 2^H + l; a partition whose bound reaches tau_q sets bit q of
-// masks[p * nw + q / 64].
-__global__ __launch_bounds__(BAUX_BS) void batch_leaf_kernel(const BatchQuery *__restrict__ qs,
-                                                             const BoundArgs *__restrict__ qb, TreeArgs t,
-                                                             uint32_t H, uint32_t L, uint32_t nw,
-                                                             const uint2 *__restrict__ pairs,
-                                                             const unsigned int *__restrict__ ctr,
-                                                             unsigned long long *__restrict__ masks) {
-    const uint32_t npairs = ctr[6];
-    const uint32_t lane = lane_id();
-    const uint32_t nwaves = gridDim.x * (BAUX_BS / 64);
-    for (uint32_t i = blockIdx.x * (BAUX_BS / 64) + threadIdx.x / 64; i < npairs; i += nwaves) {
-        const uint2 pr = pairs[i];
-        const uint32_t q = pr.x;
-        const uint32_t p = (pr.y << H) + lane;
-        if (lane < (1u << H) && p < L) {
-            const float ub = node_bound(qb[q], t, t.p2 + p);
-            if (ub > -INFINITY && ub >= qs[q].tau) atomicOr(&masks[(uint64_t)p * nw + q / 64], 1ull << (q % 64));
-        }
+    __syncthreads();
+    const uint32_t nl = nlist, per = 1u << H;
+    for (uint32_t i = tid; i < nl * per; i += BMASK_BS) {
+        const uint32_t p = (list[i >> H] << H) + (i & (per - 1));
+        if (p >= L) continue;
+        const float ub = node_bound(b, t, t.p2 + p);
+        if (ub > -INFINITY && ub >= tau) plist[(uint64_t)p * nq + atomicAdd(&pcnt[p], 1u)] = (uint16_t)q;
     }
 }
 
-// Position just past the t-th (1-based) set bit of m (t <= popcount(m)).
-__device__ __forceinline__ uint32_t after_nth_bit(unsigned long long m, uint32_t t) {
-    for (uint32_t i = 1; i < t; i++) m &= m - 1ull;
-    return (uint32_t)__builtin_ctzll(m) + 1u;
-}
-
-// One thread per partition: its bitmap is cut into query ranges of at most
-// qmax queries (cuts may fall inside a 64-query word), and every range is
-// split into object chunks of ochunk.  Ranges holding at least qmax / 2
-// queries are "heavy" and fill the item list from the front, the others
-// from the back (index cap - 1 - i), so the persistent scoring kernel starts
-// the long items first.  One pair of global atomics per workgroup reserves
-// the items; the pair counters are updated.
-__global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned long long *__restrict__ masks,
-                                                                uint32_t L, uint32_t nw, uint32_t nq, uint32_t qmax,
-                                                                uint32_t ochunk, uint64_t cap,
+// One thread per partition: its query list is cut into ranges of at most
+// qmax queries and every range into object chunks of ochunk.  Ranges holding
+// at least qmax / 2 queries are "heavy" and fill the item list from the
+// front, the others from the back (index cap - 1 - i), so the persistent
+// scoring kernel starts the long items first.  One pair of global atomics
+// per workgroup reserves the items; the pair counters are updated.
+__global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned int *__restrict__ pcnt, uint32_t L,
+                                                                uint32_t qmax, uint32_t ochunk, uint64_t cap,
                                                                 const uint32_t *__restrict__ psz,
                                                                 BatchItem *__restrict__ items,
                                                                 unsigned int *__restrict__ ctr) {
     __shared__ uint32_t wsum[BAUX_BS / 64];
     __shared__ uint32_t tot_h, tot_l, base_h, base_l;
     const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x;
-    const unsigned long long *mp = masks + (uint64_t)p * nw;
-    const uint32_t sz = p < L ? psz[p] : 0u;
+    const uint32_t cnt = p < L ? pcnt[p] : 0u;
+    const uint32_t sz = cnt ? psz[p] : 0u;
     const uint32_t nch = (sz + ochunk - 1) / ochunk;
     const uint32_t heavy_min = (qmax + 1) / 2;
-    uint32_t nh = 0, nl = 0, pairs = 0;
-    if (p < L) {
-        uint32_t run = 0;
-        for (uint32_t w0 = 0; w0 < nw; w0 += 8) {
-            unsigned long long mw[8];  // 8 independent loads in flight
-#pragma unroll
-            for (uint32_t u = 0; u < 8; u++) mw[u] = w0 + u < nw ? mp[w0 + u] : 0ull;
-#pragma unroll
-            for (uint32_t u = 0; u < 8; u++) {
-                uint32_t c = (uint32_t)__popcll(mw[u]);
-                pairs += c;
-                while (c) {
-                    const uint32_t take = c < qmax - run ? c : qmax - run;
-                    run += take;
-                    c -= take;
-                    if (run == qmax) {
-                        nh++;
-                        run = 0;
-                    }
-                }
-            }
-        }
-        if (run) {
-            if (run >= heavy_min) nh++;
-            else nl++;
-        }
-        nh *= nch;
-        nl *= nch;
-    }
+    const uint32_t nr = (cnt + qmax - 1) / qmax;
+    const uint32_t last = cnt - (nr ? (nr - 1) * qmax : 0u);
+    const uint32_t nhr = nr ? nr - 1 + (last >= heavy_min ? 1u : 0u) : 0u;
+    const uint32_t nh = nhr * nch, nl = (nr - nhr) * nch;
     const uint32_t offh = block_excl_scan<BAUX_BS>(nh, wsum, &tot_h);
     const uint32_t offl = block_excl_scan<BAUX_BS>(nl, wsum, &tot_l);
-    const uint32_t wp = wave_sum_u32(pairs);
-    unsigned long long objs = (unsigned long long)pairs * sz;
+    const uint32_t wp = wave_sum_u32(cnt);
+    unsigned long long objs = (unsigned long long)cnt * sz;
     for (int o = 32; o > 0; o >>= 1) objs += __shfl_xor(objs, o, 64);
     if (threadIdx.x == 0) {
         base_h = tot_h ? atomicAdd(&ctr[0], tot_h) : 0u;
@@ -389,46 +349,16 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned l
         atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 4), objs);
     }
     __syncthreads();
-    if (!nh && !nl) return;
+    if (!nr) return;
     uint32_t kh = base_h + offh, kl = base_l + offl;
-    auto emit = [&](uint32_t beg, uint32_t end, uint32_t cnt) {
+    for (uint32_t r = 0; r < nr; r++) {
+        const uint32_t q0 = r * qmax, qn = cnt - q0 < qmax ? cnt - q0 : qmax;
         for (uint32_t c = 0; c < nch; c++) {
-            const BatchItem it{p, beg, end, c};
-            if (cnt >= heavy_min) items[kh++] = it;
+            const BatchItem it{p, q0, qn, c};
+            if (qn >= heavy_min) items[kh++] = it;
             else items[cap - 1 - kl++] = it;
         }
-    };
-    uint32_t run = 0, beg = 0;
-    bool open = false;
-    unsigned long long mw[8];
-    for (uint32_t w = 0; w < nw; w++) {
-        if ((w & 7) == 0) {
-#pragma unroll
-            for (uint32_t u = 0; u < 8; u++) mw[u] = w + u < nw ? mp[w + u] : 0ull;
-        }
-        unsigned long long m = 0;
-#pragma unroll
-        for (uint32_t u = 0; u < 8; u++) m = (w & 7) == u ? mw[u] : m;
-        while (m) {
-            if (!open) {
-                beg = w * 64 + (uint32_t)__builtin_ctzll(m);
-                open = true;
-            }
-            const uint32_t c = (uint32_t)__popcll(m);
-            if (run + c < qmax) {
-                run += c;
-                m = 0;
-            } else {
-                const uint32_t take = qmax - run;
-                const uint32_t endbit = after_nth_bit(m, take);
-                emit(beg, w * 64 + endbit, qmax);
-                m = endbit >= 64 ? 0ull : (m & (~0ull << endbit));
-                run = 0;
-                open = false;
-            }
-        }
     }
-    if (open) emit(beg, nq, run);
 }
 
 struct ScoreArgs {
@@ -439,8 +369,8 @@ struct ScoreArgs {
     const BatchItem *items;
     uint64_t cap;       // item list capacity (light items sit at its end)
     uint32_t ochunk;    // objects per item
-    const unsigned long long *masks;
-    uint32_t nw;
+    const uint16_t *plist;
+    uint32_t nq;
     unsigned int *ctr;
     const BatchQuery *q;
     uint64_t *cand;
@@ -507,14 +437,10 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     for (int e = 0; e < VPT; e++) x[c][e] = 0.0f;
                 }
             }
-            for (uint32_t wd = I.qbeg / 64; wd * 64 < I.qend; wd++) {
-            unsigned long long mask = a.masks[(uint64_t)I.p * a.nw + wd];
-            if (wd * 64 < I.qbeg) mask &= ~0ull << (I.qbeg - wd * 64);
-            if (I.qend < wd * 64 + 64) mask &= (1ull << (I.qend - wd * 64)) - 1ull;
-            while (mask) {
-                const uint32_t bit = (uint32_t)__builtin_ctzll(mask);
-                mask &= mask - 1ull;
-                const uint32_t q = wd * 64 + bit;
+            {
+            const uint16_t *ql = a.plist + (uint64_t)I.p * a.nq + I.qbeg;
+            for (uint32_t qi = 0; qi < I.qcnt; qi++) {
+                const uint32_t q = __builtin_amdgcn_readfirstlane((uint32_t)ql[qi]);
                 const BatchQuery &Q = a.q[q];
                 float s[VPT];
 #pragma unroll
@@ -651,15 +577,15 @@ struct BatchState {
     uint64_t gen = ~0ull;   // layout generation the buffers and graphs belong to
     uint32_t nq_cap = 0;
     size_t in_cap = 0, out_cap = 0;
-    uint64_t mask_cap = 0, item_cap = 0, pair_cap = 0;
+    uint64_t plist_cap = 0, pcnt_cap = 0, item_cap = 0;
     unsigned char *d_in = nullptr, *h_in = nullptr;    // BatchQuery[nq] | BoundArgs[nq]
     unsigned char *d_out = nullptr, *h_out = nullptr;  // ids | scores | thr | status | cnt | ctr
-    unsigned long long *d_masks = nullptr;
+    uint16_t *d_plist = nullptr;     // per-partition query lists (nq entries each)
+    unsigned int *d_pcnt = nullptr;  // per-partition list length
     BatchItem *d_items = nullptr;
     unsigned int *d_ctr = nullptr;
     unsigned int *d_ccount = nullptr;
     uint64_t *d_cand = nullptr;
-    uint2 *d_pairs = nullptr;   // (query, group) pairs of the mask's first level
     int cus = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
@@ -710,12 +636,12 @@ void batch_free(topk_ctx *ctx) {
     drop_graphs(*B);
     hipFree(B->d_in);
     hipFree(B->d_out);
-    hipFree(B->d_masks);
+    hipFree(B->d_plist);
+    hipFree(B->d_pcnt);
     hipFree(B->d_items);
     hipFree(B->d_ctr);
     hipFree(B->d_ccount);
     hipFree(B->d_cand);
-    hipFree(B->d_pairs);
     if (B->h_in) hipHostFree(B->h_in);
     if (B->h_out) hipHostFree(B->h_out);
     delete B;
@@ -738,6 +664,7 @@ static int grow_dev(T *&p, uint64_t &cap, uint64_t need, bool &changed) {
 static uint32_t group_height(uint32_t p2) {
     uint32_t h = 0;
     while (h < 6 && (1u << (h + 1)) <= p2) h++;
+    while ((p2 >> h) > BMASK_MAX_G) h++;
     return h;
 }
 
@@ -783,12 +710,12 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks) {
         TK_HIP(hipHostMalloc(&B.h_out, out_need, hipHostMallocDefault));
         B.out_cap = out_need;
     }
-    int rc = grow_dev(B.d_masks, B.mask_cap, (uint64_t)L * nw, changed);
+    int rc = grow_dev(B.d_plist, B.plist_cap, (uint64_t)L * nq, changed);
+    if (!rc) rc = grow_dev(B.d_pcnt, B.pcnt_cap, (uint64_t)L, changed);
     // budget >= 16 queries, chunks >= ITEM_MIN_OBJECTS objects
     if (!rc)
         rc = grow_dev(B.d_items, B.item_cap,
                       (uint64_t)L * (4 * nw + 1) * ((P.maxsz + ITEM_MIN_OBJECTS - 1) / ITEM_MIN_OBJECTS + 1), changed);
-    if (!rc) rc = grow_dev(B.d_pairs, B.pair_cap, (uint64_t)(P.p2 >> group_height(P.p2)) * nq, changed);
     if (rc) return rc;
     if (!B.d_ctr) {
         changed = true;
@@ -847,18 +774,18 @@ static void launch_score(int D, const BatchVariant &v, hipStream_t s, const Scor
 static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
                         const BatchVariant &v, bool second) {
     const PtaLayout &P = ctx->pta;
-    const uint32_t L = P.nparts, nw = (nq + 63) / 64;
-    const uint64_t nmask = (uint64_t)L * nw;
+    const uint32_t L = P.nparts;
     const InLayout il = InLayout::make(nq);
     const OutLayout ol = OutLayout::make(nq, ks);
     BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
     BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
     uint32_t *qflag = reinterpret_cast<uint32_t *>(B.d_out + ol.fl_off);
     const uint32_t *pass_flags = second ? qflag : nullptr;
-    uint32_t zg = (uint32_t)((nmask + BAUX_BS - 1) / BAUX_BS);
-    if (zg > 2048) zg = 2048;
-    if (zg < 1) zg = 1;
-    hipLaunchKernelGGL(batch_zero_kernel, dim3(zg), dim3(BAUX_BS), 0, s, B.d_masks, nmask, B.d_ccount, nq, B.d_ctr);
+    if (second) {  // the first pass's counters were cleared by the prep kernel
+        uint32_t zg = ((L > nq ? L : nq) + BAUX_BS - 1) / BAUX_BS;
+        if (zg > 1024) zg = 1024;
+        hipLaunchKernelGGL(batch_zero_kernel, dim3(zg), dim3(BAUX_BS), 0, s, B.d_pcnt, L, B.d_ccount, nq, B.d_ctr);
+    }
     const TreeArgs ta{P.tmax, P.tmin, P.tsum, P.tminsum, P.p2, ctx->d};
     hipEvent_t evt = nullptr;
     timing_begin(ctx, s, evt);
@@ -873,18 +800,18 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                second ? "batch_tau_second" : (mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given")),
                evt);
     const uint32_t H = group_height(P.p2), G = P.p2 >> H;
-    hipLaunchKernelGGL(batch_group_kernel, dim3((G + BAUX_BS - 1) / BAUX_BS, nq), dim3(BAUX_BS), 0, s, dq, db, ta, G,
-                       B.d_pairs, B.d_ctr, pass_flags);
-    hipLaunchKernelGGL(batch_leaf_kernel, dim3(2048), dim3(BAUX_BS), 0, s, dq, db, ta, H, L, nw, B.d_pairs, B.d_ctr,
-                       B.d_masks);
+    hipLaunchKernelGGL(batch_mask_kernel, dim3(nq), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.d_pcnt,
+                       B.d_plist, pass_flags);
     const uint32_t ochunk = item_objects(v);
-    hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_masks, L,
-                       nw, nq, (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
-    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.item_cap, ochunk, B.d_masks, nw, B.d_ctr,
+    hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_pcnt, L,
+                       (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
+    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.item_cap, ochunk, B.d_plist, nq, B.d_ctr,
                  dq, B.d_cand, B.d_ccount};
+    BatchVariant vs = v;
+    if (second) vs.grid = v.grid / 8 > 32 ? v.grid / 8 : 32;  // the second pass has few items
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
-    launch_score(dims_class(ctx->d), v, s, sa);
+    launch_score(dims_class(ctx->d), vs, s, sa);
     timing_end(ctx, s, second ? "batch_score_second" : "batch_score", ev0);
     FinalArgs fa{dq,
                  B.d_cand,
@@ -916,10 +843,14 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
     BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
     TK_HIP(hipMemcpyAsync(B.d_in, B.h_in, il.host, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(batch_prep_kernel, dim3((nq + 127) / 128), dim3(128), 0, s,
+    const uint32_t L = ctx->pta.nparts;
+    uint32_t pg = (L + 63) / 64;
+    if (pg > 1024) pg = 1024;
+    if (pg < nq) pg = nq;
+    hipLaunchKernelGGL(batch_prep_kernel, dim3(pg), dim3(64), 0, s,
                        reinterpret_cast<const topk_query *>(B.d_in + il.raw_off),
                        mode == BMODE_GIVEN ? reinterpret_cast<const float *>(B.d_in + il.tau_off) : nullptr, nq, ctx->n,
-                       ctx->d, dq, db);
+                       ctx->d, dq, db, B.d_pcnt, L, B.d_ccount, B.d_ctr);
     int rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, false);
     if (!rc) rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, true);
     if (rc) return rc;
