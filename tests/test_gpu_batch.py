@@ -162,17 +162,19 @@ def test_batch_graph_replays(gpu):
                     assert np.array_equal(gi, ri) and np.array_equal(gs, rs) and gt == rt
 
 
-def test_batch_overflow_falls_back(gpu):
-    """Heavy ties push a query's candidate list past its capacity; the query
-    is answered by the exact FTE path and counted as a fallback."""
+def test_batch_tie_overflow_reruns_with_key_threshold(gpu):
+    """Heavy ties push a query's candidate list past its capacity; the second
+    pass reruns it with the k-th key of its stored candidates as threshold,
+    which keeps ties with larger ids out, so the batch settles it itself."""
     n, d = 200_000, 2
     x = np.zeros((d, n), np.float32)
     x[:, ::2] = 1.0  # 100000 objects tie at the top score
     with topk_amd.Engine(x) as eng:
         qs = [(100, np.ones(d, np.float32), np.array([0, 1], np.uint8)),
-              (10, np.array([1.0, -1.0], np.float32), np.array([1, 0], np.uint8))]
+              (10, np.array([1.0, -1.0], np.float32), np.array([1, 0], np.uint8)),
+              (1024, np.ones(d, np.float32), np.array([1], np.uint8))]
         st = check_batch(eng, x, n, qs)
-        assert st["fallback"] >= 1
+        assert st["fallback"] == 0
 
 
 def test_batch_errors(gpu):

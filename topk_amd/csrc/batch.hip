@@ -8,32 +8,41 @@
 // use).  One call is a fixed kernel sequence on one stream, captured once
 // per batch shape in a hipGraph and replayed:
 //
-//  1. threshold per query, tau_q (a score: objects >= tau_q are candidates)
-//       - seed mode: LP bounds of every (query, partition) pair; the exact
-//         k-th score among the first objects of the query's 4 best-bounded
-//         partitions is a proven lower bound of the query's k-th score.
+//  0. prep (one wave per query): the device form of every query (terms,
+//     ascending form, LP bound arguments) and cleared per-batch counters.
+//  1. threshold per query, a ranking key (tau_q, tau_id_q): objects whose
+//     (score, id) key reaches it are candidates.
+//       - seed mode: a beam search down the bound tree picks the query's 4
+//         best-bounded partitions; the exact k-th key among their first
+//         16384 objects is a lower bound of the query's k-th key.
 //       - learned mode: an MLP predicts the k-th score from (query, data
 //         sketch), lowered by a calibrated margin (learned.hip).
-//  2. mask: pair (p, q) is scored iff bound(p, q) >= tau_q; set bits are
-//     OR-ed into a per-partition query bitmap.
-//  3. compaction: non-empty (partition, query group) bitmaps become work
-//     items.
+//  2. mask (one workgroup per query): groups of 2^H partitions whose bound
+//     reaches tau_q, then their partitions; q is appended to the query list
+//     of every partition whose bound reaches tau_q.
+//  3. compaction (one thread per partition): each query list becomes work
+//     items of at most qpi queries over object chunks; items with many
+//     (object, query) pairs go to the front of the list.
 //  4. scoring (persistent workgroups taking items from a global ticket):
-//     a workgroup loads the partition's objects into registers once and
-//     scores them for every query of the item; survivors (score >= tau_q)
-//     are appended to the query's candidate list (wave-aggregated atomic).
-//     Scores use the query's own term order: a wave-uniform switch on the
-//     attribute id picks the register column, so any attribute order,
-//     subset or repeat costs the same as the plain weighted sum.
-//  5. final select per query (one workgroup each): radix select in LDS over
-//     the candidates held in registers, rank sort, best-first output.
+//     a workgroup loads the chunk's objects into registers once and scores
+//     them for every query of the item; survivors are appended to the
+//     query's candidate list (wave-aggregated atomic).  Scores use the
+//     query's own term order: a fixed ascending register walk when the terms
+//     are ascending, else a wave-uniform switch on the attribute id.
+//  5. final select per query (one workgroup each): radix select over the
+//     candidates (in registers up to 8192, else in memory), rank sort,
+//     best-first output.
+//  6. a second pass of 0-5 in the same graph for the queries the first left
+//     open: a predicted threshold with fewer than k candidates reruns with a
+//     lower retry threshold (or a seed), and a candidate list over capacity
+//     reruns with the k-th key of its stored candidates as threshold (ties
+//     past that key stay out).
 //
-// Exactness: every object with score >= tau_q lies in a partition whose
-// bound is >= tau_q, so it is a candidate.  With c >= k candidates the top k
-// are among them.  A learned tau_q above the true k-th score leaves c < k
-// (status RETRY) and a candidate list over its capacity raises OVERFLOW;
-// both queries are answered by the plain exact FTE path on the host side of
-// the call, and counted in topk_stats.fallback.
+// Exactness: every object whose key reaches the threshold lies in a
+// partition whose bound reaches tau_q, so it is a candidate.  With c >= k
+// candidates the top k are among them.  A query still open after the second
+// pass (too few candidates or too many) is answered by the plain exact FTE
+// path on the host side of the call and counted in topk_stats.fallback.
 #include <string.h>
 
 #include <array>
@@ -135,6 +144,7 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
     if (lane == 0) {
         b.tau = taus ? taus[q] : 0.0f;
         b.tau_retry = -INFINITY;
+        b.tau_id = 0xFFFFFFFFu;
         b.k = (uint32_t)(n < Q.k ? n : Q.k);
         b.kuser = Q.k;
         b.m = qq;
@@ -276,40 +286,62 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 #pragma unroll
     for (int j = 0; j < BSEED_KPT; j++) rk.k[j] = ok[j] ? rkey(sc[j], pid[pos[j]]) : 0ull;
     float tau = -INFINITY;
-    if (total >= Q.k) tau = rkey_score(block_kth_largest<BSEED_BS>(rk, total, Q.k, ss));
-    if (tid == 0) qs[q].tau = tau;
+    uint32_t tau_id = 0xFFFFFFFFu;
+    if (total >= Q.k) {
+        const uint64_t kth = block_kth_largest<BSEED_BS>(rk, total, Q.k, ss);
+        tau = rkey_score(kth);
+        tau_id = rkey_id(kth);
+    }
+    if (tid == 0) {
+        qs[q].tau = tau;
+        qs[q].tau_id = tau_id;
+    }
 }
 
-// One workgroup per query walks the bound tree two levels deep: tree nodes
-// of height H (groups of 2^H partitions, nodes G..2G-1) whose bound reaches
-// tau_q go to an LDS list; every partition of a listed group whose own
-// bound reaches tau_q gets q appended to its query list plist[p * nq ...]
-// (one atomic per (partition, query) pair on the partition's counter).
+// Workgroup (q, y) of S per query walks the bound tree two levels deep:
+// tree nodes of height H (groups of 2^H partitions, nodes G..2G-1; slice y
+// takes groups y, y + S, ...) whose bound reaches tau_q go to an LDS list;
+// every partition of a listed group whose own bound reaches tau_q gets q
+// appended to its query list plist[p * nq ...] (one atomic per (partition,
+// query) pair on the partition's counter).  The partition loop runs four
+// partitions per thread at a time so their loads and atomics overlap.
 __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *__restrict__ qs,
                                                               const BoundArgs *__restrict__ qb, TreeArgs t,
                                                               uint32_t G, uint32_t H, uint32_t L, uint32_t nq,
                                                               unsigned int *__restrict__ pcnt,
                                                               uint16_t *__restrict__ plist,
                                                               const uint32_t *__restrict__ qflag) {
+    constexpr int U = 4;
     __shared__ uint32_t list[BMASK_MAX_G];
     __shared__ uint32_t nlist;
-    const uint32_t q = blockIdx.x, tid = threadIdx.x;
+    const uint32_t q = blockIdx.x, y = blockIdx.y, S = gridDim.y, tid = threadIdx.x;
     if (qflag && qflag[q] == QF_DONE) return;  // second pass: settled queries take no part
     if (tid == 0) nlist = 0;
     __syncthreads();
     const BoundArgs &b = qb[q];
     const float tau = qs[q].tau;
-    for (uint32_t g = tid; g < G; g += BMASK_BS) {
+    for (uint32_t g = y + S * tid; g < G; g += S * BMASK_BS) {
         const float ub = node_bound(b, t, G + g);
         if (ub > -INFINITY && ub >= tau) list[atomicAdd(&nlist, 1u)] = g;
     }
     __syncthreads();
-    const uint32_t nl = nlist, per = 1u << H;
-    for (uint32_t i = tid; i < nl * per; i += BMASK_BS) {
-        const uint32_t p = (list[i >> H] << H) + (i & (per - 1));
-        if (p >= L) continue;
-        const float ub = node_bound(b, t, t.p2 + p);
-        if (ub > -INFINITY && ub >= tau) plist[(uint64_t)p * nq + atomicAdd(&pcnt[p], 1u)] = (uint16_t)q;
+    const uint32_t nl = nlist, per = 1u << H, total = nl * per;
+    for (uint32_t i0 = tid; i0 < total; i0 += U * BMASK_BS) {
+        uint32_t p[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * BMASK_BS;
+            p[u] = i < total ? (list[i >> H] << H) + (i & (per - 1)) : L;
+            const float ub = p[u] < L ? node_bound(b, t, t.p2 + p[u]) : -INFINITY;
+            in[u] = ub > -INFINITY && ub >= tau;
+        }
+        uint32_t pos[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pos[u] = in[u] ? atomicAdd(&pcnt[p[u]], 1u) : 0u;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (in[u]) plist[(uint64_t)p[u] * nq + pos[u]] = (uint16_t)q;
     }
 }
 
@@ -471,10 +503,17 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                         }
                     }
                 }
+                // admitted: key (s, id) >= (tau, tau_id); the id is read only on a tie
                 const float tau = Q.tau;
-                uint32_t np = 0;
+                const uint32_t tau_id = Q.tau_id;
+                uint32_t adm = 0;
 #pragma unroll
-                for (int e = 0; e < VPT; e++) np += ((uint32_t)e < nvalid && s[e] >= tau) ? 1u : 0u;
+                for (int e = 0; e < VPT; e++) {
+                    bool in = (uint32_t)e < nvalid && s[e] >= tau;
+                    if (in && s[e] == tau && tau_id != 0xFFFFFFFFu) in = a.pid[start + i0 + e] <= tau_id;
+                    adm |= in ? 1u << e : 0u;
+                }
+                const uint32_t np = (uint32_t)__popc(adm);
                 if (__ballot(np != 0)) {
                     const uint32_t incl = wave_incl_scan(np);
                     const uint32_t tot = __shfl(incl, 63, 64);
@@ -485,7 +524,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     uint64_t *out = a.cand + (uint64_t)q * BATCH_CAP;
 #pragma unroll
                     for (int e = 0; e < VPT; e++) {
-                        if ((uint32_t)e < nvalid && s[e] >= tau) {
+                        if ((adm >> e) & 1u) {
                             if (pos < BATCH_CAP) out[pos] = rkey(s[e], a.pid[start + i0 + e]);
                             pos++;
                         }
@@ -537,6 +576,7 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
         if (tid == 0) {
             a.status[q] = STATUS_OVERFLOW;
             a.q[q].tau = rkey_score(kth);
+            a.q[q].tau_id = rkey_id(kth);
             a.qflag[q] = QF_GIVEN;
         }
         return;
@@ -550,6 +590,7 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
                 const float tr = a.q[q].tau_retry;
                 if (tr < a.q[q].tau) {
                     a.q[q].tau = tr;
+                    a.q[q].tau_id = 0xFFFFFFFFu;
                     a.qflag[q] = QF_LOWER;
                 } else {
                     a.qflag[q] = QF_SEED;
@@ -800,7 +841,10 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                second ? "batch_tau_second" : (mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given")),
                evt);
     const uint32_t H = group_height(P.p2), G = P.p2 >> H;
-    hipLaunchKernelGGL(batch_mask_kernel, dim3(nq), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.d_pcnt,
+    // slices per query: enough workgroups to cover the CUs on small batches;
+    // the second pass reruns few queries, so each gets many
+    const uint32_t S = second ? 16u : std::min(16u, std::max(1u, (4u * (uint32_t)B.cus + nq - 1) / nq));
+    hipLaunchKernelGGL(batch_mask_kernel, dim3(nq, S), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.d_pcnt,
                        B.d_plist, pass_flags);
     const uint32_t ochunk = item_objects(v);
     hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_pcnt, L,
@@ -906,6 +950,7 @@ int topk::make_batch_query(const topk_ctx *ctx, const topk_query &Q, BatchQuery 
     }
     if (b.ascending)
         for (uint32_t j = 0; j < Q.qq; j++) b.wa[Q.attr[j]] = b.w[j];
+    b.tau_id = 0xFFFFFFFFu;
     return TOPK_OK;
 }
 

@@ -93,6 +93,10 @@ struct BatchQuery {
     // Lower threshold for a rerun when tau left fewer than k candidates
     // (written with a predicted tau; -inf: rerun with a seed threshold).
     float tau_retry;
+    // (tau, tau_id) is a ranking-key threshold: an object scoring exactly tau
+    // is a candidate only when its id <= tau_id (0xFFFFFFFF admits every tie),
+    // so a threshold taken from a known key keeps ties past it out.
+    uint32_t tau_id;
 };
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
