@@ -440,22 +440,32 @@ __device__ __forceinline__ void load_objs(const float *__restrict__ p, float (&x
 
 // Persistent scoring kernel: D = attribute registers per object (>= d),
 // VPT consecutive objects per thread, BS threads per workgroup.
+//
+// Latency: the query list of an item and, for D <= 16, every listed query's
+// ascending-form weights and threshold are loaded once per item, one query
+// per lane; the per-query loop reads them with readlane, so it waits on no
+// memory.  The ticket of the next item is taken after the current item's
+// loads are issued and is read only when the item is done.
 template <int D, int VPT, int BS>
 __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
-    __shared__ uint32_t s_it;
+    constexpr bool LANE_W = D <= 16;
+    __shared__ uint32_t s_it[2];
     const uint32_t tid = threadIdx.x, lane = lane_id();
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(a.ctr[0]);
     const uint32_t nlight = __builtin_amdgcn_readfirstlane(a.ctr[7]);
-    for (;;) {
-        if (tid == 0) s_it = atomicAdd(&a.ctr[1], 1u);
-        __syncthreads();
-        const uint32_t it = __builtin_amdgcn_readfirstlane(s_it);
-        __syncthreads();
-        if (it >= nheavy + nlight) break;
+    const uint32_t nitems = nheavy + nlight;
+    if (tid == 0) s_it[0] = atomicAdd(&a.ctr[1], 1u);
+    __syncthreads();
+    uint32_t it = __builtin_amdgcn_readfirstlane(s_it[0]);
+    uint32_t slot = 1;
+    while (it < nitems) {
         const BatchItem I = it < nheavy ? a.items[it] : a.items[a.cap - 1 - (it - nheavy)];
         const uint32_t first = I.chunk * a.ochunk, psz_ = a.psz[I.p];
         const uint32_t start = a.poff[I.p] + first;
         const uint32_t size = psz_ - first < a.ochunk ? psz_ - first : a.ochunk;
+        const uint16_t *ql = a.plist + (uint64_t)I.p * a.nq + I.qbeg;
+        uint32_t nxt = 0;
+        bool ticket = tid == 0;
         for (uint32_t b0 = 0; b0 < size; b0 += BS * VPT) {
             const uint32_t i0 = b0 + tid * VPT;
             const uint32_t nvalid = i0 < size ? (size - i0 < (uint32_t)VPT ? size - i0 : (uint32_t)VPT) : 0u;
@@ -469,70 +479,99 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     for (int e = 0; e < VPT; e++) x[c][e] = 0.0f;
                 }
             }
-            {
-            const uint16_t *ql = a.plist + (uint64_t)I.p * a.nq + I.qbeg;
-            for (uint32_t qi = 0; qi < I.qcnt; qi++) {
-                const uint32_t q = __builtin_amdgcn_readfirstlane((uint32_t)ql[qi]);
-                const BatchQuery &Q = a.q[q];
-                float s[VPT];
+            for (uint32_t q0 = 0; q0 < I.qcnt; q0 += 64) {
+                const uint32_t nqb = I.qcnt - q0 < 64u ? I.qcnt - q0 : 64u;
+                // lane j: query q0 + j of the item
+                const uint32_t lq = lane < nqb ? (uint32_t)ql[q0 + lane] : 0u;
+                const f4 lt = *reinterpret_cast<const f4 *>(&a.q[lq].tau);  // tau, tau_id, ascending, m
+                float lw[LANE_W ? D : 1];
+                if constexpr (LANE_W) {
 #pragma unroll
-                for (int e = 0; e < VPT; e++) s[e] = 0.0f;
-                if (Q.ascending) {
-                    // static register order, zero weights skipped by a uniform branch
+                    for (int c = 0; c < D; c += 4) {
+                        const f4 v = *reinterpret_cast<const f4 *>(&a.q[lq].wa[c]);
+                        lw[c] = v[0];
+                        lw[c + 1] = v[1];
+                        lw[c + 2] = v[2];
+                        lw[c + 3] = v[3];
+                    }
+                }
+                if (ticket) {
+                    nxt = atomicAdd(&a.ctr[1], 1u);
+                    ticket = false;
+                }
+                for (uint32_t qi = 0; qi < nqb; qi++) {
+                    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)lq, (int)qi);
+                    const uint32_t asc = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[2]), (int)qi);
+                    float s[VPT];
 #pragma unroll
-                    for (int c = 0; c < D; c++) {
-                        const float w = Q.wa[c];
-                        if (w != 0.0f) {
+                    for (int e = 0; e < VPT; e++) s[e] = 0.0f;
+                    if (asc) {
+                        // static register order, zero weights skipped by a uniform branch
 #pragma unroll
-                            for (int e = 0; e < VPT; e++) {
-                                const float pr = x[c][e] * w;
-                                s[e] = s[e] + pr;
+                        for (int c = 0; c < D; c++) {
+                            float w;
+                            if constexpr (LANE_W)
+                                w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lw[LANE_W ? c : 0]),
+                                                                             (int)qi));
+                            else
+                                w = a.q[q].wa[c];
+                            if (w != 0.0f) {
+#pragma unroll
+                                for (int e = 0; e < VPT; e++) {
+                                    const float pr = x[c][e] * w;
+                                    s[e] = s[e] + pr;
+                                }
+                            }
+                        }
+                    } else {
+                        const BatchQuery &Q = a.q[q];
+                        for (uint32_t j = 0; j < Q.m; j++) {
+                            const float w = Q.w[j];
+                            switch (Q.attr[j]) {
+                                TK_TERM(0) TK_TERM(1) TK_TERM(2) TK_TERM(3) TK_TERM(4) TK_TERM(5) TK_TERM(6)
+                                TK_TERM(7) TK_TERM(8) TK_TERM(9) TK_TERM(10) TK_TERM(11) TK_TERM(12) TK_TERM(13)
+                                TK_TERM(14) TK_TERM(15) TK_TERM(16) TK_TERM(17) TK_TERM(18) TK_TERM(19) TK_TERM(20)
+                                TK_TERM(21) TK_TERM(22) TK_TERM(23) TK_TERM(24) TK_TERM(25) TK_TERM(26) TK_TERM(27)
+                                TK_TERM(28) TK_TERM(29) TK_TERM(30) TK_TERM(31)
+                                default: break;
                             }
                         }
                     }
-                } else {
-                    for (uint32_t j = 0; j < Q.m; j++) {
-                        const float w = Q.w[j];
-                        switch (Q.attr[j]) {
-                            TK_TERM(0) TK_TERM(1) TK_TERM(2) TK_TERM(3) TK_TERM(4) TK_TERM(5) TK_TERM(6)
-                            TK_TERM(7) TK_TERM(8) TK_TERM(9) TK_TERM(10) TK_TERM(11) TK_TERM(12) TK_TERM(13)
-                            TK_TERM(14) TK_TERM(15) TK_TERM(16) TK_TERM(17) TK_TERM(18) TK_TERM(19) TK_TERM(20)
-                            TK_TERM(21) TK_TERM(22) TK_TERM(23) TK_TERM(24) TK_TERM(25) TK_TERM(26) TK_TERM(27)
-                            TK_TERM(28) TK_TERM(29) TK_TERM(30) TK_TERM(31)
-                            default: break;
-                        }
-                    }
-                }
-                // admitted: key (s, id) >= (tau, tau_id); the id is read only on a tie
-                const float tau = Q.tau;
-                const uint32_t tau_id = Q.tau_id;
-                uint32_t adm = 0;
-#pragma unroll
-                for (int e = 0; e < VPT; e++) {
-                    bool in = (uint32_t)e < nvalid && s[e] >= tau;
-                    if (in && s[e] == tau && tau_id != 0xFFFFFFFFu) in = a.pid[start + i0 + e] <= tau_id;
-                    adm |= in ? 1u << e : 0u;
-                }
-                const uint32_t np = (uint32_t)__popc(adm);
-                if (__ballot(np != 0)) {
-                    const uint32_t incl = wave_incl_scan(np);
-                    const uint32_t tot = __shfl(incl, 63, 64);
-                    uint32_t base = 0;
-                    if (lane == 63) base = atomicAdd(&a.ccount[q], tot);
-                    base = __shfl(base, 63, 64);
-                    uint32_t pos = base + incl - np;
-                    uint64_t *out = a.cand + (uint64_t)q * BATCH_CAP;
+                    // admitted: key (s, id) >= (tau, tau_id); the id is read only on a tie
+                    const float tau = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lt[0]), (int)qi));
+                    const uint32_t tau_id = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[1]), (int)qi);
+                    uint32_t adm = 0;
 #pragma unroll
                     for (int e = 0; e < VPT; e++) {
-                        if ((adm >> e) & 1u) {
-                            if (pos < BATCH_CAP) out[pos] = rkey(s[e], a.pid[start + i0 + e]);
-                            pos++;
+                        bool in = (uint32_t)e < nvalid && s[e] >= tau;
+                        if (in && s[e] == tau && tau_id != 0xFFFFFFFFu) in = a.pid[start + i0 + e] <= tau_id;
+                        adm |= in ? 1u << e : 0u;
+                    }
+                    const uint32_t np = (uint32_t)__popc(adm);
+                    if (__ballot(np != 0)) {
+                        const uint32_t incl = wave_incl_scan(np);
+                        const uint32_t tot = __shfl(incl, 63, 64);
+                        uint32_t base = 0;
+                        if (lane == 63) base = atomicAdd(&a.ccount[q], tot);
+                        base = __shfl(base, 63, 64);
+                        uint32_t pos = base + incl - np;
+                        uint64_t *out = a.cand + (uint64_t)q * BATCH_CAP;
+#pragma unroll
+                        for (int e = 0; e < VPT; e++) {
+                            if ((adm >> e) & 1u) {
+                                if (pos < BATCH_CAP) out[pos] = rkey(s[e], a.pid[start + i0 + e]);
+                                pos++;
+                            }
                         }
                     }
                 }
             }
-            }
         }
+        if (ticket) nxt = atomicAdd(&a.ctr[1], 1u);  // an item without queries
+        if (tid == 0) s_it[slot] = nxt;
+        __syncthreads();
+        it = __builtin_amdgcn_readfirstlane(s_it[slot]);
+        slot ^= 1u;
     }
 }
 #undef TK_TERM

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <map>
@@ -77,27 +78,35 @@ struct Tuning {
 
 // One query of a batch as the device kernels read it (prepared on the host;
 // tau is written on the device by the seed pass or the learned predictor).
-struct BatchQuery {
-    float tau;                      // objects scoring >= tau are candidates
-    uint32_t k;                     // min(k, n)
-    uint32_t m;                     // terms
-    uint32_t kuser;                 // k as requested
-    uint8_t attr[TOPK_MAX_DIMS];    // term j reads attribute attr[j]
-    float w[TOPK_MAX_DIMS];         // weight of term j
+// wa opens the record and (tau, tau_id, ascending, m) share one 16-byte
+// word, so a kernel reads them with 16-byte loads.
+struct alignas(16) BatchQuery {
     // Ascending form: when attr[] is strictly ascending, the score equals
     // the sum over all attributes a in ascending order of x[a] * wa[a] with
     // wa[a] = 0 for attributes not queried (x * 0 is +-0 and s + (+-0) == s
     // because a score is never -0).
-    uint32_t ascending;
     float wa[TOPK_MAX_DIMS];
-    // Lower threshold for a rerun when tau left fewer than k candidates
-    // (written with a predicted tau; -inf: rerun with a seed threshold).
-    float tau_retry;
+    float w[TOPK_MAX_DIMS];         // weight of term j
+    float tau;                      // objects scoring >= tau are candidates
     // (tau, tau_id) is a ranking-key threshold: an object scoring exactly tau
     // is a candidate only when its id <= tau_id (0xFFFFFFFF admits every tie),
     // so a threshold taken from a known key keeps ties past it out.
     uint32_t tau_id;
+    uint32_t ascending;
+    uint32_t m;                     // terms
+    // Lower threshold for a rerun when tau left fewer than k candidates
+    // (written with a predicted tau; -inf: rerun with a seed threshold).
+    float tau_retry;
+    uint32_t k;                     // min(k, n)
+    uint32_t kuser;                 // k as requested
+    uint32_t pad_;
+    uint8_t attr[TOPK_MAX_DIMS];    // term j reads attribute attr[j]
 };
+static_assert(sizeof(BatchQuery) % 16 == 0, "BatchQuery records stay 16-byte aligned");
+static_assert(offsetof(BatchQuery, tau) % 16 == 0 && offsetof(BatchQuery, tau_id) == offsetof(BatchQuery, tau) + 4 &&
+                  offsetof(BatchQuery, ascending) == offsetof(BatchQuery, tau) + 8 &&
+                  offsetof(BatchQuery, m) == offsetof(BatchQuery, tau) + 12,
+              "tau, tau_id, ascending, m form one 16-byte word");
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
 #define TK_SWITCH_M(m, CASE)          \
