@@ -276,6 +276,10 @@ void *topk_stream(topk_ctx *ctx);
  *   "batch_graph"     1 = capture each batch shape in a hipGraph and replay
  *   "batch_mode"      0 = learned threshold when trained, else seed;
  *                     1 = seed threshold; 2 = learned threshold
+ *   "batch_mask"      layout of the batch's partition-mask kernel:
+ *                     0 = query-major first pass, group-major second pass;
+ *                     1 = query-major (one workgroup per query);
+ *                     2 = group-major (one per partition group and query slice)
  * (0 for a batch knob = default or the autotuner's choice.)
  * Unknown names or values return TOPK_ERR_INVALID. */
 int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value);

@@ -73,8 +73,8 @@ def test_batch_parity_d16_mixed(gpu):
         qs = make_queries(rng, 1024, d, [16, 32, 64, 128, 256])
         st = check_batch(eng, x, n, qs)
         # the correlated third clips many values to exactly 1.0: a query on few
-        # attributes can tie more objects at its top score than a candidate
-        # list holds, and such a query is answered by the exact FTE path
+        # attributes ties more objects at its top score than a candidate list
+        # holds, which the second pass settles with a ranking-key threshold
         assert st["fallback"] <= 0.01 * len(qs)
 
 
@@ -111,21 +111,22 @@ def test_batch_small_stores(gpu, n):
 
 
 def test_batch_variants_agree(gpu):
-    """Every scoring shape, query grouping and digit width gives the same bits."""
+    """Every scoring shape, query grouping, digit width and mask layout gives
+    the same bits."""
     n, d = 400_000, 8
     rng = np.random.default_rng(21)
     x = topk_amd.generate(n, d, topk_amd.MIXED, 21)
     qs = make_queries(rng, 200, d, [16, 64, 256])
     ref = oracle_batch(x, n, qs)
     with topk_amd.Engine(x) as eng:
-        for vpt, bs, qpi, digit in [(4, 256, 64, 11), (8, 512, 32, 8), (16, 256, 16, 11), (8, 256, 256, 8),
-                                    (8, 256, 1024, 11)]:
+        for vpt, bs, qpi, digit, mask in [(4, 256, 64, 11, 1), (8, 512, 32, 8, 2), (16, 256, 16, 11, 0),
+                                          (8, 256, 256, 8, 2), (8, 256, 1024, 11, 1)]:
             for name, v in (("batch_vpt", vpt), ("batch_bs", bs), ("batch_qpi", qpi), ("batch_digit", digit),
-                            ("batch_grid", 300)):
+                            ("batch_grid", 300), ("batch_mask", mask)):
                 eng.set_tuning(name, v)
             got = eng.batch(qs)
             for (gi, gs, _), (ri, rs) in zip(got, ref):
-                assert np.array_equal(gi, ri) and np.array_equal(gs, rs), (vpt, bs, qpi, digit)
+                assert np.array_equal(gi, ri) and np.array_equal(gs, rs), (vpt, bs, qpi, digit, mask)
 
 
 def test_batch_autotune_keeps_results(gpu):
@@ -188,3 +189,5 @@ def test_batch_errors(gpu):
         assert e.value.status == 4
         with pytest.raises(topk_amd.TopkError):
             eng.set_tuning("batch_qpi", 8)
+        with pytest.raises(topk_amd.TopkError):
+            eng.set_tuning("batch_mask", 3)

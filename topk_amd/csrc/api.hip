@@ -428,6 +428,9 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
     } else if (key == "batch_mode") {
         if (value < 0 || value > 2) return TOPK_ERR_INVALID;
         ctx->tune.batch_mode = (int)value;
+    } else if (key == "batch_mask") {
+        if (value < 0 || value > 2) return TOPK_ERR_INVALID;
+        ctx->tune.batch_mask = (int)value;
     } else {
         return TOPK_ERR_INVALID;
     }

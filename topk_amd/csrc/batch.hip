@@ -65,15 +65,15 @@ constexpr uint32_t BSEED_T = 4;                          // seed partitions per 
 constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
 constexpr uint32_t ITEM_MIN_OBJECTS = 2048;              // smallest object chunk of a work item
-constexpr int BMASK_BS = 256;                            // mask: one workgroup per query
-constexpr uint32_t BMASK_MAX_G = 4096;                   // groups of the mask's first level (LDS list)
+constexpr int BMASK_BS = 256;                            // mask workgroup
+constexpr uint32_t BMASK_MAX_PER = 64;                   // partitions per mask group (2^H)
+constexpr uint32_t BMASK_PAIRS = 4096;                   // (query, partition) pairs a mask workgroup holds
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 // Per-query flags the first pass leaves for the second (in-graph) pass.
 enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
 // Device counters of one batch: [0] heavy work items (front of the list),
-// [1] work ticket, [2..3] scored (partition, query) pairs, [4..5] scored
-// (object, query) pairs, [6] (query, group) pairs of the mask's first
-// level, [7] light work items (back of the list).
+// [1] light work items (back of the list), [2..3] scored (partition, query)
+// pairs, [4..5] scored (object, query) pairs, [6] work ticket, [7] unused.
 constexpr uint32_t BCTR_WORDS = 8;
 
 // Work item: objects [chunk * ochunk, (chunk + 1) * ochunk) of partition p,
@@ -170,11 +170,14 @@ struct TreeArgs {
     uint32_t p2, d;
 };
 
-// Upper bound of query b's score over tree node v (-inf for an empty node).
+// Upper bound of query b's score over tree node v (-inf for an empty node);
+// NB >= b.nd.  The node's loads are issued together, not behind the
+// emptiness test.
+template <int NB>
 __device__ __forceinline__ float node_bound(const BoundArgs &b, const TreeArgs &t, uint32_t v) {
     const float ts = t.tsum[v];
-    if (ts == -INFINITY) return -INFINITY;
-    return lp_bound(b, t.tmax + (uint64_t)v * t.d, t.tmin + (uint64_t)v * t.d, ts, t.tminsum[v]);
+    const float ub = lp_bound<NB>(b, t.tmax + (uint64_t)v * t.d, t.tmin + (uint64_t)v * t.d, ts, t.tminsum[v]);
+    return ts == -INFINITY ? -INFINITY : ub;
 }
 
 // Score of the object at partition-order position pos, terms in query order.
@@ -194,6 +197,7 @@ __device__ __forceinline__ float batch_score_one(const BatchQuery &Q, const floa
 // all) are scored, and the exact k-th largest key among them is a lower
 // bound of the query's k-th key: its score is tau.  Fewer than k objects:
 // tau = -inf (every object is a candidate).
+template <int NB>
 __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__restrict__ qs,
                                                               const BoundArgs *__restrict__ qb, TreeArgs t,
                                                               uint32_t L, const float *__restrict__ pcols,
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 #pragma unroll
                 for (uint32_t i = 1; i < BSEED_T; i++) parent = lane / 2 == i ? beam[i] : parent;
                 const uint32_t c = 2 * parent + (lane & 1);
-                const float ub = node_bound(b, t, c);
+                const float ub = node_bound<NB>(b, t, c);
                 if (ub > -INFINITY) key = ((uint64_t)okey(ub) << 32) | c;
             }
             uint32_t got = 0;
@@ -298,21 +302,22 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
     }
 }
 
-// Workgroup (q, y) of S per query walks the bound tree two levels deep:
-// tree nodes of height H (groups of 2^H partitions, nodes G..2G-1; slice y
-// takes groups y, y + S, ...) whose bound reaches tau_q go to an LDS list;
-// every partition of a listed group whose own bound reaches tau_q gets q
-// appended to its query list plist[p * nq ...] (one atomic per (partition,
-// query) pair on the partition's counter).  The partition loop runs four
-// partitions per thread at a time so their loads and atomics overlap.
-__global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *__restrict__ qs,
-                                                              const BoundArgs *__restrict__ qb, TreeArgs t,
-                                                              uint32_t G, uint32_t H, uint32_t L, uint32_t nq,
-                                                              unsigned int *__restrict__ pcnt,
-                                                              uint16_t *__restrict__ plist,
-                                                              const uint32_t *__restrict__ qflag) {
-    constexpr int U = 4;
-    __shared__ uint32_t list[BMASK_MAX_G];
+// Query-major mask: workgroup (q, y) of S per query walks the bound tree
+// two levels deep.  Groups y, y + S, ... of 2^H partitions (tree nodes
+// G + g) whose bound reaches tau_q go to an LDS list; every partition of a
+// listed group whose own bound reaches tau_q gets q appended to its query
+// list plist[p * nq ...] (one atomic per (partition, query) pair on the
+// partition's counter).  Each thread evaluates 32 / NB bounds at a time so
+// their loads and atomics overlap.
+template <int NB>
+__global__ __launch_bounds__(BMASK_BS) void batch_mask_query_kernel(const BatchQuery *__restrict__ qs,
+                                                                    const BoundArgs *__restrict__ qb, TreeArgs t,
+                                                                    uint32_t G, uint32_t H, uint32_t L, uint32_t nq,
+                                                                    unsigned int *__restrict__ pcnt,
+                                                                    uint16_t *__restrict__ plist,
+                                                                    const uint32_t *__restrict__ qflag) {
+    constexpr int U = 32 / NB;
+    __shared__ uint32_t list[BMASK_PAIRS];
     __shared__ uint32_t nlist;
     const uint32_t q = blockIdx.x, y = blockIdx.y, S = gridDim.y, tid = threadIdx.x;
     if (qflag && qflag[q] == QF_DONE) return;  // second pass: settled queries take no part
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *
     const BoundArgs &b = qb[q];
     const float tau = qs[q].tau;
     for (uint32_t g = y + S * tid; g < G; g += S * BMASK_BS) {
-        const float ub = node_bound(b, t, G + g);
+        const float ub = node_bound<NB>(b, t, G + g);
         if (ub > -INFINITY && ub >= tau) list[atomicAdd(&nlist, 1u)] = g;
     }
     __syncthreads();
@@ -333,7 +338,7 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *
         for (int u = 0; u < U; u++) {
             const uint32_t i = i0 + u * BMASK_BS;
             p[u] = i < total ? (list[i >> H] << H) + (i & (per - 1)) : L;
-            const float ub = p[u] < L ? node_bound(b, t, t.p2 + p[u]) : -INFINITY;
+            const float ub = p[u] < L ? node_bound<NB>(b, t, t.p2 + p[u]) : -INFINITY;
             in[u] = ub > -INFINITY && ub >= tau;
         }
         uint32_t pos[U];
@@ -345,18 +350,86 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *
     }
 }
 
+// Group-major mask: workgroup (g, y) takes group g of 2^H partitions (tree node G + g) and
+// the query slice [y * QS, (y + 1) * QS): the queries whose bound over the
+// group reaches tau_q go to an LDS list, then every (listed query,
+// partition of the group) pair whose bound reaches tau_q is counted per
+// partition in LDS.  One global atomic per partition reserves room in its
+// query list plist[p * nq ...], so a partition that many queries reach sees
+// one atomic per workgroup, not one per query.  Each thread evaluates
+// 32 / NB pair bounds at a time so their loads overlap.
+template <int NB>
+__global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *__restrict__ qs,
+                                                              const BoundArgs *__restrict__ qb, TreeArgs t,
+                                                              uint32_t G, uint32_t H, uint32_t L, uint32_t nq,
+                                                              uint32_t QS, unsigned int *__restrict__ pcnt,
+                                                              uint16_t *__restrict__ plist,
+                                                              const uint32_t *__restrict__ qflag) {
+    constexpr int U = 32 / NB;  // bounds in flight per thread (registers: ~4 NB per bound)
+    __shared__ uint16_t lq[BMASK_PAIRS];   // listed queries
+    __shared__ uint16_t ent[BMASK_PAIRS];  // pair (i, j): 1 + its slot in partition j's share, 0 = out
+    __shared__ uint32_t lcnt[BMASK_MAX_PER], lbase[BMASK_MAX_PER];
+    __shared__ uint32_t nlist;
+    const uint32_t g = blockIdx.x, tid = threadIdx.x, per = 1u << H;
+    if (tid == 0) nlist = 0;
+    if (tid < per) lcnt[tid] = 0;
+    __syncthreads();
+    const uint32_t qa = blockIdx.y * QS, qe = nq - qa < QS ? nq : qa + QS;
+    for (uint32_t q = qa + tid; q < qe; q += BMASK_BS) {
+        if (qflag && qflag[q] == QF_DONE) continue;  // second pass: settled queries take no part
+        const float ub = node_bound<NB>(qb[q], t, G + g);
+        if (ub > -INFINITY && ub >= qs[q].tau) lq[atomicAdd(&nlist, 1u)] = (uint16_t)q;
+    }
+    __syncthreads();
+    const uint32_t nl = nlist;
+    if (nl == 0) return;
+    const uint32_t total = nl << H, p0 = g << H;
+    for (uint32_t i0 = tid; i0 < total; i0 += U * BMASK_BS) {
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * BMASK_BS, p = p0 + (i & (per - 1));
+            in[u] = false;
+            if (i < total && p < L) {
+                const uint32_t q = lq[i >> H];
+                const float ub = node_bound<NB>(qb[q], t, t.p2 + p);
+                in[u] = ub > -INFINITY && ub >= qs[q].tau;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t i = i0 + u * BMASK_BS;
+            if (i < total) ent[i] = in[u] ? (uint16_t)(atomicAdd(&lcnt[i & (per - 1)], 1u) + 1u) : (uint16_t)0;
+        }
+    }
+    __syncthreads();
+    if (tid < per) {
+        const uint32_t c = lcnt[tid];
+        lbase[tid] = c ? atomicAdd(&pcnt[p0 + tid], c) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < total; i += BMASK_BS) {
+        const uint32_t e = ent[i];
+        if (e) {
+            const uint32_t j = i & (per - 1);
+            plist[(uint64_t)(p0 + j) * nq + lbase[j] + e - 1] = lq[i >> H];
+        }
+    }
+}
+
 // One thread per partition: its query list is cut into ranges of at most
 // qmax queries and every range into object chunks of ochunk.  Ranges holding
 // at least qmax / 2 queries are "heavy" and fill the item list from the
 // front, the others from the back (index cap - 1 - i), so the persistent
-// scoring kernel starts the long items first.  One pair of global atomics
-// per workgroup reserves the items; the pair counters are updated.
+// scoring kernel starts the long items first.  One global atomic per
+// workgroup reserves its items, two more add its pair counts.
 __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned int *__restrict__ pcnt, uint32_t L,
                                                                 uint32_t qmax, uint32_t ochunk, uint64_t cap,
                                                                 const uint32_t *__restrict__ psz,
                                                                 BatchItem *__restrict__ items,
                                                                 unsigned int *__restrict__ ctr) {
     __shared__ uint32_t wsum[BAUX_BS / 64];
+    __shared__ unsigned long long wstat[BAUX_BS / 64][2];
     __shared__ uint32_t tot_h, tot_l, base_h, base_l;
     const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x;
     const uint32_t cnt = p < L ? pcnt[p] : 0u;
@@ -369,16 +442,34 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned i
     const uint32_t nh = nhr * nch, nl = (nr - nhr) * nch;
     const uint32_t offh = block_excl_scan<BAUX_BS>(nh, wsum, &tot_h);
     const uint32_t offl = block_excl_scan<BAUX_BS>(nl, wsum, &tot_l);
-    const uint32_t wp = wave_sum_u32(cnt);
-    unsigned long long objs = (unsigned long long)cnt * sz;
-    for (int o = 32; o > 0; o >>= 1) objs += __shfl_xor(objs, o, 64);
-    if (threadIdx.x == 0) {
-        base_h = tot_h ? atomicAdd(&ctr[0], tot_h) : 0u;
-        base_l = tot_l ? atomicAdd(&ctr[7], tot_l) : 0u;
+    unsigned long long pairs = cnt, objs = (unsigned long long)cnt * sz;
+    for (int o = 32; o > 0; o >>= 1) {
+        pairs += __shfl_xor(pairs, o, 64);
+        objs += __shfl_xor(objs, o, 64);
     }
-    if (lane_id() == 0 && wp) {
-        atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), (unsigned long long)wp);
-        atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 4), objs);
+    if (lane_id() == 0) {
+        wstat[threadIdx.x / 64][0] = pairs;
+        wstat[threadIdx.x / 64][1] = objs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // heavy items count in ctr[0], light in ctr[1]: one 64-bit atomic
+        // reserves both (the low word never carries)
+        const unsigned long long r =
+            tot_h | tot_l ? atomicAdd(reinterpret_cast<unsigned long long *>(ctr),
+                                      ((unsigned long long)tot_l << 32) | tot_h)
+                          : 0ull;
+        base_h = (uint32_t)r;
+        base_l = (uint32_t)(r >> 32);
+        unsigned long long sp = 0, so = 0;
+        for (int w = 0; w < BAUX_BS / 64; w++) {
+            sp += wstat[w][0];
+            so += wstat[w][1];
+        }
+        if (sp) {
+            atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 2), sp);
+            atomicAdd(reinterpret_cast<unsigned long long *>(ctr + 4), so);
+        }
     }
     __syncthreads();
     if (!nr) return;
@@ -452,9 +543,9 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
     __shared__ uint32_t s_it[2];
     const uint32_t tid = threadIdx.x, lane = lane_id();
     const uint32_t nheavy = __builtin_amdgcn_readfirstlane(a.ctr[0]);
-    const uint32_t nlight = __builtin_amdgcn_readfirstlane(a.ctr[7]);
+    const uint32_t nlight = __builtin_amdgcn_readfirstlane(a.ctr[1]);
     const uint32_t nitems = nheavy + nlight;
-    if (tid == 0) s_it[0] = atomicAdd(&a.ctr[1], 1u);
+    if (tid == 0) s_it[0] = atomicAdd(&a.ctr[6], 1u);
     __syncthreads();
     uint32_t it = __builtin_amdgcn_readfirstlane(s_it[0]);
     uint32_t slot = 1;
@@ -496,7 +587,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     }
                 }
                 if (ticket) {
-                    nxt = atomicAdd(&a.ctr[1], 1u);
+                    nxt = atomicAdd(&a.ctr[6], 1u);
                     ticket = false;
                 }
                 for (uint32_t qi = 0; qi < nqb; qi++) {
@@ -567,7 +658,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 }
             }
         }
-        if (ticket) nxt = atomicAdd(&a.ctr[1], 1u);  // an item without queries
+        if (ticket) nxt = atomicAdd(&a.ctr[6], 1u);  // an item without queries
         if (tid == 0) s_it[slot] = nxt;
         __syncthreads();
         it = __builtin_amdgcn_readfirstlane(s_it[slot]);
@@ -651,7 +742,7 @@ struct BatchVariant {
     int vpt, bs, qpi, digit, grid;
 };
 
-using GraphKey = std::array<uint64_t, 10>;
+using GraphKey = std::array<uint64_t, 11>;
 
 struct BatchState {
     uint64_t gen = ~0ull;   // layout generation the buffers and graphs belong to
@@ -741,10 +832,10 @@ static int grow_dev(T *&p, uint64_t &cap, uint64_t need, bool &changed) {
 }
 
 // Tree height of the mask's first level: groups of up to 64 partitions.
+// Height of the mask's groups: 64 partitions, fewer on small layouts.
 static uint32_t group_height(uint32_t p2) {
     uint32_t h = 0;
-    while (h < 6 && (1u << (h + 1)) <= p2) h++;
-    while ((p2 >> h) > BMASK_MAX_G) h++;
+    while ((1u << (h + 1)) <= BMASK_MAX_PER && (1u << (h + 1)) <= p2) h++;
     return h;
 }
 
@@ -870,8 +961,10 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     hipEvent_t evt = nullptr;
     timing_begin(ctx, s, evt);
     if (mode == BMODE_SEED || second) {
-        hipLaunchKernelGGL(batch_seed_kernel, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff,
-                           P.psz, P.pid, pass_flags);
+        const int NB = dims_class(ctx->d);
+        auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
+        hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz, P.pid,
+                           pass_flags);
     } else if (mode == BMODE_LEARNED) {
         const int rc = learned_tau_enqueue(ctx, s, dq, nq);
         if (rc) return rc;
@@ -880,11 +973,26 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                second ? "batch_tau_second" : (mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given")),
                evt);
     const uint32_t H = group_height(P.p2), G = P.p2 >> H;
-    // slices per query: enough workgroups to cover the CUs on small batches;
-    // the second pass reruns few queries, so each gets many
-    const uint32_t S = second ? 16u : std::min(16u, std::max(1u, (4u * (uint32_t)B.cus + nq - 1) / nq));
-    hipLaunchKernelGGL(batch_mask_kernel, dim3(nq, S), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.d_pcnt,
-                       B.d_plist, pass_flags);
+    const int NB = dims_class(ctx->d);
+    // mask layout: query-major (one workgroup per query) spreads a full
+    // batch over the CUs; group-major (one per group and query slice) keeps
+    // the few queries of a second pass from running one long chain each
+    const int layout = ctx->tune.batch_mask ? ctx->tune.batch_mask : (second ? 2 : 1);
+    if (layout == 1) {
+        // slices per query: enough workgroups to cover the CUs on small
+        // batches, and at most BMASK_PAIRS groups per workgroup
+        uint32_t S = std::min(16u, std::max(1u, (4u * (uint32_t)B.cus + nq - 1) / nq));
+        S = std::max(S, (G + BMASK_PAIRS - 1) / BMASK_PAIRS);
+        auto mask = NB == 8 ? batch_mask_query_kernel<8>
+                            : (NB == 16 ? batch_mask_query_kernel<16> : batch_mask_query_kernel<32>);
+        hipLaunchKernelGGL(mask, dim3(nq, S), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.d_pcnt, B.d_plist,
+                           pass_flags);
+    } else {
+        const uint32_t QS = BMASK_PAIRS >> H, Y = (nq + QS - 1) / QS;
+        auto mask = NB == 8 ? batch_mask_kernel<8> : (NB == 16 ? batch_mask_kernel<16> : batch_mask_kernel<32>);
+        hipLaunchKernelGGL(mask, dim3(G, Y), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, QS, B.d_pcnt,
+                           B.d_plist, pass_flags);
+    }
     const uint32_t ochunk = item_objects(v);
     hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_pcnt, L,
                        (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
@@ -945,7 +1053,8 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                         const BatchVariant &v) {
     if (!ctx->tune.batch_graph || ctx->timing_on) return batch_enqueue(ctx, B, s, nq, ks, mode, v);
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
-                       (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0};
+                       (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
+                       (uint64_t)ctx->tune.batch_mask};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1089,7 +1198,7 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     S.queries = nq;
     S.fallback = missed;
     S.retried = retried;
-    S.items = c1[0] + c1[7];
+    S.items = c1[0] + c1[1];
     S.candidates = candidates;
     S.partitions_scored = ((uint64_t)c1[2] | ((uint64_t)c1[3] << 32)) + ((uint64_t)c2[2] | ((uint64_t)c2[3] << 32));
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;

@@ -64,34 +64,43 @@ inline BoundArgs make_bound_args(uint32_t d, uint32_t qq, const float *weights, 
 }
 
 // pmax / pmin point at the partition's d per-attribute maxima / minima.
-// NMAX bounds b.nd at compile time: the queried maxima / minima are read
-// with independent loads into registers before any arithmetic.
+// NMAX bounds b.nd at compile time.  Every operand (weight, maximum,
+// minimum) is read with independent loads into registers before any
+// arithmetic, and the loops are predicated rather than cut short, so a
+// bound costs one round of memory latency.  Terms past b.nd carry w = 0 and
+// max = min = 0 and add exact zeros; the knapsack loop skips them.
 template <int NMAX = TOPK_MAX_DIMS>
 __device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__restrict__ pmax,
                                           const float *__restrict__ pmin, float psum, float pminsum) {
+    const uint32_t nd = b.nd;
     float mxv[NMAX], mnv[NMAX];
+    double wv[NMAX];
 #pragma unroll
     for (int i = 0; i < NMAX; i++) {
-        const bool on = (uint32_t)i < b.nd;
-        mxv[i] = on ? pmax[b.attr[i]] : 0.0f;
-        mnv[i] = on ? pmin[b.attr[i]] : 0.0f;
+        const bool on = (uint32_t)i < nd;
+        const uint32_t ai = on ? b.attr[i] : 0u;  // entries past nd may be stale
+        wv[i] = on ? b.w[i] : 0.0;
+        mxv[i] = on ? pmax[ai] : 0.0f;
+        mnv[i] = on ? pmin[ai] : 0.0f;
     }
     double base = 0, box = 0, absw = 0;
 #pragma unroll
     for (int i = 0; i < NMAX; i++) {
-        if ((uint32_t)i >= b.nd) break;
-        const double mx = mxv[i], mn = mnv[i], w = b.w[i];
+        const double mx = mxv[i], mn = mnv[i], w = wv[i];
         base += w * mn;
         box += w * (w >= 0 ? mx : mn);
         absw += fabs(w) * fmax(fabs(mx), fabs(mn));
     }
+    // fractional knapsack: the weights are sorted heaviest first, so the
+    // positive ones form a prefix; the budget only shrinks
     double budget = (double)psum - (double)pminsum;
 #pragma unroll
     for (int i = 0; i < NMAX; i++) {
-        if ((uint32_t)i >= b.nd || b.w[i] <= 0 || budget <= 0) break;
-        const double room = fmin((double)mxv[i] - (double)mnv[i], budget);
-        base += b.w[i] * room;
-        budget -= room;
+        if (wv[i] > 0 && budget > 0) {
+            const double room = fmin((double)mxv[i] - (double)mnv[i], budget);
+            base += wv[i] * room;
+            budget -= room;
+        }
     }
     const double lp = base < box ? base : box;
     return f32_round_up(lp + absw * 1.52587890625e-05 + 1e-30);  // margin 2^-16 * sum |W x|

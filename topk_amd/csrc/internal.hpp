@@ -74,6 +74,7 @@ struct Tuning {
     int batch_grid = 0;         // persistent scoring workgroups
     int batch_graph = 1;        // replay captured hipGraphs
     int batch_mode = 0;         // 0 auto (learned when trained), 1 seed threshold, 2 learned threshold
+    int batch_mask = 0;         // mask layout: 0 auto, 1 query-major, 2 group-major
 };
 
 // One query of a batch as the device kernels read it (prepared on the host;
