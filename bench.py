@@ -142,22 +142,23 @@ def learned_batched(eng, n, k, args):
         eng.set_tuning("batch_mode", mode)
         for i in range(warmup):
             eng.batch_run(held[i % len(held)])
-        fb = pairs = objs = cand = items = 0
+        fb = ff = pairs = objs = cand = items = 0
         t1 = time.perf_counter()
         for i in range(steps):
             eng.batch_run(held[i % len(held)])
             st = eng.stats()
             fb += st["fallback"]
+            ff += st["fte_fallback"]
             pairs += st["partitions_scored"]
             objs += st["tuple_count"]
             cand += st["candidates"]
             items += st["items"]
         el = time.perf_counter() - t1
-        return el, fb, pairs, objs, cand, items
+        return el, fb, ff, pairs, objs, cand, items
 
     # seed thresholds (the method without the network), then learned
-    s_el, s_fb, s_pairs, s_objs, _, _ = timed(max(2, args.steps // 4), 1, 1)
-    el, fb, pairs, objs, cand, items = timed(args.steps, args.warmup, 2)
+    s_el, s_fb, _, s_pairs, s_objs, _, _ = timed(max(2, args.steps // 4), 1, 1)
+    el, fb, ff, pairs, objs, cand, items = timed(args.steps, args.warmup, 2)
     steps = args.steps
     qps = steps * nb / el
 
@@ -194,7 +195,10 @@ def learned_batched(eng, n, k, args):
     return {
         "qps": qps, "ms_per_batch": el * 1e3 / steps, "steps": steps, "queries_per_batch": nb,
         "seed_batched_qps": max(2, args.steps // 4) * nb / s_el,
+        # learned threshold above the k-th score: re-run exactly in the same launch
         "fallback_rate": fb / (steps * nb),
+        # answered by the plain FTE path after both device passes
+        "fte_fallback_rate": ff / (steps * nb),
         "mismatches_vs_plain_fte": mism, "judged_queries": len(held) * nb,
         "scored_pair_frac": objs / (steps * total_pairs),
         "scored_pair_frac_seed": s_objs / (max(2, args.steps // 4) * total_pairs),

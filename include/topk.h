@@ -157,10 +157,13 @@ typedef struct topk_query {
  * (1 <= k <= k_stride <= TOPK_MAX_K).  Runs over the PTA partition layout
  * (built on first use when topk_init(TOPK_INIT_PTA) was not called): every
  * partition is loaded once per group of queries whose bound reaches the
- * query's threshold and scored for all of them.  The threshold comes from a
- * seed pass, or from the trained predictor once one is installed (see
- * topk_learned_train); queries the device path cannot settle exactly are
- * answered by topk_fte and counted in topk_stats.fallback.  Results of
+ * query's threshold and scored for all of them.  The threshold is a ranking
+ * key from a seed pass, or a score from the trained predictor once one is
+ * installed (see topk_learned_train).  A second device pass in the same
+ * launch re-runs the queries the first could not settle exactly: fewer than
+ * k candidates (a lower threshold) or more than a candidate list holds (the
+ * k-th key of the stored candidates as threshold).  A query still open after
+ * it is answered by topk_fte (topk_stats.fte_fallback).  Results of
  * query q go to out_ids/out_scores[q*k_stride ...] (min(k, n) entries) and
  * out_thresholds[q]; they equal topk_fte's for the same query. */
 int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
@@ -291,12 +294,16 @@ typedef struct topk_stats {
     uint64_t candidates;         /* keys handed to the final exact select */
     uint64_t partitions_scored;  /* PTA: partitions whose objects were scored */
     uint64_t partitions_total;
-    uint32_t fallback;           /* queries of the last call that took the exact fallback path */
+    uint32_t fallback;           /* queries of the last call whose first threshold failed: FTE
+                                    queries re-run on the exact path; batch queries whose
+                                    (predicted) threshold left fewer than k candidates, re-run
+                                    exactly, or that topk_fte answered */
     uint32_t queries;            /* queries in the last call */
     double ms_last;              /* host wall time of the last query call */
     double tt_init;              /* host wall time of the last topk_init, ms */
     uint32_t retried;            /* batch: queries re-run with a tightened threshold */
     uint32_t items;              /* batch: work items (partition loads) of the first pass */
+    uint32_t fte_fallback;       /* batch: queries neither device pass settled, answered by topk_fte */
 } topk_stats;
 
 int topk_get_stats(topk_ctx *ctx, topk_stats *out);

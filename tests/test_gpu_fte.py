@@ -225,3 +225,20 @@ def test_fte_stream_shapes(gpu, vpt):
             eng.set_tuning("fte_force_exact", 1)
             _check(eng, x, n, 32, np.ones(d, np.float32), [7, 3])
             eng.set_tuning("fte_force_exact", 0)
+
+
+def test_golden_fixtures_all_paths(gpu):
+    """FTE, PTA and batch against the pure-numpy fixtures of
+    tests/golden/make_golden.py (data generated on the device)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+    n, d = 1 << 16, 8
+    cases = ((1, 0, 16, np.ones(d, np.float32), np.array([4, 5, 6, 7], np.uint8), "c1"),
+             (0, 2, 64, g["cw_weights"], g["cw_attr"], "cw"))
+    for distr, seed, k, w, attr, tag in cases:
+        with topk_amd.Engine.generated(n, d, distr, seed) as eng:
+            eng.init(topk_amd.TOPK_INIT_PTA)
+            outs = [eng.fte(k, w, attr), eng.pta(k, w, attr), eng.batch([(k, w, attr)])[0]]
+            for ids, sc, thr in outs:
+                assert np.array_equal(ids, g[f"{tag}_ids"]), tag
+                assert np.array_equal(sc.view(np.uint32), g[f"{tag}_scores"]), tag

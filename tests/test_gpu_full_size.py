@@ -80,7 +80,7 @@ def test_learned_full_size_d16_mixed(gpu):
     n, d = 1 << 26, 16
     train = make_queries(np.random.default_rng(300), 100_000, d, KS)
     held = make_queries(np.random.default_rng(301), 4096, d, KS)
-    got, fallback = [], 0
+    got, fallback, fte = [], 0, 0
     with topk_amd.Engine.generated(n, d, topk_amd.MIXED, 5) as eng:
         rep = eng.learned_train(train)
         assert rep["n_train"] + rep["n_calib"] == len(train)
@@ -88,9 +88,11 @@ def test_learned_full_size_d16_mixed(gpu):
         eng.set_tuning("batch_mode", 2)
         for b in range(0, len(held), 1024):
             got += eng.batch(held[b:b + 1024])
-            fallback += eng.stats()["fallback"]
+            st = eng.stats()
+            fallback += st["fallback"]
+            fte += st["fte_fallback"]
         eng.set_tuning("batch_mode", 0)
-        assert fallback <= 0.02 * len(held)
+        assert fallback <= 0.02 * len(held) and fte <= 0.005 * len(held)
         mism = sum(not _same(g, eng.fte(*q)) for g, q in zip(got, held))
         assert mism == 0
     x = topk_amd.generate(n, d, topk_amd.MIXED, 5, threads=HOST_THREADS)

@@ -264,3 +264,27 @@ def test_mlp_forward_closed_form():
     p[96 * 64 + 64 + 64 * 64 + 64] = 0.5
     x = np.zeros((2, 96)); x[0, 5] = 1.5; x[1, 5] = -1.0
     assert oracle.mlp_forward(p, x).tolist() == [4.5, 0.0]
+
+
+GOLDEN = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "golden.npz"))
+
+
+@pytest.mark.parametrize("distr", [0, 1, 2, 3])
+def test_golden_generator(distr):
+    """The library's host generator against the pure-numpy restatement."""
+    import topk_amd
+    x = topk_amd.generate(300, 8, distr, 7)
+    assert np.array_equal(x.view(np.uint32), GOLDEN[f"gen_{distr}"])
+
+
+def test_golden_topk():
+    """The oracle's top-k against the pure-numpy fixtures (reduced C1 and a
+    weighted, unordered query with 49 objects tied at the top score)."""
+    import topk_amd
+    n = 1 << 16
+    x = topk_amd.generate(n, 8, 1, 0)
+    rid, rs, _ = oracle.fte_topk(x, n, 16, np.ones(8, np.float32), [4, 5, 6, 7])
+    assert np.array_equal(rid, GOLDEN["c1_ids"]) and np.array_equal(rs.view(np.uint32), GOLDEN["c1_scores"])
+    xc = topk_amd.generate(n, 8, 0, 2)
+    rid, rs, _ = oracle.fte_topk(xc, n, 64, GOLDEN["cw_weights"], GOLDEN["cw_attr"])
+    assert np.array_equal(rid, GOLDEN["cw_ids"]) and np.array_equal(rs.view(np.uint32), GOLDEN["cw_scores"])

@@ -323,6 +323,7 @@ static int fte_run_sync(topk_ctx *ctx, uint32_t k, const QCols &q) {
     uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
     ctx->stats.fallback = 0;
     ctx->stats.retried = 0;
+    ctx->stats.fte_fallback = 0;
     Lane &L = ctx->lane[0];
     int mode = ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE;
     for (int attempt = 0; attempt < 3; attempt++) {

@@ -1198,6 +1198,7 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     S.queries = nq;
     S.fallback = missed;
     S.retried = retried;
+    S.fte_fallback = (uint32_t)fte.size();
     S.items = c1[0] + c1[1];
     S.candidates = candidates;
     S.partitions_scored = ((uint64_t)c1[2] | ((uint64_t)c1[3] << 32)) + ((uint64_t)c2[2] | ((uint64_t)c2[3] << 32));

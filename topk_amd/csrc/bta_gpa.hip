@@ -528,6 +528,7 @@ int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     S.queries = 1;
     S.fallback = 0;
     S.retried = 0;
+    S.fte_fallback = 0;
     S.tuple_count = ctx->n;
     S.candidates = (uint64_t)nblk * K2;
     S.partitions_scored = nblk;
@@ -620,6 +621,7 @@ int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     S.queries = 1;
     S.fallback = 0;
     S.retried = 0;
+    S.fte_fallback = 0;
     S.candidates = last;
     S.tuple_count = n;
     S.partitions_scored = 0;

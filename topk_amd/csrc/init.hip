@@ -193,6 +193,7 @@ int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     ctx->stats.partitions_total = ctx->pta.nparts;
     ctx->stats.fallback = 0;
     ctx->stats.retried = 0;
+    ctx->stats.fte_fallback = 0;
     ctx->stats.queries = 1;
     ctx->stats.ms_last = now_ms() - t0;
     ctx->stats.tuple_count = 0;

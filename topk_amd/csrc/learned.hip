@@ -6,7 +6,9 @@
 // front from (query, data sketch); the batch path lowers the prediction by a
 // calibrated margin and uses it as the query's threshold tau in its fused
 // score + prune + compact pass.  Exactness never depends on the network:
-// fewer than k survivors send the query to the plain exact path (batch.hip).
+// fewer than k survivors re-run the query exactly with a lower threshold in
+// the batch's second pass, and a query still open after it goes to the plain
+// exact path (batch.hip).
 //
 //  * network: 96 -> 64 -> 64 -> 1, ReLU, FP32.  Parameter layout W1[64][96],
 //    b1[64], W2[64][64], b2[64], W3[64], b3 (10433 floats).
