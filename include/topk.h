@@ -169,6 +169,22 @@ typedef struct topk_query {
 int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
                uint64_t *out_ids, float *out_scores, float *out_thresholds);
 
+/* Asynchronous topk_batch (no reference counterpart; for callers that keep
+ * the GPU busy while they prepare the next batch or consume the last):
+ * validates and stages the queries (qs may be reused when it returns),
+ * launches the batch and returns its ticket.  Up to two batches are in
+ * flight per context; a submit with two in flight first completes the older
+ * one.  A batch's results are in its output arrays (which must stay valid
+ * until then) once topk_batch_wait of its ticket or of a later one returns,
+ * or once a later submit completed it. */
+int topk_batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
+                      uint64_t *out_ids, float *out_scores, float *out_thresholds, uint64_t *ticket);
+
+/* Completes every batch submitted up to and including ticket, oldest first;
+ * topk_get_stats then describes the last batch completed.  A ticket never
+ * issued is TOPK_ERR_INVALID; one already completed returns TOPK_OK. */
+int topk_batch_wait(topk_ctx *ctx, uint64_t ticket);
+
 /* Launch-shape autotuner for topk_batch (no reference counterpart; it
  * replaces the reference's compile-time block sizes, Makefile:7-54): times
  * the whole batch pipeline with HIP events on these queries for every

@@ -191,3 +191,36 @@ def test_batch_errors(gpu):
             eng.set_tuning("batch_qpi", 8)
         with pytest.raises(topk_amd.TopkError):
             eng.set_tuning("batch_mask", 3)
+
+
+def test_batch_submit_wait_pipeline(gpu):
+    """Asynchronous batches (two in flight) give the synchronous results; a
+    third submit completes the oldest; a layout rebuild and a predictor
+    change complete the batches in flight first; bad tickets are rejected."""
+    n, d = 700_000, 8
+    rng = np.random.default_rng(44)
+    x = topk_amd.generate(n, d, topk_amd.MIXED, 44)
+    with topk_amd.Engine(x) as eng:
+        pbs = [eng.prepare_batch(make_queries(rng, 300, d, [16, 64, 128])) for _ in range(5)]
+        ref = [eng.batch(pb.queries) for pb in pbs]
+        tickets = [eng.batch_submit(pb) for pb in pbs]  # the 3rd..5th submits complete older batches
+        eng.batch_wait(tickets[-1])
+        for pb, r in zip(pbs, ref):
+            for i, (ri, rs, rt) in enumerate(r):
+                gi, gs, gt = pb.result(i, n)
+                assert np.array_equal(gi, ri) and np.array_equal(gs, rs) and gt == rt
+        eng.batch_wait(tickets[0])  # already completed
+        with pytest.raises(topk_amd.TopkError):
+            eng.batch_wait(tickets[-1] + 5)
+        # in flight across a layout rebuild: completed before the layout goes
+        for pb in pbs:
+            pb.ids[:] = 0
+        t = [eng.batch_submit(pbs[0]), eng.batch_submit(pbs[1])]
+        eng.init(topk_amd.TOPK_INIT_PTA)
+        for pb, r in zip(pbs[:2], ref[:2]):
+            for i, (ri, rs, rt) in enumerate(r):
+                assert np.array_equal(pb.result(i, n)[0], ri)
+        eng.batch_wait(t[1])
+        got = eng.batch(pbs[2].queries)
+        for (gi, gs, gt), (ri, rs, rt) in zip(got, ref[2]):
+            assert np.array_equal(gi, ri) and np.array_equal(gs, rs)

@@ -742,15 +742,35 @@ struct BatchVariant {
     int vpt, bs, qpi, digit, grid;
 };
 
-using GraphKey = std::array<uint64_t, 11>;
+using GraphKey = std::array<uint64_t, 12>;
+
+// Pinned staging of one batch in flight, and where its results go.
+struct HostSlot {
+    unsigned char *h_in = nullptr, *h_out = nullptr;
+    size_t in_cap = 0, out_cap = 0;
+    hipEvent_t done = nullptr;  // recorded after the batch's result copy
+    bool busy = false;
+    uint64_t ticket = 0;
+    uint32_t nq = 0, ks = 0;
+    uint64_t *out_ids = nullptr;
+    float *out_scores = nullptr, *out_thr = nullptr;
+    double t0 = 0;
+};
 
 struct BatchState {
     uint64_t gen = ~0ull;   // layout generation the buffers and graphs belong to
     uint32_t nq_cap = 0;
-    size_t in_cap = 0, out_cap = 0;
+    size_t din_cap = 0, dout_cap = 0;
     uint64_t plist_cap = 0, pcnt_cap = 0, item_cap = 0;
-    unsigned char *d_in = nullptr, *h_in = nullptr;    // BatchQuery[nq] | BoundArgs[nq]
-    unsigned char *d_out = nullptr, *h_out = nullptr;  // ids | scores | thr | status | cnt | ctr
+    unsigned char *d_in = nullptr;   // raw queries | taus | BatchQuery[nq] | BoundArgs[nq]
+    unsigned char *d_out = nullptr;  // ids | scores | thr | status | cnt | flags | ctr
+    // Two batches may be in flight (topk_batch_submit): each stages through
+    // its own pinned slot; the device buffers are shared because the batches
+    // run one after the other on one stream.
+    HostSlot slot[2];
+    int cur = 0;                     // slot of the batch being enqueued
+    int next = 0;                    // slot the next submit uses
+    uint64_t next_ticket = 1;
     uint16_t *d_plist = nullptr;     // per-partition query lists (nq entries each)
     unsigned int *d_pcnt = nullptr;  // per-partition list length
     BatchItem *d_items = nullptr;
@@ -804,6 +824,7 @@ static void drop_graphs(BatchState &B) {
 void batch_free(topk_ctx *ctx) {
     BatchState *B = static_cast<BatchState *>(ctx->batch);
     if (!B) return;
+    hipStreamSynchronize(ctx->lane[0].stream);  // batches still in flight
     drop_graphs(*B);
     hipFree(B->d_in);
     hipFree(B->d_out);
@@ -813,8 +834,11 @@ void batch_free(topk_ctx *ctx) {
     hipFree(B->d_ctr);
     hipFree(B->d_ccount);
     hipFree(B->d_cand);
-    if (B->h_in) hipHostFree(B->h_in);
-    if (B->h_out) hipHostFree(B->h_out);
+    for (HostSlot &h : B->slot) {
+        if (h.h_in) hipHostFree(h.h_in);
+        if (h.h_out) hipHostFree(h.h_out);
+        if (h.done) hipEventDestroy(h.done);
+    }
     delete B;
     ctx->batch = nullptr;
 }
@@ -831,7 +855,6 @@ static int grow_dev(T *&p, uint64_t &cap, uint64_t need, bool &changed) {
     return TOPK_OK;
 }
 
-// Tree height of the mask's first level: groups of up to 64 partitions.
 // Height of the mask's groups: 64 partitions, fewer on small layouts.
 static uint32_t group_height(uint32_t p2) {
     uint32_t h = 0;
@@ -839,14 +862,29 @@ static uint32_t group_height(uint32_t p2) {
     return h;
 }
 
-static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks) {
+static int batch_drain(topk_ctx *ctx, BatchState &B);
+
+// Sizes the shared device buffers and slot si's staging for nq queries.
+// Growing anything first completes the batches in flight (they use the
+// buffers and the captured graphs).
+static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si) {
     const PtaLayout &P = ctx->pta;
+    HostSlot &H = B.slot[si];
+    const uint32_t L = P.nparts, nw = (nq + 63) / 64;
+    const size_t in_need = InLayout::make(nq).bytes, out_need = OutLayout::make(nq, ks).bytes;
+    const uint64_t items_need =
+        (uint64_t)L * (4 * nw + 1) * ((P.maxsz + ITEM_MIN_OBJECTS - 1) / ITEM_MIN_OBJECTS + 1);
+    const bool grow = B.gen != P.generation || nq > B.nq_cap || in_need > B.din_cap || out_need > B.dout_cap ||
+                      in_need > H.in_cap || out_need > H.out_cap || (uint64_t)L * nq > B.plist_cap ||
+                      (uint64_t)L > B.pcnt_cap || items_need > B.item_cap || !B.d_ctr || !H.done;
+    if (!grow) return TOPK_OK;
+    int rc = batch_drain(ctx, B);
+    if (rc) return rc;
     bool changed = false;
     if (B.gen != P.generation) {
         changed = true;
         B.gen = P.generation;
     }
-    const uint32_t L = P.nparts, nw = (nq + 63) / 64;
     if (nq > B.nq_cap) {
         uint32_t cap = 64;
         while (cap < nq) cap *= 2;
@@ -860,33 +898,43 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks) {
         TK_HIP(hipMalloc(&B.d_cand, sizeof(uint64_t) * BATCH_CAP * cap));
         B.nq_cap = cap;
     }
-    const size_t in_need = InLayout::make(nq).bytes, out_need = OutLayout::make(nq, ks).bytes;
-    if (in_need > B.in_cap) {
+    if (in_need > B.din_cap) {
         hipFree(B.d_in);
-        if (B.h_in) hipHostFree(B.h_in);
-        B.d_in = B.h_in = nullptr;
-        B.in_cap = 0;
+        B.d_in = nullptr;
+        B.din_cap = 0;
         changed = true;
         TK_HIP(hipMalloc(&B.d_in, in_need));
-        TK_HIP(hipHostMalloc(&B.h_in, in_need, hipHostMallocDefault));
-        B.in_cap = in_need;
+        B.din_cap = in_need;
     }
-    if (out_need > B.out_cap) {
+    if (out_need > B.dout_cap) {
         hipFree(B.d_out);
-        if (B.h_out) hipHostFree(B.h_out);
-        B.d_out = B.h_out = nullptr;
-        B.out_cap = 0;
+        B.d_out = nullptr;
+        B.dout_cap = 0;
         changed = true;
         TK_HIP(hipMalloc(&B.d_out, out_need));
-        TK_HIP(hipHostMalloc(&B.h_out, out_need, hipHostMallocDefault));
-        B.out_cap = out_need;
+        B.dout_cap = out_need;
     }
-    int rc = grow_dev(B.d_plist, B.plist_cap, (uint64_t)L * nq, changed);
+    if (in_need > H.in_cap) {
+        if (H.h_in) hipHostFree(H.h_in);
+        H.h_in = nullptr;
+        H.in_cap = 0;
+        changed = true;
+        TK_HIP(hipHostMalloc(&H.h_in, in_need, hipHostMallocDefault));
+        H.in_cap = in_need;
+    }
+    if (out_need > H.out_cap) {
+        if (H.h_out) hipHostFree(H.h_out);
+        H.h_out = nullptr;
+        H.out_cap = 0;
+        changed = true;
+        TK_HIP(hipHostMalloc(&H.h_out, out_need, hipHostMallocDefault));
+        H.out_cap = out_need;
+    }
+    if (!H.done) TK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
+    rc = grow_dev(B.d_plist, B.plist_cap, (uint64_t)L * nq, changed);
     if (!rc) rc = grow_dev(B.d_pcnt, B.pcnt_cap, (uint64_t)L, changed);
     // budget >= 16 queries, chunks >= ITEM_MIN_OBJECTS objects
-    if (!rc)
-        rc = grow_dev(B.d_items, B.item_cap,
-                      (uint64_t)L * (4 * nw + 1) * ((P.maxsz + ITEM_MIN_OBJECTS - 1) / ITEM_MIN_OBJECTS + 1), changed);
+    if (!rc) rc = grow_dev(B.d_items, B.item_cap, items_need, changed);
     if (rc) return rc;
     if (!B.d_ctr) {
         changed = true;
@@ -1033,7 +1081,7 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     const OutLayout ol = OutLayout::make(nq, ks);
     BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
     BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
-    TK_HIP(hipMemcpyAsync(B.d_in, B.h_in, il.host, hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemcpyAsync(B.d_in, B.slot[B.cur].h_in, il.host, hipMemcpyHostToDevice, s));
     const uint32_t L = ctx->pta.nparts;
     uint32_t pg = (L + 63) / 64;
     if (pg > 1024) pg = 1024;
@@ -1045,7 +1093,7 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     int rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, false);
     if (!rc) rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, true);
     if (rc) return rc;
-    TK_HIP(hipMemcpyAsync(B.h_out, B.d_out, ol.bytes, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipMemcpyAsync(B.slot[B.cur].h_out, B.d_out, ol.bytes, hipMemcpyDeviceToHost, s));
     return TOPK_OK;
 }
 
@@ -1054,7 +1102,7 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     if (!ctx->tune.batch_graph || ctx->timing_on) return batch_enqueue(ctx, B, s, nq, ks, mode, v);
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
                        (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
-                       (uint64_t)ctx->tune.batch_mask};
+                       (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1113,7 +1161,7 @@ static int stage_queries(topk_ctx *ctx, BatchState &B, uint32_t nq, const topk_q
         for (uint32_t j = 0; j < Q.qq; j++)
             if (Q.attr[j] >= ctx->d) return TOPK_ERR_INVALID;
     }
-    memcpy(B.h_in + InLayout::make(nq).raw_off, qs, sizeof(topk_query) * nq);
+    memcpy(B.slot[B.cur].h_in + InLayout::make(nq).raw_off, qs, sizeof(topk_query) * nq);
     return TOPK_OK;
 }
 
@@ -1123,43 +1171,25 @@ static int resolve_mode(const topk_ctx *ctx) {
     return learned_ready(ctx) ? BMODE_LEARNED : BMODE_SEED;
 }
 
-// Runs one batch (both passes in one launch) and copies results out; a
-// query the second pass could not settle either goes through topk_fte.
-static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
-                     float *out_scores, float *out_thr, const BatchVariant *force) {
-    if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
-    if (nq > BATCH_MAX_Q || ks > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
-    const double t0 = now_ms();
-    if (!ctx->pta.built) {
-        const int rc = pta_build(ctx);
-        if (rc) return rc;
-    }
-    if (!ctx->batch) ctx->batch = new BatchState();
-    BatchState &B = *static_cast<BatchState *>(ctx->batch);
-    const int mode = resolve_mode(ctx);
-    if (mode < 0) return TOPK_ERR_STATE;
-    int rc = ensure(ctx, B, nq, ks);
-    if (rc) return rc;
-    rc = stage_queries(ctx, B, nq, qs, ks);
-    if (rc) return rc;
-    const BatchVariant v = force ? *force : pick_variant(ctx, B);
-    if (!variant_ok(dims_class(ctx->d), v.vpt, v.bs) || v.qpi < 16 || v.qpi > 1024 ||
-        (v.digit != 8 && v.digit != 11) || v.grid < 1)
-        return TOPK_ERR_INVALID;
-    hipStream_t s = ctx->lane[0].stream;
-    rc = batch_launch(ctx, B, s, nq, ks, mode, v);
-    if (rc) return rc;
-    TK_HIP(hipStreamSynchronize(s));
-
+// Completes the batch of slot si: waits for its result copy, copies the
+// results out, answers any query the second pass could not settle either
+// through topk_fte, and sets the statistics.
+static int batch_complete(topk_ctx *ctx, BatchState &B, int si) {
+    HostSlot &H = B.slot[si];
+    if (!H.busy) return TOPK_OK;
+    H.busy = false;
+    TK_HIP(hipEventSynchronize(H.done));
+    const uint32_t nq = H.nq, ks = H.ks;
+    const topk_query *qs = reinterpret_cast<const topk_query *>(H.h_in + InLayout::make(nq).raw_off);
     const OutLayout ol = OutLayout::make(nq, ks);
-    const uint64_t *ids = reinterpret_cast<const uint64_t *>(B.h_out + ol.ids_off);
-    const float *sc = reinterpret_cast<const float *>(B.h_out + ol.sc_off);
-    const float *thr = reinterpret_cast<const float *>(B.h_out + ol.thr_off);
-    const uint32_t *st = reinterpret_cast<const uint32_t *>(B.h_out + ol.st_off);
-    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(B.h_out + ol.cnt_off);
-    const uint32_t *fl = reinterpret_cast<const uint32_t *>(B.h_out + ol.fl_off);
-    const uint32_t *c1 = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr_off);
-    const uint32_t *c2 = reinterpret_cast<const uint32_t *>(B.h_out + ol.ctr2_off);
+    const uint64_t *ids = reinterpret_cast<const uint64_t *>(H.h_out + ol.ids_off);
+    const float *sc = reinterpret_cast<const float *>(H.h_out + ol.sc_off);
+    const float *thr = reinterpret_cast<const float *>(H.h_out + ol.thr_off);
+    const uint32_t *st = reinterpret_cast<const uint32_t *>(H.h_out + ol.st_off);
+    const uint32_t *cnt = reinterpret_cast<const uint32_t *>(H.h_out + ol.cnt_off);
+    const uint32_t *fl = reinterpret_cast<const uint32_t *>(H.h_out + ol.fl_off);
+    const uint32_t *c1 = reinterpret_cast<const uint32_t *>(H.h_out + ol.ctr_off);
+    const uint32_t *c2 = reinterpret_cast<const uint32_t *>(H.h_out + ol.ctr2_off);
     uint64_t candidates = 0;
     uint32_t missed = 0, retried = 0;
     std::vector<uint32_t> fte;
@@ -1169,30 +1199,38 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
         // the device passes could not settle it
         missed += fl[i] == QF_SEED || fl[i] == QF_LOWER || st[i] ? 1u : 0u;
         retried += fl[i] == QF_GIVEN ? 1u : 0u;
-        if (st[i]) {
-            fte.push_back(i);
-            continue;
-        }
-        const uint32_t kq = qs[i].k;
-        const uint32_t c = (uint32_t)(ctx->n < kq ? ctx->n : kq);
-        const uint64_t o = (uint64_t)i * ks;
-        if (out_ids) {
-            memcpy(out_ids + o, ids + o, sizeof(uint64_t) * c);
-            for (uint32_t j = c; j < kq; j++) out_ids[o + j] = ~0ull;
-        }
-        if (out_scores) {
-            memcpy(out_scores + o, sc + o, sizeof(float) * c);
-            for (uint32_t j = c; j < kq; j++) out_scores[o + j] = -INFINITY;
-        }
-        if (out_thr) out_thr[i] = thr[i];
+        if (st[i]) fte.push_back(i);
     }
+    // results are laid out [q * ks + j] on both sides; query i owns its
+    // first k_i entries (min(k_i, n) results, then padding)
+    bool full = ctx->n >= ks;
+    for (uint32_t i = 0; full && i < nq; i++) full = qs[i].k == ks;
+    if (full) {  // every query fills its stride: one copy per array
+        if (H.out_ids) memcpy(H.out_ids, ids, sizeof(uint64_t) * nq * ks);
+        if (H.out_scores) memcpy(H.out_scores, sc, sizeof(float) * nq * ks);
+    } else {
+        for (uint32_t i = 0; i < nq; i++) {
+            const uint32_t kq = qs[i].k, c = (uint32_t)(ctx->n < kq ? ctx->n : kq);
+            const uint64_t o = (uint64_t)i * ks;
+            if (H.out_ids) {
+                memcpy(H.out_ids + o, ids + o, sizeof(uint64_t) * c);
+                for (uint32_t j = c; j < kq; j++) H.out_ids[o + j] = ~0ull;
+            }
+            if (H.out_scores) {
+                memcpy(H.out_scores + o, sc + o, sizeof(float) * c);
+                for (uint32_t j = c; j < kq; j++) H.out_scores[o + j] = -INFINITY;
+            }
+        }
+    }
+    if (H.out_thr) memcpy(H.out_thr, thr, sizeof(float) * nq);
     for (uint32_t i : fte) {
         const topk_query &Q = qs[i];
         float t = 0;
-        rc = topk_fte(ctx, Q.k, (uint8_t)Q.qq, Q.weights, Q.attr, out_ids ? out_ids + (uint64_t)i * ks : nullptr,
-                      out_scores ? out_scores + (uint64_t)i * ks : nullptr, &t);
+        const int rc = topk_fte(ctx, Q.k, (uint8_t)Q.qq, Q.weights, Q.attr,
+                                H.out_ids ? H.out_ids + (uint64_t)i * ks : nullptr,
+                                H.out_scores ? H.out_scores + (uint64_t)i * ks : nullptr, &t);
         if (rc) return rc;
-        if (out_thr) out_thr[i] = t;
+        if (H.out_thr) H.out_thr[i] = t;
     }
     topk_stats &S = ctx->stats;
     S.queries = nq;
@@ -1204,8 +1242,79 @@ static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t 
     S.partitions_scored = ((uint64_t)c1[2] | ((uint64_t)c1[3] << 32)) + ((uint64_t)c2[2] | ((uint64_t)c2[3] << 32));
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;
     S.tuple_count = ((uint64_t)c1[4] | ((uint64_t)c1[5] << 32)) + ((uint64_t)c2[4] | ((uint64_t)c2[5] << 32));
-    S.ms_last = now_ms() - t0;
+    S.ms_last = now_ms() - H.t0;
     return TOPK_OK;
+}
+
+// Completes the batches in flight with tickets <= upto, oldest first.
+static int batch_wait_upto(topk_ctx *ctx, BatchState &B, uint64_t upto) {
+    for (;;) {
+        int si = -1;
+        for (int i = 0; i < 2; i++)
+            if (B.slot[i].busy && B.slot[i].ticket <= upto && (si < 0 || B.slot[i].ticket < B.slot[si].ticket)) si = i;
+        if (si < 0) return TOPK_OK;
+        const int rc = batch_complete(ctx, B, si);
+        if (rc) return rc;
+    }
+}
+
+static int batch_drain(topk_ctx *ctx, BatchState &B) { return batch_wait_upto(ctx, B, ~0ull); }
+
+// Validates, stages and launches one batch (both passes in one launch) in
+// the next slot; a busy slot holds the older of the two batches in flight
+// and is completed first.
+static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
+                        float *out_scores, float *out_thr, const BatchVariant *force, uint64_t *ticket) {
+    if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
+    if (nq > BATCH_MAX_Q || ks > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    const double t0 = now_ms();
+    if (!ctx->pta.built) {
+        const int rc = pta_build(ctx);
+        if (rc) return rc;
+    }
+    if (!ctx->batch) ctx->batch = new BatchState();
+    BatchState &B = *static_cast<BatchState *>(ctx->batch);
+    const int mode = resolve_mode(ctx);
+    if (mode < 0) return TOPK_ERR_STATE;
+    const int si = B.next;
+    int rc = batch_complete(ctx, B, si);
+    if (!rc) rc = ensure(ctx, B, nq, ks, si);
+    if (rc) return rc;
+    const BatchVariant v = force ? *force : pick_variant(ctx, B);  // after ensure: the CU count is known
+    if (!variant_ok(dims_class(ctx->d), v.vpt, v.bs) || v.qpi < 16 || v.qpi > 1024 ||
+        (v.digit != 8 && v.digit != 11) || v.grid < 1)
+        return TOPK_ERR_INVALID;
+    B.cur = si;
+    rc = stage_queries(ctx, B, nq, qs, ks);
+    if (rc) return rc;
+    hipStream_t s = ctx->lane[0].stream;
+    rc = batch_launch(ctx, B, s, nq, ks, mode, v);
+    if (rc) return rc;
+    HostSlot &H = B.slot[si];
+    TK_HIP(hipEventRecord(H.done, s));
+    H.busy = true;
+    H.ticket = B.next_ticket++;
+    H.nq = nq;
+    H.ks = ks;
+    H.out_ids = out_ids;
+    H.out_scores = out_scores;
+    H.out_thr = out_thr;
+    H.t0 = t0;
+    B.next = si ^ 1;
+    if (ticket) *ticket = H.ticket;
+    return TOPK_OK;
+}
+
+static int batch_run(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
+                     float *out_scores, float *out_thr, const BatchVariant *force) {
+    uint64_t t = 0;
+    const int rc = batch_submit(ctx, nq, qs, ks, out_ids, out_scores, out_thr, force, &t);
+    if (rc) return rc;
+    return batch_wait_upto(ctx, *static_cast<BatchState *>(ctx->batch), t);
+}
+
+void batch_quiesce(topk_ctx *ctx) {
+    if (ctx && ctx->batch) batch_drain(ctx, *static_cast<BatchState *>(ctx->batch));
 }
 
 }  // namespace topk
@@ -1217,6 +1326,19 @@ extern "C" {
 int topk_batch(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride, uint64_t *out_ids,
                float *out_scores, float *out_thresholds) {
     return batch_run(ctx, nq, qs, k_stride, out_ids, out_scores, out_thresholds, nullptr);
+}
+
+int topk_batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride, uint64_t *out_ids,
+                      float *out_scores, float *out_thresholds, uint64_t *ticket) {
+    return batch_submit(ctx, nq, qs, k_stride, out_ids, out_scores, out_thresholds, nullptr, ticket);
+}
+
+int topk_batch_wait(topk_ctx *ctx, uint64_t ticket) {
+    if (!ctx) return TOPK_ERR_INVALID;
+    if (!ctx->batch) return ticket ? TOPK_ERR_INVALID : TOPK_OK;
+    BatchState &B = *static_cast<BatchState *>(ctx->batch);
+    if (ticket >= B.next_ticket) return TOPK_ERR_INVALID;
+    return batch_wait_upto(ctx, B, ticket);
 }
 
 int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
@@ -1231,8 +1353,11 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     BatchState &B = *static_cast<BatchState *>(ctx->batch);
     const int mode = resolve_mode(ctx);
     if (mode < 0) return TOPK_ERR_STATE;
-    int rc = ensure(ctx, B, nq, k_stride);
-    if (!rc) rc = stage_queries(ctx, B, nq, qs, k_stride);
+    int rc = batch_drain(ctx, B);
+    if (!rc) rc = ensure(ctx, B, nq, k_stride, 0);
+    if (rc) return rc;
+    B.cur = 0;
+    rc = stage_queries(ctx, B, nq, qs, k_stride);
     if (rc) return rc;
     const int D = dims_class(ctx->d);
     const BatchVariant def = pick_variant(ctx, B);

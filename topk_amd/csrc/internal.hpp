@@ -272,6 +272,8 @@ bool learned_ready(const topk_ctx *ctx);
 uint64_t learned_version(const topk_ctx *ctx);
 // Validates one query and converts it to the device form (tau left at 0).
 int make_batch_query(const topk_ctx *ctx, const topk_query &q, BatchQuery &out);
+// Completes the batches in flight (before the layout or the predictor changes).
+void batch_quiesce(topk_ctx *ctx);
 int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq);
 // Releases the batch buffers and captured graphs of a context (batch.hip).
 void batch_free(topk_ctx *ctx);

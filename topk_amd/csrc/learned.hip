@@ -444,6 +444,7 @@ void learned_free(topk_ctx *ctx) {
 }
 
 static int install(topk_ctx *ctx, const float *d_src, float margin) {
+    batch_quiesce(ctx);  // batches in flight read the parameters
     LearnedState *L = lstate(ctx);
     if (!L->d_params) TK_HIP(hipMalloc(&L->d_params, sizeof(float) * MLP_NP));
     TK_HIP(hipMemcpyAsync(L->d_params, d_src, sizeof(float) * MLP_NP, hipMemcpyDefault, ctx->stream));
@@ -691,6 +692,7 @@ int topk_learned_default_config(topk_learned_config *cfg) {
 
 int topk_learned_clear(topk_ctx *ctx) {
     if (!ctx) return TOPK_ERR_INVALID;
+    batch_quiesce(ctx);
     learned_free(ctx);
     return TOPK_OK;
 }

@@ -310,12 +310,15 @@ static void pta_free(PtaLayout &P) {
     P.generation = gen + 1;
 }
 
-void pta_release(topk_ctx *ctx) { pta_free(ctx->pta); }
+void pta_release(topk_ctx *ctx) {
+    batch_quiesce(ctx);  // batches in flight read the layout
+    pta_free(ctx->pta);
+}
 
 int pta_build(topk_ctx *ctx) {
     const double t0 = now_ms();
     PtaLayout &P = ctx->pta;
-    pta_free(P);
+    pta_release(ctx);
     const uint64_t n = ctx->n;
     const uint32_t d = ctx->d;
     hipStream_t s = ctx->stream;

@@ -203,6 +203,17 @@ class Engine:
         check(lib().topk_batch(self._h, pb.nq, _c.cast(pb.qs, _c.c_void_p), pb.kmax, _ptr(pb.ids), _ptr(pb.scores),
                                _ptr(pb.thr)), "topk_batch")
 
+    def batch_submit(self, pb):
+        """Launches a PreparedBatch without waiting; returns its ticket.  The
+        results are in pb.ids / pb.scores / pb.thr after batch_wait(ticket)."""
+        t = _c.c_uint64()
+        check(lib().topk_batch_submit(self._h, pb.nq, _c.cast(pb.qs, _c.c_void_p), pb.kmax, _ptr(pb.ids),
+                                      _ptr(pb.scores), _ptr(pb.thr), _c.byref(t)), "topk_batch_submit")
+        return t.value
+
+    def batch_wait(self, ticket):
+        check(lib().topk_batch_wait(self._h, ticket), "topk_batch_wait")
+
     def batch_autotune(self, queries):
         """Times every batch launch shape on these queries and installs the
         fastest; returns the choice as a dict."""
