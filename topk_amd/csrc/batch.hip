@@ -421,8 +421,10 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *
 // qmax queries and every range into object chunks of ochunk.  Ranges holding
 // at least qmax / 2 queries are "heavy" and fill the item list from the
 // front, the others from the back (index cap - 1 - i), so the persistent
-// scoring kernel starts the long items first.  One global atomic per
-// workgroup reserves its items, two more add its pair counts.
+// scoring kernel starts the long items first; only a list's last range can
+// be light.  One global atomic per workgroup reserves its items, two more
+// add its pair counts.  The items of a partition are written by the whole
+// wave (a partition many queries reach has a hundred or more).
 __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned int *__restrict__ pcnt, uint32_t L,
                                                                 uint32_t qmax, uint32_t ochunk, uint64_t cap,
                                                                 const uint32_t *__restrict__ psz,
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned i
     __shared__ uint32_t wsum[BAUX_BS / 64];
     __shared__ unsigned long long wstat[BAUX_BS / 64][2];
     __shared__ uint32_t tot_h, tot_l, base_h, base_l;
-    const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x;
+    const uint32_t p = blockIdx.x * BAUX_BS + threadIdx.x, lane = lane_id();
     const uint32_t cnt = p < L ? pcnt[p] : 0u;
     const uint32_t sz = cnt ? psz[p] : 0u;
     const uint32_t nch = (sz + ochunk - 1) / ochunk;
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned i
         pairs += __shfl_xor(pairs, o, 64);
         objs += __shfl_xor(objs, o, 64);
     }
-    if (lane_id() == 0) {
+    if (lane == 0) {
         wstat[threadIdx.x / 64][0] = pairs;
         wstat[threadIdx.x / 64][1] = objs;
     }
@@ -472,14 +474,18 @@ __global__ __launch_bounds__(BAUX_BS) void batch_compact_kernel(const unsigned i
         }
     }
     __syncthreads();
-    if (!nr) return;
-    uint32_t kh = base_h + offh, kl = base_l + offl;
-    for (uint32_t r = 0; r < nr; r++) {
-        const uint32_t q0 = r * qmax, qn = cnt - q0 < qmax ? cnt - q0 : qmax;
-        for (uint32_t c = 0; c < nch; c++) {
-            const BatchItem it{p, q0, qn, c};
-            if (qn >= heavy_min) items[kh++] = it;
-            else items[cap - 1 - kl++] = it;
+    const uint32_t kh0 = base_h + offh, kl0 = base_l + offl;
+    for (uint64_t m = __ballot(nr != 0); m; m &= m - 1) {
+        const int src = __ffsll((long long)m) - 1;
+        const uint32_t sp_ = __shfl(p, src, 64), scnt = __shfl(cnt, src, 64), snch = __shfl(nch, src, 64);
+        const uint32_t snr = __shfl(nr, src, 64), snhr = __shfl(nhr, src, 64);
+        const uint32_t skh = __shfl(kh0, src, 64), skl = __shfl(kl0, src, 64);
+        for (uint32_t i = lane; i < snr * snch; i += 64) {
+            const uint32_t r = i / snch, c = i - r * snch;
+            const uint32_t q0 = r * qmax, qn = scnt - q0 < qmax ? scnt - q0 : qmax;
+            const BatchItem it{sp_, q0, qn, c};
+            if (r < snhr) items[skh + i] = it;           // heavy ranges come first
+            else items[cap - 1 - (skl + c)] = it;        // the light last range
         }
     }
 }

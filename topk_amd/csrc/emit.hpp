@@ -15,13 +15,54 @@ struct FlatView {
 };
 
 // Selects the exact top k of the c candidates of view v and writes them best
-// first.  All threads of the block call it.  `sorted` may alias the view's
-// storage when c <= BS*KPT (the keys are read into registers first).
-// DB is the radix digit width of the register-resident select.
+// first.  All threads of the block call it.  `sorted` (TOPK_MAX_K keys) may
+// alias the view's storage when c <= BS*KPT (the keys are read into
+// registers first).  Up to 2*BS candidates take one rank pass in LDS; more
+// take a radix select (DB-bit digits in registers, or over memory) and a
+// rank sort of the k selected.
 template <int BS, int KPT, class View, int DB = 11>
 __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const FinalOut &o, uint64_t *top,
                                 uint64_t *sorted, SelectScratch &ss) {
     const uint32_t tid = threadIdx.x;
+    if (c <= 2u * BS && c <= (uint32_t)TOPK_MAX_K) {
+        // Few candidates: one rank pass.  Keys are distinct (ids are), so the
+        // number of larger keys is each key's output position.
+        uint64_t mk[2];
+        uint32_t r[2] = {0u, 0u};
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const uint32_t i = tid + u * BS;
+            mk[u] = i < c ? v.get(i) : 0ull;
+        }
+        __syncthreads();  // the view's storage may be reused as `sorted`
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            if (tid + u * BS < c) sorted[tid + u * BS] = mk[u];
+        __syncthreads();
+        for (uint32_t j = 0; j < c; j++) {
+            const uint64_t x = sorted[j];
+            r[0] += x > mk[0] ? 1u : 0u;
+            r[1] += x > mk[1] ? 1u : 0u;
+        }
+        const uint32_t got = c < k ? c : k;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (tid + u * BS < c && r[u] < k) {
+                o.ids[r[u]] = rkey_id(mk[u]);
+                o.scores[r[u]] = rkey_score(mk[u]);
+                if (r[u] == got - 1 && o.thr) *o.thr = rkey_score(mk[u]);
+            }
+        }
+        for (uint32_t i = got + tid; i < k; i += BS) {
+            o.ids[i] = ~0ull;
+            o.scores[i] = -INFINITY;
+        }
+        if (tid == 0) {
+            if (!got && o.thr) *o.thr = -INFINITY;
+            if (o.status) *o.status = 0;
+        }
+        return;
+    }
     uint32_t got;
     if (c <= (uint32_t)BS * KPT) {
         RegKeys<KPT> rk;
