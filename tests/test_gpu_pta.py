@@ -45,7 +45,13 @@ def test_pta_layout_invariants(gpu, distr):
         for p in range(psz.size):
             seg = pid[poff[p]:poff[p] + psz[p]]
             assert np.all(seg != 0xFFFFFFFF) and np.all(pid[poff[p] + psz[p]:poff[p + 1]] == 0xFFFFFFFF)
-            assert np.all(np.diff(seg.astype(np.int64)) > 0)  # ids ascending inside a partition
+            # inside a partition: attribute sums (float32, attribute order)
+            # non-increasing, equal sums by ascending id
+            sm = np.zeros(seg.size, np.float32)
+            for m in range(d):
+                sm = sm + x[m, seg]
+            ds, di = np.diff(sm.astype(np.float64)), np.diff(seg.astype(np.int64))
+            assert np.all((ds < 0) | ((ds == 0) & (di > 0)))
             if psz[p]:
                 assert np.array_equal(pmax[p], x[:, seg].max(axis=1))
                 assert np.array_equal(pmin[p], x[:, seg].min(axis=1))

@@ -59,6 +59,7 @@ namespace topk {
 
 constexpr uint32_t BATCH_MAX_Q = 16384;   // queries per call
 constexpr uint32_t BATCH_CAP = 32768;     // candidate keys per query
+constexpr uint32_t BATCH_MAX_GRID = 65536;  // persistent scoring workgroups
 constexpr int BFIN_BS = 512, BFIN_KPT = 16;  // lists up to 8192 keys are selected in registers, longer ones in memory
 constexpr int BSEED_BS = 512, BSEED_KPT = 32;
 constexpr uint32_t BSEED_T = 4;                          // seed partitions per query
@@ -136,9 +137,28 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
         const bool uc = __shfl((int)used, c, 64) != 0;
         r += uc && (ac > agg || (ac == agg && c < lane)) ? 1u : 0u;
     }
+    // the first 16 ranks, staged in LDS so that every ws / as slot is written
+    // once: rank r's weight (exact for ascending queries, each attribute
+    // once) and attribute, zeros past the distinct attributes
+    __shared__ float sws[16];
+    __shared__ uint32_t sas[16];
+    if (lane < 16) {
+        sws[lane] = 0.0f;
+        sas[lane] = 0u;
+    }
+    __syncthreads();
     if (used && lane < d) {
         g.attr[r] = (uint8_t)lane;
         g.w[r] = agg;
+        if (r < 16) {
+            sws[r] = (float)agg;
+            sas[r] = lane;
+        }
+    }
+    __syncthreads();
+    if (lane < 16) {
+        b.ws[lane] = sws[lane];
+        b.as[lane] = (uint8_t)sas[lane];
     }
     const uint32_t nd = (uint32_t)__popcll(__ballot(used && lane < d));
     if (lane == 0) {
@@ -504,6 +524,9 @@ struct ScoreArgs {
     const BatchQuery *q;
     uint64_t *cand;
     unsigned int *ccount;
+    const float *cstat;          // chunk statistics rows (crow floats each)
+    uint32_t crow;
+    unsigned long long *scored;  // per workgroup: (object, query) pairs scored
 };
 
 template <int VPT>
@@ -543,6 +566,13 @@ __device__ __forceinline__ void load_objs(const float *__restrict__ p, float (&x
 // per lane; the per-query loop reads them with readlane, so it waits on no
 // memory.  The ticket of the next item is taken after the current item's
 // loads are issued and is read only when the item is done.
+//
+// Chunk pruning: a wave's 64 * VPT objects lie in one or a few chunks of
+// PTA_CHUNK positions (sorted by attribute sum inside the partition).  Lane j
+// evaluates the LP bound of query j over those chunks' statistics (ascending
+// queries with D <= 16; other queries always pass); the wave scores only the
+// queries whose bound reaches their threshold, and loads its objects only if
+// one does (per group of 64 listed queries).
 template <int D, int VPT, int BS>
 __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
     constexpr bool LANE_W = D <= 16;
@@ -555,6 +585,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
     __syncthreads();
     uint32_t it = __builtin_amdgcn_readfirstlane(s_it[0]);
     uint32_t slot = 1;
+    uint32_t npairs = 0;
     while (it < nitems) {
         const BatchItem I = it < nheavy ? a.items[it] : a.items[a.cap - 1 - (it - nheavy)];
         const uint32_t first = I.chunk * a.ochunk, psz_ = a.psz[I.p];
@@ -566,16 +597,32 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
         for (uint32_t b0 = 0; b0 < size; b0 += BS * VPT) {
             const uint32_t i0 = b0 + tid * VPT;
             const uint32_t nvalid = i0 < size ? (size - i0 < (uint32_t)VPT ? size - i0 : (uint32_t)VPT) : 0u;
-            float x[D][VPT];
-#pragma unroll
-            for (int c = 0; c < D; c++) {
-                if (c < (int)a.d && nvalid) {
-                    load_objs<VPT>(a.pcols + (uint64_t)c * a.ld + start + i0, x[c]);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < VPT; e++) x[c][e] = 0.0f;
+            // this wave's objects: item positions [wb, we)
+            const uint32_t wb = __builtin_amdgcn_readfirstlane(b0 + (tid & ~63u) * VPT);
+            if (wb >= size) continue;
+            const uint32_t we = size - wb < 64u * VPT ? size : wb + 64u * VPT;
+            // lane e holds element e of the union of the wave's chunk rows
+            float cs = 0.0f;
+            if constexpr (LANE_W) {
+                const uint32_t c0 = (start + wb) / PTA_CHUNK, c1 = (start + we - 1) / PTA_CHUNK;
+                const uint32_t d = a.d;
+                for (uint32_t c = c0; c <= c1; c++) {
+                    const float v = lane < a.crow ? a.cstat[(uint64_t)c * a.crow + lane] : 0.0f;
+                    if (c == c0) cs = v;
+                    else if (lane < d || lane == 2 * d) cs = fmaxf(cs, v);  // maxima, largest sum
+                    else if (lane < 2 * d) cs = fminf(cs, v);                // minima
                 }
             }
+            const uint32_t dd = a.d;
+            const float csum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd)));
+            // the union's sum of minima is the sum of its per-attribute minima
+            // (not the smallest chunk sum), in FP64 rounded down
+            double msum = 0.0;
+#pragma unroll
+            for (int m = 0; m < D; m++)
+                if ((uint32_t)m < dd)
+                    msum += (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)dd + m));
+            const float cminsum = f32_round_down(msum);
             for (uint32_t q0 = 0; q0 < I.qcnt; q0 += 64) {
                 const uint32_t nqb = I.qcnt - q0 < 64u ? I.qcnt - q0 : 64u;
                 // lane j: query q0 + j of the item
@@ -592,11 +639,54 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                         lw[c + 3] = v[3];
                     }
                 }
+                // chunk bound of lane j's query, evaluated by every lane (the
+                // shuffles read all lanes); lanes past nqb hold query 0
+                bool pass = lane < nqb;
+                if constexpr (LANE_W) {
+                    const f4 *wsp = reinterpret_cast<const f4 *>(a.q[lq].ws);
+                    const uint4 asv = *reinterpret_cast<const uint4 *>(a.q[lq].as);
+                    const uint32_t mq = (uint32_t)__float_as_int(lt[3]);
+                    double wv[D];
+                    float mxv[D], mnv[D];
+#pragma unroll
+                    for (int i = 0; i < D; i += 4) {
+                        const f4 w4 = wsp[i / 4];
+                        const uint32_t aw = i / 4 == 0 ? asv.x : (i / 4 == 1 ? asv.y : (i / 4 == 2 ? asv.z : asv.w));
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const bool on = (uint32_t)(i + u) < mq;
+                            const int at = on ? (int)((aw >> (8 * u)) & 0xFFu) : 0;
+                            const float mx = __shfl(cs, at, 64), mn = __shfl(cs, (int)dd + at, 64);
+                            wv[i + u] = on ? (double)w4[u] : 0.0;
+                            mxv[i + u] = on ? mx : 0.0f;
+                            mnv[i + u] = on ? mn : 0.0f;
+                        }
+                    }
+                    const float ub = lp_bound_regs<D>(wv, mxv, mnv, csum, cminsum);
+                    const bool asc_l = __float_as_int(lt[2]) != 0;
+                    pass = pass && (!asc_l || ub >= lt[0]);
+                }
+                const uint64_t qmask = __ballot(pass);
                 if (ticket) {
                     nxt = atomicAdd(&a.ctr[6], 1u);
                     ticket = false;
                 }
-                for (uint32_t qi = 0; qi < nqb; qi++) {
+                if (!qmask) continue;
+                // the objects, loaded once the wave has queries to score (again
+                // per 64-query group, so they are not live during the bounds)
+                float x[D][VPT];
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    if (c < (int)a.d && nvalid) {
+                        load_objs<VPT>(a.pcols + (uint64_t)c * a.ld + start + i0, x[c]);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < VPT; e++) x[c][e] = 0.0f;
+                    }
+                }
+                for (uint64_t mm = qmask; mm; mm &= mm - 1) {
+                    const uint32_t qi = (uint32_t)__ffsll((long long)mm) - 1u;
+                    npairs += nvalid;
                     const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)lq, (int)qi);
                     const uint32_t asc = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[2]), (int)qi);
                     float s[VPT];
@@ -664,11 +754,21 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 }
             }
         }
-        if (ticket) nxt = atomicAdd(&a.ctr[6], 1u);  // an item without queries
+        if (ticket) nxt = atomicAdd(&a.ctr[6], 1u);  // no wave of tid 0 reached the queries
         if (tid == 0) s_it[slot] = nxt;
         __syncthreads();
         it = __builtin_amdgcn_readfirstlane(s_it[slot]);
         slot ^= 1u;
+    }
+    // scored (object, query) pairs of this workgroup
+    __shared__ unsigned long long wpairs[BS / 64];
+    unsigned long long t = npairs;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) wpairs[tid / 64] = t;
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < BS / 64; w++) t += wpairs[w];
+        a.scored[blockIdx.x] = t;
     }
 }
 #undef TK_TERM
@@ -687,6 +787,8 @@ struct FinalArgs {
     uint32_t *ctr_out;
     uint32_t *qflag;   // first pass: written; second pass: read (settled queries are skipped)
     int second;
+    const unsigned long long *scored;  // per scoring workgroup of this pass
+    uint32_t nscored;
 };
 
 template <int DB>
@@ -695,7 +797,21 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     __shared__ uint64_t sorted[TOPK_MAX_K];
     __shared__ SelectScratch ss;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
-    if (q == 0 && tid < BCTR_WORDS) a.ctr_out[tid] = a.ctr[tid];
+    if (q == 0) {
+        // counters of this pass; words 4..5 become the (object, query) pairs
+        // the scoring kernel actually scored
+        if (tid < BCTR_WORDS && (tid < 4 || tid > 5)) a.ctr_out[tid] = a.ctr[tid];
+        unsigned long long t = 0;
+        for (uint32_t i = tid; i < a.nscored; i += BFIN_BS) t += a.scored[i];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (lane_id() == 0) top[tid / 64] = t;
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < BFIN_BS / 64; w++) t += top[w];
+            *reinterpret_cast<unsigned long long *>(a.ctr_out + 4) = t;
+        }
+        __syncthreads();
+    }
     if (a.second && a.qflag[q] == QF_DONE) return;
     const uint32_t c = a.ccount[q], k = a.q[q].k;
     if (tid == 0) a.cnt[q] = c;
@@ -781,6 +897,7 @@ struct BatchState {
     unsigned int *d_pcnt = nullptr;  // per-partition list length
     BatchItem *d_items = nullptr;
     unsigned int *d_ctr = nullptr;
+    unsigned long long *d_scored = nullptr;  // per scoring workgroup: pairs scored
     unsigned int *d_ccount = nullptr;
     uint64_t *d_cand = nullptr;
     int cus = 0;
@@ -838,6 +955,7 @@ void batch_free(topk_ctx *ctx) {
     hipFree(B->d_pcnt);
     hipFree(B->d_items);
     hipFree(B->d_ctr);
+    hipFree(B->d_scored);
     hipFree(B->d_ccount);
     hipFree(B->d_cand);
     for (HostSlot &h : B->slot) {
@@ -945,6 +1063,7 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
     if (!B.d_ctr) {
         changed = true;
         TK_HIP(hipMalloc(&B.d_ctr, sizeof(unsigned int) * BCTR_WORDS));
+        TK_HIP(hipMalloc(&B.d_scored, sizeof(unsigned long long) * BATCH_MAX_GRID));
     }
     if (!B.cus) {
         TK_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -1050,8 +1169,8 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     const uint32_t ochunk = item_objects(v);
     hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_pcnt, L,
                        (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
-    ScoreArgs sa{P.pcols, P.ld, ctx->d, P.poff, P.psz, P.pid, B.d_items, B.item_cap, ochunk, B.d_plist, nq, B.d_ctr,
-                 dq, B.d_cand, B.d_ccount};
+    ScoreArgs sa{P.pcols, P.ld, ctx->d,    P.poff,     P.psz,  P.pid,  B.d_items, B.item_cap,
+                 ochunk,  B.d_plist, nq, B.d_ctr, dq, B.d_cand, B.d_ccount, P.cstat,  P.crow, B.d_scored};
     BatchVariant vs = v;
     if (second) vs.grid = v.grid / 8 > 32 ? v.grid / 8 : 32;  // the second pass has few items
     hipEvent_t ev0 = nullptr;
@@ -1070,7 +1189,9 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                  reinterpret_cast<uint32_t *>(B.d_out + ol.cnt_off),
                  reinterpret_cast<uint32_t *>(B.d_out + (second ? ol.ctr2_off : ol.ctr_off)),
                  qflag,
-                 second ? 1 : 0};
+                 second ? 1 : 0,
+                 B.d_scored,
+                 (uint32_t)vs.grid};
     if (v.digit == 8)
         hipLaunchKernelGGL(batch_final_kernel<8>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
     else
@@ -1153,6 +1274,11 @@ int topk::make_batch_query(const topk_ctx *ctx, const topk_query &Q, BatchQuery 
     if (b.ascending)
         for (uint32_t j = 0; j < Q.qq; j++) b.wa[Q.attr[j]] = b.w[j];
     b.tau_id = 0xFFFFFFFFu;
+    const BoundArgs g = make_bound_args(ctx->d, Q.qq, Q.weights, Q.attr);
+    for (uint32_t i = 0; i < g.nd && i < 16; i++) {
+        b.ws[i] = (float)g.w[i];
+        b.as[i] = g.attr[i];
+    }
     return TOPK_OK;
 }
 

@@ -63,26 +63,15 @@ inline BoundArgs make_bound_args(uint32_t d, uint32_t qq, const float *weights, 
     return b;
 }
 
-// pmax / pmin point at the partition's d per-attribute maxima / minima.
-// NMAX bounds b.nd at compile time.  Every operand (weight, maximum,
-// minimum) is read with independent loads into registers before any
-// arithmetic, and the loops are predicated rather than cut short, so a
-// bound costs one round of memory latency.  Terms past b.nd carry w = 0 and
-// max = min = 0 and add exact zeros; the knapsack loop skips them.
-template <int NMAX = TOPK_MAX_DIMS>
-__device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__restrict__ pmax,
-                                          const float *__restrict__ pmin, float psum, float pminsum) {
-    const uint32_t nd = b.nd;
-    float mxv[NMAX], mnv[NMAX];
-    double wv[NMAX];
-#pragma unroll
-    for (int i = 0; i < NMAX; i++) {
-        const bool on = (uint32_t)i < nd;
-        const uint32_t ai = on ? b.attr[i] : 0u;  // entries past nd may be stale
-        wv[i] = on ? b.w[i] : 0.0;
-        mxv[i] = on ? pmax[ai] : 0.0f;
-        mnv[i] = on ? pmin[ai] : 0.0f;
-    }
+// The LP bound from operands already in registers: the first nd entries of
+// wv (weights, heaviest first), mxv / mnv (their attributes' maxima /
+// minima); entries past nd must hold w = 0 and max = min = 0, which add
+// exact zeros; the knapsack loop skips them.  psum bounds every object's
+// attribute sum from above, pminsum is the sum of all attributes' minima
+// rounded down.  The loops are predicated rather than cut short.
+template <int NMAX>
+__device__ __forceinline__ float lp_bound_regs(const double (&wv)[NMAX], const float (&mxv)[NMAX],
+                                               const float (&mnv)[NMAX], float psum, float pminsum) {
     double base = 0, box = 0, absw = 0;
 #pragma unroll
     for (int i = 0; i < NMAX; i++) {
@@ -104,6 +93,27 @@ __device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__res
     }
     const double lp = base < box ? base : box;
     return f32_round_up(lp + absw * 1.52587890625e-05 + 1e-30);  // margin 2^-16 * sum |W x|
+}
+
+// pmax / pmin point at the partition's d per-attribute maxima / minima.
+// NMAX bounds b.nd at compile time.  Every operand (weight, maximum,
+// minimum) is read with independent loads into registers before any
+// arithmetic, so a bound costs one round of memory latency.
+template <int NMAX = TOPK_MAX_DIMS>
+__device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__restrict__ pmax,
+                                          const float *__restrict__ pmin, float psum, float pminsum) {
+    const uint32_t nd = b.nd;
+    float mxv[NMAX], mnv[NMAX];
+    double wv[NMAX];
+#pragma unroll
+    for (int i = 0; i < NMAX; i++) {
+        const bool on = (uint32_t)i < nd;
+        const uint32_t ai = on ? b.attr[i] : 0u;  // entries past nd may be stale
+        wv[i] = on ? b.w[i] : 0.0;
+        mxv[i] = on ? pmax[ai] : 0.0f;
+        mnv[i] = on ? pmin[ai] : 0.0f;
+    }
+    return lp_bound_regs<NMAX>(wv, mxv, mnv, psum, pminsum);
 }
 
 }  // namespace topk

@@ -101,6 +101,10 @@ struct alignas(16) BatchQuery {
     uint32_t k;                     // min(k, n)
     uint32_t kuser;                 // k as requested
     uint32_t pad_;
+    // The queried attributes (first 16) sorted by weight, heaviest first
+    // (ties by lower id): the order a bound's fractional knapsack takes.
+    float ws[16];
+    uint8_t as[16];
     uint8_t attr[TOPK_MAX_DIMS];    // term j reads attribute attr[j]
 };
 static_assert(sizeof(BatchQuery) % 16 == 0, "BatchQuery records stay 16-byte aligned");
@@ -108,6 +112,7 @@ static_assert(offsetof(BatchQuery, tau) % 16 == 0 && offsetof(BatchQuery, tau_id
                   offsetof(BatchQuery, ascending) == offsetof(BatchQuery, tau) + 8 &&
                   offsetof(BatchQuery, m) == offsetof(BatchQuery, tau) + 12,
               "tau, tau_id, ascending, m form one 16-byte word");
+static_assert(offsetof(BatchQuery, ws) % 16 == 0 && offsetof(BatchQuery, as) % 16 == 0, "16-byte loads");
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
 #define TK_SWITCH_M(m, CASE)          \
@@ -200,9 +205,20 @@ struct PtaLayout {
     float *tmin = nullptr;       // 2*p2 x d
     float *tsum = nullptr;       // 2*p2
     float *tminsum = nullptr;    // 2*p2
+    // Chunk statistics: positions [c * PTA_CHUNK, (c + 1) * PTA_CHUNK) form
+    // chunk c (partitions start on chunk boundaries); row c of cstat (crow =
+    // 2d + 2 floats) holds the chunk's maxima, minima, largest attribute sum
+    // (rounded up; -inf: no objects) and sum of minima (rounded down).
+    // Objects are ordered by attribute sum inside a partition, so the leading
+    // chunks hold the large sums.
+    uint64_t nchunks = 0;
+    uint32_t crow = 0;
+    float *cstat = nullptr;
     double tt_build_ms = 0;
 };
+constexpr uint32_t PTA_CHUNK = 256;
 constexpr uint32_t PART_ALIGN = 256;
+static_assert(PART_ALIGN % PTA_CHUNK == 0, "partitions start on chunk boundaries");
 
 }  // namespace topk
 

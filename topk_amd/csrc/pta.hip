@@ -39,7 +39,7 @@ namespace topk {
 
 constexpr int PTA_BS = 256;
 constexpr uint32_t PTA_TILE = 1024;          // objects per stream step (4 per lane)
-constexpr uint32_t PTA_SORT_MAX = 8192;      // partitions up to this size get ids sorted in LDS
+constexpr uint32_t PTA_SORT_MAX = 8192;      // partitions up to this size are sorted in LDS (larger: scatter order)
 constexpr uint32_t PTA_SEED_OBJS = 4096;     // objects of a seed partition that are scored
 constexpr uint32_t PTA_MAX_SEED = 64;
 
@@ -84,29 +84,45 @@ __global__ void pta_scatter_kernel(uint64_t n, const uint32_t *__restrict__ leaf
     perm[poff[l] + atomicAdd(&cursor[l], 1u)] = (uint32_t)i;
 }
 
-// Sorts each partition's ids ascending (deterministic layout).
-__global__ __launch_bounds__(1024) void pta_sort_ids_kernel(const uint32_t *__restrict__ poff,
-                                                            const uint32_t *__restrict__ psz,
-                                                            uint32_t *__restrict__ perm) {
-    __shared__ uint32_t a[PTA_SORT_MAX];
+// Sorts each partition's objects by attribute sum, largest first (ties by
+// lower id): consecutive positions then hold objects of similar sum, so a
+// chunk's statistics bound its objects more tightly than the partition's.
+// The sum only orders; any rounding is fine.
+__global__ __launch_bounds__(1024) void pta_sort_kernel(const float *__restrict__ cols, uint64_t ld, uint32_t d,
+                                                        const uint32_t *__restrict__ poff,
+                                                        const uint32_t *__restrict__ psz,
+                                                        uint32_t *__restrict__ perm) {
+    __shared__ uint64_t a[PTA_SORT_MAX];
     const uint32_t p = blockIdx.x, s = psz[p];
     if (s < 2 || s > PTA_SORT_MAX) return;
     uint32_t *seg = perm + poff[p];
     uint32_t n2 = 1;
     while (n2 < s) n2 <<= 1;
-    for (uint32_t i = threadIdx.x; i < n2; i += 1024) a[i] = i < s ? seg[i] : 0xFFFFFFFFu;
+    for (uint32_t i = threadIdx.x; i < n2; i += 1024) {
+        uint64_t key = 0;
+        if (i < s) {
+            const uint32_t id = seg[i];
+            float sum = 0.0f;
+            for (uint32_t m = 0; m < d; m++) sum += cols[(uint64_t)m * ld + id];
+            key = rkey(sum, id);
+        }
+        a[i] = key;
+    }
     __syncthreads();
     for (uint32_t size = 2; size <= n2; size <<= 1)
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
             for (uint32_t t = threadIdx.x; t < n2 / 2; t += 1024) {
                 const uint32_t lo = 2 * stride * (t / stride) + (t & (stride - 1)), hi = lo + stride;
-                const bool asc = (lo & size) == 0;
-                const uint32_t x = a[lo], y = a[hi];
-                if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
+                const bool desc = (lo & size) == 0;
+                const uint64_t x = a[lo], y = a[hi];
+                if ((x < y) == desc) {
+                    a[lo] = y;
+                    a[hi] = x;
+                }
             }
             __syncthreads();
         }
-    for (uint32_t i = threadIdx.x; i < s; i += 1024) seg[i] = a[i];
+    for (uint32_t i = threadIdx.x; i < s; i += 1024) seg[i] = rkey_id(a[i]);
 }
 
 __global__ void pta_gather_kernel(const float *__restrict__ cols, uint64_t ld, uint32_t d, uint64_t npos,
@@ -170,6 +186,65 @@ __global__ __launch_bounds__(PTA_BS) void pta_bounds_kernel(const float *__restr
         for (uint32_t m = 0; m < d; m++) ms += (double)colmin[m];
         psum[p] = s ? f32_round_up(best) : -INFINITY;
         pminsum[p] = s ? f32_round_down(ms) : 0.0f;
+    }
+}
+
+// One wave per chunk of PTA_CHUNK positions: row c of cstat holds the
+// chunk's per-attribute maxima [0, d) and minima [d, 2d), the largest
+// attribute sum of its objects (FP64, rounded up) at 2d and the sum of its
+// minima (FP64, rounded down) at 2d + 1.  A chunk without objects has sum
+// -inf.
+__global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict__ pcols, uint64_t ldp, uint32_t d,
+                                                        const uint32_t *__restrict__ pid, uint64_t nchunks,
+                                                        float *__restrict__ cstat) {
+    const uint64_t c = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    if (c >= nchunks) return;
+    const uint32_t lane = lane_id();
+    constexpr uint32_t PER = PTA_CHUNK / 64;
+    const uint64_t pos0 = c * PTA_CHUNK + lane * PER;
+    bool ok[PER];
+    bool any = false;
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) {
+        ok[j] = pid[pos0 + j] != 0xFFFFFFFFu;
+        any |= ok[j];
+    }
+    const bool nonempty = __ballot(any) != 0;
+    float *row = cstat + c * (2 * d + 2);
+    double best = -INFINITY, minsum = 0.0;
+    double sums[PER];
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) sums[j] = 0.0;
+    for (uint32_t m = 0; m < d; m++) {
+        float mx = -INFINITY, mn = INFINITY;
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) {
+            const float v = pcols[(uint64_t)m * ldp + pos0 + j];
+            if (ok[j]) {
+                mx = v > mx ? v : mx;
+                mn = v < mn ? v : mn;
+                sums[j] += (double)v;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+            mn = fminf(mn, __shfl_xor(mn, o, 64));
+        }
+        minsum += (double)mn;
+        if (lane == 0) {
+            row[m] = nonempty ? mx : 0.0f;
+            row[d + m] = nonempty ? mn : 0.0f;
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < PER; j++) best = ok[j] && sums[j] > best ? sums[j] : best;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(best, o, 64);
+        best = t > best ? t : best;
+    }
+    if (lane == 0) {
+        row[2 * d] = nonempty ? f32_round_up(best) : -INFINITY;
+        row[2 * d + 1] = nonempty ? f32_round_down(minsum) : 0.0f;
     }
 }
 
@@ -303,6 +378,7 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.tmin);
     hipFree(P.tsum);
     hipFree(P.tminsum);
+    hipFree(P.cstat);
     const uint32_t psize = P.psize;
     const uint64_t gen = P.generation;
     P = PtaLayout();
@@ -389,12 +465,12 @@ int pta_build(topk_ctx *ctx) {
         TK_TRY(hipMemcpyAsync(P.psz, h_cnt.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice, s));
         TK_TRY(hipStreamSynchronize(s));  // host vectors go out of scope
 
-        // 3. counting sort by leaf, ids ascending inside each partition
+        // 3. counting sort by leaf, then by attribute sum inside each partition
         TK_TRY(hipMalloc(&perm, sizeof(uint32_t) * P.ld));
         TK_TRY(hipMemsetAsync(perm, 0xFF, sizeof(uint32_t) * P.ld, s));
         hipLaunchKernelGGL(pta_scatter_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, n, leaf, P.poff,
                            cursor, perm);
-        hipLaunchKernelGGL(pta_sort_ids_kernel, dim3(L), dim3(1024), 0, s, P.poff, P.psz, perm);
+        hipLaunchKernelGGL(pta_sort_kernel, dim3(L), dim3(1024), 0, s, ctx->cols, ctx->ld, d, P.poff, P.psz, perm);
 
         // 4. partition-order columns and bounds
         const size_t pfl = P.ld * d + 8192;
@@ -410,6 +486,12 @@ int pta_build(topk_ctx *ctx) {
         TK_TRY(hipMalloc(&P.pminsum, sizeof(float) * L));
         hipLaunchKernelGGL(pta_bounds_kernel, dim3(L), dim3(PTA_BS), 0, s, P.pcols, P.ld, d, P.poff, P.psz, P.pmax,
                            P.pmin, P.psum, P.pminsum);
+        P.nchunks = P.npos / PTA_CHUNK;
+        P.crow = 2 * d + 2;
+        TK_TRY(hipMalloc(&P.cstat, sizeof(float) * (P.nchunks ? P.nchunks : 1) * P.crow));
+        if (P.nchunks)
+            hipLaunchKernelGGL(pta_chunk_kernel, dim3((uint32_t)((P.nchunks + 3) / 4)), dim3(256), 0, s, P.pcols, P.ld,
+                               d, P.pid, (uint64_t)P.nchunks, P.cstat);
         TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
         TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
         TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
