@@ -134,7 +134,7 @@ def learned_batched(eng, n, k, args):
     eng.set_tuning("batch_mode", 2)
     tuning = None
     if args.tune:
-        for name, val in zip(("batch_vpt", "batch_bs", "batch_qpi", "batch_digit", "batch_grid"),
+        for name, val in zip(("batch_vpt", "batch_bs", "batch_qpi", "batch_digit", "batch_grid", "batch_item"),
                              (int(t) for t in args.tune.split(","))):
             eng.set_tuning(name, val)
         tuning = {"fixed": args.tune}
@@ -283,7 +283,7 @@ def main():
     ap.add_argument("--train-queries", type=int, default=16384)
     ap.add_argument("--held-batches", type=int, default=4)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tune", default="", metavar="VPT,BS,QPI,DIGIT,GRID",
+    ap.add_argument("--tune", default="", metavar="VPT,BS,QPI,DIGIT,GRID[,ITEM]",
                     help="fixed batch launch shape instead of the autotuner's choice")
     ap.add_argument("--fte-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)

@@ -192,7 +192,7 @@ int topk_batch_wait(topk_ctx *ctx, uint64_t ticket);
  * per CU), query budget per work item and select digit width, and installs
  * the fastest in the context (as topk_set_tuning would). */
 typedef struct topk_batch_tuning {
-    uint32_t vpt, bs, qpi, digit, grid;
+    uint32_t vpt, bs, qpi, digit, grid, item;
     uint32_t variants;           /* shapes timed */
     double ms_best, ms_default;  /* device time of one batch: chosen / defaults */
 } topk_batch_tuning;
@@ -292,6 +292,8 @@ void *topk_stream(topk_ctx *ctx);
  *                     to this many of the queries that reach it
  *   "batch_digit"     radix digit width of the per-query select: 8 or 11
  *   "batch_grid"      persistent batch scoring workgroups
+ *   "batch_item"      objects of a partition per batch work item (a multiple
+ *                     of 256, 256..65536)
  *   "batch_graph"     1 = capture each batch shape in a hipGraph and replay
  *   "batch_mode"      0 = learned threshold when trained, else seed;
  *                     1 = seed threshold; 2 = learned threshold

@@ -119,14 +119,15 @@ def test_batch_variants_agree(gpu):
     qs = make_queries(rng, 200, d, [16, 64, 256])
     ref = oracle_batch(x, n, qs)
     with topk_amd.Engine(x) as eng:
-        for vpt, bs, qpi, digit, mask in [(4, 256, 64, 11, 1), (8, 512, 32, 8, 2), (16, 256, 16, 11, 0),
-                                          (8, 256, 256, 8, 2), (8, 256, 1024, 11, 1)]:
+        for vpt, bs, qpi, digit, mask, item in [(4, 256, 64, 11, 1, 2048), (8, 512, 32, 8, 2, 4096),
+                                                (16, 256, 16, 11, 0, 8192), (8, 256, 256, 8, 2, 256),
+                                                (8, 256, 1024, 11, 1, 16384), (4, 512, 16, 8, 0, 1280)]:
             for name, v in (("batch_vpt", vpt), ("batch_bs", bs), ("batch_qpi", qpi), ("batch_digit", digit),
-                            ("batch_grid", 300), ("batch_mask", mask)):
+                            ("batch_grid", 300), ("batch_mask", mask), ("batch_item", item)):
                 eng.set_tuning(name, v)
             got = eng.batch(qs)
             for (gi, gs, _), (ri, rs) in zip(got, ref):
-                assert np.array_equal(gi, ri) and np.array_equal(gs, rs), (vpt, bs, qpi, digit, mask)
+                assert np.array_equal(gi, ri) and np.array_equal(gs, rs), (vpt, bs, qpi, digit, mask, item)
 
 
 def test_batch_autotune_keeps_results(gpu):
@@ -139,6 +140,7 @@ def test_batch_autotune_keeps_results(gpu):
         t = eng.batch_autotune(qs)
         assert t["variants"] > 10 and 0 < t["ms_best"] <= t["ms_default"]
         assert t["qpi"] in (16, 32, 64, 128) and t["bs"] in (256, 512) and t["digit"] in (8, 11)
+        assert t["item"] in (2048, 4096, 8192, 16384)
         after = eng.batch(qs)
         for (a, b, c), (e, f, g) in zip(before, after):
             assert np.array_equal(a, e) and np.array_equal(b, f) and c == g
@@ -191,6 +193,8 @@ def test_batch_errors(gpu):
             eng.set_tuning("batch_qpi", 8)
         with pytest.raises(topk_amd.TopkError):
             eng.set_tuning("batch_mask", 3)
+        with pytest.raises(topk_amd.TopkError):
+            eng.set_tuning("batch_item", 1000)
 
 
 def test_batch_submit_wait_pipeline(gpu):

@@ -594,64 +594,67 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
         const uint16_t *ql = a.plist + (uint64_t)I.p * a.nq + I.qbeg;
         uint32_t nxt = 0;
         bool ticket = tid == 0;
-        for (uint32_t b0 = 0; b0 < size; b0 += BS * VPT) {
-            const uint32_t i0 = b0 + tid * VPT;
-            const uint32_t nvalid = i0 < size ? (size - i0 < (uint32_t)VPT ? size - i0 : (uint32_t)VPT) : 0u;
-            // this wave's objects: item positions [wb, we)
-            const uint32_t wb = __builtin_amdgcn_readfirstlane(b0 + (tid & ~63u) * VPT);
-            if (wb >= size) continue;
-            const uint32_t we = size - wb < 64u * VPT ? size : wb + 64u * VPT;
-            // lane e holds element e of the union of the wave's chunk rows
-            float cs = 0.0f;
+        for (uint32_t q0 = 0; q0 < I.qcnt; q0 += 64) {
+            const uint32_t nqb = I.qcnt - q0 < 64u ? I.qcnt - q0 : 64u;
+            // lane j: query q0 + j of the item (lanes past nqb hold query 0)
+            const uint32_t lq = lane < nqb ? (uint32_t)ql[q0 + lane] : 0u;
+            const f4 lt = *reinterpret_cast<const f4 *>(&a.q[lq].tau);  // tau, tau_id, ascending, m
+            float lw[LANE_W ? D : 1];
+            f4 lws[LANE_W ? D / 4 : 1];
+            uint4 las = {0u, 0u, 0u, 0u};
             if constexpr (LANE_W) {
-                const uint32_t c0 = (start + wb) / PTA_CHUNK, c1 = (start + we - 1) / PTA_CHUNK;
-                const uint32_t d = a.d;
-                for (uint32_t c = c0; c <= c1; c++) {
-                    const float v = lane < a.crow ? a.cstat[(uint64_t)c * a.crow + lane] : 0.0f;
-                    if (c == c0) cs = v;
-                    else if (lane < d || lane == 2 * d) cs = fmaxf(cs, v);  // maxima, largest sum
-                    else if (lane < 2 * d) cs = fminf(cs, v);                // minima
+#pragma unroll
+                for (int c = 0; c < D; c += 4) {
+                    const f4 v = *reinterpret_cast<const f4 *>(&a.q[lq].wa[c]);
+                    lw[c] = v[0];
+                    lw[c + 1] = v[1];
+                    lw[c + 2] = v[2];
+                    lw[c + 3] = v[3];
+                    lws[c / 4] = reinterpret_cast<const f4 *>(a.q[lq].ws)[c / 4];
                 }
+                las = *reinterpret_cast<const uint4 *>(a.q[lq].as);
             }
-            const uint32_t dd = a.d;
-            const float csum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd)));
-            // the union's sum of minima is the sum of its per-attribute minima
-            // (not the smallest chunk sum), in FP64 rounded down
-            double msum = 0.0;
-#pragma unroll
-            for (int m = 0; m < D; m++)
-                if ((uint32_t)m < dd)
-                    msum += (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)dd + m));
-            const float cminsum = f32_round_down(msum);
-            for (uint32_t q0 = 0; q0 < I.qcnt; q0 += 64) {
-                const uint32_t nqb = I.qcnt - q0 < 64u ? I.qcnt - q0 : 64u;
-                // lane j: query q0 + j of the item
-                const uint32_t lq = lane < nqb ? (uint32_t)ql[q0 + lane] : 0u;
-                const f4 lt = *reinterpret_cast<const f4 *>(&a.q[lq].tau);  // tau, tau_id, ascending, m
-                float lw[LANE_W ? D : 1];
-                if constexpr (LANE_W) {
-#pragma unroll
-                    for (int c = 0; c < D; c += 4) {
-                        const f4 v = *reinterpret_cast<const f4 *>(&a.q[lq].wa[c]);
-                        lw[c] = v[0];
-                        lw[c + 1] = v[1];
-                        lw[c + 2] = v[2];
-                        lw[c + 3] = v[3];
-                    }
-                }
+            if (ticket) {
+                nxt = atomicAdd(&a.ctr[6], 1u);
+                ticket = false;
+            }
+            for (uint32_t b0 = 0; b0 < size; b0 += BS * VPT) {
+                const uint32_t i0 = b0 + tid * VPT;
+                const uint32_t nvalid = i0 < size ? (size - i0 < (uint32_t)VPT ? size - i0 : (uint32_t)VPT) : 0u;
+                // this wave's objects: item positions [wb, we)
+                const uint32_t wb = __builtin_amdgcn_readfirstlane(b0 + (tid & ~63u) * VPT);
+                if (wb >= size) break;
+                const uint32_t we = size - wb < 64u * VPT ? size : wb + 64u * VPT;
                 // chunk bound of lane j's query, evaluated by every lane (the
-                // shuffles read all lanes); lanes past nqb hold query 0
+                // shuffles read all lanes)
                 bool pass = lane < nqb;
                 if constexpr (LANE_W) {
-                    const f4 *wsp = reinterpret_cast<const f4 *>(a.q[lq].ws);
-                    const uint4 asv = *reinterpret_cast<const uint4 *>(a.q[lq].as);
+                    // lane e holds element e of the union of the wave's chunk rows
+                    const uint32_t c0 = (start + wb) / PTA_CHUNK, c1 = (start + we - 1) / PTA_CHUNK;
+                    const uint32_t dd = a.d;
+                    float cs = 0.0f;
+                    for (uint32_t c = c0; c <= c1; c++) {
+                        const float v = lane < a.crow ? a.cstat[(uint64_t)c * a.crow + lane] : 0.0f;
+                        if (c == c0) cs = v;
+                        else if (lane < dd || lane == 2 * dd) cs = fmaxf(cs, v);  // maxima, largest sum
+                        else if (lane < 2 * dd) cs = fminf(cs, v);                // minima
+                    }
+                    const float csum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd)));
+                    // the union's sum of minima is the sum of its per-attribute
+                    // minima (not the smallest chunk sum), in FP64 rounded down
+                    double msum = 0.0;
+#pragma unroll
+                    for (int m = 0; m < D; m++)
+                        if ((uint32_t)m < dd)
+                            msum += (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)dd + m));
+                    const float cminsum = f32_round_down(msum);
                     const uint32_t mq = (uint32_t)__float_as_int(lt[3]);
                     double wv[D];
                     float mxv[D], mnv[D];
 #pragma unroll
                     for (int i = 0; i < D; i += 4) {
-                        const f4 w4 = wsp[i / 4];
-                        const uint32_t aw = i / 4 == 0 ? asv.x : (i / 4 == 1 ? asv.y : (i / 4 == 2 ? asv.z : asv.w));
+                        const f4 w4 = lws[i / 4];
+                        const uint32_t aw = i / 4 == 0 ? las.x : (i / 4 == 1 ? las.y : (i / 4 == 2 ? las.z : las.w));
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
                             const bool on = (uint32_t)(i + u) < mq;
@@ -667,13 +670,8 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     pass = pass && (!asc_l || ub >= lt[0]);
                 }
                 const uint64_t qmask = __ballot(pass);
-                if (ticket) {
-                    nxt = atomicAdd(&a.ctr[6], 1u);
-                    ticket = false;
-                }
                 if (!qmask) continue;
-                // the objects, loaded once the wave has queries to score (again
-                // per 64-query group, so they are not live during the bounds)
+                // the objects, loaded once the wave has queries to score
                 float x[D][VPT];
 #pragma unroll
                 for (int c = 0; c < D; c++) {
@@ -689,71 +687,71 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     npairs += nvalid;
                     const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)lq, (int)qi);
                     const uint32_t asc = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[2]), (int)qi);
-                    float s[VPT];
+                        float s[VPT];
 #pragma unroll
-                    for (int e = 0; e < VPT; e++) s[e] = 0.0f;
-                    if (asc) {
-                        // static register order, zero weights skipped by a uniform branch
+                        for (int e = 0; e < VPT; e++) s[e] = 0.0f;
+                        if (asc) {
+                            // static register order, zero weights skipped by a uniform branch
 #pragma unroll
-                        for (int c = 0; c < D; c++) {
-                            float w;
-                            if constexpr (LANE_W)
-                                w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lw[LANE_W ? c : 0]),
-                                                                             (int)qi));
-                            else
-                                w = a.q[q].wa[c];
-                            if (w != 0.0f) {
+                            for (int c = 0; c < D; c++) {
+                                float w;
+                                if constexpr (LANE_W)
+                                    w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lw[LANE_W ? c : 0]),
+                                                                                 (int)qi));
+                                else
+                                    w = a.q[q].wa[c];
+                                if (w != 0.0f) {
 #pragma unroll
-                                for (int e = 0; e < VPT; e++) {
-                                    const float pr = x[c][e] * w;
-                                    s[e] = s[e] + pr;
+                                    for (int e = 0; e < VPT; e++) {
+                                        const float pr = x[c][e] * w;
+                                        s[e] = s[e] + pr;
+                                    }
+                                }
+                            }
+                        } else {
+                            const BatchQuery &Q = a.q[q];
+                            for (uint32_t j = 0; j < Q.m; j++) {
+                                const float w = Q.w[j];
+                                switch (Q.attr[j]) {
+                                    TK_TERM(0) TK_TERM(1) TK_TERM(2) TK_TERM(3) TK_TERM(4) TK_TERM(5) TK_TERM(6)
+                                    TK_TERM(7) TK_TERM(8) TK_TERM(9) TK_TERM(10) TK_TERM(11) TK_TERM(12) TK_TERM(13)
+                                    TK_TERM(14) TK_TERM(15) TK_TERM(16) TK_TERM(17) TK_TERM(18) TK_TERM(19) TK_TERM(20)
+                                    TK_TERM(21) TK_TERM(22) TK_TERM(23) TK_TERM(24) TK_TERM(25) TK_TERM(26) TK_TERM(27)
+                                    TK_TERM(28) TK_TERM(29) TK_TERM(30) TK_TERM(31)
+                                    default: break;
                                 }
                             }
                         }
-                    } else {
-                        const BatchQuery &Q = a.q[q];
-                        for (uint32_t j = 0; j < Q.m; j++) {
-                            const float w = Q.w[j];
-                            switch (Q.attr[j]) {
-                                TK_TERM(0) TK_TERM(1) TK_TERM(2) TK_TERM(3) TK_TERM(4) TK_TERM(5) TK_TERM(6)
-                                TK_TERM(7) TK_TERM(8) TK_TERM(9) TK_TERM(10) TK_TERM(11) TK_TERM(12) TK_TERM(13)
-                                TK_TERM(14) TK_TERM(15) TK_TERM(16) TK_TERM(17) TK_TERM(18) TK_TERM(19) TK_TERM(20)
-                                TK_TERM(21) TK_TERM(22) TK_TERM(23) TK_TERM(24) TK_TERM(25) TK_TERM(26) TK_TERM(27)
-                                TK_TERM(28) TK_TERM(29) TK_TERM(30) TK_TERM(31)
-                                default: break;
-                            }
-                        }
-                    }
-                    // admitted: key (s, id) >= (tau, tau_id); the id is read only on a tie
-                    const float tau = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lt[0]), (int)qi));
-                    const uint32_t tau_id = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[1]), (int)qi);
-                    uint32_t adm = 0;
-#pragma unroll
-                    for (int e = 0; e < VPT; e++) {
-                        bool in = (uint32_t)e < nvalid && s[e] >= tau;
-                        if (in && s[e] == tau && tau_id != 0xFFFFFFFFu) in = a.pid[start + i0 + e] <= tau_id;
-                        adm |= in ? 1u << e : 0u;
-                    }
-                    const uint32_t np = (uint32_t)__popc(adm);
-                    if (__ballot(np != 0)) {
-                        const uint32_t incl = wave_incl_scan(np);
-                        const uint32_t tot = __shfl(incl, 63, 64);
-                        uint32_t base = 0;
-                        if (lane == 63) base = atomicAdd(&a.ccount[q], tot);
-                        base = __shfl(base, 63, 64);
-                        uint32_t pos = base + incl - np;
-                        uint64_t *out = a.cand + (uint64_t)q * BATCH_CAP;
+                        // admitted: key (s, id) >= (tau, tau_id); the id is read only on a tie
+                        const float tau = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lt[0]), (int)qi));
+                        const uint32_t tau_id = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[1]), (int)qi);
+                        uint32_t adm = 0;
 #pragma unroll
                         for (int e = 0; e < VPT; e++) {
-                            if ((adm >> e) & 1u) {
-                                if (pos < BATCH_CAP) out[pos] = rkey(s[e], a.pid[start + i0 + e]);
-                                pos++;
+                            bool in = (uint32_t)e < nvalid && s[e] >= tau;
+                            if (in && s[e] == tau && tau_id != 0xFFFFFFFFu) in = a.pid[start + i0 + e] <= tau_id;
+                            adm |= in ? 1u << e : 0u;
+                        }
+                        const uint32_t np = (uint32_t)__popc(adm);
+                        if (__ballot(np != 0)) {
+                            const uint32_t incl = wave_incl_scan(np);
+                            const uint32_t tot = __shfl(incl, 63, 64);
+                            uint32_t base = 0;
+                            if (lane == 63) base = atomicAdd(&a.ccount[q], tot);
+                            base = __shfl(base, 63, 64);
+                            uint32_t pos = base + incl - np;
+                            uint64_t *out = a.cand + (uint64_t)q * BATCH_CAP;
+#pragma unroll
+                            for (int e = 0; e < VPT; e++) {
+                                if ((adm >> e) & 1u) {
+                                    if (pos < BATCH_CAP) out[pos] = rkey(s[e], a.pid[start + i0 + e]);
+                                    pos++;
+                                }
                             }
                         }
                     }
                 }
             }
-        }
         if (ticket) nxt = atomicAdd(&a.ctr[6], 1u);  // no wave of tid 0 reached the queries
         if (tid == 0) s_it[slot] = nxt;
         __syncthreads();
@@ -862,9 +860,10 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
 // ---------------------------------------------------------------------------
 struct BatchVariant {
     int vpt, bs, qpi, digit, grid;
+    int item;  // objects per work item (a multiple of PTA_CHUNK)
 };
 
-using GraphKey = std::array<uint64_t, 12>;
+using GraphKey = std::array<uint64_t, 13>;
 
 // Pinned staging of one batch in flight, and where its results go.
 struct HostSlot {
@@ -1075,11 +1074,12 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
 
 static int dims_class(uint32_t d) { return d <= 8 ? 8 : (d <= 16 ? 16 : 32); }
 
-// Objects of one work item: one scoring pass of the shape, at least
-// ITEM_MIN_OBJECTS (bounds the item list).
+// Objects of one work item: the variant's item size, at least one scoring
+// pass of the shape and ITEM_MIN_OBJECTS (which bounds the item list).
 static uint32_t item_objects(const BatchVariant &v) {
-    const uint32_t pass = (uint32_t)(v.bs * v.vpt);
-    return pass > ITEM_MIN_OBJECTS ? pass : ITEM_MIN_OBJECTS;
+    uint32_t o = (uint32_t)(v.bs * v.vpt);
+    o = o > (uint32_t)v.item ? o : (uint32_t)v.item;
+    return o > ITEM_MIN_OBJECTS ? o : ITEM_MIN_OBJECTS;
 }
 
 static BatchVariant pick_variant(const topk_ctx *ctx, const BatchState &B) {
@@ -1090,6 +1090,7 @@ static BatchVariant pick_variant(const topk_ctx *ctx, const BatchState &B) {
     v.qpi = ctx->tune.batch_qpi ? ctx->tune.batch_qpi : 128;
     v.digit = ctx->tune.batch_digit ? ctx->tune.batch_digit : 11;
     v.grid = ctx->tune.batch_grid ? ctx->tune.batch_grid : B.cus * (v.bs == 256 ? 4 : 2);
+    v.item = ctx->tune.batch_item ? ctx->tune.batch_item : 4096;
     return v;
 }
 
@@ -1229,7 +1230,7 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     if (!ctx->tune.batch_graph || ctx->timing_on) return batch_enqueue(ctx, B, s, nq, ks, mode, v);
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
                        (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
-                       (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur};
+                       (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur, (uint64_t)v.item};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1414,7 +1415,8 @@ static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     if (rc) return rc;
     const BatchVariant v = force ? *force : pick_variant(ctx, B);  // after ensure: the CU count is known
     if (!variant_ok(dims_class(ctx->d), v.vpt, v.bs) || v.qpi < 16 || v.qpi > 1024 ||
-        (v.digit != 8 && v.digit != 11) || v.grid < 1)
+        (v.digit != 8 && v.digit != 11) || v.grid < 1 || v.grid > (int)BATCH_MAX_GRID || v.item < (int)PTA_CHUNK ||
+        v.item % PTA_CHUNK)
         return TOPK_ERR_INVALID;
     B.cur = si;
     rc = stage_queries(ctx, B, nq, qs, ks);
@@ -1499,7 +1501,8 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
         for (int bs : {256, 512}) {
             if (!variant_ok(D, vpt, bs)) continue;
             for (int qpi : qpis)
-                for (int pc : per_cu) cand.push_back(BatchVariant{vpt, bs, qpi, def.digit, B.cus * pc * 256 / bs});
+                for (int pc : per_cu)
+                    cand.push_back(BatchVariant{vpt, bs, qpi, def.digit, B.cus * pc * 256 / bs, def.item});
         }
     hipStream_t s = ctx->lane[0].stream;
     hipEvent_t e0, e1;
@@ -1531,12 +1534,26 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
             best = cand[i];
         }
     }
+    uint32_t extra = 0;
     for (int dg : {8, 11}) {  // the final select's digit width, with the best shape
         if (rc || dg == best.digit) continue;
         BatchVariant v = best;
         v.digit = dg;
         double ms;
         rc = time_one(v, ms);
+        extra++;
+        if (!rc && ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    for (int it : {2048, 4096, 8192, 16384}) {  // objects per work item, with the best shape
+        if (rc || it == best.item) continue;
+        BatchVariant v = best;
+        v.item = it;
+        double ms;
+        rc = time_one(v, ms);
+        extra++;
         if (!rc && ms < best_ms) {
             best_ms = ms;
             best = v;
@@ -1550,13 +1567,15 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     ctx->tune.batch_qpi = best.qpi;
     ctx->tune.batch_digit = best.digit;
     ctx->tune.batch_grid = best.grid;
+    ctx->tune.batch_item = best.item;
     if (out) {
         out->vpt = (uint32_t)best.vpt;
         out->bs = (uint32_t)best.bs;
         out->qpi = (uint32_t)best.qpi;
         out->digit = (uint32_t)best.digit;
         out->grid = (uint32_t)best.grid;
-        out->variants = (uint32_t)cand.size() + 2;
+        out->item = (uint32_t)best.item;
+        out->variants = (uint32_t)cand.size() + 1 + extra;
         out->ms_best = best_ms;
         out->ms_default = ms_def;
     }
