@@ -294,6 +294,9 @@ void *topk_stream(topk_ctx *ctx);
  *   "batch_grid"      persistent batch scoring workgroups
  *   "batch_item"      objects of a partition per batch work item (a multiple
  *                     of 256, 256..65536)
+ *   "batch_retry"     a learned threshold that left fewer than k candidates
+ *                     is rerun (second pass) with: 1 = a threshold far below
+ *                     the prediction; 2 = a seed threshold (0 = default)
  *   "batch_graph"     1 = capture each batch shape in a hipGraph and replay
  *   "batch_mode"      0 = learned threshold when trained, else seed;
  *                     1 = seed threshold; 2 = learned threshold

@@ -427,6 +427,9 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
     } else if (key == "batch_item") {
         if (value != 0 && (value < 256 || value > 65536 || value % 256)) return TOPK_ERR_INVALID;
         ctx->tune.batch_item = (int)value;
+    } else if (key == "batch_retry") {
+        if (value < 0 || value > 2) return TOPK_ERR_INVALID;
+        ctx->tune.batch_retry = (int)value;
     } else if (key == "batch_graph") {
         ctx->tune.batch_graph = value ? 1 : 0;
     } else if (key == "batch_mode") {

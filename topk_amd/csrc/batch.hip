@@ -174,16 +174,6 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
     }
 }
 
-__global__ __launch_bounds__(BAUX_BS) void batch_zero_kernel(unsigned int *__restrict__ pcnt, uint32_t L,
-                                                             unsigned int *__restrict__ ccount, uint32_t nq,
-                                                             unsigned int *__restrict__ ctr) {
-    const uint64_t stride = (uint64_t)gridDim.x * BAUX_BS;
-    const uint64_t i0 = (uint64_t)blockIdx.x * BAUX_BS + threadIdx.x;
-    for (uint64_t i = i0; i < L; i += stride) pcnt[i] = 0u;
-    for (uint64_t i = i0; i < nq; i += stride) ccount[i] = 0u;
-    if (i0 < BCTR_WORDS) ctr[i0] = 0u;
-}
-
 // Bound tree of the partition layout as the batch kernels read it.
 struct TreeArgs {
     const float *tmax, *tmin, *tsum, *tminsum;
@@ -774,8 +764,8 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
 struct FinalArgs {
     BatchQuery *q;
     const uint64_t *cand;
-    const unsigned int *ccount;
-    const unsigned int *ctr;
+    unsigned int *ccount;    // first pass: read, then cleared for the second
+    unsigned int *ctr;       // first pass: copied out, then cleared for the second
     uint32_t kstride;
     uint64_t *ids;
     float *scores;
@@ -787,6 +777,9 @@ struct FinalArgs {
     int second;
     const unsigned long long *scored;  // per scoring workgroup of this pass
     uint32_t nscored;
+    int seed2;               // the second pass runs a seed threshold pass
+    unsigned int *pcnt;      // first pass: cleared here for the second
+    uint32_t L;
 };
 
 template <int DB>
@@ -798,7 +791,10 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     if (q == 0) {
         // counters of this pass; words 4..5 become the (object, query) pairs
         // the scoring kernel actually scored
-        if (tid < BCTR_WORDS && (tid < 4 || tid > 5)) a.ctr_out[tid] = a.ctr[tid];
+        if (tid < BCTR_WORDS) {
+            if (tid < 4 || tid > 5) a.ctr_out[tid] = a.ctr[tid];
+            if (!a.second) a.ctr[tid] = 0u;  // counters of the second pass
+        }
         unsigned long long t = 0;
         for (uint32_t i = tid; i < a.nscored; i += BFIN_BS) t += a.scored[i];
         for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
@@ -813,6 +809,13 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     if (a.second && a.qflag[q] == QF_DONE) return;
     const uint32_t c = a.ccount[q], k = a.q[q].k;
     if (tid == 0) a.cnt[q] = c;
+    if (!a.second) {
+        // clear what the second pass counts: this query's candidates and a
+        // share of the partition counters (the compaction has read them)
+        __syncthreads();
+        if (tid == 0) a.ccount[q] = 0u;
+        for (uint64_t i = (uint64_t)q * BFIN_BS + tid; i < a.L; i += (uint64_t)gridDim.x * BFIN_BS) a.pcnt[i] = 0u;
+    }
     if (c > BATCH_CAP) {
         if (a.second) {
             if (tid == 0) a.status[q] = STATUS_OVERFLOW;
@@ -843,7 +846,9 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
                     a.q[q].tau_id = 0xFFFFFFFFu;
                     a.qflag[q] = QF_LOWER;
                 } else {
-                    a.qflag[q] = QF_SEED;
+                    // no lower threshold: a seed pass, or (without one) the
+                    // host's exact path (status RETRY stays)
+                    a.qflag[q] = a.seed2 ? QF_SEED : QF_DONE;
                 }
             }
         }
@@ -863,7 +868,7 @@ struct BatchVariant {
     int item;  // objects per work item (a multiple of PTA_CHUNK)
 };
 
-using GraphKey = std::array<uint64_t, 13>;
+using GraphKey = std::array<uint64_t, 14>;
 
 // Pinned staging of one batch in flight, and where its results go.
 struct HostSlot {
@@ -1126,20 +1131,19 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
     uint32_t *qflag = reinterpret_cast<uint32_t *>(B.d_out + ol.fl_off);
     const uint32_t *pass_flags = second ? qflag : nullptr;
-    if (second) {  // the first pass's counters were cleared by the prep kernel
-        uint32_t zg = ((L > nq ? L : nq) + BAUX_BS - 1) / BAUX_BS;
-        if (zg > 1024) zg = 1024;
-        hipLaunchKernelGGL(batch_zero_kernel, dim3(zg), dim3(BAUX_BS), 0, s, B.d_pcnt, L, B.d_ccount, nq, B.d_ctr);
-    }
+    // counters: cleared by the prep kernel for the first pass, by the first
+    // pass's final kernel for the second
     const TreeArgs ta{P.tmax, P.tmin, P.tsum, P.tminsum, P.p2, ctx->d};
+    // a second-pass seed only serves learned misses that rerun with seeds
+    const bool seed2 = mode == BMODE_LEARNED && ctx->tune.batch_retry == 2;
     hipEvent_t evt = nullptr;
     timing_begin(ctx, s, evt);
-    if (mode == BMODE_SEED || second) {
+    if (second ? seed2 : mode == BMODE_SEED) {
         const int NB = dims_class(ctx->d);
         auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
         hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz, P.pid,
                            pass_flags);
-    } else if (mode == BMODE_LEARNED) {
+    } else if (mode == BMODE_LEARNED && !second) {
         const int rc = learned_tau_enqueue(ctx, s, dq, nq);
         if (rc) return rc;
     }
@@ -1192,7 +1196,10 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                  qflag,
                  second ? 1 : 0,
                  B.d_scored,
-                 (uint32_t)vs.grid};
+                 (uint32_t)vs.grid,
+                 seed2 ? 1 : 0,
+                 B.d_pcnt,
+                 L};
     if (v.digit == 8)
         hipLaunchKernelGGL(batch_final_kernel<8>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
     else
@@ -1230,7 +1237,8 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     if (!ctx->tune.batch_graph || ctx->timing_on) return batch_enqueue(ctx, B, s, nq, ks, mode, v);
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
                        (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
-                       (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur, (uint64_t)v.item};
+                       (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur, (uint64_t)v.item,
+                       (uint64_t)ctx->tune.batch_retry};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -287,7 +287,7 @@ __global__ void mlp_adam_kernel(float *__restrict__ P, float *__restrict__ M, fl
 // the parameter load, so waves may take different numbers of queries).
 __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict__ P, BatchQuery *__restrict__ qs,
                                                          uint32_t nq, Sketch S, float margin,
-                                                         float *__restrict__ pred) {
+                                                         float *__restrict__ pred, float retry_width) {
     __shared__ float W1T[MLP_IN][MLP_H];
     __shared__ float W2T[MLP_H][MLP_H];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid / 64;
@@ -312,8 +312,10 @@ __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict
         if (lane == 0) {
             const float y = o + b3;
             qs[q].tau = sc > 0.0f ? (y - margin) * sc : -INFINITY;
-            const float wide = 9.0f * fabsf(margin);  // a miss reruns far below the prediction
-            qs[q].tau_retry = sc > 0.0f ? (y - wide) * sc : -INFINITY;
+            // a miss reruns far below the prediction, or (retry_width < 0)
+            // with a seed threshold
+            const float wide = retry_width * fmaxf(fabsf(margin), 1e-3f);  // strictly below tau
+            qs[q].tau_retry = retry_width < 0.0f ? INFINITY : (sc > 0.0f ? (y - wide) * sc : -INFINITY);
             if (pred) pred[q] = y * sc;
         }
     }
@@ -429,8 +431,9 @@ int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq
     if (!L || !L->ready) return TOPK_ERR_STATE;
     uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
     if (g > 1024) g = 1024;
+    const float width = ctx->tune.batch_retry == 2 ? -1.0f : 9.0f;
     hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, s, L->d_params, q, nq, L->sk, L->margin,
-                       (float *)nullptr);
+                       (float *)nullptr, width);
     TK_HIP(hipGetLastError());
     return TOPK_OK;
 }
@@ -751,7 +754,7 @@ int topk_learned_predict(topk_ctx *ctx, uint32_t nq, const topk_query *qs, float
     uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
     if (g > 1024) g = 1024;
     hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, ctx->stream, L->d_params, B.q, nq, L->sk, L->margin,
-                       B.out);
+                       B.out, 9.0f);
     TK_HIP(hipGetLastError());
     TK_HIP(hipStreamSynchronize(ctx->stream));
     if (pred) TK_HIP(hipMemcpy(pred, B.out, sizeof(float) * nq, hipMemcpyDeviceToHost));
