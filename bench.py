@@ -197,12 +197,15 @@ def learned_batched(eng, n, k, args):
         eng.batch_run(held[i % len(held)])
         tobjs += eng.stats()["tuple_count"]
     score_ms, launches = eng.timing_get("batch_score")
+    score2_ms, launches2 = eng.timing_get("batch_score_second")
+    score_ms, launches = score_ms + score2_ms, launches + launches2  # both passes of every batch
     seed_ms, _ = eng.timing_get("batch_tau_seed")
     lrn_ms, _ = eng.timing_get("batch_tau_learned")
     eng.timing(False)
     avg_ms = score_ms / max(1, launches)
-    # mul + add per term (m = 8 terms) for every scored (object, query) pair,
-    # over all scoring launches (a batch's second pass included)
+    # mul + add per term (m = 8 terms) for every (object, query) pair the
+    # scoring kernel scored (counted on the device, chunk-pruned pairs not
+    # included), over all its launches (first and second passes)
     flops = 2.0 * D * tobjs / max(1, launches)
     tflops = 2.0 * D * tobjs / (score_ms * 1e-3) / 1e12 if score_ms > 0 else 0.0
 
