@@ -287,6 +287,7 @@ void *topk_stream(topk_ctx *ctx);
  *   "batch_vpt"       objects per thread of the batch scoring kernel
  *                     (d <= 8: 4, 8, 16; d <= 16: 2, 4, 8; else 2, 4)
  *   "batch_bs"        threads per batch scoring workgroup: 256 or 512
+ *                     (1024 for d <= 8 with batch_vpt 4)
  *   "batch_qpi"       query budget of a batch work item (16..1024): a
  *                     partition is loaded once per item and scored for up
  *                     to this many of the queries that reach it

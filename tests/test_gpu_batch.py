@@ -121,7 +121,8 @@ def test_batch_variants_agree(gpu):
     with topk_amd.Engine(x) as eng:
         for vpt, bs, qpi, digit, mask, item in [(4, 256, 64, 11, 1, 2048), (8, 512, 32, 8, 2, 4096),
                                                 (16, 256, 16, 11, 0, 8192), (8, 256, 256, 8, 2, 256),
-                                                (8, 256, 1024, 11, 1, 16384), (4, 512, 16, 8, 0, 1280)]:
+                                                (8, 256, 1024, 11, 1, 16384), (4, 512, 16, 8, 0, 1280),
+                                                (4, 1024, 32, 11, 0, 4096)]:
             for name, v in (("batch_vpt", vpt), ("batch_bs", bs), ("batch_qpi", qpi), ("batch_digit", digit),
                             ("batch_grid", 300), ("batch_mask", mask), ("batch_item", item)):
                 eng.set_tuning(name, v)
@@ -139,7 +140,7 @@ def test_batch_autotune_keeps_results(gpu):
         before = eng.batch(qs)
         t = eng.batch_autotune(qs)
         assert t["variants"] > 10 and 0 < t["ms_best"] <= t["ms_default"]
-        assert t["qpi"] in (16, 32, 64, 128) and t["bs"] in (256, 512) and t["digit"] in (8, 11)
+        assert t["qpi"] in (16, 32, 64, 128) and t["bs"] in (256, 512, 1024) and t["digit"] in (8, 11)
         assert t["item"] in (2048, 4096, 8192, 16384)
         after = eng.batch(qs)
         for (a, b, c), (e, f, g) in zip(before, after):

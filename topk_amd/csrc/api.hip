@@ -413,7 +413,7 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
         if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16) return TOPK_ERR_INVALID;
         ctx->tune.batch_vpt = (int)value;
     } else if (key == "batch_bs") {
-        if (value != 0 && value != 256 && value != 512) return TOPK_ERR_INVALID;
+        if (value != 0 && value != 256 && value != 512 && value != 1024) return TOPK_ERR_INVALID;
         ctx->tune.batch_bs = (int)value;
     } else if (key == "batch_qpi") {
         if (value != 0 && (value < 16 || value > 1024)) return TOPK_ERR_INVALID;

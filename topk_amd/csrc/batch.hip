@@ -557,12 +557,14 @@ __device__ __forceinline__ void load_objs(const float *__restrict__ p, float (&x
 // memory.  The ticket of the next item is taken after the current item's
 // loads are issued and is read only when the item is done.
 //
-// Chunk pruning: a wave's 64 * VPT objects lie in one or a few chunks of
-// PTA_CHUNK positions (sorted by attribute sum inside the partition).  Lane j
-// evaluates the LP bound of query j over those chunks' statistics (ascending
+// Chunk pruning: objects are sorted by attribute sum inside a partition and
+// the statistics row of a chunk of PTA_CHUNK positions bounds that chunk and
+// every later one of the partition (suffix statistics).  Lane j evaluates
+// the LP bound of query j from the row of the wave's first chunk (ascending
 // queries with D <= 16; other queries always pass); the wave scores only the
-// queries whose bound reaches their threshold, and loads its objects only if
-// one does (per group of 64 listed queries).
+// queries whose bound reaches their threshold, loads its objects only if one
+// does, and a query that fails stays out of the wave's later tiles, which
+// ends the wave's walk through the item once every query has failed.
 template <int D, int VPT, int BS>
 __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
     constexpr bool LANE_W = D <= 16;
@@ -608,36 +610,26 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 nxt = atomicAdd(&a.ctr[6], 1u);
                 ticket = false;
             }
+            // lanes whose query may still reach its threshold on the wave's
+            // remaining objects (suffix bounds: once below, below for good)
+            bool alive = lane < nqb;
             for (uint32_t b0 = 0; b0 < size; b0 += BS * VPT) {
                 const uint32_t i0 = b0 + tid * VPT;
                 const uint32_t nvalid = i0 < size ? (size - i0 < (uint32_t)VPT ? size - i0 : (uint32_t)VPT) : 0u;
-                // this wave's objects: item positions [wb, we)
+                // this wave's objects: item positions [wb, ...)
                 const uint32_t wb = __builtin_amdgcn_readfirstlane(b0 + (tid & ~63u) * VPT);
-                if (wb >= size) break;
-                const uint32_t we = size - wb < 64u * VPT ? size : wb + 64u * VPT;
-                // chunk bound of lane j's query, evaluated by every lane (the
-                // shuffles read all lanes)
-                bool pass = lane < nqb;
+                if (wb >= size || !__ballot(alive)) break;
+                // suffix bound of lane j's query from the row of the wave's
+                // first chunk, evaluated by every lane (the shuffles read all
+                // lanes)
+                bool pass = alive;
                 if constexpr (LANE_W) {
-                    // lane e holds element e of the union of the wave's chunk rows
-                    const uint32_t c0 = (start + wb) / PTA_CHUNK, c1 = (start + we - 1) / PTA_CHUNK;
                     const uint32_t dd = a.d;
-                    float cs = 0.0f;
-                    for (uint32_t c = c0; c <= c1; c++) {
-                        const float v = lane < a.crow ? a.cstat[(uint64_t)c * a.crow + lane] : 0.0f;
-                        if (c == c0) cs = v;
-                        else if (lane < dd || lane == 2 * dd) cs = fmaxf(cs, v);  // maxima, largest sum
-                        else if (lane < 2 * dd) cs = fminf(cs, v);                // minima
-                    }
+                    const uint32_t c0 = (start + wb) / PTA_CHUNK;
+                    const float cs = lane < a.crow ? a.cstat[(uint64_t)c0 * a.crow + lane] : 0.0f;  // lane e: element e
                     const float csum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd)));
-                    // the union's sum of minima is the sum of its per-attribute
-                    // minima (not the smallest chunk sum), in FP64 rounded down
-                    double msum = 0.0;
-#pragma unroll
-                    for (int m = 0; m < D; m++)
-                        if ((uint32_t)m < dd)
-                            msum += (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)dd + m));
-                    const float cminsum = f32_round_down(msum);
+                    const float cminsum =
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd + 1)));
                     const uint32_t mq = (uint32_t)__float_as_int(lt[3]);
                     double wv[D];
                     float mxv[D], mnv[D];
@@ -658,6 +650,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     const float ub = lp_bound_regs<D>(wv, mxv, mnv, csum, cminsum);
                     const bool asc_l = __float_as_int(lt[2]) != 0;
                     pass = pass && (!asc_l || ub >= lt[0]);
+                    alive = pass;  // a failing suffix bound covers every later object of the partition
                 }
                 const uint64_t qmask = __ballot(pass);
                 if (!qmask) continue;
@@ -1100,6 +1093,8 @@ static BatchVariant pick_variant(const topk_ctx *ctx, const BatchState &B) {
 }
 
 static bool variant_ok(int D, int vpt, int bs) {
+    // 1024-thread workgroups (16 waves, one CU) need <= 128 VGPRs a thread
+    if (bs == 1024) return D == 8 && vpt == 4;
     if (bs != 256 && bs != 512) return false;
     if (D == 8) return vpt == 4 || vpt == 8 || vpt == 16;
     if (D == 16) return vpt == 2 || vpt == 4 || vpt == 8;
@@ -1113,14 +1108,15 @@ static void launch_score(int D, const BatchVariant &v, hipStream_t s, const Scor
         return;                                                                                                \
     }
     TK_SC(8, 4, 256) TK_SC(8, 8, 256) TK_SC(8, 16, 256) TK_SC(8, 4, 512) TK_SC(8, 8, 512) TK_SC(8, 16, 512)
+    TK_SC(8, 4, 1024)
     TK_SC(16, 2, 256) TK_SC(16, 4, 256) TK_SC(16, 8, 256) TK_SC(16, 2, 512) TK_SC(16, 4, 512) TK_SC(16, 8, 512)
     TK_SC(32, 2, 256) TK_SC(32, 4, 256) TK_SC(32, 2, 512) TK_SC(32, 4, 512)
 #undef TK_SC
 }
 
-// Enqueues one scoring pass: zero, threshold stage, two-level mask,
-// compaction, scoring, final select.  The second pass works only on the
-// queries the first one flagged.
+// Enqueues one scoring pass: threshold stage (first pass; a second pass
+// only for seeded reruns), two-level mask, compaction, scoring, final
+// select.  The second pass works only on the queries the first one flagged.
 static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
                         const BatchVariant &v, bool second) {
     const PtaLayout &P = ctx->pta;
@@ -1506,7 +1502,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     std::vector<BatchVariant> cand;
     static const int vpts[] = {2, 4, 8, 16}, qpis[] = {16, 32, 64, 128}, per_cu[] = {2, 4, 8};
     for (int vpt : vpts)
-        for (int bs : {256, 512}) {
+        for (int bs : {256, 512, 1024}) {
             if (!variant_ok(D, vpt, bs)) continue;
             for (int qpi : qpis)
                 for (int pc : per_cu)

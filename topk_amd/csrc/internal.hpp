@@ -209,10 +209,11 @@ struct PtaLayout {
     float *tminsum = nullptr;    // 2*p2
     // Chunk statistics: positions [c * PTA_CHUNK, (c + 1) * PTA_CHUNK) form
     // chunk c (partitions start on chunk boundaries); row c of cstat (crow =
-    // 2d + 2 floats) holds the chunk's maxima, minima, largest attribute sum
-    // (rounded up; -inf: no objects) and sum of minima (rounded down).
-    // Objects are ordered by attribute sum inside a partition, so the leading
-    // chunks hold the large sums.
+    // 2d + 2 floats) holds the maxima, minima, largest attribute sum (rounded
+    // up; -inf: no objects) and sum of minima (rounded down) of chunk c and
+    // every later chunk of its partition (suffix statistics: the bounds never
+    // increase along a partition).  Objects are ordered by attribute sum
+    // inside a partition, so the leading chunks hold the large sums.
     uint64_t nchunks = 0;
     uint32_t crow = 0;
     float *cstat = nullptr;

@@ -248,6 +248,31 @@ __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict_
     }
 }
 
+// One thread per partition turns its chunk rows into suffix rows: row c
+// then bounds chunks c, c + 1, ... of the partition (per-attribute maxima and
+// minima, the largest attribute sum, and the sum of the union's minima in
+// FP64 rounded down), so a partition's row bounds never increase with c.
+__global__ __launch_bounds__(256) void pta_chunk_suffix_kernel(uint32_t L, uint32_t d,
+                                                               const uint32_t *__restrict__ poff,
+                                                               const uint32_t *__restrict__ psz,
+                                                               float *__restrict__ cstat) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= L || psz[p] == 0) return;
+    const uint32_t crow = 2 * d + 2, c0 = poff[p] / PTA_CHUNK, n = (psz[p] + PTA_CHUNK - 1) / PTA_CHUNK;
+    for (uint32_t i = n - 1; i > 0; i--) {
+        const float *nx = cstat + (uint64_t)(c0 + i) * crow;
+        float *row = cstat + (uint64_t)(c0 + i - 1) * crow;
+        double ms = 0.0;
+        for (uint32_t m = 0; m < d; m++) {
+            row[m] = fmaxf(row[m], nx[m]);
+            row[d + m] = fminf(row[d + m], nx[d + m]);
+            ms += (double)row[d + m];
+        }
+        row[2 * d] = fmaxf(row[2 * d], nx[2 * d]);
+        row[2 * d + 1] = f32_round_down(ms);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host: k-d tree on the sample
 // ---------------------------------------------------------------------------
@@ -492,6 +517,9 @@ int pta_build(topk_ctx *ctx) {
         if (P.nchunks)
             hipLaunchKernelGGL(pta_chunk_kernel, dim3((uint32_t)((P.nchunks + 3) / 4)), dim3(256), 0, s, P.pcols, P.ld,
                                d, P.pid, (uint64_t)P.nchunks, P.cstat);
+        if (P.nchunks)
+            hipLaunchKernelGGL(pta_chunk_suffix_kernel, dim3((L + 255) / 256), dim3(256), 0, s, L, d, P.poff, P.psz,
+                               P.cstat);
         TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
         TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
         TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
