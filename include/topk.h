@@ -296,8 +296,10 @@ void *topk_stream(topk_ctx *ctx);
  *   "batch_item"      objects of a partition per batch work item (a multiple
  *                     of 256, 256..65536)
  *   "batch_retry"     a learned threshold that left fewer than k candidates
- *                     is rerun (second pass) with: 1 = a threshold far below
- *                     the prediction; 2 = a seed threshold (0 = default)
+ *                     is rerun (second pass) with: 1 = a lower threshold
+ *                     below the prediction; 2 = a seed threshold (0 = 1)
+ *   "batch_retry_width" distance of that threshold below the prediction,
+ *                     in calibrated margins: 1..64 (0 = default, 3)
  *   "batch_graph"     1 = capture each batch shape in a hipGraph and replay
  *   "batch_mode"      0 = learned threshold when trained, else seed;
  *                     1 = seed threshold; 2 = learned threshold

@@ -430,6 +430,9 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
     } else if (key == "batch_retry") {
         if (value < 0 || value > 2) return TOPK_ERR_INVALID;
         ctx->tune.batch_retry = (int)value;
+    } else if (key == "batch_retry_width") {
+        if (value < 0 || value > 64) return TOPK_ERR_INVALID;
+        ctx->tune.batch_retry_width = (int)value;
     } else if (key == "batch_graph") {
         ctx->tune.batch_graph = value ? 1 : 0;
     } else if (key == "batch_mode") {

@@ -861,7 +861,7 @@ struct BatchVariant {
     int item;  // objects per work item (a multiple of PTA_CHUNK)
 };
 
-using GraphKey = std::array<uint64_t, 14>;
+using GraphKey = std::array<uint64_t, 15>;
 
 // Pinned staging of one batch in flight, and where its results go.
 struct HostSlot {
@@ -1234,7 +1234,7 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
                        (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
                        (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur, (uint64_t)v.item,
-                       (uint64_t)ctx->tune.batch_retry};
+                       (uint64_t)ctx->tune.batch_retry, (uint64_t)ctx->tune.batch_retry_width};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -73,7 +73,8 @@ struct Tuning {
     int batch_digit = 0;        // radix digit width of the per-query select (8 / 11)
     int batch_grid = 0;         // persistent scoring workgroups
     int batch_item = 0;         // objects per work item
-    int batch_retry = 0;        // learned misses rerun with: 0 default, 1 a wide threshold, 2 a seed
+    int batch_retry = 0;        // learned misses rerun with: 0 default, 1 a lower threshold, 2 a seed
+    int batch_retry_width = 0;  // the lower threshold's distance below the prediction, in margins (0: 3)
     int batch_graph = 1;        // replay captured hipGraphs
     int batch_mode = 0;         // 0 auto (learned when trained), 1 seed threshold, 2 learned threshold
     int batch_mask = 0;         // mask layout: 0 auto, 1 query-major, 2 group-major

@@ -312,8 +312,8 @@ __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict
         if (lane == 0) {
             const float y = o + b3;
             qs[q].tau = sc > 0.0f ? (y - margin) * sc : -INFINITY;
-            // a miss reruns far below the prediction, or (retry_width < 0)
-            // with a seed threshold
+            // a miss reruns retry_width margins below the prediction, or
+            // (retry_width < 0) with a seed threshold
             const float wide = retry_width * fmaxf(fabsf(margin), 1e-3f);  // strictly below tau
             qs[q].tau_retry = retry_width < 0.0f ? INFINITY : (sc > 0.0f ? (y - wide) * sc : -INFINITY);
             if (pred) pred[q] = y * sc;
@@ -431,7 +431,8 @@ int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq
     if (!L || !L->ready) return TOPK_ERR_STATE;
     uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
     if (g > 1024) g = 1024;
-    const float width = ctx->tune.batch_retry == 2 ? -1.0f : 9.0f;
+    const float width =
+        ctx->tune.batch_retry == 2 ? -1.0f : (ctx->tune.batch_retry_width ? (float)ctx->tune.batch_retry_width : 3.0f);
     hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, s, L->d_params, q, nq, L->sk, L->margin,
                        (float *)nullptr, width);
     TK_HIP(hipGetLastError());
@@ -754,7 +755,7 @@ int topk_learned_predict(topk_ctx *ctx, uint32_t nq, const topk_query *qs, float
     uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
     if (g > 1024) g = 1024;
     hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, ctx->stream, L->d_params, B.q, nq, L->sk, L->margin,
-                       B.out, 9.0f);
+                       B.out, 3.0f);
     TK_HIP(hipGetLastError());
     TK_HIP(hipStreamSynchronize(ctx->stream));
     if (pred) TK_HIP(hipMemcpy(pred, B.out, sizeof(float) * nq, hipMemcpyDeviceToHost));
