@@ -273,12 +273,132 @@ def cpu_baseline(x, n, k, threads, budget_s, held, gpu_check):
             "median_s": float(np.median(times))}, mism
 
 
+def subset_queries(rng, count, d, ks):
+    """Random attribute subsets of 2..d attributes (ascending), weights
+    U(0.1, 1) by attribute id, k drawn from ks."""
+    out = []
+    for _ in range(count):
+        m = int(rng.integers(2, d + 1))
+        attr = np.sort(rng.choice(d, size=m, replace=False)).astype(np.uint8)
+        out.append((int(rng.choice(ks)), rng.uniform(0.1, 1.0, d).astype(np.float32), attr))
+    return out
+
+
+def pipelined(eng, pbs, steps):
+    """Batches per second with two in flight; returns (q/s, fallback, fte fallback)."""
+    import torch
+    for pb in pbs[:2]:
+        eng.batch_run(pb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pend, fb, ff, nq = [], 0, 0, 0
+    for i in range(steps):
+        pend.append(eng.batch_submit(pbs[i % len(pbs)]))
+        nq += pbs[i % len(pbs)].nq
+        if len(pend) == 2:
+            eng.batch_wait(pend.pop(0))
+            st = eng.stats()
+            fb, ff = fb + st["fallback"], ff + st["fte_fallback"]
+    for t in pend:
+        eng.batch_wait(t)
+        st = eng.stats()
+        fb, ff = fb + st["fallback"], ff + st["fte_fallback"]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return nq / el, fb / nq, ff / nq
+
+
+def other_configs(threads):
+    """The other workloads of the scope table, measured once each (one JSON
+    line; the headline line is the default mode):
+    * PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
+      weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128;
+    * batched: N = 2^25, d = 16 mixed (seed 4), 1024 queries per launch on
+      random subsets of 2-16 attributes, k from {16..256} (query seed 200);
+    * learned: N = 2^26, d = 16 mixed (seed 5), 100k training queries (seed
+      300), 4096 held-out queries (seed 301) in launches of 1024."""
+    import oracle
+    import topk_amd
+    ks = [16, 32, 64, 128, 256]
+    res = {}
+    n3, d = 1 << 28, 8
+    rng = np.random.default_rng(100)
+    ws = [rng.uniform(0.1, 1.0, d).astype(np.float32) for _ in range(20)]
+    attr = np.arange(d, dtype=np.uint8)
+    with topk_amd.Engine.generated(n3, d, topk_amd.ANTICORRELATED, 3) as eng:
+        eng.set_partition_size(4096)
+        t0 = time.perf_counter()
+        eng.init(topk_amd.TOPK_INIT_PTA)
+        init_s = time.perf_counter() - t0
+        eng.pta(128, ws[0], attr)
+        t0 = time.perf_counter()
+        scored = 0
+        for w in ws:
+            eng.pta(128, w, attr)
+            st = eng.stats()
+            scored += st["partitions_scored"] / max(1, st["partitions_total"])
+        pta_s = (time.perf_counter() - t0) / len(ws)
+        t0 = time.perf_counter()
+        for w in ws[:5]:
+            eng.fte(128, w, attr)
+        fte_s = (time.perf_counter() - t0) / 5
+        got = eng.pta(128, ws[1], attr)
+    x = topk_amd.generate(n3, d, topk_amd.ANTICORRELATED, 3, threads=threads)
+    t0 = time.perf_counter()
+    ref = oracle.tpac_threads(x, n3, 128, ws[1], attr, threads)
+    cpu_s = time.perf_counter() - t0
+    del x
+    res["pta_n2e28_anticorrelated_k128"] = {
+        "pta_qps": 1.0 / pta_s, "fte_qps": 1.0 / fte_s, "partitions_scored_frac": scored / len(ws),
+        "layout_build_s": init_s, "cpu_tpac_threads_qps": 1.0 / cpu_s, "cpu_threads": threads,
+        "matches_oracle": bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))}
+
+    with topk_amd.Engine.generated(1 << 25, 16, topk_amd.MIXED, 4) as eng:
+        pbs = [eng.prepare_batch(subset_queries(np.random.default_rng(200 + i), 1024, 16, ks)) for i in range(4)]
+        tune = eng.batch_autotune(pbs[0])
+        qps, fb, ff = pipelined(eng, pbs, 20)
+        res["batch_n2e25_d16_mixed"] = {"qps": qps, "fte_fallback_rate": ff, "autotune": tune,
+                                        "threshold": "seed (no predictor)"}
+
+    with topk_amd.Engine.generated(1 << 26, 16, topk_amd.MIXED, 5) as eng:
+        train = subset_queries(np.random.default_rng(300), 100_000, 16, ks)
+        t0 = time.perf_counter()
+        rep = eng.learned_train(train)
+        train_s = time.perf_counter() - t0
+        held = subset_queries(np.random.default_rng(301), 4096, 16, ks)
+        pbs = [eng.prepare_batch(held[i:i + 1024]) for i in range(0, 4096, 1024)]
+        eng.set_tuning("batch_mode", 2)
+        tune = eng.batch_autotune(pbs[0])
+        qps, fb, ff = pipelined(eng, pbs, 20)
+        eng.set_tuning("batch_mode", 1)
+        sqps, _, _ = pipelined(eng, pbs, 8)
+        eng.set_tuning("batch_mode", 2)
+        mism = 0
+        for pb in pbs:
+            eng.batch_run(pb)
+            for i, q in enumerate(pb.queries):
+                gi, gs, gt = pb.result(i, 1 << 26)
+                ri, rs, rt = eng.fte(*q)
+                mism += int(not (np.array_equal(gi, ri) and np.array_equal(gs, rs) and gt == rt))
+        t0 = time.perf_counter()
+        for q in held[:50]:
+            eng.fte(*q)
+        fte_qps = 50 / (time.perf_counter() - t0)
+        res["learned_n2e26_d16_mixed"] = {
+            "learned_qps": qps, "seed_qps": sqps, "plain_fte_qps": fte_qps, "speedup_vs_plain_fte": qps / fte_qps,
+            "fallback_rate": fb, "fte_fallback_rate": ff, "mismatches_vs_plain_fte": mism, "judged_queries": 4096,
+            "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
+                         "train_mse": rep["train_mse"], "wall_s": train_s}, "autotune": tune}
+    print(json.dumps({"configs": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte"], default="learned")
+    ap.add_argument("--mode", choices=["learned", "fte", "configs"], default="learned",
+                    help="learned: the headline line; fte: plain FTE alone; configs: the other workloads")
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--m", type=int, default=D, help="--mode fte: query width")
     ap.add_argument("--k", type=int, default=K)
@@ -306,6 +426,10 @@ def main():
 
     torch.cuda.init()  # torch's HIP runtime first; the engine shares it
     import topk_amd
+
+    if args.mode == "configs":
+        other_configs(min(os.cpu_count() or 1, 16))
+        return
 
     n, k = args.n, args.k
     eng = topk_amd.Engine.generated(n, D, topk_amd.INDEPENDENT, 0)

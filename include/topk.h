@@ -225,6 +225,8 @@ typedef struct topk_learned_report {
     float margin;                /* installed margin, score / sum|W| units */
     float calib_fallback;        /* calibration queries whose threshold would exceed their k-th score */
     double ms_label, ms_train;   /* wall time of labelling (exact batch path) and of training */
+    uint32_t saturated;          /* 1: labels reach the largest score the data allows (ties there);
+                                    queries predicted near it take exact seed thresholds */
 } topk_learned_report;
 
 int topk_learned_default_config(topk_learned_config *cfg);

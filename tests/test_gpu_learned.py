@@ -105,6 +105,9 @@ def trained():
 def test_learned_training_report(gpu, trained):
     eng, _, _, rep, _ = trained
     assert rep["n_train"] + rep["n_calib"] == 6000
+    # the correlated third clips values at 1.0: many queries' k-th score is
+    # the largest the data allows, so the store counts as saturated
+    assert rep["saturated"] == 1
     assert rep["margin"] > 0 and np.isfinite(rep["train_mse"]) and np.isfinite(rep["calib_mse"])
     assert rep["calib_fallback"] <= 0.02
 
@@ -122,12 +125,30 @@ def test_learned_batch_is_exact(gpu, trained):
     eng.set_tuning("batch_mode", 0)
 
 
-def test_learned_gain_over_seed(gpu, trained):
+@pytest.fixture(scope="module")
+def trained_unsaturated():
+    """N=2^20, d=16 independent store (no ties at the top) with a predictor
+    trained on 6000 queries."""
+    if not __import__("torch").cuda.is_available():
+        pytest.skip("no GPU")
+    n, d = 1 << 20, 16
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 78)
+    eng = topk_amd.Engine(x)
+    rng = np.random.default_rng(4321)
+    rep = eng.learned_train(make_queries(rng, 6000, d, [16, 32, 64, 128, 256]), epochs=40)
+    test = make_queries(rng, 512, d, [16, 32, 64, 128, 256])
+    yield eng, x, n, rep, test
+    eng.close()
+
+
+def test_learned_gain_over_seed(gpu, trained_unsaturated):
     """What the network buys: its thresholds replace the exact seed pass
     (4 best-bounded partitions scored per query, then a k-th select) with one
     small MLP evaluation per query, while pruning about as much and giving
-    the same answers."""
-    eng, _, _, _, test = trained
+    the same answers (a store without ties at the top, so no query needs a
+    seed)."""
+    eng, _, _, rep, test = trained_unsaturated
+    assert rep["saturated"] == 0
     res, stats, tau_ms = {}, {}, {}
     for mode, name in ((1, "seed"), (2, "learned")):
         eng.set_tuning("batch_mode", mode)
@@ -159,10 +180,11 @@ def test_learned_thresholds_below_kth(gpu, trained):
     assert np.median(np.abs(pred - kth) / np.abs(kth)) < 0.05
 
 
-def test_forced_fallback_stays_exact(gpu, trained):
+def test_forced_fallback_stays_exact(gpu, trained_unsaturated):
     """A predictor whose thresholds sit above every k-th score: every query
-    falls back to the exact path and the answers do not change."""
-    eng, x, n, _, test = trained
+    falls back to the exact path and the answers do not change (a store
+    without ties at the top, so no query takes a seed instead)."""
+    eng, x, n, _, test = trained_unsaturated
     params, margin = eng.learned_get()
     sub = test[:40]
     eng.learned_set(params, -1.0)  # tau = (pred + 1) * sum|W|: above any score

@@ -165,6 +165,7 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
         b.tau = taus ? taus[q] : 0.0f;
         b.tau_retry = -INFINITY;
         b.tau_id = 0xFFFFFFFFu;
+        b.seed = 0u;
         b.k = (uint32_t)(n < Q.k ? n : Q.k);
         b.kuser = Q.k;
         b.m = qq;
@@ -214,12 +215,13 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
                                                               uint64_t ld, const uint32_t *__restrict__ poff,
                                                               const uint32_t *__restrict__ psz,
                                                               const uint32_t *__restrict__ pid,
-                                                              const uint32_t *__restrict__ qflag) {
+                                                              const uint32_t *__restrict__ qflag, int marked) {
     __shared__ SelectScratch ss;
     __shared__ uint32_t best[BSEED_T];
     __shared__ uint32_t nbest;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     if (qflag && qflag[q] != QF_SEED) return;  // second pass: only queries that need a seed
+    if (marked && !qs[q].seed) return;         // learned first pass: only the queries the predictor marked
     const BatchQuery &Q = qs[q];
     if (tid < 64) {
         const BoundArgs &b = qb[q];
@@ -1138,10 +1140,16 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
         const int NB = dims_class(ctx->d);
         auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
         hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz, P.pid,
-                           pass_flags);
+                           pass_flags, 0);
     } else if (mode == BMODE_LEARNED && !second) {
         const int rc = learned_tau_enqueue(ctx, s, dq, nq);
         if (rc) return rc;
+        if (learned_saturated(ctx)) {  // queries predicted at their largest score: a seed (key) threshold
+            const int NB = dims_class(ctx->d);
+            auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
+            hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz,
+                               P.pid, (const uint32_t *)nullptr, 1);
+        }
     }
     timing_end(ctx, s,
                second ? "batch_tau_second" : (mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given")),

@@ -103,7 +103,7 @@ struct alignas(16) BatchQuery {
     float tau_retry;
     uint32_t k;                     // min(k, n)
     uint32_t kuser;                 // k as requested
-    uint32_t pad_;
+    uint32_t seed;                  // learned mode: take a seed threshold (set by the predictor)
     // The queried attributes (first 16) sorted by weight, heaviest first
     // (ties by lower id): the order a bound's fractional knapsack takes.
     float ws[16];
@@ -289,6 +289,8 @@ void learned_free(topk_ctx *ctx);
 // installed; learned_tau_enqueue writes q[i].tau for i < nq on stream s
 // (kernels only, so it can be captured in a graph).
 bool learned_ready(const topk_ctx *ctx);
+// The installed predictor marks queries for seed thresholds (saturated store).
+bool learned_saturated(const topk_ctx *ctx);
 uint64_t learned_version(const topk_ctx *ctx);
 // Validates one query and converts it to the device form (tau left at 0).
 int make_batch_query(const topk_ctx *ctx, const topk_query &q, BatchQuery &out);

@@ -60,6 +60,11 @@ struct LearnedState {
     float margin = 0.0f;       // in units of score / sum |W|
     Sketch sk{};
     bool sk_valid = false;
+    // The store saturates: training labels at the largest score the data
+    // allows (many objects tied there, e.g. values clipped at a bound).  A
+    // score threshold admits every tie, so queries predicted near their
+    // maximum take an exact seed threshold (a key) instead.
+    bool saturated = false;
 };
 
 static LearnedState *lstate(topk_ctx *ctx) {
@@ -287,7 +292,8 @@ __global__ void mlp_adam_kernel(float *__restrict__ P, float *__restrict__ M, fl
 // the parameter load, so waves may take different numbers of queries).
 __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict__ P, BatchQuery *__restrict__ qs,
                                                          uint32_t nq, Sketch S, float margin,
-                                                         float *__restrict__ pred, float retry_width) {
+                                                         float *__restrict__ pred, float retry_width,
+                                                         int saturated) {
     __shared__ float W1T[MLP_IN][MLP_H];
     __shared__ float W2T[MLP_H][MLP_H];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid / 64;
@@ -316,6 +322,9 @@ __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict
             // (retry_width < 0) with a seed threshold
             const float wide = retry_width * fmaxf(fabsf(margin), 1e-3f);  // strictly below tau
             qs[q].tau_retry = retry_width < 0.0f ? INFINITY : (sc > 0.0f ? (y - wide) * sc : -INFINITY);
+            // within two margins of the query's largest score (feature 91):
+            // ties likely, seed this query
+            qs[q].seed = saturated && sc > 0.0f && y + 2.0f * fabsf(margin) >= f[91] ? 1u : 0u;
             if (pred) pred[q] = y * sc;
         }
     }
@@ -426,6 +435,11 @@ uint64_t learned_version(const topk_ctx *ctx) {
     return L ? L->version : 0;
 }
 
+bool learned_saturated(const topk_ctx *ctx) {
+    const LearnedState *L = static_cast<const LearnedState *>(ctx->learned);
+    return L && L->ready && L->saturated;
+}
+
 int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq) {
     LearnedState *L = static_cast<LearnedState *>(ctx->learned);
     if (!L || !L->ready) return TOPK_ERR_STATE;
@@ -434,7 +448,7 @@ int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq
     const float width =
         ctx->tune.batch_retry == 2 ? -1.0f : (ctx->tune.batch_retry_width ? (float)ctx->tune.batch_retry_width : 3.0f);
     hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, s, L->d_params, q, nq, L->sk, L->margin,
-                       (float *)nullptr, width);
+                       (float *)nullptr, width, L->saturated ? 1 : 0);
     TK_HIP(hipGetLastError());
     return TOPK_OK;
 }
@@ -532,6 +546,20 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
         if (scale[i] > 0 && std::isfinite(y[i])) usable.push_back(i);
     }
     if (usable.size() < 16) return TOPK_ERR_INVALID;
+    // saturation: labels at the query's largest score the sketch allows
+    // (sum of normalised weight x sample maximum, feature 91)
+    {
+        uint32_t sat = 0;
+        for (uint32_t i : usable) {
+            float wa[LRN_MAX_D] = {0.0f};
+            for (uint32_t j = 0; j < qs[i].qq; j++) wa[qs[i].attr[j]] += qs[i].weights[qs[i].attr[j]];
+            float sabs = 0.0f, smax = 0.0f;
+            for (int a = 0; a < LRN_MAX_D; a++) sabs += fabsf(wa[a]);
+            for (int a = 0; a < LRN_MAX_D; a++) smax += wa[a] / sabs * Ls->sk.cmax[a];
+            sat += y[i] >= smax - 1e-5f * fabsf(smax) ? 1u : 0u;
+        }
+        Ls->saturated = sat >= usable.size() / 200 + 1;  // at least 0.5% of the labels
+    }
     std::mt19937_64 rng(cfg.seed);
     std::shuffle(usable.begin(), usable.end(), rng);
     uint32_t ncal = (uint32_t)(cfg.calib_fraction * usable.size());
@@ -679,6 +707,7 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
         rep->calib_fallback = (float)over / (float)ncal;
         rep->ms_label = t1 - t0;
         rep->ms_train = t2 - t1;
+        rep->saturated = Ls->saturated ? 1u : 0u;
     }
     return TOPK_OK;
 }
@@ -755,7 +784,7 @@ int topk_learned_predict(topk_ctx *ctx, uint32_t nq, const topk_query *qs, float
     uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
     if (g > 1024) g = 1024;
     hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, ctx->stream, L->d_params, B.q, nq, L->sk, L->margin,
-                       B.out, 3.0f);
+                       B.out, 3.0f, 0);
     TK_HIP(hipGetLastError());
     TK_HIP(hipStreamSynchronize(ctx->stream));
     if (pred) TK_HIP(hipMemcpy(pred, B.out, sizeof(float) * nq, hipMemcpyDeviceToHost));
