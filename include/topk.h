@@ -55,7 +55,9 @@ typedef enum topk_status {
     TOPK_ERR_NONFINITE = 5,    /* the store holds NaN or infinity */
     TOPK_ERR_STATE = 6,        /* e.g. topk_pta before the partitions are built */
     TOPK_ERR_NO_DEVICE = 7,    /* no usable GPU */
-    TOPK_ERR_OVERFLOW = 8      /* internal candidate buffer overflowed (retried on the exact path) */
+    TOPK_ERR_OVERFLOW = 8,     /* internal candidate buffer overflowed (retried on the exact path) */
+    TOPK_ERR_FORMAT = 9,       /* a text or column file is malformed, or holds fewer rows than asked */
+    TOPK_ERR_IO = 10           /* a file could not be opened, read or written */
 } topk_status;
 
 typedef struct topk_ctx topk_ctx;
@@ -83,6 +85,61 @@ int topk_ctx_create(topk_ctx **out, const float *data, uint64_t n, uint32_t d, u
 int topk_ctx_create_generated(topk_ctx **out, uint64_t n, uint32_t d, int distr, uint64_t seed);
 
 int topk_ctx_destroy(topk_ctx *ctx);
+
+/* ---- data ingest (reference: File<T>, input/File.h) ----
+ *
+ * Text files hold one object per line, d values separated by ',' (the
+ * "%f,%f,...,%f" rows of File::line_specifier, input/File.h:86-104).  Every
+ * value converts exactly as fscanf's "%f" converts it (correctly rounded
+ * binary32): the GPU parses the text, lines it cannot convert exactly on the
+ * device are parsed on the host with strtof.  Blank lines are skipped (as
+ * fscanf skips them); a line with another field count, a value fscanf could
+ * not read, or white space before a ',' is TOPK_ERR_FORMAT (the reference's
+ * loop would silently misalign the rest of the file). */
+
+typedef struct topk_ingest_report {
+    uint32_t format;        /* 0 text, 1 column file */
+    uint32_t fields;        /* values per object (d) */
+    uint64_t bytes;         /* file size */
+    uint64_t lines;         /* text lines, blank ones included */
+    uint64_t rows;          /* objects loaded (n) */
+    uint64_t slow_lines;    /* lines the host parsed (exact conversion not possible on the device) */
+    double ms_read;         /* file to HBM (pinned staging) */
+    double ms_parse;        /* line index and conversion (text) */
+    double ms_total;
+} topk_ingest_report;
+
+/* Counts the non-blank lines of a text file and the fields of its first
+ * non-blank line.  Replaces File<T>::count (input/File.h:107-128), which
+ * counts newlines (a last line without one, or blank lines, differ). */
+int topk_csv_count(const char *path, uint64_t *rows, uint32_t *fields);
+
+/* Reads the first n rows of d fields (0 = all rows / the first line's field
+ * count) into a host n x d table, column-major out[m*n + i]
+ * (File::read_scanf_t, input/File.h:264-295) or row-major with
+ * TOPK_ROWMAJOR (File::read_scanf, :233-262), parsing on the current GPU.
+ * rep may be NULL. */
+int topk_csv_read(const char *path, uint64_t n, uint32_t d, uint32_t flags, float *out, topk_ingest_report *rep);
+
+/* Writes an n x d table as text rows of "%f" values (File::write_printf_t,
+ * input/File.h:415-430, which writes nothing for odd d; any d here). */
+int topk_csv_write(const char *path, const float *data, uint64_t n, uint32_t d, uint32_t flags);
+
+/* Binary column file (no reference counterpart): a 4096-byte header
+ * ("TOPKCOL1", u32 version 1, u32 d, u64 n, u64 data offset) followed by d
+ * columns of n little-endian binary32 values. */
+int topk_colfile_write(const char *path, const float *data, uint64_t n, uint32_t d, uint32_t flags);
+int topk_colfile_info(const char *path, uint64_t *n, uint32_t *d);
+
+/* Builds the column store from a file: a column file streams straight into
+ * HBM, a text file is uploaded and parsed on the GPU.  n / d of 0 take the
+ * whole file; otherwise the first n objects (and, for a column file, the
+ * first d attributes).  Replaces File(fname, ...) + File::load(get_cdata())
+ * + the algorithm's init upload (validation/bench_cpu.h:156-162). flags: 0.
+ * rep may be NULL. */
+int topk_ctx_create_file(topk_ctx **out, const char *path, uint64_t n, uint32_t d, uint32_t flags,
+                         topk_ingest_report *rep);
+
 
 uint64_t topk_ctx_n(const topk_ctx *ctx);
 uint32_t topk_ctx_d(const topk_ctx *ctx);

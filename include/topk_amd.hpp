@@ -30,6 +30,7 @@
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "topk.h"
@@ -211,6 +212,63 @@ private:
     std::vector<topk_query> qs_;
     std::vector<float> thr_;
     uint32_t ks_ = 1;
+};
+
+/* Data files with the reference's loader interface (File<T>,
+ * input/File.h:14-83, float only): load() parses the text on the GPU
+ * (topk_csv_read), gen() runs the synthetic generator, store() writes "%f"
+ * rows.  With set_transpose(true) the table is column-major x[m*n + i]
+ * (read_scanf_t, :264-295), else row-major (read_scanf, :233-262).  Errors
+ * throw topk_amd::error where the reference exits. */
+class File {
+public:
+    File(std::string fname, bool /*gpu*/, uint64_t n, uint64_t d) : fname_(std::move(fname)), n_(n), d_(d) {}
+    File(std::string fname, bool /*gpu*/) : fname_(std::move(fname)) {
+        uint32_t d = 0;
+        check(topk_csv_count(fname_.c_str(), &n_, &d), "topk_csv_count");
+        d_ = d;
+    }
+    uint64_t items() const { return d_; }
+    uint64_t rows() const { return n_; }
+    void set_transpose(bool t) { transpose_ = t; }
+    void set_seed(uint64_t s) { seed_ = s; }
+    const topk_ingest_report &report() const { return report_; }
+
+    void load(float *&data) {
+        if (!data) data = alloc();
+        check(topk_csv_read(fname_.c_str(), n_, (uint32_t)d_, transpose_ ? 0u : TOPK_ROWMAJOR, data, &report_),
+              "topk_csv_read");
+    }
+    void gen(float *&data, int8_t type) {
+        if (type < 0 || type > 3)
+            throw error(TOPK_ERR_INVALID, "gen: type 0 (correlated), 1 (independent), 2 (anti), 3 (mixed)");
+        if (!data) data = alloc();
+        if (transpose_) {
+            check(topk_generate(data, n_, (uint32_t)d_, n_, type, seed_, 0, n_, 0), "topk_generate");
+        } else {
+            std::vector<float> c(n_ * d_);
+            check(topk_generate(c.data(), n_, (uint32_t)d_, n_, type, seed_, 0, n_, 0), "topk_generate");
+            for (uint64_t i = 0; i < n_; i++)
+                for (uint64_t m = 0; m < d_; m++) data[i * d_ + m] = c[m * n_ + i];
+        }
+    }
+    void store(const std::string &fname, const float *data) {
+        check(topk_csv_write(fname.c_str(), data, n_, (uint32_t)d_, transpose_ ? 0u : TOPK_ROWMAJOR),
+              "topk_csv_write");
+    }
+
+private:
+    float *alloc() const {
+        const size_t bytes = ((sizeof(float) * n_ * d_ + 1023) / 1024) * 1024;
+        float *p = static_cast<float *>(std::aligned_alloc(1024, bytes));
+        if (!p) throw error(TOPK_ERR_NOMEM, "File: host buffer");
+        return p;
+    }
+    std::string fname_;
+    uint64_t n_ = 0, d_ = 0;
+    bool transpose_ = false;
+    uint64_t seed_ = 0;
+    topk_ingest_report report_{};
 };
 
 }  // namespace topk_amd

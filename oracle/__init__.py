@@ -63,6 +63,8 @@ def lib():
         "or_mlp_nparams": (_c.c_uint32, []),
         "or_mlp_forward": (None, [_f64p, _f64p, _c.c_uint32, _f64p]),
         "or_mlp_backward": (_c.c_double, [_f64p, _f64p, _f64p, _c.c_uint32, _f64p]),
+        "or_csv_count": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint32)]),
+        "or_csv_read": (_c.c_int64, [_c.c_char_p, _c.c_uint64, _c.c_uint32, _c.c_int, _f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -256,3 +258,25 @@ def mlp_backward(params, x, y):
     g = np.zeros(p.size, dtype=np.float64)
     loss = lib().or_mlp_backward(p, xx.ravel(), yy, xx.shape[0], g)
     return loss, g
+
+
+def csv_count(path):
+    """(n, d) as the reference's File::count computes them (input/File.h:107-128)."""
+    n = _c.c_uint64()
+    d = _c.c_uint32()
+    if lib().or_csv_count(os.fsencode(path), _c.byref(n), _c.byref(d)) != 0:
+        raise OSError(path)
+    return n.value, d.value
+
+
+def csv_read(path, n, d, colmajor=True):
+    """The reference's fscanf row loop (input/File.h:233-295): returns the
+    table (d x n column-major as a 2-D array, or n x d) and the rows read;
+    raises ValueError for a row with fewer than d values."""
+    out = np.zeros((d, n) if colmajor else (n, d), np.float32)
+    r = lib().or_csv_read(os.fsencode(path), n, d, 1 if colmajor else 0, out)
+    if r == -2:
+        raise OSError(path)
+    if r < 0:
+        raise ValueError("row with fewer than %d values" % d)
+    return out, int(r)

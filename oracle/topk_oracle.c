@@ -591,3 +591,56 @@ double or_mlp_backward(const double *p, const double *x, const double *y, uint32
     }
     return loss / batch;
 }
+
+/* ------------------------------------------------------------------ */
+/* text loader (File<T>, input/File.h)                                 */
+/* ------------------------------------------------------------------ */
+#include <stdio.h>
+
+/* File<T>::count (input/File.h:107-128): d = commas on the first line + 1,
+   n = number of '\n' bytes in the file. */
+int or_csv_count(const char *path, uint64_t *n, uint32_t *d)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    uint64_t nl = 0;
+    uint32_t dd = 1;
+    int first = 1, c;
+    while ((c = fgetc(f)) != EOF) {
+        if (first && c == ',') dd++;
+        if (c == '\n') { nl++; first = 0; }
+    }
+    fclose(f);
+    *n = nl;
+    *d = dd;
+    return 0;
+}
+
+/* File<T>::read_scanf_t / read_scanf (input/File.h:233-295): rows read
+   with fscanf and the "%f,%f,...,%f" specifier of line_specifier
+   (:86-104), stored x[m*n+i] (colmajor) or x[i*d+m], until n rows are read
+   or a row's first value fails.  Returns the rows read, or -1 for a row
+   that ended after fewer than d values (the reference keeps the previous
+   row's values there) and -2 if the file cannot be opened. */
+int64_t or_csv_read(const char *path, uint64_t n, uint32_t d, int colmajor, float *out)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return -2;
+    uint64_t i = 0;
+    float v[64];
+    while (i < n) {
+        int got = fscanf(f, "%f", &v[0]);
+        if (got != 1) break;
+        uint32_t m = 1;
+        for (; m < d; m++)
+            if (fscanf(f, ",%f", &v[m]) != 1) break;
+        if (m < d) { fclose(f); return -1; }
+        for (m = 0; m < d; m++) {
+            if (colmajor) out[(uint64_t)m * n + i] = v[m];
+            else out[i * d + m] = v[m];
+        }
+        i++;
+    }
+    fclose(f);
+    return (int64_t)i;
+}

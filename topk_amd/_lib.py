@@ -22,6 +22,8 @@ TOPK_ERR_NONFINITE = 5
 TOPK_ERR_STATE = 6
 TOPK_ERR_NO_DEVICE = 7
 TOPK_ERR_OVERFLOW = 8
+TOPK_ERR_FORMAT = 9
+TOPK_ERR_IO = 10
 
 TOPK_ROWMAJOR = 0x1
 TOPK_INIT_NORMALIZE = 0x1
@@ -63,6 +65,12 @@ class TopkLearnedReport(ctypes.Structure):
                 ("ms_label", _c.c_double), ("ms_train", _c.c_double), ("saturated", _c.c_uint32)]
 
 
+class TopkIngestReport(ctypes.Structure):
+    _fields_ = [("format", _c.c_uint32), ("fields", _c.c_uint32), ("bytes", _c.c_uint64),
+                ("lines", _c.c_uint64), ("rows", _c.c_uint64), ("slow_lines", _c.c_uint64),
+                ("ms_read", _c.c_double), ("ms_parse", _c.c_double), ("ms_total", _c.c_double)]
+
+
 MLP_NPARAMS = 10433
 MLP_IN = 96
 
@@ -74,6 +82,14 @@ SIGNATURES = {
     "topk_ctx_create_generated": (_c.c_int, [_c.POINTER(_vp), _c.c_uint64, _c.c_uint32, _c.c_int,
                                              _c.c_uint64]),
     "topk_ctx_destroy": (_c.c_int, [_vp]),
+    "topk_csv_count": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint32)]),
+    "topk_csv_read": (_c.c_int, [_c.c_char_p, _c.c_uint64, _c.c_uint32, _c.c_uint32, _vp,
+                                 _c.POINTER(TopkIngestReport)]),
+    "topk_csv_write": (_c.c_int, [_c.c_char_p, _vp, _c.c_uint64, _c.c_uint32, _c.c_uint32]),
+    "topk_colfile_write": (_c.c_int, [_c.c_char_p, _vp, _c.c_uint64, _c.c_uint32, _c.c_uint32]),
+    "topk_colfile_info": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint32)]),
+    "topk_ctx_create_file": (_c.c_int, [_c.POINTER(_vp), _c.c_char_p, _c.c_uint64, _c.c_uint32, _c.c_uint32,
+                                        _c.POINTER(TopkIngestReport)]),
     "topk_ctx_n": (_c.c_uint64, [_vp]),
     "topk_ctx_d": (_c.c_uint32, [_vp]),
     "topk_ctx_read_columns": (_c.c_int, [_vp, _vp]),

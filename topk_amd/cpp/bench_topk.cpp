@@ -32,6 +32,7 @@ struct Args {
     uint32_t workload = 1024;
     bool normalize = true;
     bool check = false;
+    std::string file;  // load this text file instead of generating (the reference's LD != 1)
 };
 
 bool parse(int argc, char **argv, Args &a) {
@@ -53,6 +54,7 @@ bool parse(int argc, char **argv, Args &a) {
         else if (s == "-r" && next(v)) a.seed = v;
         else if (s == "-w" && next(v)) a.workload = (uint32_t)v;
         else if (s == "-a" && i + 1 < argc) a.algo = argv[++i];
+        else if (s == "-f" && i + 1 < argc) a.file = argv[++i];
         else if (s == "--no-normalize") a.normalize = false;
         else if (s == "--check") a.check = true;
         else return false;
@@ -62,8 +64,15 @@ bool parse(int argc, char **argv, Args &a) {
 
 template <class Algo>
 void fill(Algo &al, const Args &a) {
-    float *&c = al.get_cdata();
-    topk_amd::check(topk_generate(c, a.n, a.d, a.n, a.distr, a.seed, 0, a.n, 0), "topk_generate");
+    topk_amd::File f(a.file, false, a.n, a.d);
+    f.set_transpose(true);
+    f.set_seed(a.seed);
+    if (!a.file.empty()) {
+        std::cout << "Loading data from file !!!" << std::endl;
+        f.load(al.get_cdata());
+    } else {
+        f.gen(al.get_cdata(), (int8_t)a.distr);
+    }
     al.set_normalize(a.normalize);
 }
 
@@ -108,7 +117,7 @@ int main(int argc, char **argv) {
     if (!parse(argc, argv, a)) {
         std::fprintf(stderr,
                      "usage: bench_topk [-n N] [-d D] [-s KS] [-e KE] [-q QD] [-i ITER] [-t DISTR] [-r SEED] "
-                     "[-a fte|pta|mq] [-w WORKLOAD] [--no-normalize] [--check]\n");
+                     "[-a fte|pta|mq] [-w WORKLOAD] [-f TEXTFILE] [--no-normalize] [--check]\n");
         return 2;
     }
     const std::vector<float> w(a.d, 1.0f);

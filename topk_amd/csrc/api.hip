@@ -104,7 +104,7 @@ __global__ void nonfinite_kernel(const float *__restrict__ cols, uint64_t n, uin
 static uint32_t grid_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 // Allocates the column store and per-query scratch of a fresh context.
-static int ctx_alloc(topk_ctx *ctx) {
+int ctx_alloc(topk_ctx *ctx) {
     ctx->ld = ((ctx->n + 8191) / 8192) * 8192;
     TK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const size_t col_floats = ctx->ld * ctx->d + 8192;
@@ -133,7 +133,7 @@ static int ctx_alloc(topk_ctx *ctx) {
     return TOPK_OK;
 }
 
-static int check_finite(topk_ctx *ctx) {
+int check_finite(topk_ctx *ctx) {
     unsigned int *flag = reinterpret_cast<unsigned int *>(ctx->d_res + ctx->res.status_off);
     TK_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), ctx->stream));
     hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, ctx->stream, ctx->cols, ctx->n, ctx->d, ctx->ld,
@@ -174,6 +174,8 @@ const char *topk_strerror(int s) {
         case TOPK_ERR_STATE: return "context not initialised for this call";
         case TOPK_ERR_NO_DEVICE: return "no usable GPU";
         case TOPK_ERR_OVERFLOW: return "candidate buffer overflow";
+        case TOPK_ERR_FORMAT: return "malformed file";
+        case TOPK_ERR_IO: return "file input/output error";
         default: return "unknown status";
     }
 }

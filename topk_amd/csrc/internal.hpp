@@ -310,6 +310,11 @@ void timing_end(topk_ctx *ctx, hipStream_t s, const char *name, hipEvent_t ev0);
 
 int hip_status(hipError_t e);
 
+// Allocates the column store and query scratch of a context whose n and d
+// are set; rejects stores holding NaN or infinity (api.hip).
+int ctx_alloc(topk_ctx *ctx);
+int check_finite(topk_ctx *ctx);
+
 }  // namespace topk
 
 #define TK_HIP(call)                                       \

@@ -1,15 +1,17 @@
 """Python host side over the C ABI (include/topk.h).
 
 :class:`Engine` owns one ``topk_ctx`` (a column store in HBM plus a HIP
-stream).  :mod:`topk_amd.compat` mirrors the reference's algorithm-object
-interface (TPAc / PTA / BTA / GPA) on top of it.
+stream); the module functions cover data ingest (text and column files).
+The reference's algorithm-object interface (TPAc / PTA / File) is mirrored
+for C++ callers in include/topk_amd.hpp.
 """
 import ctypes
+import os
 
 import numpy as np
 
-from ._lib import (MLP_IN, MLP_NPARAMS, TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkBatchTuning, TopkLearnedConfig,
-                   TopkLearnedReport, TopkQuery, TopkStats, check, lib)
+from ._lib import (MLP_IN, MLP_NPARAMS, TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkBatchTuning, TopkIngestReport,
+                   TopkLearnedConfig, TopkLearnedReport, TopkQuery, TopkStats, check, lib)
 
 _c = ctypes
 
@@ -26,6 +28,56 @@ def generate(n, d, distr, seed, i0=0, count=None, threads=0):
     out = np.empty((d, count), dtype=np.float32)
     check(lib().topk_generate(_ptr(out), n, d, count, distr, seed, i0, count, threads), "topk_generate")
     return out
+
+
+def _report(rep):
+    return {name: getattr(rep, name) for name, _ in TopkIngestReport._fields_}
+
+
+def csv_count(path):
+    """(rows, fields) of a text file: non-blank lines, values on the first one."""
+    n = _c.c_uint64()
+    d = _c.c_uint32()
+    check(lib().topk_csv_count(os.fsencode(path), _c.byref(n), _c.byref(d)), "topk_csv_count")
+    return n.value, d.value
+
+
+def read_csv(path, n=0, d=0, rowmajor=False):
+    """Parses a text file on the GPU: returns (table, report), the table
+    (d, n) column-major or (n, d) with rowmajor=True."""
+    if n == 0 or d == 0:
+        cn, cd = csv_count(path)
+        n = n or cn
+        d = d or cd
+    out = np.empty((n, d) if rowmajor else (d, n), dtype=np.float32)
+    rep = TopkIngestReport()
+    check(lib().topk_csv_read(os.fsencode(path), n, d, TOPK_ROWMAJOR if rowmajor else 0, _ptr(out),
+                              _c.byref(rep)), "topk_csv_read")
+    return out, _report(rep)
+
+
+def write_csv(path, data, rowmajor=False):
+    """Writes a (d, n) column-major table (or (n, d) with rowmajor=True) as
+    text rows of "%f" values."""
+    a = np.ascontiguousarray(data, dtype=np.float32)
+    n, d = (a.shape if rowmajor else a.shape[::-1])
+    check(lib().topk_csv_write(os.fsencode(path), _ptr(a), n, d, TOPK_ROWMAJOR if rowmajor else 0),
+          "topk_csv_write")
+
+
+def write_colfile(path, data, rowmajor=False):
+    """Writes a binary column file (4096-byte header, then the columns)."""
+    a = np.ascontiguousarray(data, dtype=np.float32)
+    n, d = (a.shape if rowmajor else a.shape[::-1])
+    check(lib().topk_colfile_write(os.fsencode(path), _ptr(a), n, d, TOPK_ROWMAJOR if rowmajor else 0),
+          "topk_colfile_write")
+
+
+def colfile_info(path):
+    n = _c.c_uint64()
+    d = _c.c_uint32()
+    check(lib().topk_colfile_info(os.fsencode(path), _c.byref(n), _c.byref(d)), "topk_colfile_info")
+    return n.value, d.value
 
 
 def _query_arrays(d, weights, attr):
@@ -101,6 +153,19 @@ class Engine:
         h = _c.c_void_p()
         check(lib().topk_ctx_create_generated(_c.byref(h), n, d, distr, seed), "topk_ctx_create_generated")
         return cls(_handle=h)
+
+    @classmethod
+    def from_file(cls, path, n=0, d=0):
+        """Column store from a text file (one object per line, ',' between
+        values; parsed on the GPU) or a binary column file; n / d of 0 take
+        the whole file.  The ingest report is kept as ``ingest``."""
+        h = _c.c_void_p()
+        rep = TopkIngestReport()
+        check(lib().topk_ctx_create_file(_c.byref(h), os.fsencode(path), n, d, 0, _c.byref(rep)),
+              "topk_ctx_create_file")
+        e = cls(_handle=h)
+        e.ingest = _report(rep)
+        return e
 
     def close(self):
         if self._h:
