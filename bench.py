@@ -24,6 +24,9 @@ Store: N = 2^26 objects, d = 8 FP32 attributes, independent U[0,1) data
 
 ``--mode fte`` runs the plain FTE part alone (one step = one query); it is
 the command the FTE rocprofv3 summaries under profiles/ were taken with.
+``--mode ingest`` times data ingest alone (one step = one file load), and
+``--mode configs`` the other workloads (PTA at N = 2^28, batched and learned
+d = 16, ingest).
 """
 import argparse
 import json
@@ -316,7 +319,8 @@ def other_configs(threads):
     * batched: N = 2^25, d = 16 mixed (seed 4), 1024 queries per launch on
       random subsets of 2-16 attributes, k from {16..256} (query seed 200);
     * learned: N = 2^26, d = 16 mixed (seed 5), 100k training queries (seed
-      300), 4096 held-out queries (seed 301) in launches of 1024."""
+      300), 4096 held-out queries (seed 301) in launches of 1024;
+    * ingest: N = 2^24, d = 8 text and column files (ingest_bench)."""
     import oracle
     import topk_amd
     ks = [16, 32, 64, 128, 256]
@@ -389,7 +393,67 @@ def other_configs(threads):
             "fallback_rate": fb, "fte_fallback_rate": ff, "mismatches_vs_plain_fte": mism, "judged_queries": 4096,
             "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
                          "train_mse": rep["train_mse"], "wall_s": train_s}, "autotune": tune}
+    res["ingest_n2e24_d8"] = ingest_bench(1 << 24, 8, 3)
     print(json.dumps({"configs": res}), flush=True)
+
+
+def ingest_bench(n, d, steps):
+    """Data ingest: an N x d independent store (seed 0) written as text rows
+    of "%f" values (the reference's writer, input/File.h:415-430) and as a
+    binary column file in a temporary directory (page cache warm), each
+    loaded into a fresh column store ``steps`` times.  The CPU baseline is the
+    oracle's fscanf row loop (input/File.h:264-295, one thread) on the first
+    2^20 rows; parity: the loaded columns equal the oracle's on that sample
+    and the column file loads unchanged."""
+    import shutil
+    import tempfile
+
+    import oracle
+    import topk_amd
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 0)
+    tmp = tempfile.mkdtemp(prefix="topk_ingest_")
+    try:
+        csv = os.path.join(tmp, "d_%d_%d_i.csv" % (n, d))
+        col = os.path.join(tmp, "d_%d_%d_i.col" % (n, d))
+        t0 = time.perf_counter()
+        topk_amd.write_csv(csv, x)
+        write_s = time.perf_counter() - t0
+        topk_amd.write_colfile(col, x)
+        text_runs, col_runs = [], []
+        for _ in range(steps):
+            with topk_amd.Engine.from_file(csv) as e:
+                text_runs.append(e.ingest)
+        sample = min(n, 1 << 20)
+        with topk_amd.Engine.from_file(csv) as e:
+            cols = e.read_columns()
+        t0 = time.perf_counter()
+        want, r = oracle.csv_read(csv, sample, d)
+        cpu_s = time.perf_counter() - t0
+        text_ok = r == sample and np.array_equal(cols[:, :sample].view(np.uint32), want.view(np.uint32))
+        text_close = float(np.max(np.abs(cols - x)))
+        for _ in range(steps):
+            with topk_amd.Engine.from_file(col) as e:
+                col_runs.append(e.ingest)
+        with topk_amd.Engine.from_file(col) as e:
+            col_ok = np.array_equal(e.read_columns(), x)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    tb = sorted(text_runs, key=lambda r: r["ms_total"])[len(text_runs) // 2]
+    cb = sorted(col_runs, key=lambda r: r["ms_total"])[len(col_runs) // 2]
+    sample_bytes = tb["bytes"] * sample / n
+    return {
+        "text": {"bytes": tb["bytes"], "rows": tb["rows"], "ms_total": tb["ms_total"], "ms_read": tb["ms_read"],
+                 "ms_parse": tb["ms_parse"], "GB_per_s": tb["bytes"] / tb["ms_total"] / 1e6,
+                 "rows_per_s": tb["rows"] / tb["ms_total"] * 1e3, "slow_lines": tb["slow_lines"],
+                 "matches_oracle_sample": bool(text_ok), "max_abs_vs_source": text_close,
+                 "write_s": write_s},
+        "column_file": {"bytes": cb["bytes"], "ms_total": cb["ms_total"],
+                        "GB_per_s": cb["bytes"] / cb["ms_total"] / 1e6, "loads_unchanged": bool(col_ok)},
+        "cpu_baseline": {"kind": "port", "cores": 1, "sample": "first %d rows, fscanf row loop" % sample,
+                         "rows_per_s": sample / cpu_s, "GB_per_s": sample_bytes / cpu_s / 1e9},
+        "speedup_vs_cpu_rows": (tb["rows"] / tb["ms_total"] * 1e3) / (sample / cpu_s),
+        "steps": steps,
+    }
 
 
 def main():
@@ -397,8 +461,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs"], default="learned",
-                    help="learned: the headline line; fte: plain FTE alone; configs: the other workloads")
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest"], default="learned",
+                    help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
+                         "ingest: data ingest alone")
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--m", type=int, default=D, help="--mode fte: query width")
     ap.add_argument("--k", type=int, default=K)
@@ -429,6 +494,10 @@ def main():
 
     if args.mode == "configs":
         other_configs(min(os.cpu_count() or 1, 16))
+        return
+    if args.mode == "ingest":
+        n = args.n if args.n != N_DEFAULT else 1 << 24
+        print(json.dumps({"ingest_n%d_d8" % n: ingest_bench(n, D, max(1, args.steps))}), flush=True)
         return
 
     n, k = args.n, args.k

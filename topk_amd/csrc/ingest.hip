@@ -12,8 +12,9 @@
 //   csv_scan_kernel    exclusive prefix over tiles (one workgroup)
 //   csv_index_kernel   byte offset of every newline, in order
 //   csv_rows_kernel    non-blank lines per 256-line group; scanned again
-//   csv_parse_kernel   one thread per line: fields to binary32, written
-//                      column-major (coalesced across a wave's lines)
+//   csv_parse_kernel   one thread per line over the workgroup's lines staged
+//                      in LDS: fields to binary32, written column-major
+//                      (coalesced across a wave's lines)
 //   csv_patch_kernel   rows the host parsed (below) written into place
 //
 // Exactness: a field with at most 19 significant digits w and decimal
@@ -49,7 +50,7 @@ namespace topk {
 
 constexpr uint32_t CSV_BS = 256;
 constexpr uint32_t CSV_TILE = 65536;             // text bytes per workgroup in the newline passes
-constexpr uint32_t CSV_SEG = CSV_TILE / CSV_BS;  // contiguous bytes per thread in csv_index_kernel
+constexpr uint32_t CSV_LDS_WORDS = 8192;        // text staged in LDS per parse workgroup (32 KiB)
 constexpr uint32_t CSV_SCAN_BS = 1024;
 constexpr uint64_t STAGE_BYTES = 64ull << 20;    // pinned staging buffer (two alternate)
 constexpr uint32_t IO_THREADS = 8;
@@ -136,39 +137,42 @@ __global__ __launch_bounds__(CSV_SCAN_BS) void csv_scan_kernel(const uint32_t *c
     }
 }
 
-// Writes the byte offset of every newline, in file order: thread t of tile
-// b covers bytes [b*TILE + t*SEG, +SEG).
+// Writes the byte offset of every newline, in file order.  The tile is
+// read in coalesced rounds of CSV_BS 16-byte words; a block scan per round
+// orders the round's newlines (word order is file order).
 __global__ __launch_bounds__(CSV_BS) void csv_index_kernel(const uint4 *text, const uint64_t *tile_base,
                                                            uint64_t *ends) {
-    const uint64_t seg = (uint64_t)blockIdx.x * CSV_TILE + (uint64_t)threadIdx.x * CSV_SEG;
-    const uint4 *p = text + seg / 16;
-    uint32_t c = 0;
-#pragma unroll 4
-    for (uint32_t j = 0; j < CSV_SEG / 16; j++) {
-        const uint4 v = p[j];
-        c += __popc(nl_mask(v.x)) + __popc(nl_mask(v.y)) + __popc(nl_mask(v.z)) + __popc(nl_mask(v.w));
-    }
+    const uint4 *t = text + (uint64_t)blockIdx.x * (CSV_TILE / 16);
     __shared__ uint32_t wsum[CSV_BS / 64];
     __shared__ uint32_t total;
-    const uint32_t ex = block_excl_scan<CSV_BS>(c, wsum, &total);
-    if (!c) return;
-    uint64_t o = tile_base[blockIdx.x] + ex;
-    for (uint32_t j = 0; j < CSV_SEG / 16; j++) {
-        const uint4 v = p[j];
+    uint64_t o = tile_base[blockIdx.x];
+    for (uint32_t r = 0; r < CSV_TILE / 16 / CSV_BS; r++) {
+        const uint32_t wi = r * CSV_BS + threadIdx.x;
+        const uint4 v = t[wi];
         const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+        uint32_t mk[4], c = 0;
 #pragma unroll
         for (int h = 0; h < 4; h++) {
-            uint32_t m = nl_mask(w4[h]);
+            mk[h] = nl_mask(w4[h]);
+            c += __popc(mk[h]);
+        }
+        const uint32_t ex = block_excl_scan<CSV_BS>(c, wsum, &total);
+        uint64_t q = o + ex;
+        const uint64_t byte0 = (uint64_t)blockIdx.x * CSV_TILE + (uint64_t)wi * 16;
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            uint32_t m = mk[h];
             while (m) {
-                const uint32_t byte = (uint32_t)__builtin_ctz(m) >> 3;
-                ends[o++] = seg + j * 16 + h * 4 + byte;
+                ends[q++] = byte0 + h * 4 + ((uint32_t)__builtin_ctz(m) >> 3);
                 m &= m - 1;
             }
         }
+        o += total;
     }
 }
 
-// Reads the text a byte at a time through aligned 32-bit loads.
+// Reads the text a byte at a time through aligned 32-bit loads, from HBM
+// or from a copy of words [base, ...) in LDS.
 struct ByteReader {
     const uint32_t *w;
     uint64_t cur = ~0ull;
@@ -183,6 +187,12 @@ struct ByteReader {
     }
 };
 
+struct LdsByteReader {
+    const uint8_t *lds;  // LDS bytes holding text bytes [base, ...)
+    uint64_t base;
+    __device__ __forceinline__ uint32_t operator()(uint64_t p) const { return lds[(uint32_t)(p - base)]; }
+};
+
 // Line j spans [start, end): end is its newline, or the text's end for the
 // line after the last newline.
 __device__ __forceinline__ void csv_line(const uint64_t *ends, uint64_t nl, uint64_t size, uint64_t j,
@@ -191,7 +201,8 @@ __device__ __forceinline__ void csv_line(const uint64_t *ends, uint64_t nl, uint
     end = j < nl ? ends[j] : size;
 }
 
-__device__ __forceinline__ bool csv_nonblank(ByteReader &rd, uint64_t start, uint64_t end) {
+template <class Reader>
+__device__ __forceinline__ bool csv_nonblank(Reader &rd, uint64_t start, uint64_t end) {
     for (uint64_t p = start; p < end; p++)
         if (!csv_ws(rd(p))) return true;
     return false;
@@ -214,8 +225,8 @@ __global__ __launch_bounds__(CSV_BS) void csv_rows_kernel(const uint32_t *text, 
 
 // Converts one field (see the exactness note at the top); false sends the
 // line to the host.
-__device__ __forceinline__ bool csv_finish(uint64_t w, int q, bool neg, bool inexact, bool digits, float &out) {
-    if (inexact || !digits) return false;
+__device__ __forceinline__ bool csv_finish(uint64_t w, int q, bool neg, bool digits, float &out) {
+    if (!digits) return false;
     if (w == 0) {
         out = neg ? -0.0f : 0.0f;
         return true;
@@ -236,129 +247,120 @@ __device__ __forceinline__ bool csv_finish(uint64_t w, int q, bool neg, bool ine
     return true;
 }
 
+// Parses line [start, end) into row `row`; false if the host must.  One
+// straight pass per field (white space, sign, integer digits, fraction,
+// exponent), so the lanes of a wave walk their lines in step.
+template <class Reader>
+__device__ __forceinline__ bool csv_parse_line(Reader &rd, uint64_t start, uint64_t end, uint32_t d, float *out,
+                                               uint64_t cs, uint64_t rs, uint64_t row) {
+    uint64_t p = start;
+    auto at = [&](uint64_t i) -> uint32_t { return i < end ? rd(i) : (uint32_t)'\n'; };
+    uint32_t c = at(p);
+    for (uint32_t m = 0; m < d; m++) {
+        while (csv_ws(c)) c = at(++p);
+        bool neg = false;
+        if (c == '+' || c == '-') {
+            neg = c == '-';
+            c = at(++p);
+        }
+        uint64_t w = 0;
+        int q = 0, nd = 0;
+        bool digits = false;
+        for (uint32_t dig = c - '0'; dig < 10; dig = c - '0') {  // integer digits
+            digits = true;
+            if (w || dig) {
+                if (nd < 19) {
+                    w = w * 10 + dig;
+                    nd++;
+                } else {
+                    q++;  // past 19 digits w >= 2^53: only the scale matters
+                }
+            }
+            c = at(++p);
+        }
+        if (c == '.') {
+            c = at(++p);
+            for (uint32_t dig = c - '0'; dig < 10; dig = c - '0') {  // fraction digits
+                digits = true;
+                if (w || dig) {
+                    if (nd < 19) {
+                        w = w * 10 + dig;
+                        nd++;
+                        q--;
+                    }
+                } else {
+                    q--;  // leading zeros scale only
+                }
+                c = at(++p);
+            }
+        }
+        if (!digits) return false;  // empty field, inf / nan, or garbage
+        if (c == 'e' || c == 'E') {
+            c = at(++p);
+            bool eneg = false;
+            if (c == '+' || c == '-') {
+                eneg = c == '-';
+                c = at(++p);
+            }
+            if (c - '0' >= 10u) return false;
+            int e = 0;
+            for (uint32_t dig = c - '0'; dig < 10; dig = c - '0') {
+                e = e < 100000 ? e * 10 + (int)dig : e;
+                c = at(++p);
+            }
+            q += eneg ? -e : e;
+        }
+        float v;
+        if (!csv_finish(w, q, neg, digits, v)) return false;
+        out[m * cs + row * rs] = v;
+        if (m + 1 < d) {
+            if (c != ',') return false;  // also white space before the separator
+            c = at(++p);
+        }
+    }
+    while (csv_ws(c)) c = at(++p);
+    return p >= end;
+}
+
 // One thread per line; row = rank of the line among the non-blank lines.
-// Value m of row i goes to out[m*cs + i*rs].
+// Value m of row i goes to out[m*cs + i*rs].  A workgroup's lines are
+// staged in LDS (coalesced 32-bit loads) when they fit in CSV_LDS_WORDS.
 __global__ __launch_bounds__(CSV_BS) void csv_parse_kernel(const uint32_t *text, const uint64_t *ends, uint64_t nl,
                                                            uint64_t size, const uint64_t *row_base, uint64_t nwant,
                                                            uint32_t d, float *out, uint64_t cs, uint64_t rs,
                                                            unsigned long long *slow_cnt, CsvSlow *slow) {
-    const uint64_t j = (uint64_t)blockIdx.x * CSV_BS + threadIdx.x;
-    uint64_t start = 0, end = 0;
-    bool f = false;
-    ByteReader rd{text};
-    if (j <= nl) {
-        csv_line(ends, nl, size, j, start, end);
-        f = csv_nonblank(rd, start, end);
-    }
+    __shared__ uint32_t lds[CSV_LDS_WORDS];
     __shared__ uint32_t wsum[CSV_BS / 64];
     __shared__ uint32_t total;
+    const uint64_t j0 = (uint64_t)blockIdx.x * CSV_BS;
+    const uint64_t j = j0 + threadIdx.x;
+    const uint64_t jl = min(j0 + CSV_BS - 1, nl);  // the group's last line
+    uint64_t gs, ge;
+    {
+        uint64_t s0, e0, s1, e1;
+        csv_line(ends, nl, size, j0, s0, e0);
+        csv_line(ends, nl, size, jl, s1, e1);
+        gs = s0;
+        ge = e1;
+    }
+    const uint64_t wb = gs >> 2, we = (ge + 3) >> 2;
+    const bool staged = we - wb <= CSV_LDS_WORDS;
+    if (staged)
+        for (uint64_t w = wb + threadIdx.x; w < we; w += CSV_BS) lds[(uint32_t)(w - wb)] = text[w];
+    __syncthreads();
+    uint64_t start = 0, end = 0;
+    bool f = false;
+    ByteReader grd{text};
+    LdsByteReader lrd{reinterpret_cast<const uint8_t *>(lds), wb * 4};
+    if (j <= nl) {
+        csv_line(ends, nl, size, j, start, end);
+        f = staged ? csv_nonblank(lrd, start, end) : csv_nonblank(grd, start, end);
+    }
     const uint32_t rank = block_excl_scan<CSV_BS>(f ? 1u : 0u, wsum, &total);
     const uint64_t row = row_base[blockIdx.x] + rank;
     if (!f || row >= nwant) return;
-
-    // states: 0 leading space, 1 integer digits, 2 fraction digits,
-    // 3 exponent sign, 4 first exponent digit, 5 exponent digits, 6 trailing space
-    int state = 0;
-    uint32_t m = 0;
-    uint64_t w = 0;
-    int q = 0, e = 0, nd = 0;
-    bool neg = false, eneg = false, inexact = false, digits = false, ok = true;
-    for (uint64_t p = start; p <= end && ok; p++) {
-        const uint32_t c = p < end ? rd(p) : '\n';
-        const uint32_t dig = c - '0';
-        const bool isd = dig < 10;
-        bool fin = false;
-        switch (state) {
-            case 0:
-                if (csv_ws(c)) break;
-                if (c == '+' || c == '-') {
-                    neg = c == '-';
-                    state = 1;
-                } else if (isd) {
-                    state = 1;
-                    --p;  // the digit is taken in state 1
-                } else if (c == '.') {
-                    state = 2;
-                } else {
-                    ok = false;  // empty field, inf / nan / hex, or garbage
-                }
-                break;
-            case 1:
-            case 2:
-                if (isd) {
-                    digits = true;
-                    if (w == 0 && dig == 0) {
-                        if (state == 2) q--;  // leading zeros of the fraction scale only
-                    } else if (nd < 19) {
-                        w = w * 10 + dig;
-                        nd++;
-                        if (state == 2) q--;
-                    } else {
-                        if (dig) inexact = true;  // digits past the 19th
-                        if (state == 1) q++;
-                    }
-                } else if (c == '.' && state == 1) {
-                    state = 2;
-                } else if ((c == 'e' || c == 'E') && digits) {
-                    state = 3;
-                } else {
-                    fin = true;
-                }
-                break;
-            case 3:
-                if (c == '+' || c == '-') {
-                    eneg = c == '-';
-                    state = 4;
-                } else if (isd) {
-                    state = 5;
-                    e = dig;
-                } else {
-                    ok = false;
-                }
-                break;
-            case 4:
-                if (isd) {
-                    state = 5;
-                    e = dig;
-                } else {
-                    ok = false;
-                }
-                break;
-            case 5:
-                if (isd) {
-                    e = e < 100000 ? e * 10 + (int)dig : e;
-                } else {
-                    fin = true;
-                }
-                break;
-            default:  // 6: only white space may follow the last field
-                if (c == '\n') {
-                    ok = m == d;
-                } else if (!csv_ws(c)) {
-                    ok = false;
-                }
-                break;
-        }
-        if (!fin) continue;
-        float v;
-        ok = m < d && csv_finish(w, q + (eneg ? -e : e), neg, inexact, digits, v);
-        if (!ok) break;
-        out[m * cs + row * rs] = v;
-        m++;
-        if (c == ',') {
-            ok = m < d;
-            state = 0;
-            w = 0;
-            q = e = nd = 0;
-            neg = eneg = inexact = digits = false;
-        } else if (c == '\n') {
-            ok = m == d;
-        } else if (csv_ws(c)) {
-            state = 6;
-        } else {
-            ok = false;
-        }
-    }
+    const bool ok = staged ? csv_parse_line(lrd, start, end, d, out, cs, rs, row)
+                           : csv_parse_line(grd, start, end, d, out, cs, rs, row);
     if (!ok) {
         const unsigned long long s = atomicAdd(slow_cnt, 1ull);
         slow[s] = CsvSlow{start, (uint32_t)(end - start), (uint32_t)row};
