@@ -398,6 +398,43 @@ int topk_timing_enable(topk_ctx *ctx, int on);
 int topk_timing_get(topk_ctx *ctx, const char *kernel, double *total_ms, uint64_t *launches);
 int topk_timing_reset(topk_ctx *ctx);
 
+/* ---- stores larger than HBM (reference: GPAm, gpu/GPAm.h:30-89) ----
+ *
+ * The column store lives in pinned host memory and each query streams it
+ * through HBM in object-range chunks (only the queried columns): two chunk
+ * stores alternate, the copy of the next chunk overlapping the exact FTE
+ * pass over the current one; per-chunk top-k lists merge by ranking key
+ * with global ids, so results equal topk_fte on the whole store.  A batch
+ * streams every chunk once for all of its queries. */
+
+typedef struct topk_hstore topk_hstore;
+
+typedef struct topk_stream_report {
+    uint64_t chunks;       /* chunks streamed by the last call */
+    uint64_t bytes;        /* bytes copied host -> HBM by the last call */
+    uint64_t queries;
+    double ms_total;
+} topk_stream_report;
+
+/* Allocates a pinned host store of n x d floats and two chunk stores of
+ * `chunk` objects in HBM (0: 2^26; at least 8192).  Replaces GPAm::alloc
+ * (gpu/GPAm.h:30-35, cudaMallocHost + device buffer). */
+int topk_hstore_create(topk_hstore **out, uint64_t n, uint32_t d, uint64_t chunk);
+/* The host store to fill, column-major x[m*n + i] (AA::get_cdata, cpu/AA.h:77). */
+float *topk_hstore_data(topk_hstore *hs);
+uint64_t topk_hstore_n(const topk_hstore *hs);
+uint32_t topk_hstore_d(const topk_hstore *hs);
+int topk_hstore_destroy(topk_hstore *hs);
+/* One query over the whole host store (GPAm::findTopK, gpu/GPAm.h:38-89);
+ * outputs as topk_fte.  TOPK_ERR_NONFINITE if a streamed column holds NaN
+ * or infinity. */
+int topk_hstore_fte(topk_hstore *hs, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
+                    uint64_t *out_ids, float *out_scores, float *out_threshold);
+/* Many queries, each chunk streamed once for all of them; outputs as topk_batch. */
+int topk_hstore_batch(topk_hstore *hs, uint32_t nq, const topk_query *qs, uint32_t k_stride, uint64_t *out_ids,
+                      float *out_scores, float *out_thresholds);
+int topk_hstore_report(const topk_hstore *hs, topk_stream_report *rep);
+
 /* ---- utilities ---- */
 
 /* Deterministic synthetic generator (replaces the time-seeded

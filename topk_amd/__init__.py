@@ -8,7 +8,7 @@ library ``libtopk_amd.so`` behind the C ABI of ``include/topk.h``.
 """
 from ._lib import (ANTICORRELATED, CORRELATED, INDEPENDENT, MIXED, TOPK_INIT_NORMALIZE, TOPK_INIT_PTA,
                    TOPK_MAX_DIMS, TOPK_MAX_K, TopkError, lib)
-from .engine import Engine, colfile_info, csv_count, generate, read_csv, write_colfile, write_csv
+from .engine import Engine, HostStore, colfile_info, csv_count, generate, read_csv, write_colfile, write_csv
 
-__all__ = ["Engine", "generate", "lib", "csv_count", "read_csv", "write_csv", "write_colfile", "colfile_info", "TopkError", "CORRELATED", "INDEPENDENT", "ANTICORRELATED", "MIXED",
+__all__ = ["Engine", "HostStore", "generate", "lib", "csv_count", "read_csv", "write_csv", "write_colfile", "colfile_info", "TopkError", "CORRELATED", "INDEPENDENT", "ANTICORRELATED", "MIXED",
            "TOPK_INIT_NORMALIZE", "TOPK_INIT_PTA", "TOPK_MAX_DIMS", "TOPK_MAX_K"]

@@ -71,6 +71,11 @@ class TopkIngestReport(ctypes.Structure):
                 ("ms_read", _c.c_double), ("ms_parse", _c.c_double), ("ms_total", _c.c_double)]
 
 
+class TopkStreamReport(ctypes.Structure):
+    _fields_ = [("chunks", _c.c_uint64), ("bytes", _c.c_uint64), ("queries", _c.c_uint64),
+                ("ms_total", _c.c_double)]
+
+
 MLP_NPARAMS = 10433
 MLP_IN = 96
 
@@ -88,6 +93,14 @@ SIGNATURES = {
     "topk_csv_write": (_c.c_int, [_c.c_char_p, _vp, _c.c_uint64, _c.c_uint32, _c.c_uint32]),
     "topk_colfile_write": (_c.c_int, [_c.c_char_p, _vp, _c.c_uint64, _c.c_uint32, _c.c_uint32]),
     "topk_colfile_info": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint32)]),
+    "topk_hstore_create": (_c.c_int, [_c.POINTER(_vp), _c.c_uint64, _c.c_uint32, _c.c_uint64]),
+    "topk_hstore_data": (_vp, [_vp]),
+    "topk_hstore_n": (_c.c_uint64, [_vp]),
+    "topk_hstore_d": (_c.c_uint32, [_vp]),
+    "topk_hstore_destroy": (_c.c_int, [_vp]),
+    "topk_hstore_fte": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp, _vp]),
+    "topk_hstore_batch": (_c.c_int, [_vp, _c.c_uint32, _vp, _c.c_uint32, _vp, _vp, _vp]),
+    "topk_hstore_report": (_c.c_int, [_vp, _c.POINTER(TopkStreamReport)]),
     "topk_ctx_create_file": (_c.c_int, [_c.POINTER(_vp), _c.c_char_p, _c.c_uint64, _c.c_uint32, _c.c_uint32,
                                         _c.POINTER(TopkIngestReport)]),
     "topk_ctx_n": (_c.c_uint64, [_vp]),

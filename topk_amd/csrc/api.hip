@@ -21,9 +21,15 @@ namespace topk {
 void pta_release(topk_ctx *ctx);
 
 int hip_status(hipError_t e) {
+    if (e == hipSuccess) return TOPK_OK;
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return TOPK_ERR_NOMEM;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return TOPK_ERR_NO_DEVICE;
     return TOPK_ERR_HIP;
+}
+
+int hip_fail(hipError_t e, const char *call, const char *file, int line) {
+    if (getenv("TOPK_DEBUG")) fprintf(stderr, "topk %s:%d %s: %s\n", file, line, call, hipGetErrorString(e));
+    return hip_status(e);
 }
 
 void timing_begin(topk_ctx *ctx, hipStream_t s, hipEvent_t &ev0) {

@@ -309,6 +309,8 @@ void timing_begin(topk_ctx *ctx, hipStream_t s, hipEvent_t &ev0);
 void timing_end(topk_ctx *ctx, hipStream_t s, const char *name, hipEvent_t ev0);
 
 int hip_status(hipError_t e);
+// hip_status, naming the failed call on stderr when TOPK_DEBUG is set.
+int hip_fail(hipError_t e, const char *call, const char *file, int line);
 
 // Allocates the column store and query scratch of a context whose n and d
 // are set; rejects stores holding NaN or infinity (api.hip).
@@ -317,8 +319,8 @@ int check_finite(topk_ctx *ctx);
 
 }  // namespace topk
 
-#define TK_HIP(call)                                       \
-    do {                                                   \
-        hipError_t e_ = (call);                            \
-        if (e_ != hipSuccess) return topk::hip_status(e_); \
+#define TK_HIP(call)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess) return topk::hip_fail(e_, #call, __FILE__, __LINE__); \
     } while (0)

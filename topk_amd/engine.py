@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from ._lib import (MLP_IN, MLP_NPARAMS, TOPK_MAX_DIMS, TOPK_ROWMAJOR, TopkBatchTuning, TopkIngestReport,
-                   TopkLearnedConfig, TopkLearnedReport, TopkQuery, TopkStats, check, lib)
+                   TopkLearnedConfig, TopkLearnedReport, TopkQuery, TopkStats, TopkStreamReport, check, lib)
 
 _c = ctypes
 
@@ -384,3 +384,67 @@ class Engine:
         check(lib().topk_radix_select_u32(self._h, _c.c_void_p(d_keys), n, r, _c.byref(out)),
               "topk_radix_select_u32")
         return out.value
+
+
+class HostStore:
+    """A column store in pinned host memory, streamed through HBM in chunks
+    by every query (stores larger than HBM).  ``data`` is the (d, n)
+    column-major host array to fill before querying."""
+
+    def __init__(self, n, d, chunk=0):
+        self._h = _c.c_void_p()
+        check(lib().topk_hstore_create(_c.byref(self._h), n, d, chunk), "topk_hstore_create")
+        self.n, self.d = n, d
+        ptr = lib().topk_hstore_data(self._h)
+        self.data = np.ctypeslib.as_array((_c.c_float * (n * d)).from_address(ptr)).reshape(d, n)
+
+    def close(self):
+        if self._h:
+            self.data = None
+            lib().topk_hstore_destroy(self._h)
+            self._h = _c.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def fte(self, k, weights, attr):
+        """Exact top-k over the whole host store: (ids, scores, threshold), best first."""
+        w, at = _query_arrays(self.d, weights, attr)
+        ids = np.empty(k, dtype=np.uint64)
+        sc = np.empty(k, dtype=np.float32)
+        thr = np.zeros(1, dtype=np.float32)
+        check(lib().topk_hstore_fte(self._h, k, at.size, _ptr(w), _ptr(at), _ptr(ids), _ptr(sc), _ptr(thr)),
+              "topk_hstore_fte")
+        c = min(k, self.n)
+        return ids[:c], sc[:c], float(thr[0])
+
+    def batch(self, queries):
+        """queries: list of (k, weights_by_attr_id, attr); each chunk crosses
+        PCIe once for all of them.  Returns (ids, scores, threshold) per query."""
+        nq = len(queries)
+        qs = query_array(queries)
+        kmax = max(1, max(int(q[0]) for q in queries))
+        ids = np.empty(nq * kmax, dtype=np.uint64)
+        sc = np.empty(nq * kmax, dtype=np.float32)
+        thr = np.empty(nq, dtype=np.float32)
+        check(lib().topk_hstore_batch(self._h, nq, _c.cast(qs, _c.c_void_p), kmax, _ptr(ids), _ptr(sc), _ptr(thr)),
+              "topk_hstore_batch")
+        out = []
+        for i, (k, _, _) in enumerate(queries):
+            c = min(k, self.n)
+            out.append((ids[i * kmax:i * kmax + c].copy(), sc[i * kmax:i * kmax + c].copy(), float(thr[i])))
+        return out
+
+    def report(self):
+        r = TopkStreamReport()
+        check(lib().topk_hstore_report(self._h, _c.byref(r)), "topk_hstore_report")
+        return {name: getattr(r, name) for name, _ in TopkStreamReport._fields_}
