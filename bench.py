@@ -24,9 +24,10 @@ Store: N = 2^26 objects, d = 8 FP32 attributes, independent U[0,1) data
 
 ``--mode fte`` runs the plain FTE part alone (one step = one query); it is
 the command the FTE rocprofv3 summaries under profiles/ were taken with.
-``--mode ingest`` times data ingest alone (one step = one file load), and
+``--mode ingest`` times data ingest alone (one step = one file load),
+``--mode stream`` a store in host memory streamed through HBM, and
 ``--mode configs`` the other workloads (PTA at N = 2^28, batched and learned
-d = 16, ingest).
+d = 16, ingest, host-memory store).
 """
 import argparse
 import json
@@ -320,7 +321,8 @@ def other_configs(threads):
       random subsets of 2-16 attributes, k from {16..256} (query seed 200);
     * learned: N = 2^26, d = 16 mixed (seed 5), 100k training queries (seed
       300), 4096 held-out queries (seed 301) in launches of 1024;
-    * ingest: N = 2^24, d = 8 text and column files (ingest_bench)."""
+    * ingest: N = 2^24, d = 8 text and column files (ingest_bench);
+    * host-memory store: N = 2^28, d = 8 streamed through HBM (stream_bench)."""
     import oracle
     import topk_amd
     ks = [16, 32, 64, 128, 256]
@@ -394,6 +396,7 @@ def other_configs(threads):
             "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
                          "train_mse": rep["train_mse"], "wall_s": train_s}, "autotune": tune}
     res["ingest_n2e24_d8"] = ingest_bench(1 << 24, 8, 3)
+    res["stream_n2e28_d8_host"] = stream_bench(1 << 28, 8, 1 << 26, 64)
     print(json.dumps({"configs": res}), flush=True)
 
 
@@ -456,14 +459,66 @@ def ingest_bench(n, d, steps):
     }
 
 
+PCIE_PEAK_GBS = 64.0   # PCIe 5.0 x16, one direction
+
+
+def stream_bench(n, d, chunk, nbatch):
+    """Stores larger than HBM: an N x d independent store (seed 0) in pinned
+    host memory streamed through HBM in chunks.  Single queries (k = 64, all
+    d attributes, weight 1) and one batch of ``nbatch`` queries (weights
+    U(0.1, 1), k = 64) that streams each chunk once; the in-HBM FTE path on the
+    same store is the comparison and the parity check."""
+    import topk_amd
+    rng = np.random.default_rng(500)
+    attr = list(range(d))
+    w1 = np.ones(d, np.float32)
+    batch = [(K, rng.uniform(0.1, 1.0, d).astype(np.float32), attr) for _ in range(nbatch)]
+    with topk_amd.HostStore(n, d, chunk) as hs:
+        t0 = time.perf_counter()
+        hs.generate(topk_amd.INDEPENDENT, 0, threads=16)
+        fill_s = time.perf_counter() - t0
+        hs.fte(K, w1, attr)  # warm-up
+        t0 = time.perf_counter()
+        single = [hs.fte(K, w1, attr) for _ in range(3)]
+        single_s = (time.perf_counter() - t0) / 3
+        rep1 = hs.report()
+        t0 = time.perf_counter()
+        got = hs.batch(batch)
+        batch_s = time.perf_counter() - t0
+        repb = hs.report()
+    with topk_amd.Engine.generated(n, d, topk_amd.INDEPENDENT, 0) as e:
+        ref = e.fte(K, w1, attr)
+        e.fte(K, w1, attr)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            e.fte(K, w1, attr)
+        hbm_qps = 10 / (time.perf_counter() - t0)
+        mism = int(not all(np.array_equal(a, b) for a, b in zip(single[0], ref)))
+        for q, g in zip(batch[:8], got[:8]):
+            r = e.fte(*q)
+            mism += int(not (np.array_equal(g[0], r[0]) and np.array_equal(g[1], r[1]) and g[2] == r[2]))
+    pcie1 = rep1["bytes"] / single_s / 1e9
+    return {
+        "store": {"n": n, "d": d, "bytes": n * d * 4, "chunk_objects": chunk, "chunks": rep1["chunks"],
+                  "fill_s": fill_s},
+        "single_query": {"qps": 1.0 / single_s, "ms": single_s * 1e3, "bytes_streamed": rep1["bytes"],
+                         "pcie_GB_per_s": pcie1, "pcie_frac": pcie1 / PCIE_PEAK_GBS},
+        "batch": {"queries": nbatch, "qps": nbatch / batch_s, "ms": batch_s * 1e3,
+                  "bytes_streamed": repb["bytes"], "pcie_GB_per_s": repb["bytes"] / batch_s / 1e9},
+        "in_hbm_fte_qps": hbm_qps,
+        "mismatches_vs_in_hbm_fte": mism, "judged_queries": 9,
+        "pcie_peak_GB_per_s": PCIE_PEAK_GBS,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest"], default="learned",
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream"], default="learned",
                     help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
-                         "ingest: data ingest alone")
+                         "ingest: data ingest alone; stream: a host-memory store streamed through HBM")
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--m", type=int, default=D, help="--mode fte: query width")
     ap.add_argument("--k", type=int, default=K)
@@ -494,6 +549,10 @@ def main():
 
     if args.mode == "configs":
         other_configs(min(os.cpu_count() or 1, 16))
+        return
+    if args.mode == "stream":
+        n = args.n if args.n != N_DEFAULT else 1 << 28
+        print(json.dumps({"stream_n%d_d8" % n: stream_bench(n, D, 1 << 26, 64)}), flush=True)
         return
     if args.mode == "ingest":
         n = args.n if args.n != N_DEFAULT else 1 << 24

@@ -416,6 +416,12 @@ class HostStore:
     def __exit__(self, *a):
         self.close()
 
+    def generate(self, distr, seed, threads=0):
+        """Fills the host store with the synthetic generator (same bits as
+        Engine.generated)."""
+        check(lib().topk_generate(lib().topk_hstore_data(self._h), self.n, self.d, self.n, distr, seed, 0, self.n,
+                                  threads), "topk_generate")
+
     def fte(self, k, weights, attr):
         """Exact top-k over the whole host store: (ids, scores, threshold), best first."""
         w, at = _query_arrays(self.d, weights, attr)
