@@ -312,21 +312,12 @@ def pipelined(eng, pbs, steps):
     return nq / el, fb / nq, ff / nq
 
 
-def other_configs(threads):
-    """The other workloads of the scope table, measured once each (one JSON
-    line; the headline line is the default mode):
-    * PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
-      weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128;
-    * batched: N = 2^25, d = 16 mixed (seed 4), 1024 queries per launch on
-      random subsets of 2-16 attributes, k from {16..256} (query seed 200);
-    * learned: N = 2^26, d = 16 mixed (seed 5), 100k training queries (seed
-      300), 4096 held-out queries (seed 301) in launches of 1024;
-    * ingest: N = 2^24, d = 8 text and column files (ingest_bench);
-    * host-memory store: N = 2^28, d = 8 streamed through HBM (stream_bench)."""
+def pta_bench(threads, cpu=True):
+    """PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
+    weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128; the plain
+    FTE path and the oracle's threaded TPAc on the same store beside it."""
     import oracle
     import topk_amd
-    ks = [16, 32, 64, 128, 256]
-    res = {}
     n3, d = 1 << 28, 8
     rng = np.random.default_rng(100)
     ws = [rng.uniform(0.1, 1.0, d).astype(np.float32) for _ in range(20)]
@@ -349,15 +340,35 @@ def other_configs(threads):
             eng.fte(128, w, attr)
         fte_s = (time.perf_counter() - t0) / 5
         got = eng.pta(128, ws[1], attr)
-    x = topk_amd.generate(n3, d, topk_amd.ANTICORRELATED, 3, threads=threads)
-    t0 = time.perf_counter()
-    ref = oracle.tpac_threads(x, n3, 128, ws[1], attr, threads)
-    cpu_s = time.perf_counter() - t0
-    del x
-    res["pta_n2e28_anticorrelated_k128"] = {
+    cpu_s, match = float("nan"), None
+    if cpu:
+        x = topk_amd.generate(n3, d, topk_amd.ANTICORRELATED, 3, threads=threads)
+        t0 = time.perf_counter()
+        ref = oracle.tpac_threads(x, n3, 128, ws[1], attr, threads)
+        cpu_s = time.perf_counter() - t0
+        del x
+        match = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
+    return {
         "pta_qps": 1.0 / pta_s, "fte_qps": 1.0 / fte_s, "partitions_scored_frac": scored / len(ws),
         "layout_build_s": init_s, "cpu_tpac_threads_qps": 1.0 / cpu_s, "cpu_threads": threads,
-        "matches_oracle": bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))}
+        "matches_oracle": match}
+
+
+def other_configs(threads):
+    """The other workloads of the scope table, measured once each (one JSON
+    line; the headline line is the default mode):
+    * PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
+      weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128;
+    * batched: N = 2^25, d = 16 mixed (seed 4), 1024 queries per launch on
+      random subsets of 2-16 attributes, k from {16..256} (query seed 200);
+    * learned: N = 2^26, d = 16 mixed (seed 5), 100k training queries (seed
+      300), 4096 held-out queries (seed 301) in launches of 1024;
+    * ingest: N = 2^24, d = 8 text and column files (ingest_bench);
+    * host-memory store: N = 2^28, d = 8 streamed through HBM (stream_bench)."""
+    import topk_amd
+    ks = [16, 32, 64, 128, 256]
+    res = {}
+    res["pta_n2e28_anticorrelated_k128"] = pta_bench(threads)
 
     with topk_amd.Engine.generated(1 << 25, 16, topk_amd.MIXED, 4) as eng:
         pbs = [eng.prepare_batch(subset_queries(np.random.default_rng(200 + i), 1024, 16, ks)) for i in range(4)]
@@ -516,7 +527,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream"], default="learned",
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta"], default="learned",
                     help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
                          "ingest: data ingest alone; stream: a host-memory store streamed through HBM")
     ap.add_argument("--n", type=int, default=N_DEFAULT)
@@ -549,6 +560,9 @@ def main():
 
     if args.mode == "configs":
         other_configs(min(os.cpu_count() or 1, 16))
+        return
+    if args.mode == "pta":
+        print(json.dumps({"pta_n2e28_anticorrelated_k128": pta_bench(16, cpu=not args.no_cpu)}), flush=True)
         return
     if args.mode == "stream":
         n = args.n if args.n != N_DEFAULT else 1 << 28

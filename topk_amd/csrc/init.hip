@@ -20,7 +20,7 @@ namespace topk {
 int pta_build(topk_ctx *ctx);
 void pta_release(topk_ctx *ctx);
 int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *d_ids,
-              float *d_scores, float *d_thr, uint32_t *d_status, uint64_t *scored_out);
+              float *d_scores, float *d_thr, uint32_t *d_status);
 
 // Per column max (starting at 0) and min (starting at FLT_MAX) as monotone
 // keys, so atomicMax / atomicMin on uint32 implement float max / min.
@@ -166,13 +166,14 @@ int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     float *d_sc = reinterpret_cast<float *>(ctx->d_res + ctx->res.scores_off);
     float *d_thr = reinterpret_cast<float *>(ctx->d_res + ctx->res.thr_off);
     uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
-    uint64_t scored = 0;
-    int rc = pta_query(ctx, k, qq, weights, attr, d_ids, d_sc, d_thr, d_status, &scored);
+    int rc = pta_query(ctx, k, qq, weights, attr, d_ids, d_sc, d_thr, d_status);
     if (rc) return rc;
     Lane &L = ctx->lane[0];
     TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
     TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
     TK_HIP(hipStreamSynchronize(L.stream));
+    uint64_t scored = 0;
+    for (uint32_t i = 0; i < ctx->pta.nscored; i++) scored += ctx->pta.h_scored[i];
     const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
     if (status) {
         // candidate storage overflowed (pathological data): the exact FTE path answers

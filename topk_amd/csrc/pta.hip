@@ -399,6 +399,8 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.pbound);
     hipFree(P.bmax);
     hipFree(P.pscored);
+    hipFree(P.pcut);
+    if (P.h_scored) hipHostFree(P.h_scored);
     hipFree(P.tmax);
     hipFree(P.tmin);
     hipFree(P.tsum);
@@ -523,6 +525,8 @@ int pta_build(topk_ctx *ctx) {
         TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
         TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
         TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
+        TK_TRY(hipMalloc(&P.pcut, sizeof(uint32_t) * L));
+        TK_TRY(hipHostMalloc(&P.h_scored, sizeof(uint32_t) * MAX_STREAM_GRID, hipHostMallocDefault));
         TK_TRY(hipGetLastError());
         TK_TRY(hipStreamSynchronize(s));
         rc = build_bound_tree(P, d, s);
@@ -566,6 +570,44 @@ __global__ __launch_bounds__(PTA_BS) void pta_bound_kernel(BoundArgs b, uint32_t
         for (int j = 1; j < PTA_BS / 64; j++) r = wm[j] > r ? wm[j] : r;
         bmax[blockIdx.x] = r;
     }
+}
+
+// Per partition whose bound reaches tau: the first chunk whose suffix row
+// bound falls below tau (every later object of the partition scores below
+// tau too), as a count of chunks to score; 0 for partitions ruled out.
+// One wave per partition, one chunk row per lane (64 bounds per round); tau
+// is the seed kernel's, read on the device.
+template <int NB>
+__global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L, const uint64_t *__restrict__ pbound,
+                                                         const uint32_t *__restrict__ poff,
+                                                         const uint32_t *__restrict__ psz,
+                                                         const float *__restrict__ cstat, uint32_t crow,
+                                                         const FteState *__restrict__ st, uint32_t *__restrict__ cut) {
+    const uint32_t p = blockIdx.x * (PTA_BS / 64) + threadIdx.x / 64;
+    if (p >= L) return;
+    const uint32_t lane = lane_id();
+    const uint32_t tau_hi = (uint32_t)(st->tau >> 32);
+    if ((uint32_t)(pbound[p] >> 32) < tau_hi) {
+        if (lane == 0) cut[p] = 0;
+        return;
+    }
+    const uint32_t c0 = poff[p] / PTA_CHUNK, nc = (psz[p] + PTA_CHUNK - 1) / PTA_CHUNK;
+    const uint32_t d = b.d;
+    uint32_t first = nc;
+    for (uint32_t cb = 0; cb < nc; cb += 64) {
+        const uint32_t c = cb + lane;
+        bool fail = false;
+        if (c < nc) {
+            const float *row = cstat + (uint64_t)(c0 + c) * crow;
+            fail = okey(lp_bound<NB>(b, row, row + d, row[2 * d], row[2 * d + 1])) < tau_hi;
+        }
+        const uint64_t m = __ballot(fail);
+        if (m) {
+            first = cb + (uint32_t)__builtin_ctzll(m);
+            break;
+        }
+    }
+    if (lane == 0) cut[p] = first;
 }
 
 // Block t scores (up to 4096 objects of) the partition holding the t-th
@@ -653,7 +695,7 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
 // Grid-stride over partitions: skip a partition whose bound score is below
 // tau's score, stream the others in 1024-object steps and keep keys >= tau.
 template <int M>
-__global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L, const uint64_t *__restrict__ pbound,
+__global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L, const uint32_t *__restrict__ cut,
                                                             const uint32_t *__restrict__ poff,
                                                             const uint32_t *__restrict__ psz,
                                                             const uint32_t *__restrict__ pid,
@@ -668,9 +710,10 @@ __global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L,
     uint64_t *my = cs.keys + (uint64_t)blockIdx.x * cs.slot;
     uint32_t scored = 0;
     for (uint32_t p = blockIdx.x; p < L; p += gridDim.x) {
-        if ((uint32_t)(pbound[p] >> 32) < tau_hi) continue;
+        const uint32_t nc = cut[p];
+        if (!nc) continue;
         scored++;
-        const uint32_t start = poff[p], end = start + psz[p];
+        const uint32_t start = poff[p], end = start + min(psz[p], nc * PTA_CHUNK);
         for (uint32_t t0 = start; t0 < end; t0 += PTA_TILE) {
             const uint64_t v = t0 / 4 + threadIdx.x;
             f4 sc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -743,7 +786,7 @@ __global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L,
 // host: query
 // ---------------------------------------------------------------------------
 int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *d_ids,
-              float *d_scores, float *d_thr, uint32_t *d_status, uint64_t *scored_out) {
+              float *d_scores, float *d_thr, uint32_t *d_status) {
     PtaLayout &P = ctx->pta;
     if (!P.built) return TOPK_ERR_STATE;
     QCols q;
@@ -777,8 +820,18 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     cs.spill_cap = ctx->spill_cap;
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
-#define TK_STREAM(MM)                                                                                        \
-    hipLaunchKernelGGL(pta_stream_kernel<MM>, dim3(g), dim3(PTA_BS), 0, s, q, L, P.pbound, P.poff, P.psz, \
+    const uint32_t nw = (L + PTA_BS / 64 - 1) / (PTA_BS / 64);  // one wave per partition
+    if (ctx->d <= 8)
+        hipLaunchKernelGGL(pta_cut_kernel<8>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff, P.psz, P.cstat,
+                           P.crow, Ln.st, P.pcut);
+    else if (ctx->d <= 16)
+        hipLaunchKernelGGL(pta_cut_kernel<16>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff, P.psz,
+                           P.cstat, P.crow, Ln.st, P.pcut);
+    else
+        hipLaunchKernelGGL(pta_cut_kernel<TOPK_MAX_DIMS>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff,
+                           P.psz, P.cstat, P.crow, Ln.st, P.pcut);
+#define TK_STREAM(MM)                                                                                      \
+    hipLaunchKernelGGL(pta_stream_kernel<MM>, dim3(g), dim3(PTA_BS), 0, s, q, L, P.pcut, P.poff, P.psz, \
                        P.pid, Ln.st, cs, P.pscored)
     TK_SWITCH_M(q.m, TK_STREAM);
 #undef TK_STREAM
@@ -786,14 +839,9 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     const uint32_t kreq = (uint32_t)(ctx->n < k ? ctx->n : k);
     final_enqueue(ctx, Ln, cs, k, kreq, FinalOut{d_ids, d_scores, d_thr, d_status}, false);
     TK_HIP(hipGetLastError());
-    if (scored_out) {
-        std::vector<uint32_t> sc(g);
-        TK_HIP(hipMemcpyAsync(sc.data(), P.pscored, sizeof(uint32_t) * g, hipMemcpyDeviceToHost, s));
-        TK_HIP(hipStreamSynchronize(s));
-        uint64_t tot = 0;
-        for (uint32_t v : sc) tot += v;
-        *scored_out = tot;
-    }
+    // partitions scored per stream block: read by the caller after its synchronisation
+    TK_HIP(hipMemcpyAsync(P.h_scored, P.pscored, sizeof(uint32_t) * g, hipMemcpyDeviceToHost, s));
+    P.nscored = g;
     return TOPK_OK;
 }
 
