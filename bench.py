@@ -370,12 +370,33 @@ def other_configs(threads):
     res = {}
     res["pta_n2e28_anticorrelated_k128"] = pta_bench(threads)
 
-    with topk_amd.Engine.generated(1 << 25, 16, topk_amd.MIXED, 4) as eng:
-        pbs = [eng.prepare_batch(subset_queries(np.random.default_rng(200 + i), 1024, 16, ks)) for i in range(4)]
+    n4 = 1 << 25
+    with topk_amd.Engine.generated(n4, 16, topk_amd.MIXED, 4) as eng:
+        qlists = [subset_queries(np.random.default_rng(200 + i), 1024, 16, ks) for i in range(4)]
+        pbs = [eng.prepare_batch(ql) for ql in qlists]
         tune = eng.batch_autotune(pbs[0])
         qps, fb, ff = pipelined(eng, pbs, 20)
-        res["batch_n2e25_d16_mixed"] = {"qps": qps, "fte_fallback_rate": ff, "autotune": tune,
-                                        "threshold": "seed (no predictor)"}
+        sample = qlists[0][:32]
+        gpu_sample = eng.batch(sample)
+    # CPU baseline: the oracle's findTopKsimdMQ restatement (one scan per
+    # query, queries over the host threads) on the first 32 queries
+    import oracle
+    x = topk_amd.generate(n4, 16, topk_amd.MIXED, 4, threads=threads)
+    amax = max(len(q[2]) for q in sample)
+    attrs = np.zeros((len(sample), amax), np.uint8)
+    for i, q in enumerate(sample):
+        attrs[i, :len(q[2])] = q[2]
+    t0 = time.perf_counter()
+    ref = oracle.mq_threads(x, n4, [q[0] for q in sample], [len(q[2]) for q in sample],
+                            np.stack([q[1] for q in sample]), attrs, threads)
+    cpu_s = time.perf_counter() - t0
+    del x
+    mism = sum(int(not (np.array_equal(g[0], r[0]) and np.array_equal(g[1], r[1]))) for g, r in zip(gpu_sample, ref))
+    res["batch_n2e25_d16_mixed"] = {"qps": qps, "fte_fallback_rate": ff, "autotune": tune,
+                                    "threshold": "seed (no predictor)",
+                                    "cpu_baseline": {"kind": "port", "cores": threads, "qps": len(sample) / cpu_s,
+                                                     "sample": "first 32 queries of the first batch"},
+                                    "mismatches_vs_oracle_sample": mism}
 
     with topk_amd.Engine.generated(1 << 26, 16, topk_amd.MIXED, 5) as eng:
         train = subset_queries(np.random.default_rng(300), 100_000, 16, ks)

@@ -53,7 +53,7 @@ constexpr uint32_t CSV_TILE = 65536;             // text bytes per workgroup in 
 constexpr uint32_t CSV_LDS_WORDS = 8192;        // text staged in LDS per parse workgroup (32 KiB)
 constexpr uint32_t CSV_SCAN_BS = 1024;
 constexpr uint64_t STAGE_BYTES = 64ull << 20;    // pinned staging buffer (two alternate)
-constexpr uint32_t IO_THREADS = 8;
+constexpr uint32_t IO_THREADS = 16;
 
 // Status of a runtime call; with TOPK_DEBUG set in the environment a
 // failure also names the call on stderr.
