@@ -25,7 +25,8 @@ Store: N = 2^26 objects, d = 8 FP32 attributes, independent U[0,1) data
 ``--mode fte`` runs the plain FTE part alone (one step = one query); it is
 the command the FTE rocprofv3 summaries under profiles/ were taken with.
 ``--mode ingest`` times data ingest alone (one step = one file load),
-``--mode stream`` a store in host memory streamed through HBM, and
+``--mode stream`` a store in host memory streamed through HBM, ``--mode
+kernels`` BTA, GPA, radix select, normalisation and reorder_attr, and
 ``--mode configs`` the other workloads (PTA at N = 2^28, batched and learned
 d = 16, ingest, host-memory store).
 """
@@ -427,6 +428,7 @@ def other_configs(threads):
             "fallback_rate": fb, "fte_fallback_rate": ff, "mismatches_vs_plain_fte": mism, "judged_queries": 4096,
             "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
                          "train_mse": rep["train_mse"], "wall_s": train_s}, "autotune": tune}
+    res["kernels_n2e26_d8"] = kernels_bench(threads)
     res["ingest_n2e24_d8"] = ingest_bench(1 << 24, 8, 3)
     res["stream_n2e28_d8_host"] = stream_bench(1 << 28, 8, 1 << 26, 64)
     print(json.dumps({"configs": res}), flush=True)
@@ -494,6 +496,91 @@ def ingest_bench(n, d, steps):
 PCIE_PEAK_GBS = 64.0   # PCIe 5.0 x16, one direction
 
 
+def kernels_bench(threads):
+    """The reference's GPU kernel sketches and preprocessing on N = 2^26,
+    d = 8 (independent, seed 0), each timed with HIP events on the engine's
+    stream against its algorithmic bytes, checked against the oracle (or the
+    plain FTE answer), with the oracle's CPU restatement timed beside it:
+    BTA and GPA top-k (k = 64, all 8 attributes, weight 1), radix select
+    (2^26 random 32-bit keys, rank N - 64), normalisation and reorder_attr."""
+    import torch
+
+    import oracle
+    import topk_amd
+    n, d, k = 1 << 26, 8, 64
+    w = np.ones(d, np.float32)
+    attr = list(range(d))
+    res = {}
+
+    def per_launch(e, name):
+        ms, cnt = e.timing_get(name)
+        return ms / max(1, cnt)
+
+    def same(a, b):
+        return bool(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2])
+
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 0, threads=threads)
+    t0 = time.perf_counter()
+    cpu_ref = oracle.tpac_threads(x, n, k, w, attr, threads)
+    cpu_q = time.perf_counter() - t0
+    with topk_amd.Engine.generated(n, d, topk_amd.INDEPENDENT, 0) as e:
+        e.timing(True)
+        ref = e.fte(k, w, attr)
+        for name, fn, kern in (("bta", e.bta, "bta_block"), ("gpa", e.gpa, "gpa")):
+            fn(k, w, attr)
+            e.timing_reset()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                got = fn(k, w, attr)
+            q_s = (time.perf_counter() - t0) / 5
+            kms = per_launch(e, kern)
+            res[name] = {"qps": 1.0 / q_s, "kernel": kern, "kernel_ms": kms,
+                         "scoring_GB_per_s": 4.0 * n * d / kms / 1e6,
+                         "hbm_frac": 4.0 * n * d / kms / 1e6 / HBM_PEAK_GBS,
+                         "matches_fte": same(got, ref),
+                         "cpu_tpac_threads_qps": 1.0 / cpu_q, "cpu_threads": threads,
+                         "matches_cpu": bool(np.array_equal(got[0], cpu_ref[0]) and np.array_equal(got[1], cpu_ref[1]))}
+        keys = torch.randint(-2 ** 31, 2 ** 31 - 1, (n,), dtype=torch.int32, device="cuda",
+                             generator=torch.Generator(device="cuda").manual_seed(7))
+        hk = keys.cpu().numpy().view(np.uint32)
+        r = n - k
+        e.radix_select_u32(keys.data_ptr(), n, r)
+        e.timing_reset()
+        for _ in range(5):
+            got = e.radix_select_u32(keys.data_ptr(), n, r)
+        kms = per_launch(e, "radix_select")
+        t0 = time.perf_counter()
+        want = oracle.radix_select_u32(hk, r)
+        cpu_s = time.perf_counter() - t0
+        res["radix_select_u32"] = {"keys": n, "rank": r, "ms": kms, "GB_per_s_3_passes": 12.0 * n / kms / 1e6,
+                                   "hbm_frac": 12.0 * n / kms / 1e6 / HBM_PEAK_GBS,
+                                   "matches_oracle": bool(got == want and want == np.partition(hk, r)[r]),
+                                   "cpu_ms": cpu_s * 1e3, "cpu_cores": 1}
+        del keys
+        e.timing_reset()
+        e.init(topk_amd.TOPK_INIT_NORMALIZE)
+        nms = per_launch(e, "normalize")
+        t0 = time.perf_counter()
+        xn = oracle.normalize(x, n)
+        cpu_n = time.perf_counter() - t0
+        norm_ok = bool(np.array_equal(e.read_columns().view(np.uint32), xn.view(np.uint32)))
+        res["normalize"] = {"ms": nms, "GB_per_s": 12.0 * n * d / nms / 1e6,
+                            "hbm_frac": 12.0 * n * d / nms / 1e6 / HBM_PEAK_GBS, "matches_oracle": norm_ok,
+                            "cpu_ms": cpu_n * 1e3, "cpu_cores": 1}
+        e.timing_reset()
+        e.reorder_attr()
+        rms = per_launch(e, "reorder_attr")
+        t0 = time.perf_counter()
+        xr = oracle.reorder_attr(xn, n)
+        cpu_r = time.perf_counter() - t0
+        res["reorder_attr"] = {"ms": rms, "GB_per_s": 8.0 * n * d / rms / 1e6,
+                               "hbm_frac": 8.0 * n * d / rms / 1e6 / HBM_PEAK_GBS,
+                               "matches_oracle": bool(np.array_equal(e.read_columns().view(np.uint32),
+                                                                     xr.view(np.uint32))),
+                               "cpu_ms": cpu_r * 1e3, "cpu_cores": 1}
+    return res
+
+
 def stream_bench(n, d, chunk, nbatch):
     """Stores larger than HBM: an N x d independent store (seed 0) in pinned
     host memory streamed through HBM in chunks.  Single queries (k = 64, all
@@ -548,7 +635,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta"], default="learned",
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels"],
+                    default="learned",
                     help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
                          "ingest: data ingest alone; stream: a host-memory store streamed through HBM")
     ap.add_argument("--n", type=int, default=N_DEFAULT)
@@ -581,6 +669,9 @@ def main():
 
     if args.mode == "configs":
         other_configs(min(os.cpu_count() or 1, 16))
+        return
+    if args.mode == "kernels":
+        print(json.dumps({"kernels_n2e26_d8": kernels_bench(16)}), flush=True)
         return
     if args.mode == "pta":
         print(json.dumps({"pta_n2e28_anticorrelated_k128": pta_bench(16, cpu=not args.no_cpu)}), flush=True)

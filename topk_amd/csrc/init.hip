@@ -93,11 +93,14 @@ static int normalize_store(topk_ctx *ctx) {
     }
     init[2 * ctx->d] = 0;
     hipError_t e = hipMemcpyAsync(k, init.data(), sizeof(uint32_t) * init.size(), hipMemcpyHostToDevice, s);
+    hipEvent_t ev0 = nullptr;
+    timing_begin(ctx, s, ev0);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(colminmax_kernel, dim3(512, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
                            k + ctx->d);
         hipLaunchKernelGGL(normalize_kernel, dim3(2048, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
                            k + ctx->d);
+        timing_end(ctx, s, "normalize", ev0);
         hipLaunchKernelGGL(nonfinite_check_kernel, dim3(1024), dim3(256), 0, s, ctx->cols, ctx->n, ctx->d, ctx->ld,
                            k + 2 * ctx->d);
         e = hipGetLastError();
@@ -149,8 +152,11 @@ int topk_reorder_attr(topk_ctx *ctx) {
     if (!ctx) return TOPK_ERR_INVALID;
     pta_release(ctx);
     ctx->store_gen++;
+    hipEvent_t ev0 = nullptr;
+    timing_begin(ctx, ctx->stream, ev0);
     hipLaunchKernelGGL(reorder_attr_kernel, dim3((uint32_t)((ctx->n + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->cols, ctx->n, ctx->d, ctx->ld);
+    timing_end(ctx, ctx->stream, "reorder_attr", ev0);
     TK_HIP(hipGetLastError());
     TK_HIP(hipStreamSynchronize(ctx->stream));
     return TOPK_OK;
