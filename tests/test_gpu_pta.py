@@ -119,10 +119,12 @@ def test_pta_partition_size_and_state(gpu):
         _check_pta(eng, x, n, 50, RNG.uniform(0.1, 1, d).astype(np.float32), [3, 0, 1])
 
 
-def test_normalize_matches_oracle(gpu):
-    n, d = 100_000, 5
+@pytest.mark.parametrize("n", [100_000, 100_003])
+def test_normalize_matches_oracle(gpu, n):
+    d = 5
     x = (topk_amd.generate(n, d, topk_amd.INDEPENDENT, 5) * 7.0 - 2.0).astype(np.float32)
     x[3] = -np.abs(x[3]) - 1.0  # an all-negative column keeps max 0 (common/common.h:36)
+    x[0, n - 1], x[1, n - 1] = 50.0, -50.0  # column extremes in the last object (past the last whole quad)
     with topk_amd.Engine(x) as eng:
         eng.init(topk_amd.TOPK_INIT_NORMALIZE | topk_amd.TOPK_INIT_PTA)
         y = eng.read_columns()

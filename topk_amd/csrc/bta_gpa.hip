@@ -290,47 +290,84 @@ __global__ void gpa_thr_kernel(const DselState *st, GpaStep g, float *thr) {
 
 // Keeps tuples whose upper bound reaches thr, adding the next term (col, w)
 // to the survivors' partial scores (col == nullptr: keep the score).
+// Stable compaction in two kernels over the same grid: workgroup b owns the
+// contiguous range [b*per, (b+1)*per) of the current survivors; the first
+// kernel counts what it keeps, the second writes its survivors at the sum of
+// the earlier workgroups' counts, in order (no global atomics).
+__device__ __forceinline__ bool gpa_keep(float s, const GpaStep &g, float thr) {
+    return g.last ? s >= thr : (double)s + g.hi + gpa_err(s, g.r) >= (double)thr;
+}
+
+__device__ __forceinline__ void gpa_range(uint32_t n, uint64_t &b, uint64_t &e) {
+    const uint64_t per = ((uint64_t)n + gridDim.x - 1) / gridDim.x;
+    b = (uint64_t)blockIdx.x * per;
+    e = b + per < n ? b + per : n;
+}
+
+__global__ __launch_bounds__(GPA_BS) void gpa_count_kernel(const float *__restrict__ s_in,
+                                                           const uint32_t *__restrict__ cnt_in, GpaStep g,
+                                                           const float *__restrict__ thr_p,
+                                                           uint32_t *__restrict__ bcount) {
+    const uint32_t n = *cnt_in;
+    const float thr = *thr_p;
+    uint64_t b, e;
+    gpa_range(n, b, e);
+    uint32_t c = 0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += GPA_BS) c += gpa_keep(s_in[i], g, thr) ? 1u : 0u;
+    __shared__ uint32_t ws[GPA_BS / 64];
+    c = wave_sum_u32(c);
+    if (lane_id() == 0) ws[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < GPA_BS / 64; w++) t += ws[w];
+        bcount[blockIdx.x] = t;
+    }
+}
+
 __global__ __launch_bounds__(GPA_BS) void gpa_prune_kernel(const uint32_t *__restrict__ ids_in,
                                                            const float *__restrict__ s_in,
                                                            const uint32_t *__restrict__ cnt_in, GpaStep g,
                                                            const float *__restrict__ thr_p,
+                                                           const uint32_t *__restrict__ bcount,
                                                            const float *__restrict__ col, float w,
                                                            uint32_t *__restrict__ ids_out, float *__restrict__ s_out,
                                                            uint32_t *__restrict__ cnt_out) {
     const uint32_t n = *cnt_in;
     const float thr = *thr_p;
-    const uint32_t lane = lane_id();
-    const uint64_t stride = (uint64_t)gridDim.x * GPA_BS;
-    // uniform trip count per wave so the ballot sees every lane
-    for (uint64_t i0 = (uint64_t)blockIdx.x * GPA_BS; i0 < n; i0 += stride) {
+    uint64_t b, e;
+    gpa_range(n, b, e);
+    __shared__ uint32_t wsum[GPA_BS / 64];
+    __shared__ uint32_t total;
+    // this workgroup's output offset: the earlier workgroups' counts
+    uint32_t part = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += GPA_BS) part += bcount[j];
+    block_excl_scan<GPA_BS>(part, wsum, &total);
+    uint32_t out = total;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *cnt_out = out + bcount[blockIdx.x];
+    for (uint64_t i0 = b; i0 < e; i0 += GPA_BS) {
         const uint64_t i = i0 + threadIdx.x;
         bool keep = false;
-        float s = 0.0f;
+        float sc = 0.0f;
         uint32_t id = 0;
-        if (i < n) {
-            s = s_in[i];
+        if (i < e) {
+            sc = s_in[i];
             id = ids_in[i];
-            keep = g.last ? s >= thr : (double)s + g.hi + gpa_err(s, g.r) >= (double)thr;
+            keep = gpa_keep(sc, g, thr);
         }
-        const uint64_t bal = __ballot(keep);
-        if (!bal) continue;
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(cnt_out, (uint32_t)__popcll(bal));
-        base = __shfl(base, 0, 64);
+        const uint32_t ex = block_excl_scan<GPA_BS>(keep ? 1u : 0u, wsum, &total);
         if (keep) {
-            const uint32_t pos = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-            float ns = s;
+            float ns = sc;
             if (col) {
                 const float p = col[id] * w;
-                ns = s + p;
+                ns = sc + p;
             }
-            ids_out[pos] = id;
-            s_out[pos] = ns;
+            ids_out[out + ex] = id;
+            s_out[out + ex] = ns;
         }
+        out += total;
     }
 }
-
-__global__ void zero_u32_kernel(uint32_t *p) { *p = 0; }
 
 struct GpaView {
     const float *s;
@@ -378,6 +415,7 @@ struct AuxState {
     float *gs[2] = {nullptr, nullptr};
     uint32_t *gcnt = nullptr;   // [2] survivor counts
     float *gthr = nullptr;
+    uint32_t *gbcount = nullptr;  // survivors kept per compaction workgroup
     uint64_t gcap = 0;
     uint32_t *cstat = nullptr;  // okeys: max[d] | min[d]
     std::vector<float> cmax, cmin;
@@ -402,6 +440,7 @@ void aux_free(topk_ctx *ctx) {
     hipFree(A->gs[1]);
     hipFree(A->gcnt);
     hipFree(A->gthr);
+    hipFree(A->gbcount);
     hipFree(A->cstat);
     delete A;
     ctx->aux = nullptr;
@@ -567,6 +606,7 @@ int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     }
     if (!A.gcnt) TK_HIP(hipMalloc(&A.gcnt, sizeof(uint32_t) * 2));
     if (!A.gthr) TK_HIP(hipMalloc(&A.gthr, sizeof(float)));
+    if (!A.gbcount) TK_HIP(hipMalloc(&A.gbcount, sizeof(uint32_t) * GPA_MAX_GRID));
     // remaining-term bounds after each term, FP64
     const uint32_t m = q.m;
     std::vector<GpaStep> steps(m);
@@ -593,11 +633,12 @@ int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     for (uint32_t j = 0; j < m; j++) {
         dsel_enqueue(s, ScoreSrc{A.gs[cur]}, A.gcnt + cur, n, 0, kreq, A.dsel);
         hipLaunchKernelGGL(gpa_thr_kernel, dim3(1), dim3(1), 0, s, A.dsel, steps[j], A.gthr);
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, A.gcnt + (cur ^ 1));
         const bool more = j + 1 < m;
+        hipLaunchKernelGGL(gpa_count_kernel, dim3(g), dim3(GPA_BS), 0, s, A.gs[cur], A.gcnt + cur, steps[j], A.gthr,
+                           A.gbcount);
         hipLaunchKernelGGL(gpa_prune_kernel, dim3(g), dim3(GPA_BS), 0, s, A.gids[cur], A.gs[cur], A.gcnt + cur,
-                           steps[j], A.gthr, more ? q.col[j + 1] : (const float *)nullptr, more ? q.w[j + 1] : 0.0f,
-                           A.gids[cur ^ 1], A.gs[cur ^ 1], A.gcnt + (cur ^ 1));
+                           steps[j], A.gthr, A.gbcount, more ? q.col[j + 1] : (const float *)nullptr,
+                           more ? q.w[j + 1] : 0.0f, A.gids[cur ^ 1], A.gs[cur ^ 1], A.gcnt + (cur ^ 1));
         cur ^= 1;
     }
     const FinalOut fo{reinterpret_cast<uint64_t *>(ctx->d_res + ctx->res.ids_off),

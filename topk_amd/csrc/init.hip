@@ -29,11 +29,23 @@ __global__ __launch_bounds__(256) void colminmax_kernel(const float *__restrict_
     const uint32_t m = blockIdx.y;
     const float *c = cols + (uint64_t)m * ld;
     float mx = 0.0f, mn = 3.40282347e+38f;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-        const float v = c[i];
+    auto take = [&](float v) {
         mx = mx < v ? v : mx;   // std::max(mmax, value)
         mn = v < mn ? v : mn;   // std::min(mmin, value)
+    };
+    // 16-byte loads over whole quads (columns start 32 KiB aligned), then the
+    // tail; the result is the same in any order (the cross-workgroup merge
+    // takes the extremes of monotone keys)
+    const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
+    const uint64_t n4 = n / 4;
+    for (uint64_t i = gid; i < n4; i += stride) {
+        const float4 v = reinterpret_cast<const float4 *>(c)[i];
+        take(v.x);
+        take(v.y);
+        take(v.z);
+        take(v.w);
     }
+    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) take(c[i]);
     for (int o = 32; o > 0; o >>= 1) {
         const float a = __shfl_xor(mx, o, 64), b = __shfl_xor(mn, o, 64);
         mx = mx < a ? a : mx;
@@ -51,25 +63,45 @@ __global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t 
     const float mx = okey_inv(kmax[m]), mn = okey_inv(kmin[m]);
     const float mm = mx - mn;
     float *c = cols + (uint64_t)m * ld;
-    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-        c[i] = (c[i] - mn) / mm;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
+    const uint64_t n4 = n / 4;
+    for (uint64_t i = gid; i < n4; i += stride) {
+        float4 v = reinterpret_cast<float4 *>(c)[i];
+        v.x = (v.x - mn) / mm;
+        v.y = (v.y - mn) / mm;
+        v.z = (v.z - mn) / mm;
+        v.w = (v.w - mn) / mm;
+        reinterpret_cast<float4 *>(c)[i] = v;
+    }
+    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) c[i] = (c[i] - mn) / mm;
 }
 
-__global__ void reorder_attr_kernel(float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Per object, its d values sorted descending across the columns
+// (gpu/reorder_attr.h:13-236).  A stable odd-even transposition network over
+// D register slots (D >= d; slots past d hold -inf and stay last): adjacent
+// pairs swap only when strictly out of order, so equal values (0 and -0)
+// keep their column order, as in an insertion sort.
+template <int D>
+__global__ __launch_bounds__(256) void reorder_attr_kernel(float *__restrict__ cols, uint64_t n, uint32_t d,
+                                                           uint64_t ld) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    float v[TOPK_MAX_DIMS];
-    for (uint32_t m = 0; m < d; m++) v[m] = cols[(uint64_t)m * ld + i];
-    for (uint32_t a = 1; a < d; a++) {  // insertion sort, descending
-        const float x = v[a];
-        int b = (int)a - 1;
-        while (b >= 0 && v[b] < x) {
-            v[b + 1] = v[b];
-            b--;
+    float v[D];
+#pragma unroll
+    for (int m = 0; m < D; m++) v[m] = (uint32_t)m < d ? cols[(uint64_t)m * ld + i] : -INFINITY;
+#pragma unroll
+    for (int r = 0; r < D; r++) {
+#pragma unroll
+        for (int a = r & 1; a + 1 < D; a += 2) {
+            const float x = v[a], y = v[a + 1];
+            const bool sw = x < y;
+            v[a] = sw ? y : x;
+            v[a + 1] = sw ? x : y;
         }
-        v[b + 1] = x;
     }
-    for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = v[m];
+#pragma unroll
+    for (int m = 0; m < D; m++)
+        if ((uint32_t)m < d) cols[(uint64_t)m * ld + i] = v[m];
 }
 
 __global__ void nonfinite_check_kernel(const float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld,
@@ -154,8 +186,15 @@ int topk_reorder_attr(topk_ctx *ctx) {
     ctx->store_gen++;
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, ctx->stream, ev0);
-    hipLaunchKernelGGL(reorder_attr_kernel, dim3((uint32_t)((ctx->n + 255) / 256)), dim3(256), 0, ctx->stream,
-                       ctx->cols, ctx->n, ctx->d, ctx->ld);
+    const dim3 grid((uint32_t)((ctx->n + 255) / 256));
+    if (ctx->d <= 8)
+        hipLaunchKernelGGL(reorder_attr_kernel<8>, grid, dim3(256), 0, ctx->stream, ctx->cols, ctx->n, ctx->d, ctx->ld);
+    else if (ctx->d <= 16)
+        hipLaunchKernelGGL(reorder_attr_kernel<16>, grid, dim3(256), 0, ctx->stream, ctx->cols, ctx->n, ctx->d,
+                           ctx->ld);
+    else
+        hipLaunchKernelGGL(reorder_attr_kernel<TOPK_MAX_DIMS>, grid, dim3(256), 0, ctx->stream, ctx->cols, ctx->n,
+                           ctx->d, ctx->ld);
     timing_end(ctx, ctx->stream, "reorder_attr", ev0);
     TK_HIP(hipGetLastError());
     TK_HIP(hipStreamSynchronize(ctx->stream));
