@@ -35,6 +35,9 @@ struct topk_hstore {
     hipStream_t copy = nullptr;
     hipEvent_t ready[2] = {nullptr, nullptr};
     topk_stream_report last{};
+    uint64_t *d_ids = nullptr;    // per-chunk answers of a batch's queries (device)
+    float *d_scores = nullptr;
+    uint64_t d_cap = 0;           // entries of d_ids / d_scores
 };
 
 using namespace topk;
@@ -117,6 +120,8 @@ int topk_hstore_destroy(topk_hstore *hs) {
     }
     if (hs->copy) (void)hipStreamDestroy(hs->copy);
     if (hs->host) (void)hipHostFree(hs->host);
+    (void)hipFree(hs->d_ids);
+    (void)hipFree(hs->d_scores);
     delete hs;
     return TOPK_OK;
 }
@@ -136,10 +141,23 @@ int topk_hstore_batch(topk_hstore *hs, uint32_t nq, const topk_query *qs, uint32
     }
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t bytes = 0;
+    uint32_t kmax = 1;
+    for (uint32_t q = 0; q < nq; q++) kmax = std::max(kmax, qs[q].k);
+    const uint64_t need = (uint64_t)nq * kmax;
+    if (need > hs->d_cap) {
+        (void)hipFree(hs->d_ids);
+        (void)hipFree(hs->d_scores);
+        hs->d_ids = nullptr;
+        hs->d_scores = nullptr;
+        hs->d_cap = 0;
+        TK_HIP(hipMalloc(&hs->d_ids, sizeof(uint64_t) * need));
+        TK_HIP(hipMalloc(&hs->d_scores, sizeof(float) * need));
+        hs->d_cap = need;
+    }
     // running best keys per query, best first
     std::vector<std::vector<uint64_t>> best(nq), merged(nq);
-    std::vector<uint64_t> ids(TOPK_MAX_K);
-    std::vector<float> sc(TOPK_MAX_K);
+    std::vector<uint64_t> ids(need);
+    std::vector<float> sc(need);
     int rc = issue_copy(hs, 0, 0, mask, bytes);
     if (!rc && hs->nchunks > 1) rc = issue_copy(hs, 1, 1, mask, bytes);
     for (uint64_t c = 0; c < hs->nchunks && !rc; c++) {
@@ -151,18 +169,34 @@ int topk_hstore_batch(topk_hstore *hs, uint32_t nq, const topk_query *qs, uint32
         X->n = std::min(hs->chunk, hs->n - i0);  // objects past n are never candidates
         X->store_gen++;
         rc = check_finite(X);
+        // the chunk's queries pipelined over the chunk store's lanes; a
+        // candidate overflow (reported by the sync) re-runs them exactly
+        for (uint32_t q = 0; q < nq && !rc; q++)
+            rc = topk_fte_async(X, qs[q].k, (uint8_t)qs[q].qq, qs[q].weights, qs[q].attr,
+                                hs->d_ids + (uint64_t)q * kmax, hs->d_scores + (uint64_t)q * kmax);
+        if (!rc) rc = topk_sync(X);
+        if (rc == TOPK_ERR_OVERFLOW) {
+            rc = TOPK_OK;
+            for (uint32_t q = 0; q < nq && !rc; q++) {
+                float thr;
+                rc = topk_fte(X, qs[q].k, (uint8_t)qs[q].qq, qs[q].weights, qs[q].attr,
+                              ids.data() + (uint64_t)q * kmax, sc.data() + (uint64_t)q * kmax, &thr);
+            }
+        } else if (!rc) {
+            rc = hip_status(hipMemcpy(ids.data(), hs->d_ids, sizeof(uint64_t) * need, hipMemcpyDeviceToHost));
+            if (!rc) rc = hip_status(hipMemcpy(sc.data(), hs->d_scores, sizeof(float) * need, hipMemcpyDeviceToHost));
+        }
         for (uint32_t q = 0; q < nq && !rc; q++) {
             const topk_query &Q = qs[q];
-            float thr;
-            rc = topk_fte(X, Q.k, (uint8_t)Q.qq, Q.weights, Q.attr, ids.data(), sc.data(), &thr);
-            if (rc) break;
+            const uint64_t *qi = ids.data() + (uint64_t)q * kmax;
+            const float *qsc = sc.data() + (uint64_t)q * kmax;
             const uint32_t got = (uint32_t)std::min<uint64_t>(Q.k, X->n);
             std::vector<uint64_t> &b = best[q], &m = merged[q];
             m.clear();
             size_t bi = 0;
             for (uint32_t j = 0; j < got || bi < b.size();) {
                 if (m.size() == Q.k) break;
-                const uint64_t kc = j < got ? rkey_from_okey(okey(sc[j]), (uint32_t)(ids[j] + i0)) : 0ull;
+                const uint64_t kc = j < got ? rkey_from_okey(okey(qsc[j]), (uint32_t)(qi[j] + i0)) : 0ull;
                 if (bi < b.size() && (j >= got || b[bi] > kc)) {
                     m.push_back(b[bi++]);
                 } else {
