@@ -1155,6 +1155,8 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                second ? "batch_tau_second" : (mode == BMODE_SEED ? "batch_tau_seed" : (mode == BMODE_LEARNED ? "batch_tau_learned" : "batch_tau_given")),
                evt);
     const uint32_t H = group_height(P.p2), G = P.p2 >> H;
+    // a query's group list and a group's partition counters live in LDS
+    if (G > BMASK_PAIRS) return TOPK_ERR_UNSUPPORTED;  // over 262144 partitions: use larger partitions
     const int NB = dims_class(ctx->d);
     // mask layout: query-major (one workgroup per query) spreads a full
     // batch over the CUs; group-major (one per group and query slice) keeps
