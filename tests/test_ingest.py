@@ -1,7 +1,8 @@
 """Data ingest on the CPU: the oracle's restatement of the reference's text
 loader (File<T>::count / read_scanf_t, input/File.h:107-128,233-295) pinned
-against exact rational arithmetic, and the library's host-side ingest calls
-(row count, text and column-file writers, column-file header)."""
+against exact rational arithmetic, the library's host-side ingest calls
+(row count, text and column-file writers, column-file header), and the
+argument checks of the host-memory store."""
 import os
 from fractions import Fraction
 
@@ -171,3 +172,18 @@ def test_text_read_needs_a_gpu(tmp_path):
     with pytest.raises(topk_amd.TopkError) as e:
         topk_amd.Engine.from_file(str(p))
     assert e.value.status == _lib.TOPK_ERR_NO_DEVICE
+
+
+def test_host_store_errors_without_a_gpu():
+    from conftest import gpu_available
+    L = topk_amd.lib()
+    import ctypes
+    h = ctypes.c_void_p()
+    assert L.topk_hstore_create(ctypes.byref(h), 0, 4, 0) == _lib.TOPK_ERR_INVALID      # n = 0
+    assert L.topk_hstore_create(ctypes.byref(h), 100, 33, 0) == _lib.TOPK_ERR_INVALID    # d > 32
+    assert L.topk_hstore_destroy(None) == _lib.TOPK_ERR_INVALID
+    assert L.topk_hstore_fte(None, 4, 1, None, None, None, None, None) == _lib.TOPK_ERR_INVALID
+    if not gpu_available():
+        with pytest.raises(topk_amd.TopkError) as e:
+            topk_amd.HostStore(1000, 4)
+        assert e.value.status == _lib.TOPK_ERR_NO_DEVICE
