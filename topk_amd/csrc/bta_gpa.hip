@@ -54,13 +54,20 @@ struct DselState {
     uint32_t hist[2048];
 };
 
+// Key sources: one key, or keys 4q..4q+3 with one 16-byte load (the base
+// pointer is 16-byte aligned when `quads` is used).
 struct U32Src {
     const uint32_t *k;
     __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return k[i]; }
+    __device__ __forceinline__ uint4 quad(uint64_t q) const { return reinterpret_cast<const uint4 *>(k)[q]; }
 };
 struct ScoreSrc {
     const float *s;
     __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return okey(s[i]); }
+    __device__ __forceinline__ uint4 quad(uint64_t q) const {
+        const float4 v = reinterpret_cast<const float4 *>(s)[q];
+        return make_uint4(okey(v.x), okey(v.y), okey(v.z), okey(v.w));
+    }
 };
 
 // rank = (cnt ? *cnt : base) - sub
@@ -76,17 +83,26 @@ __global__ void dsel_init_kernel(DselState *st, const uint32_t *cnt, uint64_t ba
 
 template <class Src>
 __global__ __launch_bounds__(DSEL_BS) void dsel_hist_kernel(Src src, const uint32_t *cnt, uint64_t n_host,
-                                                            DselState *st, int shift, int width) {
+                                                            DselState *st, int shift, int width, int quads) {
     __shared__ uint32_t h[2048];
     const uint32_t nb = 1u << width;
     for (uint32_t i = threadIdx.x; i < nb; i += DSEL_BS) h[i] = 0;
     __syncthreads();
     const uint64_t n = cnt ? *cnt : n_host;
     const uint32_t prefix = st->prefix, mask = st->mask;
-    for (uint64_t i = (uint64_t)blockIdx.x * DSEL_BS + threadIdx.x; i < n; i += (uint64_t)gridDim.x * DSEL_BS) {
-        const uint32_t k = src(i);
+    auto take = [&](uint32_t k) {
         if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & (nb - 1)], 1u);
+    };
+    const uint64_t gid = (uint64_t)blockIdx.x * DSEL_BS + threadIdx.x, stride = (uint64_t)gridDim.x * DSEL_BS;
+    const uint64_t n4 = quads ? n / 4 : 0;
+    for (uint64_t q = gid; q < n4; q += stride) {
+        const uint4 k = src.quad(q);
+        take(k.x);
+        take(k.y);
+        take(k.z);
+        take(k.w);
     }
+    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) take(src(i));
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += DSEL_BS)
         if (h[i]) atomicAdd(&st->hist[i], h[i]);
@@ -121,14 +137,14 @@ __global__ __launch_bounds__(64) void dsel_pick_kernel(DselState *st, int shift,
 
 template <class Src>
 static void dsel_enqueue(hipStream_t s, Src src, const uint32_t *cnt, uint64_t n_bound, uint64_t base, uint64_t sub,
-                         DselState *st) {
-    uint32_t g = (uint32_t)((n_bound + DSEL_BS - 1) / DSEL_BS);
+                         DselState *st, bool aligned) {
+    uint32_t g = (uint32_t)((n_bound / (aligned ? 4 : 1) + DSEL_BS - 1) / DSEL_BS);
     if (g > DSEL_MAX_GRID) g = DSEL_MAX_GRID;
     if (g < 1) g = 1;
     hipLaunchKernelGGL(dsel_init_kernel, dim3(1), dim3(256), 0, s, st, cnt, base, sub);
     for (int p = 0; p < DSEL_PASSES; p++) {
         hipLaunchKernelGGL((dsel_hist_kernel<Src>), dim3(g), dim3(DSEL_BS), 0, s, src, cnt, n_bound, st,
-                           DSEL_SHIFT[p], DSEL_WIDTH[p]);
+                           DSEL_SHIFT[p], DSEL_WIDTH[p], aligned ? 1 : 0);
         hipLaunchKernelGGL(dsel_pick_kernel, dim3(1), dim3(64), 0, s, st, DSEL_SHIFT[p], DSEL_WIDTH[p]);
     }
 }
@@ -510,7 +526,7 @@ int topk_radix_select_u32(topk_ctx *ctx, const uint32_t *d_keys, uint64_t n, uin
     hipStream_t s = ctx->stream;
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
-    dsel_enqueue(s, U32Src{d_keys}, nullptr, n, r, 0, A.dsel);
+    dsel_enqueue(s, U32Src{d_keys}, nullptr, n, r, 0, A.dsel, ((uintptr_t)d_keys & 15u) == 0);
     timing_end(ctx, s, "radix_select", ev0);
     TK_HIP(hipGetLastError());
     DselState h;
@@ -631,7 +647,7 @@ int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
                        A.gcnt);
     int cur = 0;
     for (uint32_t j = 0; j < m; j++) {
-        dsel_enqueue(s, ScoreSrc{A.gs[cur]}, A.gcnt + cur, n, 0, kreq, A.dsel);
+        dsel_enqueue(s, ScoreSrc{A.gs[cur]}, A.gcnt + cur, n, 0, kreq, A.dsel, true);  // hipMalloc'd: aligned
         hipLaunchKernelGGL(gpa_thr_kernel, dim3(1), dim3(1), 0, s, A.dsel, steps[j], A.gthr);
         const bool more = j + 1 < m;
         hipLaunchKernelGGL(gpa_count_kernel, dim3(g), dim3(GPA_BS), 0, s, A.gs[cur], A.gcnt + cur, steps[j], A.gthr,
