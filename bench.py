@@ -658,8 +658,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the sharded design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): a multi-GPU run would split "
+              "the store by object range and merge each query's per-GPU top-k", file=sys.stderr)
         sys.exit(2)
 
     import torch

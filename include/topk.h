@@ -62,7 +62,7 @@ typedef enum topk_status {
 
 typedef struct topk_ctx topk_ctx;
 
-/* Data distributions of the synthetic generator. */
+/* Data distributions of the synthetic generator (the benchmark configurations). */
 typedef enum topk_distr {
     TOPK_CORRELATED = 0,       /* u~U(0,1) per object, x = clip(u + N(0,0.05), 0, 1) */
     TOPK_INDEPENDENT = 1,      /* x ~ U[0,1) i.i.d. */

@@ -177,7 +177,7 @@ def test_fte_async_matches_sync(gpu):
 
 
 def test_fte_full_size_parity(gpu):
-    # BASELINE headline shape: N = 2^26, d = 8, k = 64, all distributions
+    # headline shape: N = 2^26, d = 8, k = 64, all distributions
     n, d = 1 << 26, 8
     for distr, m in ((topk_amd.INDEPENDENT, 8), (topk_amd.CORRELATED, 2), (topk_amd.ANTICORRELATED, 5)):
         x = topk_amd.generate(n, d, distr, distr, threads=16)

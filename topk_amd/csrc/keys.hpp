@@ -40,7 +40,7 @@ TK_HD uint32_t rkey_id(uint64_t k) { return 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFF
 TK_HD float rkey_score(uint64_t k) { return okey_inv((uint32_t)(k >> 32)); }
 
 // ---------------------------------------------------------------------------
-// Synthetic generator (BASELINE distributions).  Counter-based: the value of
+// Synthetic generator (the benchmark distributions).  Counter-based: the value of
 // (object i, attribute m) depends only on (seed, i, m), so any object range
 // can be generated independently, on the host or on the device.
 // ---------------------------------------------------------------------------
