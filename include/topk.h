@@ -51,7 +51,7 @@ typedef enum topk_status {
     TOPK_ERR_INVALID = 1,      /* bad argument (k, qq, attr id, n, d, null pointer) */
     TOPK_ERR_NOMEM = 2,        /* device or pinned-host allocation failed */
     TOPK_ERR_HIP = 3,          /* a HIP runtime call failed */
-    TOPK_ERR_UNSUPPORTED = 4,  /* request outside this build's limits */
+    TOPK_ERR_UNSUPPORTED = 4,  /* request outside this library's limits */
     TOPK_ERR_NONFINITE = 5,    /* the store holds NaN or infinity */
     TOPK_ERR_STATE = 6,        /* e.g. topk_pta before the partitions are built */
     TOPK_ERR_NO_DEVICE = 7,    /* no usable GPU */

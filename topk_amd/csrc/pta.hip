@@ -2,17 +2,19 @@
 //
 // Reference: cpu/PTA.h:88-267 (partition build: polar angles, sorted lists,
 // per-block threshold vectors) and cpu/PTA.h:357-422, 569-671 (scan with a
-// per-block bound, stop once the heap top beats it).  This build keeps the
+// per-block bound, stop once the heap top beats it).  This file keeps the
 // idea -- group similar objects, bound each group, skip groups that cannot
 // reach the running k-th score -- with its own partitioning and bound:
 //
 //  * layout (topk_init with TOPK_INIT_PTA): a k-d tree whose split planes
 //    come from a deterministic sample (median splits on the axis of largest
 //    spread, built on the host); every object is routed to its leaf on the
-//    GPU, objects are counting-sorted by leaf (ids ascending inside a leaf),
-//    the columns are gathered into partition order and each partition keeps
-//    per-attribute max/min, the max of its objects' attribute sums and the
-//    sum of its minima.
+//    GPU, objects are counting-sorted by leaf, then ordered inside each
+//    partition by attribute sum (largest first, ties by id); the columns are
+//    gathered into partition order and each partition keeps per-attribute
+//    max/min, the max of its objects' attribute sums and the sum of its
+//    minima, and each 256-object chunk the same statistics over itself and
+//    every later chunk of its partition (suffix rows).
 //  * bound: the largest weighted sum any point of the partition's box can
 //    reach under the attribute-sum limit (a fractional-knapsack LP, solved
 //    greedily in FP64), plus a margin covering binary32 rounding of a score,
@@ -20,8 +22,9 @@
 //  * query: bound keys for all partitions; the exact k-th key among the best
 //    partitions' objects (seed) is a proven lower bound tau of the global
 //    k-th key; the stream pass scores only partitions whose bound reaches
-//    tau's score (skip iff bound < tau strictly) and keeps keys >= tau; the
-//    FTE final select finishes.  Results equal topk_fte's.
+//    tau's score (skip iff bound < tau strictly), each up to its first
+//    suffix row whose bound falls below tau, and keeps keys >= tau; the FTE
+//    final select finishes.  Results equal topk_fte's.
 #include <string.h>
 
 #include <algorithm>
