@@ -43,7 +43,9 @@ namespace topk {
 constexpr int PTA_BS = 256;
 constexpr uint32_t PTA_TILE = 1024;          // objects per stream step (4 per lane)
 constexpr uint32_t PTA_SORT_MAX = 8192;      // partitions up to this size are sorted in LDS (larger: scatter order)
-constexpr uint32_t PTA_SEED_OBJS = 4096;     // objects of a seed partition that are scored
+constexpr uint32_t PTA_SEED_OBJS = 1024;     // leading objects (largest attribute sums) of a seed partition
+constexpr uint32_t PTA_MIN_SEED = 16;        // seed partitions at least
+constexpr uint32_t PTA_SEED_RANK = 1024;     // block maxima ranked in LDS by the seed kernel
 constexpr uint32_t PTA_MAX_SEED = 64;
 
 struct KdNode {
@@ -613,7 +615,8 @@ __global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L
     if (lane == 0) cut[p] = first;
 }
 
-// Block t scores (up to 4096 objects of) the partition holding the t-th
+// Block t scores the leading PTA_SEED_OBJS objects (largest attribute sums)
+// of the partition holding the t-th
 // largest block-maximum bound and keeps its top k; the last block to finish
 // takes the k-th largest of all seed keys as tau.
 template <int M>
@@ -625,8 +628,22 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
     __shared__ SelectScratch ss;
     __shared__ uint64_t pick;
     __shared__ uint32_t is_last, nvalid;
+    __shared__ uint64_t bm[PTA_SEED_RANK];
     const uint32_t tid = threadIdx.x;
-    if (tid < 64) {
+    if (tid == 0) pick = 0;
+    if (nb <= PTA_SEED_RANK) {
+        // the (t+1)-th largest block maximum: the nonzero key with exactly t
+        // larger ones (keys carry partition ids, so nonzero keys are distinct)
+        for (uint32_t i = tid; i < nb; i += PTA_BS) bm[i] = bmax[i];
+        __syncthreads();
+        for (uint32_t i = tid; i < nb; i += PTA_BS) {
+            const uint64_t v = bm[i];
+            if (!v) continue;
+            uint32_t r = 0;
+            for (uint32_t j = 0; j < nb; j++) r += bm[j] > v ? 1u : 0u;
+            if (r == blockIdx.x) pick = v;
+        }
+    } else if (tid < 64) {
         // (t+1)-th largest block maximum by wave-max rounds
         uint64_t best = 0;
         for (uint32_t r = 0; r <= blockIdx.x; r++) {
@@ -803,8 +820,8 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     const uint32_t nb = (L + PTA_BS - 1) / PTA_BS;
     hipLaunchKernelGGL(pta_bound_kernel, dim3(nb), dim3(PTA_BS), 0, s, b, L, P.psz, P.pmax, P.pmin, P.psum,
                        P.pminsum, P.pbound, P.bmax);
-    uint32_t T = (4 * k + P.psize - 1) / P.psize;
-    if (T < 4) T = 4;
+    uint32_t T = (4 * k + PTA_SEED_OBJS - 1) / PTA_SEED_OBJS;
+    if (T < PTA_MIN_SEED) T = PTA_MIN_SEED;
     if (T > PTA_MAX_SEED) T = PTA_MAX_SEED;
     if (T > nb) T = nb;
 #define TK_SEED(MM)                                                                                              \
