@@ -255,6 +255,57 @@ def learned_batched(eng, n, k, args):
     }
 
 
+def host_info():
+    """Logical CPUs and CPU model of the host the CPU baselines run on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def c1_bench(threads):
+    """The reference's CPU-runnable case: N = 2^20, d = 8, independent (seed
+    0), attributes {4, 5, 6, 7}, weight 1, k = 16 (validation/bench_cpu.h
+    row qq = 4).  The oracle's SIMD+threads TPAc restatement on the host's
+    cores (1 warm-up + 10 timed calls, mean and median, as cpu/AA.h:202) and
+    the GPU paths on the same store."""
+    import oracle
+    import topk_amd
+    n, d, k = 1 << 20, 8, 16
+    w = np.ones(d, np.float32)
+    attr = [4, 5, 6, 7]
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 0, threads=threads)
+    ref = oracle.tpac_threads(x, n, k, w, attr, threads)
+    ts = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        oracle.tpac_threads(x, n, k, w, attr, threads)
+        ts.append(time.perf_counter() - t0)
+    with topk_amd.Engine(x) as e:
+        e.fte(k, w, attr)
+        t0 = time.perf_counter()
+        for _ in range(100):
+            got = e.fte(k, w, attr)
+        fte_q = (time.perf_counter() - t0) / 100
+        e.init(topk_amd.TOPK_INIT_PTA)
+        e.pta(k, w, attr)
+        t0 = time.perf_counter()
+        for _ in range(100):
+            gp = e.pta(k, w, attr)
+        pta_q = (time.perf_counter() - t0) / 100
+    same = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]) and
+                np.array_equal(gp[0], ref[0]) and np.array_equal(gp[1], ref[1]))
+    return {"cpu_tpac_threads_ms_mean": float(np.mean(ts)) * 1e3, "cpu_tpac_threads_ms_median": float(np.median(ts)) * 1e3,
+            "cpu_threads": threads, "host": host_info(), "gpu_fte_ms": fte_q * 1e3, "gpu_pta_ms": pta_q * 1e3,
+            "gpu_matches_cpu": same}
+
+
 def cpu_baseline(x, n, k, threads, budget_s, held, gpu_check):
     """Oracle TPAc (SIMD + threads) on held-out queries within budget_s."""
     import oracle
@@ -275,7 +326,7 @@ def cpu_baseline(x, n, k, threads, budget_s, held, gpu_check):
             "sample": f"{len(times)} held-out queries of the benchmark (N={n}, d={D}, all 8 attributes, "
                       f"k={k}, same store), 1 warm-up, mean; oracle or_tpac_threads (AVX2 16-wide + "
                       f"{threads} threads, cpu/TPAc.h:256-358 restated)",
-            "median_s": float(np.median(times))}, mism
+            "median_s": float(np.median(times)), "host": host_info()}, mism
 
 
 def subset_queries(rng, count, d, ks):
@@ -358,6 +409,7 @@ def pta_bench(threads, cpu=True):
 def other_configs(threads):
     """The other workloads of the scope table, measured once each (one JSON
     line; the headline line is the default mode):
+    * C1: the reference's CPU-runnable case (c1_bench);
     * PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
       weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128;
     * batched: N = 2^25, d = 16 mixed (seed 4), 1024 queries per launch on
@@ -369,6 +421,7 @@ def other_configs(threads):
     import topk_amd
     ks = [16, 32, 64, 128, 256]
     res = {}
+    res["c1_n2e20_d8_m4_k16"] = c1_bench(threads)
     res["pta_n2e28_anticorrelated_k128"] = pta_bench(threads)
 
     n4 = 1 << 25
@@ -635,7 +688,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels"],
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels", "c1"],
                     default="learned",
                     help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
                          "ingest: data ingest alone; stream: a host-memory store streamed through HBM")
@@ -669,6 +722,9 @@ def main():
 
     if args.mode == "configs":
         other_configs(min(os.cpu_count() or 1, 16))
+        return
+    if args.mode == "c1":
+        print(json.dumps({"c1_n2e20_d8_m4_k16": c1_bench(16)}), flush=True)
         return
     if args.mode == "kernels":
         print(json.dumps({"kernels_n2e26_d8": kernels_bench(16)}), flush=True)

@@ -616,9 +616,10 @@ __global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L
 }
 
 // Block t scores the leading PTA_SEED_OBJS objects (largest attribute sums)
-// of the partition holding the t-th
-// largest block-maximum bound and keeps its top k; the last block to finish
-// takes the k-th largest of all seed keys as tau.
+// of the partition whose bound key is the (t+1)-th largest of bmax[0..nb)
+// (the partitions' own bound keys on small layouts, else the maxima of
+// blocks of PTA_BS partitions) and keeps its top k; the last block to
+// finish takes the k-th largest of all seed keys as tau.
 template <int M>
 __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_t *__restrict__ bmax, uint32_t nb,
                                                           const uint32_t *__restrict__ poff,
@@ -820,12 +821,18 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     const uint32_t nb = (L + PTA_BS - 1) / PTA_BS;
     hipLaunchKernelGGL(pta_bound_kernel, dim3(nb), dim3(PTA_BS), 0, s, b, L, P.psz, P.pmax, P.pmin, P.psum,
                        P.pminsum, P.pbound, P.bmax);
+    // seed partitions are ranked by their own bounds when there are few of
+    // them (so a small layout still seeds from several partitions), else by
+    // the block maxima
+    const bool own = L <= PTA_SEED_RANK;
+    const uint64_t *rkeys = own ? P.pbound : P.bmax;
+    const uint32_t nr = own ? L : nb;
     uint32_t T = (4 * k + PTA_SEED_OBJS - 1) / PTA_SEED_OBJS;
     if (T < PTA_MIN_SEED) T = PTA_MIN_SEED;
     if (T > PTA_MAX_SEED) T = PTA_MAX_SEED;
-    if (T > nb) T = nb;
+    if (T > nr) T = nr;
 #define TK_SEED(MM)                                                                                              \
-    hipLaunchKernelGGL(pta_seed_kernel<MM>, dim3(T), dim3(PTA_BS), 0, s, q, P.bmax, nb, P.poff, P.psz, P.pid, k, \
+    hipLaunchKernelGGL(pta_seed_kernel<MM>, dim3(T), dim3(PTA_BS), 0, s, q, rkeys, nr, P.poff, P.psz, P.pid, k, \
                        Ln.blk, Ln.st)
     TK_SWITCH_M(q.m, TK_SEED);
 #undef TK_SEED
