@@ -156,3 +156,20 @@ def test_reorder_attr_matches_oracle(gpu):
         ids, sc, _ = eng.fte(16, np.ones(d, np.float32), list(range(8)))
         rid, rsc, _ = oracle.fte_topk(ref, n, 16, np.ones(d, np.float32), list(range(8)))
         assert np.array_equal(ids, rid) and np.array_equal(sc, rsc)
+
+
+def test_small_layout_seeds_from_several_partitions(gpu):
+    """N = 2^20 (256 partitions) with a 4-of-8 attribute query: the seed
+    ranks the partitions' own bounds, so tau leaves a small candidate set
+    (one seed partition, ranked by block maxima, left a low tau and a 16 ms
+    final select)."""
+    n, d = 1 << 20, 8
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 0)
+    w = np.ones(d, np.float32)
+    with topk_amd.Engine(x) as eng:
+        eng.init(topk_amd.TOPK_INIT_PTA)
+        got = eng.pta(16, w, [4, 5, 6, 7])
+        st = eng.stats()
+        ref = oracle.fte_topk(x, n, 16, w, [4, 5, 6, 7])
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        assert st["candidates"] < 2000, st
