@@ -161,8 +161,8 @@ def test_reorder_attr_matches_oracle(gpu):
 def test_small_layout_seeds_from_several_partitions(gpu):
     """N = 2^20 (256 partitions) with a 4-of-8 attribute query: the seed
     ranks the partitions' own bounds, so tau leaves a small candidate set
-    (one seed partition, ranked by block maxima, left a low tau and a 16 ms
-    final select)."""
+    (seeding from the one partition the block maxima rank first made this
+    query's final select take 16 ms)."""
     n, d = 1 << 20, 8
     x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 0)
     w = np.ones(d, np.float32)
