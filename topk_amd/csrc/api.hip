@@ -97,14 +97,25 @@ __global__ void generate_kernel(uint64_t n, uint32_t d, uint64_t ld, int distr, 
     for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = v[m];
 }
 
-__global__ void nonfinite_kernel(const float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld,
-                                 unsigned int *__restrict__ flag) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+// Sets *flag if any of the n values of column blockIdx.y is NaN or
+// infinite: 16-byte loads over whole quads (columns start 32 KiB aligned),
+// then the tail.
+__global__ __launch_bounds__(256) void nonfinite_kernel(const float *__restrict__ cols, uint64_t n, uint64_t ld,
+                                                        unsigned int *__restrict__ flag) {
+    const float *c = cols + (uint64_t)blockIdx.y * ld;
+    const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
+    const uint64_t n4 = n / 4;
     bool bad = false;
-    for (uint32_t m = 0; m < d; m++)
-        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-            bad |= !isfinite(cols[(uint64_t)m * ld + i]);
+    for (uint64_t i = gid; i < n4; i += stride) {
+        const float4 v = reinterpret_cast<const float4 *>(c)[i];
+        bad |= !isfinite(v.x) || !isfinite(v.y) || !isfinite(v.z) || !isfinite(v.w);
+    }
+    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) bad |= !isfinite(c[i]);
     if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
+}
+
+void enqueue_finite_check(const topk_ctx *ctx, hipStream_t s, unsigned int *flag) {
+    hipLaunchKernelGGL(nonfinite_kernel, dim3(512, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, flag);
 }
 
 static uint32_t grid_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
@@ -142,8 +153,7 @@ int ctx_alloc(topk_ctx *ctx) {
 int check_finite(topk_ctx *ctx) {
     unsigned int *flag = reinterpret_cast<unsigned int *>(ctx->d_res + ctx->res.status_off);
     TK_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), ctx->stream));
-    hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, ctx->stream, ctx->cols, ctx->n, ctx->d, ctx->ld,
-                       flag);
+    enqueue_finite_check(ctx, ctx->stream, flag);
     TK_HIP(hipGetLastError());
     unsigned int h = 0;
     TK_HIP(hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

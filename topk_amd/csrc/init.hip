@@ -104,16 +104,6 @@ __global__ __launch_bounds__(256) void reorder_attr_kernel(float *__restrict__ c
         if ((uint32_t)m < d) cols[(uint64_t)m * ld + i] = v[m];
 }
 
-__global__ void nonfinite_check_kernel(const float *__restrict__ cols, uint64_t n, uint32_t d, uint64_t ld,
-                                       unsigned int *__restrict__ flag) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    bool bad = false;
-    for (uint32_t m = 0; m < d; m++)
-        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-            bad |= !isfinite(cols[(uint64_t)m * ld + i]);
-    if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1u);
-}
-
 static int normalize_store(topk_ctx *ctx) {
     hipStream_t s = ctx->stream;
     uint32_t *k = nullptr;
@@ -133,8 +123,7 @@ static int normalize_store(topk_ctx *ctx) {
         hipLaunchKernelGGL(normalize_kernel, dim3(2048, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
                            k + ctx->d);
         timing_end(ctx, s, "normalize", ev0);
-        hipLaunchKernelGGL(nonfinite_check_kernel, dim3(1024), dim3(256), 0, s, ctx->cols, ctx->n, ctx->d, ctx->ld,
-                           k + 2 * ctx->d);
+        enqueue_finite_check(ctx, s, k + 2 * ctx->d);
         e = hipGetLastError();
     }
     uint32_t flag = 0;

@@ -319,6 +319,8 @@ int hip_fail(hipError_t e, const char *call, const char *file, int line);
 // are set; rejects stores holding NaN or infinity (api.hip).
 int ctx_alloc(topk_ctx *ctx);
 int check_finite(topk_ctx *ctx);
+// Queues the NaN / infinity check of the store; *flag (device) is or-ed with 1.
+void enqueue_finite_check(const topk_ctx *ctx, hipStream_t s, unsigned int *flag);
 
 }  // namespace topk
 
