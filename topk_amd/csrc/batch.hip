@@ -14,7 +14,8 @@
 //     (score, id) key reaches it are candidates.
 //       - seed mode: a beam search down the bound tree picks the query's 4
 //         best-bounded partitions; the exact k-th key among their first
-//         16384 objects is a lower bound of the query's k-th key.
+//         4096 objects (the leading 1024 of each: largest attribute sums)
+//         is a lower bound of the query's k-th key.
 //       - learned mode: an MLP predicts the k-th score from (query, data
 //         sketch), lowered by a calibrated margin (learned.hip).
 //  2. mask (one workgroup per query): groups of 2^H partitions whose bound
@@ -61,9 +62,11 @@ constexpr uint32_t BATCH_MAX_Q = 16384;   // queries per call
 constexpr uint32_t BATCH_CAP = 32768;     // candidate keys per query
 constexpr uint32_t BATCH_MAX_GRID = 65536;  // persistent scoring workgroups
 constexpr int BFIN_BS = 512, BFIN_KPT = 16;  // lists up to 8192 keys are selected in registers, longer ones in memory
-constexpr int BSEED_BS = 512, BSEED_KPT = 32;
+constexpr int BSEED_BS = 512, BSEED_KPT = 8;
 constexpr uint32_t BSEED_T = 4;                          // seed partitions per query
-constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;    // objects a seed scores (16384)
+// objects a seed scores (4096): the leading objects of each seed partition,
+// which hold its largest attribute sums
+constexpr uint32_t BSEED_OBJS = BSEED_BS * BSEED_KPT;
 constexpr int BAUX_BS = 256;                             // bound / mask / compaction kernels
 constexpr uint32_t ITEM_MIN_OBJECTS = 2048;              // smallest object chunk of a work item
 constexpr int BMASK_BS = 256;                            // mask workgroup
