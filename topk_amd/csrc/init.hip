@@ -104,19 +104,25 @@ __global__ __launch_bounds__(256) void reorder_attr_kernel(float *__restrict__ c
         if ((uint32_t)m < d) cols[(uint64_t)m * ld + i] = v[m];
 }
 
+// Column maxima start at 0 (common/common.h:36), minima at FLT_MAX; the
+// non-finite flag at 0.
+__global__ void minmax_init_kernel(uint32_t *k, uint32_t d) {
+    const uint32_t m = threadIdx.x;
+    if (m < d) {
+        k[m] = okey(0.0f);
+        k[d + m] = okey(3.40282347e+38f);
+    }
+    if (m == 0) k[2 * d] = 0;
+}
+
 static int normalize_store(topk_ctx *ctx) {
     hipStream_t s = ctx->stream;
     uint32_t *k = nullptr;
     TK_HIP(hipMalloc(&k, sizeof(uint32_t) * (2 * ctx->d + 1)));
-    std::vector<uint32_t> init(2 * ctx->d + 1);
-    for (uint32_t m = 0; m < ctx->d; m++) {
-        init[m] = okey(0.0f);
-        init[ctx->d + m] = okey(3.40282347e+38f);
-    }
-    init[2 * ctx->d] = 0;
-    hipError_t e = hipMemcpyAsync(k, init.data(), sizeof(uint32_t) * init.size(), hipMemcpyHostToDevice, s);
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
+    hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(TOPK_MAX_DIMS), 0, s, k, ctx->d);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(colminmax_kernel, dim3(512, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
                            k + ctx->d);
