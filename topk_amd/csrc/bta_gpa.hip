@@ -153,7 +153,8 @@ static void dsel_enqueue(hipStream_t s, Src src, const uint32_t *cnt, uint64_t n
 // BTA
 // ---------------------------------------------------------------------------
 constexpr int BTA_BS = 256;
-constexpr uint32_t BTA_OBJS = 4096;
+constexpr uint32_t BTA_OBJS = 2048;  // objects per block (32 KiB of LDS per workgroup: 4 workgroups per CU)
+static_assert(BTA_OBJS >= 2 * TOPK_MAX_K, "a merge workgroup combines at least two runs of up to TOPK_MAX_K keys");
 
 __device__ __forceinline__ void cmpx(uint64_t *a, uint32_t lo, uint32_t hi, bool desc) {
     const uint64_t x = a[lo], y = a[hi];
