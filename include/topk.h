@@ -188,7 +188,7 @@ int topk_fte(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
 int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
              uint64_t *out_ids, float *out_scores, float *out_threshold);
 
-/* Bitonic top-k aggregation: per 4096-object block top-k in LDS, then a
+/* Bitonic top-k aggregation: per 2048-object block top-k in LDS, then a
  * merge (gpu/BTA.h:84-255).  Same results as topk_fte. */
 int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
              uint64_t *out_ids, float *out_scores, float *out_threshold);

@@ -9,12 +9,13 @@
 //    kernel picks the digit on the device, so a select is six launches and
 //    no host round trip.
 //  * BTA (gpu/BTA.h:57-73 local_sort2, :124-167 local_sort3, :207-255): a
-//    workgroup scores 4096 objects into LDS keys, bitonic-sorts them into
+//    workgroup scores BTA_OBJS (2048) objects into LDS keys (the reference
+//    blocks 4096), bitonic-sorts them into
 //    runs of K2 = next_pow2(k) with alternating direction, then halves the
 //    run count with the max-of-pairs step (a descending run against an
 //    ascending one keeps the K2 largest as a bitonic sequence) and a bitonic
 //    clean, until one run remains.  Block runs are merged the same way by
-//    tree launches of 4096 / K2 runs per workgroup.  Keys carry ids (the
+//    tree launches of BTA_OBJS / K2 runs per workgroup.  Keys carry ids (the
 //    reference keeps scores only, gpu/BTA.h:17).
 //  * GPA (gpu/GPA.h:161-227, gpu/prune_tuples.h:10-118): tuples (id,
 //    partial score) over the query's terms in order.  After term j the k-th
@@ -164,7 +165,7 @@ __device__ __forceinline__ void cmpx(uint64_t *a, uint32_t lo, uint32_t hi, bool
     }
 }
 
-// Bitonic sort of every run of K2 keys of a[0..4096); run r ends descending
+// Bitonic sort of every run of K2 keys of a[0..BTA_OBJS); run r ends descending
 // when r is even and ascending when r is odd.
 __device__ void bta_sort_runs(uint64_t *a, uint32_t K2) {
     for (uint32_t size = 2; size <= K2; size <<= 1)
@@ -205,7 +206,7 @@ __device__ uint64_t *bta_merge_runs(uint64_t *src, uint64_t *dst, uint32_t nruns
     return src;
 }
 
-// One workgroup per 4096 objects: scores -> keys -> the block's K2 best.
+// One workgroup per BTA_OBJS objects: scores -> keys -> the block's K2 best.
 __global__ __launch_bounds__(BTA_BS) void bta_block_kernel(QCols q, uint64_t n, uint32_t K2,
                                                            uint64_t *__restrict__ runs) {
     __shared__ uint64_t a[BTA_OBJS], b[BTA_OBJS];
