@@ -668,10 +668,6 @@ static int read_col_header(int fd, ColHeader &h, uint64_t fsize) {
     return TOPK_OK;
 }
 
-static bool is_col_file(const MappedFile &f) {
-    return f.size >= sizeof(ColHeader) && memcmp(f.map, COL_MAGIC, 8) == 0;
-}
-
 static void fill_report(topk_ingest_report *rep, int format, uint64_t bytes, uint64_t lines, uint64_t rows,
                         uint32_t fields, uint64_t slow, double ms_read, double ms_parse, double ms_total) {
     if (!rep) return;
@@ -703,6 +699,83 @@ static int check_device_ingest() {
     int cnt = 0;
     if (hipGetDeviceCount(&cnt) != hipSuccess || cnt < 1) return TOPK_ERR_NO_DEVICE;
     return TOPK_OK;
+}
+
+// A file descriptor closed on scope exit.
+struct FileFd {
+    int fd = -1;
+    ~FileFd() {
+        if (fd >= 0) ::close(fd);
+    }
+};
+
+// col = the file starts with the column-file magic (else it is read as text).
+static int sniff_format(const char *path, bool &col) {
+    FileFd f;
+    f.fd = ::open(path, O_RDONLY);
+    if (f.fd < 0) return TOPK_ERR_IO;
+    char m[8] = {0};
+    col = ::pread(f.fd, m, sizeof(m), 0) == (ssize_t)sizeof(m) && memcmp(m, COL_MAGIC, 8) == 0;
+    return TOPK_OK;
+}
+
+// Column file -> the store of ctx (first n objects, first d attributes).
+static int load_colfile(topk_ctx *ctx, const char *path, uint64_t n, uint32_t d, topk_ingest_report *rep,
+                        std::chrono::steady_clock::time_point t0) {
+    FileFd f;
+    f.fd = ::open(path, O_RDONLY);
+    if (f.fd < 0) return TOPK_ERR_IO;
+    struct stat st;
+    if (fstat(f.fd, &st) != 0) return TOPK_ERR_IO;
+    ColHeader h;
+    int rc = read_col_header(f.fd, h, (uint64_t)st.st_size);
+    if (rc) return rc;
+    if (d == 0) d = h.d;
+    if (n == 0) n = h.n;
+    if (d > h.d || n > h.n || n >= 0xFFFFFFFFull) return TOPK_ERR_FORMAT;
+    ctx->n = n;
+    ctx->d = d;
+    rc = ctx_alloc(ctx);
+    if (!rc) {
+        Uploader up;
+        rc = up.init(ctx->stream);
+        int slot = 0;
+        for (uint32_t m = 0; m < d && !rc; m++)
+            rc = up.copy(f.fd, h.data_offset + (uint64_t)m * h.n * 4, n * 4,
+                         reinterpret_cast<char *>(ctx->cols + (uint64_t)m * ctx->ld), slot);
+    }  // drains the copies
+    const double ms_read = ms_since(t0);
+    if (!rc) rc = check_finite(ctx);
+    if (!rc) fill_report(rep, 1, (uint64_t)st.st_size, 0, n, d, 0, ms_read, 0, ms_since(t0));
+    return rc;
+}
+
+// Text file -> uploaded, indexed and parsed into the store of ctx.
+static int load_text(topk_ctx *ctx, const char *path, uint64_t n, uint32_t d, topk_ingest_report *rep,
+                     std::chrono::steady_clock::time_point t0) {
+    hipStream_t ls;
+    int rc = IK_ST(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+    if (rc) return rc;
+    {
+        CsvText t;  // released (its stream drained) before the stream is destroyed
+        rc = t.load(path, ls);
+        if (!rc) rc = csv_shape(t, n, d);
+        if (!rc) {
+            ctx->n = n;
+            ctx->d = d;
+            rc = ctx_alloc(ctx);
+        }
+        uint64_t slow = 0;
+        double ms_parse = 0;
+        if (!rc) rc = IK_ST(hipStreamSynchronize(ctx->stream));  // the store is zeroed first
+        if (!rc) rc = t.parse(n, d, ctx->cols, ctx->ld, 1, slow, ms_parse);
+        if (!rc) rc = check_finite(ctx);
+        if (!rc)
+            fill_report(rep, 0, t.f.size, t.nl + (t.f.size && t.f.map[t.f.size - 1] != '\n' ? 1 : 0), n, d, slow,
+                        t.ms_read, t.ms_index + ms_parse, ms_since(t0));
+    }
+    (void)hipStreamDestroy(ls);
+    return rc;
 }
 
 extern "C" {
@@ -822,11 +895,13 @@ int topk_colfile_write(const char *path, const float *data, uint64_t n, uint32_t
 
 int topk_colfile_info(const char *path, uint64_t *n, uint32_t *d) {
     if (!path || !n || !d) return TOPK_ERR_INVALID;
-    MappedFile f;
-    int rc = f.open(path);
-    if (rc) return rc;
+    FileFd f;
+    f.fd = ::open(path, O_RDONLY);
+    if (f.fd < 0) return TOPK_ERR_IO;
+    struct stat st;
+    if (fstat(f.fd, &st) != 0) return TOPK_ERR_IO;
     ColHeader h;
-    rc = read_col_header(f.fd, h, f.size);
+    const int rc = read_col_header(f.fd, h, (uint64_t)st.st_size);
     if (rc) return rc;
     *n = h.n;
     *d = h.d;
@@ -840,63 +915,12 @@ int topk_ctx_create_file(topk_ctx **out, const char *path, uint64_t n, uint32_t 
     int rc = check_device_ingest();
     if (rc) return rc;
     const auto t0 = std::chrono::steady_clock::now();
+    bool col = false;
+    rc = sniff_format(path, col);
+    if (rc) return rc;
     topk_ctx *ctx = new topk_ctx();
     hipGetDevice(&ctx->device);
-    CsvText *t = new CsvText();
-    rc = t->f.open(path);
-    if (!rc && is_col_file(t->f)) {
-        ColHeader h;
-        rc = read_col_header(t->f.fd, h, t->f.size);
-        if (!rc) {
-            if (d == 0) d = h.d;
-            if (n == 0) n = h.n;
-            if (d > h.d || n > h.n || n >= 0xFFFFFFFFull) rc = TOPK_ERR_FORMAT;
-        }
-        if (!rc) {
-            ctx->n = n;
-            ctx->d = d;
-            rc = ctx_alloc(ctx);
-        }
-        if (!rc) {
-            Uploader up;
-            rc = up.init(ctx->stream);
-            int slot = 0;
-            for (uint32_t m = 0; m < d && !rc; m++)
-                rc = up.copy(t->f.fd, h.data_offset + (uint64_t)m * h.n * 4, n * 4,
-                             reinterpret_cast<char *>(ctx->cols + (uint64_t)m * ctx->ld), slot);
-        }
-        const double ms_read = ms_since(t0);
-        if (!rc) rc = check_finite(ctx);
-        if (!rc) fill_report(rep, 1, t->f.size, 0, n, d, 0, ms_read, 0, ms_since(t0));
-    } else if (!rc) {
-        // text: reopened by load (the mapping above only told the formats apart)
-        delete t;
-        t = new CsvText();
-        hipStream_t ls;
-        rc = IK_ST(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
-        if (!rc) {
-            rc = t->load(path, ls);
-            if (!rc) rc = csv_shape(*t, n, d);
-            if (!rc) {
-                ctx->n = n;
-                ctx->d = d;
-                rc = ctx_alloc(ctx);
-            }
-            uint64_t slow = 0;
-            double ms_parse = 0;
-            if (!rc) rc = IK_ST(hipStreamSynchronize(ctx->stream));  // the store is zeroed first
-            if (!rc) rc = t->parse(n, d, ctx->cols, ctx->ld, 1, slow, ms_parse);
-            if (!rc) rc = check_finite(ctx);
-            if (!rc)
-                fill_report(rep, 0, t->f.size, t->nl + (t->f.size && t->f.map[t->f.size - 1] != '\n' ? 1 : 0), n,
-                            d, slow, t->ms_read, t->ms_index + ms_parse, ms_since(t0));
-            hipStreamSynchronize(ls);
-            delete t;
-            t = nullptr;
-            hipStreamDestroy(ls);
-        }
-    }
-    delete t;
+    rc = col ? load_colfile(ctx, path, n, d, rep, t0) : load_text(ctx, path, n, d, rep, t0);
     if (rc) {
         if (ctx->stream) topk_ctx_destroy(ctx);
         else delete ctx;
