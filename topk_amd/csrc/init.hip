@@ -51,9 +51,21 @@ __global__ __launch_bounds__(256) void colminmax_kernel(const float *__restrict_
         mx = mx < a ? a : mx;
         mn = b < mn ? b : mn;
     }
+    // one pair of global atomics per workgroup, on the monotone keys
+    __shared__ uint32_t wmx[4], wmn[4];
     if (lane_id() == 0) {
-        atomicMax(&kmax[m], okey(mx));
-        atomicMin(&kmin[m], okey(mn));
+        wmx[threadIdx.x / 64] = okey(mx);
+        wmn[threadIdx.x / 64] = okey(mn);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a = wmx[0], b = wmn[0];
+        for (int w = 1; w < 4; w++) {
+            a = wmx[w] > a ? wmx[w] : a;
+            b = wmn[w] < b ? wmn[w] : b;
+        }
+        atomicMax(&kmax[m], a);
+        atomicMin(&kmin[m], b);
     }
 }
 
