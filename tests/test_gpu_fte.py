@@ -176,18 +176,26 @@ def test_fte_async_matches_sync(gpu):
             assert np.array_equal(sc_h[i * k:(i + 1) * k], sc)
 
 
-def test_fte_full_size_parity(gpu):
-    # headline shape: N = 2^26, d = 8, k = 64, all distributions
+@pytest.mark.parametrize("distr,seed", [(topk_amd.INDEPENDENT, 0), (topk_amd.CORRELATED, 1),
+                                        (topk_amd.ANTICORRELATED, 2)])
+def test_fte_full_size_sweep(gpu, distr, seed):
+    """The plain-FTE benchmark sweep at its full size: N = 2^26, d = 8, the
+    last m = 2..8 attributes (validation/bench_cpu.h:41-44, QM=0 rows),
+    weights 1.0, k in {16, 32, 64, 128}, every point against the oracle's
+    SIMD+threads TPAc restatement (cpu/TPAc.h:256-358) on the same bits."""
     n, d = 1 << 26, 8
-    for distr, m in ((topk_amd.INDEPENDENT, 8), (topk_amd.CORRELATED, 2), (topk_amd.ANTICORRELATED, 5)):
-        x = topk_amd.generate(n, d, distr, distr, threads=16)
-        attr = list(range(d - m, d))
-        w = np.ones(d, np.float32)
-        with topk_amd.Engine.generated(n, d, distr, distr) as eng:
-            ids, sc, thr = eng.fte(64, w, attr)
-        rid, rsc, rthr = oracle.tpac_threads(x, n, 64, w, attr, 16)
-        assert np.array_equal(ids, rid) and np.array_equal(sc, rsc) and thr == rthr
-        del x
+    x = topk_amd.generate(n, d, distr, seed, threads=16)
+    w = np.ones(d, np.float32)
+    with topk_amd.Engine.generated(n, d, distr, seed) as eng:
+        for m in range(2, d + 1):
+            attr = list(range(d - m, d))
+            for k in (16, 32, 64, 128):
+                ids, sc, thr = eng.fte(k, w, attr)
+                rid, rsc, rthr = oracle.tpac_threads(x, n, k, w, attr, 16)
+                assert np.array_equal(ids, rid), (distr, m, k)
+                assert np.array_equal(sc.view(np.uint32), rsc.view(np.uint32)), (distr, m, k)
+                assert np.float32(thr).tobytes() == np.float32(rthr).tobytes(), (distr, m, k)
+    del x
 
 
 def test_fte_exact_seed_mode_parity(gpu):

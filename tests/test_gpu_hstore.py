@@ -8,6 +8,7 @@ import pytest
 import oracle
 import topk_amd
 from topk_amd import _lib
+from topk_amd.engine import _ptr
 
 pytestmark = pytest.mark.gpu
 
@@ -109,3 +110,31 @@ def test_full_size_host_store(gpu):
         assert _same(hs.fte(64, w, list(range(d))), e.fte(64, w, list(range(d))))
         rep = hs.report()
         assert rep["chunks"] == 4 and rep["bytes"] == 4 * n * d
+
+
+def test_weights_read_only_at_queried_attributes(gpu):
+    # weights are indexed by attribute id; an array sized to max(attr) + 1 is
+    # enough, as for topk_fte (entries past it are never read)
+    n, d = 50_000, 8
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 4)
+    at = np.array([2, 0], np.uint8)
+    w_short = np.array([0.5, 9.0, 0.25], np.float32)   # attribute 1 is not queried
+    k = 20
+    ids = np.empty(k, np.uint64)
+    sc = np.empty(k, np.float32)
+    thr = np.zeros(1, np.float32)
+    with _host(x, 1 << 14) as hs:
+        _lib.check(_lib.lib().topk_hstore_fte(hs._h, k, 2, _ptr(w_short),
+                                                  _ptr(at), _ptr(ids),
+                                                  _ptr(sc), _ptr(thr)),
+                       "topk_hstore_fte")
+        w_full = np.zeros(d, np.float32)
+        w_full[[0, 2]] = [0.5, 0.25]
+        ri, rs, rt = oracle.fte_topk(x, n, k, w_full, [2, 0])
+        assert _same((ids, sc, float(thr[0])), (ri, rs, rt))
+        bad = np.array([9], np.uint8)
+        with pytest.raises(topk_amd.TopkError):
+            _lib.check(_lib.lib().topk_hstore_fte(hs._h, k, 1, _ptr(w_short),
+                                                      _ptr(bad), _ptr(ids),
+                                                      _ptr(sc), _ptr(thr)),
+                           "topk_hstore_fte")

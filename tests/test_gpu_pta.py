@@ -136,12 +136,40 @@ def test_normalize_matches_oracle(gpu, n):
         _check_pta(eng, ref, n, 20, np.ones(d, np.float32), [0, 1, 2, 3, 4])
 
 
-def test_normalize_constant_column_is_nonfinite(gpu):
-    x = np.ones((2, 1000), np.float32)
+@pytest.mark.parametrize("case", ["constant", "range_overflow"])
+def test_normalize_failure_leaves_store_untouched(gpu, case):
+    n, d = 1000, 3
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 9)
+    if case == "constant":
+        x[1] = 0.75  # max - min == 0: every value would be NaN
+    else:
+        x[2, 5], x[2, 7] = 3.0e38, -3.0e38  # max - min overflows to infinity
     with topk_amd.Engine(x) as eng:
         with pytest.raises(topk_amd.TopkError) as e:
             eng.init(topk_amd.TOPK_INIT_NORMALIZE)
         assert e.value.status == 5
+        # nothing was written: the store and later queries are the originals
+        assert np.array_equal(eng.read_columns().view(np.uint32), x.view(np.uint32))
+        w = np.ones(d, np.float32)
+        ids, sc, _ = eng.fte(10, w, [0, 1])
+        rid, rsc, _ = oracle.fte_topk(x, n, 10, w, [0, 1])
+        assert np.array_equal(ids, rid) and np.array_equal(sc, rsc)
+
+
+def test_normalize_zero_sign_follows_first_zero(gpu):
+    # The column minimum is zero with both signs present: the reference's
+    # sequential std::min keeps the first zero it meets (common/common.h:29-60),
+    # which decides the sign of the normalised zeros.
+    n = 70_001
+    x = topk_amd.generate(n, 2, topk_amd.INDEPENDENT, 12) + np.float32(0.5)
+    x[0, 40_000], x[0, 50_000] = 0.0, -0.0   # +0 first
+    x[1, 123], x[1, 60_000] = -0.0, 0.0      # -0 first
+    with topk_amd.Engine(x) as eng:
+        eng.init(topk_amd.TOPK_INIT_NORMALIZE)
+        y = eng.read_columns()
+    ref = oracle.normalize(x, n)
+    assert np.signbit(ref[0, 50_000]) and not np.signbit(ref[1, 123])
+    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
 
 
 def test_reorder_attr_matches_oracle(gpu):

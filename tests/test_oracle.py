@@ -145,6 +145,16 @@ def test_normalize_closed_form():
     assert np.array_equal(y[1], np.array([0.0, 0.5, 0.25], np.float32))
 
 
+def test_normalize_zero_sign_known_answer():
+    # std::min(mmin, v) keeps mmin when v == mmin, so the first zero met fixes
+    # the minimum's sign: (-0 - +0) / 1 = -0, while (-0 - -0) = +0.
+    x = np.array([[0.0, -0.0, 1.0], [-0.0, 0.0, 2.0]], dtype=np.float32)
+    y = oracle.normalize(x, 3)
+    assert np.signbit(y[0, 1]) and not np.signbit(y[0, 0])
+    assert not np.signbit(y[1, 0]) and not np.signbit(y[1, 1])
+    assert np.array_equal(y[:, 2], np.ones(2, np.float32))
+
+
 def test_reorder_attr_invariants():
     n, d = 3000, 8
     x = RNG.random((d, n), dtype=np.float32)

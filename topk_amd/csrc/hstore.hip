@@ -240,8 +240,13 @@ int topk_hstore_fte(topk_hstore *hs, uint32_t k, uint8_t qq, const float *weight
     topk_query q{};
     q.k = k;
     q.qq = qq;
-    for (uint32_t j = 0; j < qq; j++) q.attr[j] = attr[j];
-    for (uint32_t a = 0; a < hs->d; a++) q.weights[a] = weights[a];
+    // weights are indexed by attribute id and read only at the queried ids
+    // (as topk_fte reads them), so an array sized to max(attr) + 1 suffices
+    for (uint32_t j = 0; j < qq; j++) {
+        if (attr[j] >= hs->d) return TOPK_ERR_INVALID;
+        q.attr[j] = attr[j];
+        q.weights[attr[j]] = weights[attr[j]];
+    }
     return topk_hstore_batch(hs, 1, &q, k, out_ids, out_scores, out_thr);
 }
 

@@ -664,7 +664,9 @@ static int read_col_header(int fd, ColHeader &h, uint64_t fsize) {
     if (::pread(fd, &h, sizeof(h), 0) != (ssize_t)sizeof(h)) return TOPK_ERR_FORMAT;
     if (memcmp(h.magic, COL_MAGIC, 8) != 0 || h.version != 1 || h.d < 1 || h.d > TOPK_MAX_DIMS || h.n < 1)
         return TOPK_ERR_FORMAT;
-    if (h.data_offset < sizeof(h) || h.data_offset + h.n * h.d * 4 > fsize) return TOPK_ERR_FORMAT;
+    // (no 64-bit wrap for a corrupt header: n is checked against what the file can hold)
+    if (h.data_offset < sizeof(h) || h.data_offset > fsize || h.n > (fsize - h.data_offset) / (4ull * h.d))
+        return TOPK_ERR_FORMAT;
     return TOPK_OK;
 }
 
@@ -688,6 +690,7 @@ static int csv_shape(const CsvText &t, uint64_t &n, uint32_t &d) {
     if (d != t.fields) return TOPK_ERR_FORMAT;
     if (n == 0) n = t.rows;
     if (n == 0 || n > t.rows) return TOPK_ERR_FORMAT;
+    if (n >= 0xFFFFFFFFull) return TOPK_ERR_UNSUPPORTED;  // as topk_ctx_create: ids are 32-bit
     return TOPK_OK;
 }
 

@@ -23,57 +23,74 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
               float *d_scores, float *d_thr, uint32_t *d_status);
 
 // Per column max (starting at 0) and min (starting at FLT_MAX) as monotone
-// keys, so atomicMax / atomicMin on uint32 implement float max / min.
+// keys, so atomicMax / atomicMin on uint32 implement float max / min, and
+// the index of the column's first zero (either sign).  The monotone keys
+// order -0 below +0, where the reference's sequential std::min keeps
+// whichever zero it meets first; when the minimum is a zero, the host takes
+// the sign of the first zero instead (nothing else depends on the order).
 __global__ __launch_bounds__(256) void colminmax_kernel(const float *__restrict__ cols, uint64_t n, uint64_t ld,
-                                                        uint32_t *__restrict__ kmax, uint32_t *__restrict__ kmin) {
+                                                        uint32_t *__restrict__ kmax, uint32_t *__restrict__ kmin,
+                                                        unsigned long long *__restrict__ zfirst) {
     const uint32_t m = blockIdx.y;
     const float *c = cols + (uint64_t)m * ld;
     float mx = 0.0f, mn = 3.40282347e+38f;
-    auto take = [&](float v) {
+    uint64_t z = ~0ull;
+    auto take = [&](float v, uint64_t i) {
         mx = mx < v ? v : mx;   // std::max(mmax, value)
         mn = v < mn ? v : mn;   // std::min(mmin, value)
+        z = (v == 0.0f && i < z) ? i : z;
     };
     // 16-byte loads over whole quads (columns start 32 KiB aligned), then the
     // tail; the result is the same in any order (the cross-workgroup merge
-    // takes the extremes of monotone keys)
+    // takes the extremes of monotone keys and the smallest zero index)
     const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
     const uint64_t n4 = n / 4;
     for (uint64_t i = gid; i < n4; i += stride) {
         const float4 v = reinterpret_cast<const float4 *>(c)[i];
-        take(v.x);
-        take(v.y);
-        take(v.z);
-        take(v.w);
+        take(v.x, 4 * i);
+        take(v.y, 4 * i + 1);
+        take(v.z, 4 * i + 2);
+        take(v.w, 4 * i + 3);
     }
-    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) take(c[i]);
+    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) take(c[i], i);
     for (int o = 32; o > 0; o >>= 1) {
         const float a = __shfl_xor(mx, o, 64), b = __shfl_xor(mn, o, 64);
         mx = mx < a ? a : mx;
         mn = b < mn ? b : mn;
     }
-    // one pair of global atomics per workgroup, on the monotone keys
+    z = wave_min_u64(z);
+    // one global atomic per statistic and workgroup, on the monotone keys
     __shared__ uint32_t wmx[4], wmn[4];
+    __shared__ uint64_t wz[4];
     if (lane_id() == 0) {
         wmx[threadIdx.x / 64] = okey(mx);
         wmn[threadIdx.x / 64] = okey(mn);
+        wz[threadIdx.x / 64] = z;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = wmx[0], b = wmn[0];
+        uint64_t zz = wz[0];
         for (int w = 1; w < 4; w++) {
             a = wmx[w] > a ? wmx[w] : a;
             b = wmn[w] < b ? wmn[w] : b;
+            zz = wz[w] < zz ? wz[w] : zz;
         }
         atomicMax(&kmax[m], a);
         atomicMin(&kmin[m], b);
+        if (zz != ~0ull) atomicMin(&zfirst[m], (unsigned long long)zz);
     }
 }
 
-__global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t ld, const uint32_t *__restrict__ kmax,
-                                 const uint32_t *__restrict__ kmin) {
+// Per column minimum and range (max - min), computed on the host.
+struct NormArgs {
+    float mn[TOPK_MAX_DIMS];
+    float mm[TOPK_MAX_DIMS];
+};
+
+__global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t ld, NormArgs a) {
     const uint32_t m = blockIdx.y;
-    const float mx = okey_inv(kmax[m]), mn = okey_inv(kmin[m]);
-    const float mm = mx - mn;
+    const float mn = a.mn[m], mm = a.mm[m];
     float *c = cols + (uint64_t)m * ld;
     const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
     const uint64_t n4 = n / 4;
@@ -116,40 +133,62 @@ __global__ __launch_bounds__(256) void reorder_attr_kernel(float *__restrict__ c
         if ((uint32_t)m < d) cols[(uint64_t)m * ld + i] = v[m];
 }
 
-// Column maxima start at 0 (common/common.h:36), minima at FLT_MAX; the
-// non-finite flag at 0.
-__global__ void minmax_init_kernel(uint32_t *k, uint32_t d) {
+// Column maxima start at 0 (common/common.h:36), minima at FLT_MAX, first
+// zero indices at "none".
+__global__ void minmax_init_kernel(uint32_t *k, unsigned long long *z, uint32_t d) {
     const uint32_t m = threadIdx.x;
     if (m < d) {
         k[m] = okey(0.0f);
         k[d + m] = okey(3.40282347e+38f);
+        z[m] = ~0ull;
     }
-    if (m == 0) k[2 * d] = 0;
 }
 
+// The store holds only finite values (checked when it is built), and
+// mx >= every value >= mn, so 0 <= x - mn <= mx - mn = mm for every x of a
+// column (binary32 subtraction is monotone): every (x - mn) / mm is finite
+// exactly when mm is finite and > 0.  The ranges are checked on the host
+// before anything is written, so a column that cannot be normalised
+// (constant, or a range past FLT_MAX) leaves the whole store untouched.
 static int normalize_store(topk_ctx *ctx) {
     hipStream_t s = ctx->stream;
-    uint32_t *k = nullptr;
-    TK_HIP(hipMalloc(&k, sizeof(uint32_t) * (2 * ctx->d + 1)));
+    const uint32_t d = ctx->d;
+    unsigned long long *z = nullptr;
+    TK_HIP(hipMalloc(&z, sizeof(unsigned long long) * d + sizeof(uint32_t) * 2 * d));
+    uint32_t *k = reinterpret_cast<uint32_t *>(z + d);
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
-    hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(TOPK_MAX_DIMS), 0, s, k, ctx->d);
+    hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(TOPK_MAX_DIMS), 0, s, k, z, d);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(colminmax_kernel, dim3(512, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
-                           k + ctx->d);
-        hipLaunchKernelGGL(normalize_kernel, dim3(2048, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k,
-                           k + ctx->d);
-        timing_end(ctx, s, "normalize", ev0);
-        enqueue_finite_check(ctx, s, k + 2 * ctx->d);
+        hipLaunchKernelGGL(colminmax_kernel, dim3(512, d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k, k + d, z);
         e = hipGetLastError();
     }
-    uint32_t flag = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&flag, k + 2 * ctx->d, sizeof(flag), hipMemcpyDeviceToHost, s);
+    std::vector<uint32_t> hk(2 * d);
+    std::vector<unsigned long long> hz(d);
+    if (e == hipSuccess) e = hipMemcpyAsync(hk.data(), k, sizeof(uint32_t) * 2 * d, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hz.data(), z, sizeof(unsigned long long) * d, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(k);
+    NormArgs a;
+    bool ok = true;
+    for (uint32_t m = 0; e == hipSuccess && m < d; m++) {
+        float mn = okey_inv(hk[d + m]);
+        if (mn == 0.0f && hz[m] != ~0ull)  // the first zero of the column, with its sign
+            e = hipMemcpy(&mn, ctx->cols + (uint64_t)m * ctx->ld + hz[m], sizeof(float), hipMemcpyDeviceToHost);
+        const float mm = okey_inv(hk[m]) - mn;
+        a.mn[m] = mn;
+        a.mm[m] = mm;
+        ok = ok && mm > 0.0f && mm <= 3.40282347e+38f;
+    }
+    if (e == hipSuccess && ok) {
+        hipLaunchKernelGGL(normalize_kernel, dim3(2048, d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, a);
+        timing_end(ctx, s, "normalize", ev0);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    hipFree(z);
     TK_HIP(e);
-    return flag ? TOPK_ERR_NONFINITE : TOPK_OK;
+    return ok ? TOPK_OK : TOPK_ERR_NONFINITE;
 }
 
 }  // namespace topk
