@@ -163,6 +163,7 @@ public:
             return topk_pta(ctx_, (uint32_t)k, qq, weights, attr, ids, sc, thr);
         });
     }
+    void findTopKscalar(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
     void findTopKsimd(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
     void findTopKthreads(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
     void findTopKthreads2(uint64_t k, uint8_t qq, const float *w, const uint8_t *a) { findTopK(k, qq, w, a); }
@@ -207,6 +208,21 @@ public:
         return r;
     }
     float get_thres(size_t q) const { return thr_[q]; }
+
+    /* The multi-query form of AA::benchmark (cpu/AA.h:204-211, IMP >= 3):
+     * tt_procesing is the slowest worker's time per iteration (here one
+     * batch call per iteration: the call's time; the reference divides that
+     * figure by iter once more on this line), tuples_per_second is
+     * workload / that time in seconds. */
+    void benchmark_mq(uint64_t workload) const {
+        const double per_iter = tt_processing_ / iter_;
+        std::cout << std::fixed << std::setprecision(8);
+        std::cout << "< Benchmark for " << algo_ << " algorithm >" << std::endl;
+        std::cout << "tt_init: " << tt_init_ << std::endl;
+        std::cout << "tt_procesing: " << per_iter << std::endl;
+        std::cout << "tuples_per_second: " << (per_iter > 0 ? workload / (per_iter / 1000.0) : 0.0) << std::endl;
+        std::cout << "tuple_count: " << tuple_count_ << std::endl;
+    }
 
 private:
     std::vector<topk_query> qs_;

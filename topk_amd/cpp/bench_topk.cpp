@@ -1,15 +1,29 @@
 // Reference-style benchmark driver over the C++ mirror (include/topk_amd.hpp):
 // the loops and stdout keys of validation/bench_cpu.h (bench_tpac :155-237,
-// bench_pta :323-402, multi-query IMP=3 :198-215), on the GPU engine.
+// bench_pta :323-402), on the GPU engine.
 //
 //   bench_topk [-n N] [-d D] [-s KS] [-e KE] [-q QD] [-i ITER] [-t DISTR] [-r SEED]
-//              [-a fte|pta|mq] [-w WORKLOAD] [--no-normalize] [--check]
+//              [-a fte|pta|mq|mq4] [-p IMP] [-m QM] [-w WORKLOAD] [-f TEXTFILE]
+//              [--no-normalize] [--check] [--mq-thresholds]
 //
-// k doubles from KS to KE; qq runs 2..d in steps of QD over the last qq
-// attributes (the QM=0 tables, bench_cpu.h:25-49) with weights 1.0; one
-// warm-up call, then ITER timed calls.  --check runs TPAc and PTA on the same
-// store and compares ids and thresholds like AA::compare / compare_t
-// (cpu/AA.h:155-189), printing PASSED or FAILED.
+// The compile-time knobs of the reference's Makefile (:18-35) are flags:
+//   -a  algorithm object: fte (TPAc), pta (PTA); mq and mq4 are shorthands
+//       for -p 3 and -p 4
+//   -p  IMP: 0 scalar, 1 SIMD, 2 threads (one GPU call each), 3 random-
+//       attribute multiple queries (bench_cpu.h:198-215: WORKLOAD queries,
+//       each on a random qq in [2, d], one report per k), 4 same-attribute
+//       multiple queries (:216-235: WORKLOAD queries on the same qq, one
+//       report per (k, qq))
+//   -m  QM: 0 = the last qq attributes ascending, 1 = the first qq
+//       (the attr[][] tables, bench_cpu.h:25-68)
+// k doubles from KS to KE; qq runs 2..d in steps of QD; weights 1.0 (the
+// Q0 vector, bench_cpu.h:19); one warm-up call, then ITER timed calls.
+// Multiple-query modes run ITER batch calls of the WORKLOAD queries.
+// Each per-query report ends with "threshold: " (9 significant digits: the
+// binary32 value round-trips); --mq-thresholds prints one
+// "mq_threshold: j qq k value" line per query of the last batch.  --check
+// runs TPAc and PTA on the same store and compares ids and thresholds like
+// AA::compare / compare_t (cpu/AA.h:155-189), printing PASSED or FAILED.
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -29,9 +43,12 @@ struct Args {
     int distr = 1;
     uint64_t seed = 0;
     std::string algo = "fte";
+    int imp = 2;
+    int qm = 0;
     uint32_t workload = 1024;
     bool normalize = true;
     bool check = false;
+    bool mq_thresholds = false;
     std::string file;  // load this text file instead of generating (the reference's LD != 1)
 };
 
@@ -53,13 +70,20 @@ bool parse(int argc, char **argv, Args &a) {
         else if (s == "-t" && next(v)) a.distr = (int)v;
         else if (s == "-r" && next(v)) a.seed = v;
         else if (s == "-w" && next(v)) a.workload = (uint32_t)v;
+        else if (s == "-p" && next(v)) a.imp = (int)v;
+        else if (s == "-m" && next(v)) a.qm = (int)v;
         else if (s == "-a" && i + 1 < argc) a.algo = argv[++i];
         else if (s == "-f" && i + 1 < argc) a.file = argv[++i];
         else if (s == "--no-normalize") a.normalize = false;
         else if (s == "--check") a.check = true;
+        else if (s == "--mq-thresholds") a.mq_thresholds = true;
         else return false;
     }
-    return a.d >= 2 && a.d <= TOPK_MAX_DIMS && a.n > 0 && a.ks >= 1 && a.ke >= a.ks;
+    if (a.algo == "mq") a.imp = 3;
+    if (a.algo == "mq4") a.imp = 4;
+    return a.d >= 2 && a.d <= TOPK_MAX_DIMS && a.n > 0 && a.ks >= 1 && a.ke >= a.ks && a.imp >= 0 &&
+           a.imp <= 4 && (a.qm == 0 || a.qm == 1) && a.workload >= 1 &&
+           (a.algo == "fte" || a.algo == "pta" || a.algo == "mq" || a.algo == "mq4");
 }
 
 template <class Algo>
@@ -76,25 +100,98 @@ void fill(Algo &al, const Args &a) {
     al.set_normalize(a.normalize);
 }
 
-std::vector<uint8_t> last_attrs(uint32_t d, uint32_t qq) {
+// Row qq - 2 of the reference's attr[][] table: QM = 0 lists the last qq
+// attributes ascending, QM = 1 the first qq (bench_cpu.h:25-68).
+std::vector<uint8_t> query_attrs(uint32_t d, uint32_t qq, int qm) {
     std::vector<uint8_t> at(qq);
-    for (uint32_t j = 0; j < qq; j++) at[j] = (uint8_t)(d - qq + j);
+    for (uint32_t j = 0; j < qq; j++) at[j] = (uint8_t)(qm == 0 ? d - qq + j : j);
     return at;
 }
 
+void print_threshold(const char *key, float t) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.9g", (double)t);
+    std::cout << key << buf << std::endl;
+}
+
+void header(uint64_t n, uint32_t x, uint64_t k) {
+    std::cout << "Benchmark <<<-------------" << n << "," << x << "," << k << "------------->>> " << std::endl;
+}
+
+// IMP 0..2: one query at a time.
 template <class Algo>
 void sweep(Algo &al, const Args &a, const std::vector<float> &w) {
+    auto call = [&](uint64_t k, uint32_t qq, const uint8_t *at) {
+        if (a.imp == 0) al.findTopKscalar(k, (uint8_t)qq, w.data(), at);
+        else if (a.imp == 1) al.findTopKsimd(k, (uint8_t)qq, w.data(), at);
+        else al.findTopKthreads(k, (uint8_t)qq, w.data(), at);
+    };
     for (uint64_t k = a.ks; k <= a.ke; k *= 2) {
         for (uint32_t qq = 2; qq <= a.d; qq += a.qd) {
-            const auto at = last_attrs(a.d, qq);
-            std::cout << "Benchmark <<<-------------" << a.n << "," << qq << "," << k << "------------->>> "
-                      << std::endl;
-            al.findTopK(k, (uint8_t)qq, w.data(), at.data());  // warm-up
+            const auto at = query_attrs(a.d, qq, a.qm);
+            header(a.n, qq, k);
+            call(k, qq, at.data());  // warm-up
             al.reset_clocks();
             al.set_iter(a.iter);
-            for (uint32_t it = 0; it < a.iter; it++) al.findTopK(k, (uint8_t)qq, w.data(), at.data());
+            for (uint32_t it = 0; it < a.iter; it++) call(k, qq, at.data());
             al.benchmark();
-            std::cout << "threshold: " << al.get_thres() << std::endl;
+            print_threshold("threshold: ", al.get_thres());
+        }
+    }
+}
+
+// One timed multiple-query measurement: a warm-up batch, then ITER batches.
+void run_mq(topk_amd::MultiQuery &mq, const Args &a, const std::vector<uint32_t> &qqs, uint64_t k) {
+    mq.run_all();  // warm-up
+    mq.reset_clocks();
+    mq.set_iter(a.iter);
+    for (uint32_t it = 0; it < a.iter; it++) mq.run_all();
+    mq.benchmark_mq(a.workload);
+    if (a.mq_thresholds) {
+        for (size_t j = 0; j < mq.size(); j++) {
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), "mq_threshold: %zu %u %llu %.9g", j, qqs[j], (unsigned long long)k,
+                          (double)mq.get_thres(j));
+            std::cout << buf << std::endl;
+        }
+    }
+}
+
+// IMP 3 (random attribute counts, bench_cpu.h:198-215) and IMP 4 (the same
+// attributes for every query of the workload, :216-235).
+void multi_query(const Args &a, const std::vector<float> &w) {
+    topk_amd::MultiQuery mq(a.n, a.d);
+    fill(mq, a);
+    mq.init();
+    std::vector<std::vector<uint8_t>> keep;  // attr rows outlive the batch
+    if (a.imp == 3) {
+        // random_workload (bench_cpu.h:61-67) draws rand() % (d - 1) + 2 with a
+        // time seed; here a seeded generator, one draw per query
+        std::mt19937_64 rng(a.seed + 1);
+        std::vector<uint32_t> qqs(a.workload);
+        for (auto &q : qqs) q = 2 + (uint32_t)(rng() % (a.d - 1));
+        std::cout << "<<<Random Attribute Multiple Queries - (gpu batch) >>>" << std::endl;
+        for (uint64_t k = a.ks; k <= a.ke; k *= 2) {
+            mq.clear();
+            keep.clear();
+            for (uint32_t j = 0; j < a.workload; j++) {
+                keep.push_back(query_attrs(a.d, qqs[j], a.qm));
+                mq.add(k, (uint8_t)qqs[j], w.data(), keep.back().data());
+            }
+            header(a.n, a.d, k);
+            run_mq(mq, a, qqs, k);
+        }
+    } else {
+        std::cout << "<<<Same Attribute Multiple Queries - (gpu batch) >>>" << std::endl;
+        for (uint64_t k = a.ks; k <= a.ke; k *= 2) {
+            for (uint32_t qq = 2; qq <= a.d; qq += a.qd) {
+                mq.clear();
+                keep.assign(1, query_attrs(a.d, qq, a.qm));
+                for (uint32_t j = 0; j < a.workload; j++) mq.add(k, (uint8_t)qq, w.data(), keep[0].data());
+                header(a.n, qq, k);
+                run_mq(mq, a, std::vector<uint32_t>(a.workload, qq), k);
+                mq.reset_clocks();
+            }
         }
     }
 }
@@ -117,7 +214,8 @@ int main(int argc, char **argv) {
     if (!parse(argc, argv, a)) {
         std::fprintf(stderr,
                      "usage: bench_topk [-n N] [-d D] [-s KS] [-e KE] [-q QD] [-i ITER] [-t DISTR] [-r SEED] "
-                     "[-a fte|pta|mq] [-w WORKLOAD] [-f TEXTFILE] [--no-normalize] [--check]\n");
+                     "[-a fte|pta|mq|mq4] [-p IMP 0..4] [-m QM 0|1] [-w WORKLOAD] [-f TEXTFILE] "
+                     "[--no-normalize] [--check] [--mq-thresholds]\n");
         return 2;
     }
     const std::vector<float> w(a.d, 1.0f);
@@ -132,50 +230,25 @@ int main(int argc, char **argv) {
             int bad = 0;
             for (uint64_t k = a.ks; k <= a.ke; k *= 2)
                 for (uint32_t qq = 2; qq <= a.d; qq += a.qd) {
-                    const auto at = last_attrs(a.d, qq);
+                    const auto at = query_attrs(a.d, qq, a.qm);
                     t.findTopK(k, (uint8_t)qq, w.data(), at.data());
                     p.findTopK(k, (uint8_t)qq, w.data(), at.data());
                     bad |= compare(t, p);
                 }
             return bad;
         }
-        if (a.algo == "fte") {
+        if (a.imp >= 3) {
+            multi_query(a, w);
+        } else if (a.algo == "fte") {
             topk_amd::TPAc t(a.n, a.d);
             fill(t, a);
             t.init();
             sweep(t, a, w);
-        } else if (a.algo == "pta") {
+        } else {
             topk_amd::PTA p(a.n, a.d);
             fill(p, a);
             p.init();
             sweep(p, a, w);
-        } else if (a.algo == "mq") {
-            // random qq in [2, d] per query (bench_cpu.h:61-67 random_workload), seeded
-            topk_amd::MultiQuery mq(a.n, a.d);
-            fill(mq, a);
-            mq.init();
-            std::mt19937_64 rng(a.seed + 1);
-            for (uint64_t k = a.ks; k <= a.ke; k *= 2) {
-                mq.clear();
-                std::vector<std::vector<uint8_t>> keep;
-                for (uint32_t j = 0; j < a.workload; j++) {
-                    const uint32_t qq = 2 + (uint32_t)(rng() % (a.d - 1));
-                    keep.push_back(last_attrs(a.d, qq));
-                    mq.add(k, (uint8_t)qq, w.data(), keep.back().data());
-                }
-                std::cout << "Benchmark <<<-------------" << a.n << "," << a.d << "," << k
-                          << "------------->>> " << std::endl;
-                mq.run_all();  // warm-up
-                mq.reset_clocks();
-                mq.set_iter(a.iter);
-                for (uint32_t it = 0; it < a.iter; it++) mq.run_all();
-                mq.benchmark();
-                std::cout << "tuples_per_second: " << a.workload / (mq.tt_processing() / a.iter / 1000.0)
-                          << std::endl;
-            }
-        } else {
-            std::fprintf(stderr, "unknown algorithm %s\n", a.algo.c_str());
-            return 2;
         }
     } catch (const topk_amd::error &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
