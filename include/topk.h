@@ -325,6 +325,26 @@ int topk_mlp_eval(topk_ctx *ctx, const float *params, const float *x, const floa
 int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr,
                    uint64_t *d_ids, float *d_scores);
 
+/* Allocation-free FTE over a caller's device-resident store.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory in the call (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No context: column m of the store is at
+ * d_cols + m*ld (ld >= n, a multiple of 4; d_cols 16-byte aligned), and the
+ * buffer must hold d*ld + 8192 floats, since stream tiles read up to 8191
+ * values past a column's n-th (ignored).  The same results as topk_fte
+ * (best first into d_ids / d_scores, k entries; d_threshold may be NULL),
+ * written to device memory; *d_status (device) becomes TOPK_OK, or
+ * TOPK_ERR_OVERFLOW when more than 2^20 candidates beat even the exact
+ * prefix threshold (outputs then untouched).  Enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) without a host wait, a hipMalloc or
+ * a memset node.  Scratch is one 11.0 MiB device array per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU (different streams, or threads) race.  Limits: n <
+ * 2^32 - 1, k <= TOPK_MAX_K (TOPK_ERR_UNSUPPORTED beyond). */
+int topk_fte_noalloc(const float *d_cols, uint64_t n, uint32_t d, uint64_t ld, uint32_t k, uint8_t qq,
+                     const float *weights, const uint8_t *attr, uint64_t *d_ids, float *d_scores,
+                     float *d_threshold, uint32_t *d_status, void *stream);
+
 /* Waits for the context's stream; returns the first device-side error
  * since the last sync (TOPK_OK if none) and re-runs nothing. */
 int topk_sync(topk_ctx *ctx);

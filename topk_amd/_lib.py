@@ -119,6 +119,8 @@ SIGNATURES = {
     "topk_batch_submit": (_c.c_int, [_vp, _c.c_uint32, _vp, _c.c_uint32, _vp, _vp, _vp, _c.POINTER(_c.c_uint64)]),
     "topk_batch_wait": (_c.c_int, [_vp, _c.c_uint64]),
     "topk_fte_async": (_c.c_int, [_vp, _c.c_uint32, _c.c_uint8, _vp, _vp, _vp, _vp]),
+    "topk_fte_noalloc": (_c.c_int, [_vp, _c.c_uint64, _c.c_uint32, _c.c_uint64, _c.c_uint32, _c.c_uint8, _vp, _vp,
+                                    _vp, _vp, _vp, _vp, _vp]),
     "topk_sync": (_c.c_int, [_vp]),
     "topk_stream": (_vp, [_vp]),
     "topk_set_tuning": (_c.c_int, [_vp, _c.c_char_p, _c.c_int64]),

@@ -27,6 +27,11 @@
 // candidates.  Too few candidates (a sample guess that was too high) raise
 // STATUS_RETRY, too many for the buffer STATUS_OVERFLOW; the host then runs
 // the exact mode, with an n-entry buffer if needed.
+#include <string.h>
+
+#include <cmath>
+#include <mutex>
+
 #include "device.hpp"
 #include "emit.hpp"
 #include "internal.hpp"
@@ -77,9 +82,11 @@ __global__ __launch_bounds__(SEED_BS) void fte_seed_kernel(QCols q, uint64_t n_e
                                                            uint32_t ks, uint32_t total_valid,
                                                            uint64_t *__restrict__ blk,
                                                            uint64_t *__restrict__ pre,
-                                                           FteState *__restrict__ st) {
+                                                           FteState *__restrict__ st,
+                                                           const uint32_t *__restrict__ gate) {
     __shared__ SelectScratch ss;
     __shared__ uint32_t is_last;
+    if (gate && *gate == 0) return;  // the pass this one backs up settled the query
     const uint32_t tid = threadIdx.x;
     const uint64_t base = (uint64_t)blockIdx.x * chunk_stride;
 
@@ -217,9 +224,11 @@ template <int M, int VPT>
 __global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t n, uint64_t obj_begin,
                                                                uint64_t ntiles, FteState *__restrict__ st,
                                                                CandSet cs, const uint64_t *__restrict__ seed,
-                                                               uint32_t nseed, uint32_t ks) {
+                                                               uint32_t nseed, uint32_t ks,
+                                                               const uint32_t *__restrict__ gate) {
     constexpr uint32_t TILE = STREAM_BS * 4 * VPT;
     __shared__ uint32_t lcount;
+    if (gate && *gate == 0) return;
     if (threadIdx.x == 0) lcount = 0;
     __syncthreads();
     const uint64_t tau = nseed ? wave_seed_tau(seed, nseed, ks) : st->tau;
@@ -302,8 +311,10 @@ __global__ __launch_bounds__(STREAM_BS) void fte_stream_kernel(QCols q, uint64_t
 // one of its ~2000 blocks; each is an L2 write-back on gfx950 and together
 // they cost more than a launch.)
 __global__ __launch_bounds__(FINAL_BS) void fte_final_kernel(FteState *__restrict__ st, CandSet cs, uint32_t k,
-                                                             uint32_t kreq, FinalOut fo) {
+                                                             uint32_t kreq, FinalOut fo,
+                                                             const uint32_t *__restrict__ gate) {
     __shared__ uint32_t off[MAX_STREAM_GRID + 2];
+    if (gate && *gate == 0) return;
     __shared__ uint64_t top[TOPK_MAX_K];
     __shared__ uint64_t sorted[TOPK_MAX_K];
     __shared__ uint32_t wsum[FINAL_BS / 64];
@@ -413,8 +424,8 @@ static uint32_t exact_prefix_blocks(uint64_t n, uint32_t k) {
     return (uint32_t)(need < g ? need : g);
 }
 
-static uint32_t stream_grid(const topk_ctx *ctx, uint64_t ntiles) {
-    const uint64_t cap = ctx->tune.stream_grid ? (uint64_t)ctx->tune.stream_grid : 2048;
+static uint32_t stream_grid(uint32_t grid_cap, uint64_t ntiles) {
+    const uint64_t cap = grid_cap ? (uint64_t)grid_cap : 2048;
     const uint64_t g = ntiles < cap ? ntiles : cap;
     return (uint32_t)(g < 1 ? 1 : g);
 }
@@ -431,9 +442,8 @@ static int stream_vpt(uint32_t m) {
 // Chooses the seed geometry for a query: sample mode when n is large enough
 // that a spread sample leaves many candidates, exact prefix mode otherwise.
 // Returns true for the sample geometry.
-static bool seed_plan(const topk_ctx *ctx, uint32_t k, bool exact, uint32_t &blocks, uint64_t &stride,
-                      uint64_t &n_eff, uint32_t &ks, uint32_t &total_valid) {
-    const uint64_t n = ctx->n;
+static bool seed_plan(uint64_t n, uint32_t k, bool exact, uint32_t &blocks, uint64_t &stride, uint64_t &n_eff,
+                      uint32_t &ks, uint32_t &total_valid) {
     const uint64_t s = (uint64_t)SAMPLE_CHUNKS * SAMPLE_TILE;
     if (!exact && n > 4 * s) {
         blocks = SAMPLE_CHUNKS;
@@ -456,27 +466,47 @@ static bool seed_plan(const topk_ctx *ctx, uint32_t k, bool exact, uint32_t &blo
     }
 }
 
-int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
-                float *d_thr, uint32_t *d_status, int mode) {
-    const uint64_t n = ctx->n;
+// Everything one run of the FTE pipeline needs besides the query: the store
+// size, the stream, the scratch its kernels use and the launch shape.
+struct FteRun {
+    hipStream_t s;
+    uint64_t n;
+    FteState *st;
+    uint64_t *blk;        // seed-pass block winners (SAMPLE_CHUNKS * TOPK_MAX_K keys)
+    uint64_t *pre;        // exact-mode prefix keys (TOPK_MAX_K)
+    uint32_t *counts;     // per-stream-block counts (MAX_STREAM_GRID)
+    uint64_t *cand;       // candidate slots
+    uint64_t cand_cap;
+    bool big;             // cand holds every object of every block (exact re-run after an overflow)
+    uint64_t *spill;
+    uint32_t spill_cap;
+    int vpt;              // 0: chosen from the query width
+    uint32_t grid_cap;    // 0: default
+    const uint32_t *gate; // non-null: every kernel exits at once unless *gate != 0
+    topk_ctx *timed;      // non-null: bracket the stream pass for topk_timing_*
+};
+
+// Enqueues seed -> stream -> final select for one query on r.s.  Returns the
+// candidate set the final select read (pta.hip re-uses the final stage).
+static CandSet fte_launch(const FteRun &r, uint32_t k, const QCols &q, const FinalOut &fo, int mode) {
+    const uint64_t n = r.n;
     const bool exact = mode != FTE_SAMPLE;
-    const bool big = mode == FTE_EXACT_BIG;
-    hipStream_t s = L.stream;
+    hipStream_t s = r.s;
 
     uint32_t blocks, ks, total_valid;
     uint64_t stride, n_eff;
-    const bool sampled = seed_plan(ctx, k, exact, blocks, stride, n_eff, ks, total_valid);
-    uint64_t *pre = exact ? L.pre : nullptr;
+    const bool sampled = seed_plan(n, k, exact, blocks, stride, n_eff, ks, total_valid);
+    uint64_t *pre = exact ? r.pre : nullptr;
     // sample mode with few winners: no merge in the seed pass, every stream
     // wave reduces the winners to tau itself
     const bool wave_tau = sampled && ks <= 4 && blocks * ks <= SEED_WAVE_MAX;
 #define TK_SEED(MM, OPT, MERGE)                                                                           \
     hipLaunchKernelGGL((fte_seed_kernel<MM, OPT, MERGE>), dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, \
-                       ks, total_valid, L.blk, pre, L.st)
+                       ks, total_valid, r.blk, pre, r.st, r.gate)
 #define TK_SEED_SAMPLE(MM) TK_SEED(MM, SAMPLE_TILE / SEED_BS, true)
 #define TK_SEED_EXACT(MM) TK_SEED(MM, SEED_TILE / SEED_BS, true)
 #define TK_SAMPLE(MM) \
-    hipLaunchKernelGGL(fte_sample_kernel<MM>, dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, ks, L.blk)
+    hipLaunchKernelGGL(fte_sample_kernel<MM>, dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, ks, r.blk)
     if (wave_tau) {
         TK_SWITCH_M(q.m, TK_SAMPLE);
     } else if (sampled) {
@@ -493,30 +523,25 @@ int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_
     // exact mode streams only the objects after the prefix
     const uint64_t first = exact ? n_eff : 0;
     const uint32_t kreq = (uint32_t)(n < k ? n : k);
-    const FinalOut fo{d_ids, d_scores, d_thr, d_status};
-    const int vpt = ctx->tune.stream_vpt ? ctx->tune.stream_vpt : stream_vpt(q.m);
+    const int vpt = r.vpt ? r.vpt : stream_vpt(q.m);
     const uint64_t tile = (uint64_t)STREAM_BS * 4 * vpt;
     const uint64_t ntiles = n > first ? (n - first + tile - 1) / tile : 0;
-    const uint32_t g = ntiles ? stream_grid(ctx, ntiles) : 0;
+    const uint32_t g = ntiles ? stream_grid(r.grid_cap, ntiles) : 0;
     CandSet cs;
-    cs.counts = L.counts;
+    cs.keys = r.cand;
+    cs.counts = r.counts;
     cs.nblk = g;
     cs.pre = pre;
-    cs.spill = L.spill;
-    cs.spill_cap = ctx->spill_cap;
-    if (big) {
-        cs.keys = ctx->cand_big;
-        cs.slot = g ? (uint32_t)(((ntiles + g - 1) / g) * tile) : 0;  // a block's every object fits
-    } else {
-        cs.keys = L.cand;
-        cs.slot = g ? (uint32_t)(ctx->cand_cap / g) : 0;
-    }
+    cs.spill = r.spill;
+    cs.spill_cap = r.spill_cap;
+    if (r.big) cs.slot = g ? (uint32_t)(((ntiles + g - 1) / g) * tile) : 0;  // a block's every object fits
+    else cs.slot = g ? (uint32_t)(r.cand_cap / g) : 0;
     if (g) {
         hipEvent_t ev0 = nullptr;
-        timing_begin(ctx, s, ev0);
+        if (r.timed) timing_begin(r.timed, s, ev0);
 #define TK_STREAM(MM, VPT)                                                                                   \
     hipLaunchKernelGGL((fte_stream_kernel<MM, VPT>), dim3(g), dim3(STREAM_BS), 0, s, q, n, first, ntiles, \
-                       L.st, cs, L.blk, nseed, ks)
+                       r.st, cs, r.blk, nseed, ks, r.gate)
 #define TK_STREAM1(MM) TK_STREAM(MM, 1)
 #define TK_STREAM2(MM) TK_STREAM(MM, 2)
 #define TK_STREAM4(MM) TK_STREAM(MM, 4)
@@ -532,9 +557,33 @@ int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_
 #undef TK_STREAM2
 #undef TK_STREAM4
 #undef TK_STREAM8
-        timing_end(ctx, s, "fte_stream", ev0);
+        if (r.timed) timing_end(r.timed, s, "fte_stream", ev0);
     }
-    final_enqueue(ctx, L, cs, k, kreq, fo, big);
+    if (!r.big) hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, r.st, cs, k, kreq, fo, r.gate);
+    return cs;
+}
+
+int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
+                float *d_thr, uint32_t *d_status, int mode) {
+    FteRun r{};
+    r.s = L.stream;
+    r.n = ctx->n;
+    r.st = L.st;
+    r.blk = L.blk;
+    r.pre = L.pre;
+    r.counts = L.counts;
+    r.big = mode == FTE_EXACT_BIG;
+    r.cand = r.big ? ctx->cand_big : L.cand;
+    r.cand_cap = ctx->cand_cap;
+    r.spill = L.spill;
+    r.spill_cap = ctx->spill_cap;
+    r.vpt = ctx->tune.stream_vpt;
+    r.grid_cap = ctx->tune.stream_grid;
+    r.timed = ctx;
+    const uint32_t kreq = (uint32_t)(ctx->n < k ? ctx->n : k);
+    const FinalOut fo{d_ids, d_scores, d_thr, d_status};
+    const CandSet cs = fte_launch(r, k, q, fo, mode);
+    if (r.big) final_enqueue(ctx, L, cs, k, kreq, fo, true);
     TK_HIP(hipGetLastError());
     return TOPK_OK;
 }
@@ -548,8 +597,126 @@ void final_enqueue(topk_ctx *ctx, Lane &L, const CandSet &cs, uint32_t k, uint32
             hipLaunchKernelGGL(cand_compact_kernel, dim3(cs.nblk), dim3(256), 0, s, cs, ctx->offsets, ctx->flat);
         hipLaunchKernelGGL(fte_final_flat_kernel, dim3(1), dim3(FLAT_BS), 0, s, L.st, ctx->flat, k, kreq, fo);
     } else {
-        hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, L.st, cs, k, kreq, fo);
+        hipLaunchKernelGGL(fte_final_kernel, dim3(1), dim3(FINAL_BS), 0, s, L.st, cs, k, kreq, fo, nullptr);
     }
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free FTE over a caller's device store (topk_fte_noalloc)
+// ---------------------------------------------------------------------------
+// The per-query scratch lives in one file-scope device array per GPU, held
+// until the process ends: FTE state, the two passes' status words, seed
+// winners, prefix keys, per-block counts, candidate slots and spill region.
+// A query is at most two pipeline runs, both enqueued up front: the first
+// (sample-seeded above 2^19 objects, exact-prefix-seeded below) and an
+// exact-prefix run whose kernels exit at once unless the first reported a
+// failure.  Nothing waits on the host, so the call can be captured in a graph.
+constexpr uint64_t NA_CAND = 1ull << 20;   // candidate slots (8 MiB)
+constexpr uint32_t NA_SPILL = 1u << 18;    // spill region (2 MiB)
+constexpr uint64_t NA_BLK = (uint64_t)SAMPLE_CHUNKS * TOPK_MAX_K;  // seed winners, either mode (1 MiB)
+struct NoAllocScratch {
+    FteState st;
+    uint32_t pass[2];  // final-select status of the first and the second run (0: settled)
+    uint32_t counts[MAX_STREAM_GRID];
+    uint64_t pre[TOPK_MAX_K];
+    uint64_t blk[NA_BLK];
+    uint64_t cand[NA_CAND];
+    uint64_t spill[NA_SPILL];
+};
+static_assert(offsetof(NoAllocScratch, pre) % 8 == 0 && offsetof(NoAllocScratch, blk) % 8 == 0 &&
+                  offsetof(NoAllocScratch, cand) % 8 == 0 && offsetof(NoAllocScratch, spill) % 8 == 0,
+              "key arrays stay 8-byte aligned");
+constexpr size_t NA_SCRATCH_BYTES = sizeof(NoAllocScratch);
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+// Device address of `scratch` per GPU (the array's name on the host is only
+// a stand-in for the symbol), looked up once per device.
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex na_mu;
+static NoAllocScratch *na_ptr[NA_MAX_DEVICES];
+
+static int noalloc_scratch(NoAllocScratch **out) {
+    int dev = 0;
+    TK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return TOPK_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(na_mu);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        TK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return TOPK_ERR_HIP;
+        na_ptr[dev] = static_cast<NoAllocScratch *>(p);
+    }
+    *out = na_ptr[dev];
+    return TOPK_OK;
+}
+
+// Clears what the pipeline expects at zero (a kernel, not a memset node).
+__global__ void noalloc_clear_kernel(NoAllocScratch *__restrict__ sc) {
+    if (threadIdx.x == 0) {
+        sc->st = FteState{};
+        sc->pass[0] = 0;
+        sc->pass[1] = 0;
+    }
+}
+
+// Caller-visible status: TOPK_OK when either run settled the query.
+__global__ void noalloc_status_kernel(const NoAllocScratch *__restrict__ sc, bool second,
+                                      uint32_t *__restrict__ status) {
+    if (threadIdx.x != 0) return;
+    const uint32_t last = second && sc->pass[0] ? sc->pass[1] : sc->pass[0];
+    *status = last == 0 ? (uint32_t)TOPK_OK : (uint32_t)TOPK_ERR_OVERFLOW;
+}
+
 }  // namespace topk
+
+using namespace topk;
+
+extern "C" int topk_fte_noalloc(const float *d_cols, uint64_t n, uint32_t d, uint64_t ld, uint32_t k, uint8_t qq,
+                                const float *weights, const uint8_t *attr, uint64_t *d_ids, float *d_scores,
+                                float *d_threshold, uint32_t *d_status, void *stream) {
+    if (!d_cols || !weights || !attr || !d_ids || !d_scores || !d_status) return TOPK_ERR_INVALID;
+    if (n < 1 || d < 1 || d > TOPK_MAX_DIMS || ld < n || ld % 4 || k < 1) return TOPK_ERR_INVALID;
+    if (reinterpret_cast<uintptr_t>(d_cols) % 16) return TOPK_ERR_INVALID;
+    if (n >= 0xFFFFFFFFull || k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    if (qq < 1 || qq > d) return TOPK_ERR_INVALID;
+    QCols q;
+    memset(&q, 0, sizeof(q));
+    for (uint32_t j = 0; j < qq; j++) {
+        if (attr[j] >= d) return TOPK_ERR_INVALID;
+        const float wj = weights[attr[j]];
+        if (!std::isfinite(wj)) return TOPK_ERR_INVALID;
+        q.col[j] = d_cols + (uint64_t)attr[j] * ld;
+        q.w[j] = wj;
+    }
+    q.m = qq;
+    NoAllocScratch *sc = nullptr;
+    int rc = noalloc_scratch(&sc);
+    if (rc) return rc;
+
+    FteRun r{};
+    r.s = static_cast<hipStream_t>(stream);
+    r.n = n;
+    r.st = &sc->st;
+    r.blk = sc->blk;
+    r.pre = sc->pre;
+    r.counts = sc->counts;
+    r.cand = sc->cand;
+    r.cand_cap = NA_CAND;
+    r.spill = sc->spill;
+    r.spill_cap = NA_SPILL;
+    hipLaunchKernelGGL(noalloc_clear_kernel, dim3(1), dim3(64), 0, r.s, sc);
+    uint32_t blocks, ks, total_valid;
+    uint64_t stride, n_eff;
+    const bool sampled = seed_plan(n, k, false, blocks, stride, n_eff, ks, total_valid);
+    fte_launch(r, k, q, FinalOut{d_ids, d_scores, d_threshold, &sc->pass[0]}, FTE_SAMPLE);
+    if (sampled) {
+        // too few candidates (the sample guessed high) or a full buffer: the
+        // exact-prefix run, skipped on the device when the first one settled
+        r.gate = &sc->pass[0];
+        fte_launch(r, k, q, FinalOut{d_ids, d_scores, d_threshold, &sc->pass[1]}, FTE_EXACT);
+    }
+    hipLaunchKernelGGL(noalloc_status_kernel, dim3(1), dim3(64), 0, r.s, sc, sampled, d_status);
+    TK_HIP(hipGetLastError());
+    return TOPK_OK;
+}
