@@ -386,6 +386,15 @@ void *topk_stream(topk_ctx *ctx);
  *                     0 = query-major first pass, group-major second pass;
  *                     1 = query-major (one workgroup per query);
  *                     2 = group-major (one per partition group and query slice)
+ *   "score_divide"    1 = real-data scores: the running score is divided by
+ *                     qq after every term, each step rounded to binary32
+ *                     (the reference's LD == 2 build, cpu/TPAc.h:143-146,
+ *                     237-240, 291-294).  Applies to topk_fte, topk_fte_async
+ *                     and topk_batch (whose queries then each run the FTE
+ *                     path; topk_batch_autotune and topk_learned_train return
+ *                     TOPK_ERR_UNSUPPORTED).  topk_pta, topk_bta, topk_gpa
+ *                     keep plain sums, as the reference's PTA and GPU code
+ *                     do.  0 = plain sums (default)
  * (0 for a batch knob = default or the autotuner's choice.)
  * Unknown names or values return TOPK_ERR_INVALID. */
 int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value);

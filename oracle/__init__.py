@@ -38,8 +38,11 @@ def lib():
         "or_okey_inv": (_c.c_float, [_c.c_uint32]),
         "or_key": (_c.c_uint64, [_c.c_float, _c.c_uint64]),
         "or_score": (None, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _f32p, _u8p, _f32p]),
+        "or_score_div": (None, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _f32p, _u8p, _f32p]),
         "or_fte_topk": (_c.c_uint32, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _c.c_uint32,
                                       _f32p, _u8p, _u64p, _f32p, _f32p]),
+        "or_fte_topk_div": (_c.c_uint32, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _c.c_uint32,
+                                          _f32p, _u8p, _u64p, _f32p, _f32p]),
         "or_tpac_threads": (_c.c_uint32, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _c.c_uint32,
                                           _f32p, _u8p, _c.c_int, _u64p, _f32p, _f32p]),
         "or_mq_threads": (_c.c_int, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _u32p, _u32p,
@@ -98,23 +101,26 @@ def key(s, i):
     return lib().or_key(float(s), int(i))
 
 
-def score(data, n, weights, attr):
-    """Per-object scores, cpu/TPAc.h:129-142 order."""
+def score(data, n, weights, attr, divide=False):
+    """Per-object scores, cpu/TPAc.h:129-142 order; divide=True divides the
+    running score by qq after every term (LD == 2, cpu/TPAc.h:143-146)."""
     a, d, ld = _cols(data)
     w, at = _query(d, weights, attr)
     out = np.empty(n, dtype=np.float32)
-    lib().or_score(a.ravel(), n, ld, at.size, w, at, out)
+    (lib().or_score_div if divide else lib().or_score)(a.ravel(), n, ld, at.size, w, at, out)
     return out
 
 
-def fte_topk(data, n, k, weights, attr):
-    """Scalar FTE top-k (cpu/TPAc.h:35-207 restated).  Returns (ids, scores, thr)."""
+def fte_topk(data, n, k, weights, attr, divide=False):
+    """Scalar FTE top-k (cpu/TPAc.h:35-207 restated).  Returns (ids, scores, thr).
+    divide=True: real-data scores (score(..., divide=True))."""
     a, d, ld = _cols(data)
     w, at = _query(d, weights, attr)
     ids = np.zeros(k, dtype=np.uint64)
     sc = np.zeros(k, dtype=np.float32)
     thr = np.zeros(1, dtype=np.float32)
-    c = lib().or_fte_topk(a.ravel(), n, ld, k, at.size, w, at, ids, sc, thr)
+    fn = lib().or_fte_topk_div if divide else lib().or_fte_topk
+    c = fn(a.ravel(), n, ld, k, at.size, w, at, ids, sc, thr)
     return ids[:c], sc[:c], float(thr[0])
 
 

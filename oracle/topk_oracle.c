@@ -88,6 +88,30 @@ void or_score(const float *cols, uint64_t n, uint64_t ld, uint32_t qq,
     for (uint64_t i = 0; i < n; i++) out[i] = score_one(cols, ld, i, qq, w, attr);
 }
 
+/* Real-data scoring (the reference's LD == 2 build, cpu/TPAc.h:143-146,
+ * 237-240, 291-294): after every term the running score is divided by qq,
+ * each step rounded to binary32.  Parity pinned by exact powers of two
+ * (qq = 2, 4: every division is exact) and a float32 step-by-step
+ * restatement for other qq (tests/test_oracle.py). */
+static inline float score_one_div(const float *cols, uint64_t ld, uint64_t i,
+                                  uint32_t qq, const float *w, const uint8_t *attr)
+{
+    const float q = (float)qq;
+    float acc = 0.0f;
+    for (uint32_t m = 0; m < qq; m++) {
+        float p = cols[(uint64_t)attr[m] * ld + i] * w[attr[m]];
+        acc = acc + p;
+        acc = acc / q;
+    }
+    return acc;
+}
+
+void or_score_div(const float *cols, uint64_t n, uint64_t ld, uint32_t qq,
+                  const float *w, const uint8_t *attr, float *out)
+{
+    for (uint64_t i = 0; i < n; i++) out[i] = score_one_div(cols, ld, i, qq, w, attr);
+}
+
 /* ------------------------------------------------------------------ */
 /* size-k min-heap on 64-bit keys (cpu/AA.h:53-58 MaxCMP semantics:   */
 /* the top is the worst kept element; replace only if strictly better) */
@@ -149,6 +173,19 @@ uint32_t or_fte_topk(const float *cols, uint64_t n, uint64_t ld, uint32_t k,
     kheap h = { (uint64_t *)malloc(sizeof(uint64_t) * (k ? k : 1)), 0, k };
     for (uint64_t i = 0; i < n; i++)
         heap_push(&h, or_key(score_one(cols, ld, i, qq, w, attr), i));
+    uint32_t c = heap_emit(&h, ids, scores, thr);
+    free(h.a);
+    return c;
+}
+
+/* or_fte_topk over real-data scores (score_one_div). */
+uint32_t or_fte_topk_div(const float *cols, uint64_t n, uint64_t ld, uint32_t k,
+                         uint32_t qq, const float *w, const uint8_t *attr,
+                         uint64_t *ids, float *scores, float *thr)
+{
+    kheap h = { (uint64_t *)malloc(sizeof(uint64_t) * (k ? k : 1)), 0, k };
+    for (uint64_t i = 0; i < n; i++)
+        heap_push(&h, or_key(score_one_div(cols, ld, i, qq, w, attr), i));
     uint32_t c = heap_emit(&h, ids, scores, thr);
     free(h.a);
     return c;

@@ -250,3 +250,39 @@ def test_golden_fixtures_all_paths(gpu):
             for ids, sc, thr in outs:
                 assert np.array_equal(ids, g[f"{tag}_ids"]), tag
                 assert np.array_equal(sc.view(np.uint32), g[f"{tag}_scores"]), tag
+
+
+@pytest.mark.parametrize("n", [100_003, 3_000_017])
+def test_fte_real_data_scores_match_oracle(gpu, n):
+    """topk_set_tuning "score_divide": the running score is divided by qq
+    after every term (the reference's LD == 2 build, cpu/TPAc.h:143-146,
+    237-240), bit-exact against the oracle's restatement; batches in that
+    mode answer every query through the FTE path; off again, plain sums."""
+    d = 8
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 9)
+    rng = np.random.default_rng(5)
+    cases = [(16, 2), (64, 3), (100, 5), (128, 8), (7, 1)]
+    qs = []
+    with topk_amd.Engine(x) as eng:
+        eng.set_tuning("score_divide", 1)
+        for k, qq in cases:
+            attr = [int(a) for a in rng.permutation(d)[:qq]]
+            w = rng.uniform(0.1, 1.0, d).astype(np.float32)
+            qs.append((k, w, attr))
+            ids, sc, thr = eng.fte(k, w, attr)
+            rid, rsc, rthr = oracle.fte_topk(x, n, k, w, attr, divide=True)
+            assert np.array_equal(ids, rid), (n, k, qq)
+            assert np.array_equal(sc.view(np.uint32), rsc.view(np.uint32)), (n, k, qq)
+            assert np.float32(thr).tobytes() == np.float32(rthr).tobytes()
+        for (k, w, attr), (gi, gs, gt) in zip(qs, eng.batch(qs)):
+            rid, rsc, rthr = oracle.fte_topk(x, n, k, w, attr, divide=True)
+            assert np.array_equal(gi, rid) and np.array_equal(gs.view(np.uint32), rsc.view(np.uint32))
+            assert np.float32(gt).tobytes() == np.float32(rthr).tobytes()
+        assert eng.stats()["fte_fallback"] == len(qs)
+        with pytest.raises(topk_amd.TopkError):
+            eng.batch_autotune(qs)
+        eng.set_tuning("score_divide", 0)
+        k, w, attr = qs[3]
+        ids, sc, _ = eng.fte(k, w, attr)
+        rid, rsc, _ = oracle.fte_topk(x, n, k, w, attr)
+        assert np.array_equal(ids, rid) and np.array_equal(sc, rsc)

@@ -84,6 +84,47 @@ def test_fte_topk_bruteforce_with_ties(k):
     assert thr == sc[-1]
 
 
+def test_score_divide_powers_of_two_closed_form():
+    """Real-data scores (LD == 2): with qq = 2 or 4, w = 1 and x = a/256,
+    every division is by a power of two and every step is exact, so term j
+    (0-based) of qq ends up divided qq - j times:
+    score = sum_j a_j / 256 / qq^(qq - j)."""
+    rng = np.random.default_rng(7)
+    n, d = 4000, 8
+    a = rng.integers(0, 256, size=(d, n))
+    x = (a / 256.0).astype(np.float32)
+    w = np.ones(d, np.float32)
+    for qq in (2, 4):
+        attr = rng.permutation(d)[:qq].astype(np.uint8)
+        got = oracle.score(x, n, w, attr, divide=True)
+        want = sum(a[attr[j]] / 256.0 / float(qq) ** (qq - j) for j in range(qq))
+        assert np.array_equal(got.astype(np.float64), want)
+    # qq = 1 divides by one: the plain score
+    assert np.array_equal(oracle.score(x, n, w, [3], divide=True), oracle.score(x, n, w, [3]))
+
+
+def test_score_divide_matches_float32_steps():
+    """Other qq: numpy's binary32 mul, add and divide (each correctly
+    rounded) applied term by term in attribute order."""
+    rng = np.random.default_rng(8)
+    n, d = 3000, 8
+    x = rng.random((d, n), dtype=np.float32)
+    w = rng.uniform(0.1, 1.0, d).astype(np.float32)
+    for qq in (3, 5, 7, 8):
+        attr = rng.permutation(d)[:qq].astype(np.uint8)
+        acc = np.zeros(n, np.float32)
+        q = np.float32(qq)
+        for j in range(qq):
+            acc = (acc + (x[attr[j]] * w[attr[j]]).astype(np.float32)).astype(np.float32)
+            acc = (acc / q).astype(np.float32)
+        got = oracle.score(x, n, w, attr, divide=True)
+        assert np.array_equal(got.view(np.uint32), acc.view(np.uint32)), qq
+        ids, sc, thr = oracle.fte_topk(x, n, 25, w, attr, divide=True)
+        order = np.lexsort((np.arange(n), -acc.astype(np.float64)))[:25]
+        assert np.array_equal(ids, order.astype(np.uint64)) and np.array_equal(sc, acc[order])
+        assert thr == acc[order[-1]]
+
+
 def test_fte_topk_n_smaller_than_k():
     x = (RNG.integers(0, 100, size=(2, 5)) / 128.0).astype(np.float32)
     ids, sc, thr = oracle.fte_topk(x, 5, 16, np.ones(2, np.float32), [0, 1])

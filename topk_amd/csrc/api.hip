@@ -371,6 +371,7 @@ int topk_fte(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     QCols q;
     rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
     if (rc) return rc;
+    if (ctx->score_div) q.div = (float)qq;
     const double t0 = now_ms();
     rc = fte_run_sync(ctx, k, q);
     if (rc) return rc;
@@ -389,6 +390,7 @@ int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, 
     QCols q;
     rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
     if (rc) return rc;
+    if (ctx->score_div) q.div = (float)qq;
     // alternate lanes so consecutive queries overlap (timing runs serialise on lane 0)
     const uint32_t lanes = ctx->timing_on ? 1u : (uint32_t)ctx->tune.lanes;
     Lane &L = ctx->lane[ctx->next_lane % lanes];
@@ -459,6 +461,10 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
     } else if (key == "batch_mask") {
         if (value < 0 || value > 2) return TOPK_ERR_INVALID;
         ctx->tune.batch_mask = (int)value;
+    } else if (key == "score_divide") {
+        if (value < 0 || value > 1) return TOPK_ERR_INVALID;
+        batch_quiesce(ctx);  // batches in flight complete under the mode they ran with
+        ctx->score_div = value != 0;
     } else {
         return TOPK_ERR_INVALID;
     }

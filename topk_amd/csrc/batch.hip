@@ -1413,10 +1413,52 @@ static int batch_drain(topk_ctx *ctx, BatchState &B) { return batch_wait_upto(ct
 // Validates, stages and launches one batch (both passes in one launch) in
 // the next slot; a busy slot holds the older of the two batches in flight
 // and is completed first.
+// Real-data scores (divided by qq after every term, topk_set_tuning
+// "score_divide"; the reference's LD == 2 findTopKsimdMQ, cpu/TPAc.h:237-240)
+// have no partition bounds here: every query of the batch is answered by
+// the FTE path, synchronously, and its ticket is complete on return.
+static int batch_submit_divided(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
+                                float *out_scores, float *out_thr, uint64_t *ticket) {
+    const double t0 = now_ms();
+    if (!ctx->batch) ctx->batch = new BatchState();
+    BatchState &B = *static_cast<BatchState *>(ctx->batch);
+    int rc = batch_drain(ctx, B);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < nq; i++) {
+        const topk_query &Q = qs[i];
+        if (Q.k < 1 || Q.k > ks || Q.qq < 1 || Q.qq > TOPK_MAX_DIMS) return TOPK_ERR_INVALID;
+        if (Q.k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    }
+    uint64_t candidates = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        const topk_query &Q = qs[i];
+        float t = 0;
+        rc = topk_fte(ctx, Q.k, (uint8_t)Q.qq, Q.weights, Q.attr, out_ids ? out_ids + (uint64_t)i * ks : nullptr,
+                      out_scores ? out_scores + (uint64_t)i * ks : nullptr, &t);
+        if (rc) return rc;
+        if (out_thr) out_thr[i] = t;
+        candidates += ctx->stats.candidates;
+    }
+    topk_stats &S = ctx->stats;
+    S = topk_stats{};
+    S.queries = nq;
+    S.fte_fallback = nq;
+    S.candidates = candidates;
+    S.tuple_count = (uint64_t)nq * ctx->n;
+    S.ms_last = now_ms() - t0;
+    if (ticket) *ticket = B.next_ticket;
+    B.next_ticket++;
+    return TOPK_OK;
+}
+
 static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t ks, uint64_t *out_ids,
                         float *out_scores, float *out_thr, const BatchVariant *force, uint64_t *ticket) {
     if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
     if (nq > BATCH_MAX_Q || ks > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    if (ctx->score_div) {
+        if (force) return TOPK_ERR_UNSUPPORTED;  // no launch shapes to time
+        return batch_submit_divided(ctx, nq, qs, ks, out_ids, out_scores, out_thr, ticket);
+    }
     const double t0 = now_ms();
     if (!ctx->pta.built) {
         const int rc = pta_build(ctx);
@@ -1496,6 +1538,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
                         topk_batch_tuning *out) {
     if (!ctx || !qs || nq == 0 || k_stride == 0) return TOPK_ERR_INVALID;
     if (nq > BATCH_MAX_Q || k_stride > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    if (ctx->score_div) return TOPK_ERR_UNSUPPORTED;  // real-data batches run the FTE path
     if (!ctx->pta.built) {
         const int rc = pta_build(ctx);
         if (rc) return rc;

@@ -67,6 +67,7 @@ __device__ __forceinline__ f4 score_vec(const QCols &q, uint64_t v) {
             f4 xv = ld_stream(q.col[j] + v * 4);
             f4 p = xv * q.w[j];
             acc = acc + p;
+            if (q.div != 0.0f) acc = acc / q.div;
         }
     }
     return acc;
@@ -500,6 +501,9 @@ static CandSet fte_launch(const FteRun &r, uint32_t k, const QCols &q, const Fin
     // sample mode with few winners: no merge in the seed pass, every stream
     // wave reduces the winners to tau itself
     const bool wave_tau = sampled && ks <= 4 && blocks * ks <= SEED_WAVE_MAX;
+    // kernels unrolled for the query width, or the runtime-width ones (the
+    // only ones that divide real-data scores)
+    const uint32_t msw = q.div != 0.0f ? 0u : q.m;
 #define TK_SEED(MM, OPT, MERGE)                                                                           \
     hipLaunchKernelGGL((fte_seed_kernel<MM, OPT, MERGE>), dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, \
                        ks, total_valid, r.blk, pre, r.st, r.gate)
@@ -508,11 +512,11 @@ static CandSet fte_launch(const FteRun &r, uint32_t k, const QCols &q, const Fin
 #define TK_SAMPLE(MM) \
     hipLaunchKernelGGL(fte_sample_kernel<MM>, dim3(blocks), dim3(SEED_BS), 0, s, q, n_eff, stride, ks, r.blk)
     if (wave_tau) {
-        TK_SWITCH_M(q.m, TK_SAMPLE);
+        TK_SWITCH_M(msw, TK_SAMPLE);
     } else if (sampled) {
-        TK_SWITCH_M(q.m, TK_SEED_SAMPLE);
+        TK_SWITCH_M(msw, TK_SEED_SAMPLE);
     } else {
-        TK_SWITCH_M(q.m, TK_SEED_EXACT);
+        TK_SWITCH_M(msw, TK_SEED_EXACT);
     }
 #undef TK_SEED
 #undef TK_SAMPLE
@@ -547,10 +551,10 @@ static CandSet fte_launch(const FteRun &r, uint32_t k, const QCols &q, const Fin
 #define TK_STREAM4(MM) TK_STREAM(MM, 4)
 #define TK_STREAM8(MM) TK_STREAM(MM, 8)
         switch (vpt) {
-            case 1: TK_SWITCH_M(q.m, TK_STREAM1); break;
-            case 2: TK_SWITCH_M(q.m, TK_STREAM2); break;
-            case 4: TK_SWITCH_M(q.m, TK_STREAM4); break;
-            default: TK_SWITCH_M(q.m, TK_STREAM8); break;
+            case 1: TK_SWITCH_M(msw, TK_STREAM1); break;
+            case 2: TK_SWITCH_M(msw, TK_STREAM2); break;
+            case 4: TK_SWITCH_M(msw, TK_STREAM4); break;
+            default: TK_SWITCH_M(msw, TK_STREAM8); break;
         }
 #undef TK_STREAM
 #undef TK_STREAM1

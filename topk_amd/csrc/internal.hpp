@@ -20,6 +20,10 @@ struct QCols {
     const float *col[TOPK_MAX_DIMS];
     float w[TOPK_MAX_DIMS];
     uint32_t m;
+    // Real-data scores (the reference's LD == 2 build, cpu/TPAc.h:143-146):
+    // non-zero = divide the running score by this (qq) after every term.
+    // Only the runtime-width kernels (M = 0) read it.
+    float div;
 };
 
 // Device-side state of one FTE / PTA query pipeline.
@@ -261,6 +265,8 @@ struct topk_ctx {
     topk_stats stats{};
     bool timing_on = false;
     topk::Tuning tune;
+    // FTE scores divide by qq after every term (topk_set_tuning "score_divide")
+    bool score_div = false;
     std::map<std::string, topk::Timing> timing;
     topk::PtaLayout pta;
     // learned predictor (owned by learned.hip)
