@@ -252,6 +252,7 @@ typedef struct topk_batch_tuning {
     uint32_t vpt, bs, qpi, digit, grid, item;
     uint32_t variants;           /* shapes timed */
     double ms_best, ms_default;  /* device time of one batch: chosen / defaults */
+    uint32_t fused;              /* 1: the query-major fused pass was chosen (batch_fused 2) */
 } topk_batch_tuning;
 
 int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32_t k_stride,
@@ -386,6 +387,12 @@ void *topk_stream(topk_ctx *ctx);
  *                     0 = query-major first pass, group-major second pass;
  *                     1 = query-major (one workgroup per query);
  *                     2 = group-major (one per partition group and query slice)
+ *   "batch_fused"     1 = partition-major work items (mask, compaction, a
+ *                     persistent scoring kernel); 2 = the query-major fused
+ *                     pass: one workgroup per query bounds the groups,
+ *                     partitions and chunks and scores the chunks that
+ *                     reach the threshold, with no work list (0 = 1, or the
+ *                     autotuner's choice)
  *   "score_divide"    1 = real-data scores: the running score is divided by
  *                     qq after every term, each step rounded to binary32
  *                     (the reference's LD == 2 build, cpu/TPAc.h:143-146,

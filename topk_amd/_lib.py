@@ -51,7 +51,7 @@ class TopkQuery(ctypes.Structure):
 class TopkBatchTuning(ctypes.Structure):
     _fields_ = [("vpt", _c.c_uint32), ("bs", _c.c_uint32), ("qpi", _c.c_uint32), ("digit", _c.c_uint32),
                 ("grid", _c.c_uint32), ("item", _c.c_uint32), ("variants", _c.c_uint32), ("ms_best", _c.c_double),
-                ("ms_default", _c.c_double)]
+                ("ms_default", _c.c_double), ("fused", _c.c_uint32)]
 
 
 class TopkLearnedConfig(ctypes.Structure):

@@ -461,6 +461,9 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
     } else if (key == "batch_mask") {
         if (value < 0 || value > 2) return TOPK_ERR_INVALID;
         ctx->tune.batch_mask = (int)value;
+    } else if (key == "batch_fused") {
+        if (value < 0 || value > 2) return TOPK_ERR_INVALID;
+        ctx->tune.batch_fused = (int)value;
     } else if (key == "score_divide") {
         if (value < 0 || value > 1) return TOPK_ERR_INVALID;
         batch_quiesce(ctx);  // batches in flight complete under the mode they ran with

@@ -101,12 +101,13 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
                                                         uint32_t d, BatchQuery *__restrict__ out,
                                                         BoundArgs *__restrict__ outb, unsigned int *__restrict__ pcnt,
                                                         uint32_t L, unsigned int *__restrict__ ccount,
-                                                        unsigned int *__restrict__ ctr) {
+                                                        unsigned int *__restrict__ ctr, unsigned int *__restrict__ flist) {
     const uint32_t lane = lane_id();
     const uint64_t stride = (uint64_t)gridDim.x * 64;
     for (uint64_t i = (uint64_t)blockIdx.x * 64 + lane; i < L; i += stride) pcnt[i] = 0u;
     for (uint64_t i = (uint64_t)blockIdx.x * 64 + lane; i < nq; i += stride) ccount[i] = 0u;
     if (blockIdx.x == 0 && lane < BCTR_WORDS) ctr[lane] = 0u;
+    if (blockIdx.x == 0 && lane == 0 && flist) flist[0] = 0u;
     const uint32_t q = blockIdx.x;
     if (q >= nq) return;
     const topk_query &Q = raw[q];
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
 
 // Bound tree of the partition layout as the batch kernels read it.
 struct TreeArgs {
-    const float *tmax, *tmin, *tsum, *tminsum;
+    const float *tmax, *tmin, *tsum, *tminsum;  // tmax / tmin attribute-major: [a * 2*p2 + v]
     uint32_t p2, d;
 };
 
@@ -190,7 +191,7 @@ struct TreeArgs {
 template <int NB>
 __device__ __forceinline__ float node_bound(const BoundArgs &b, const TreeArgs &t, uint32_t v) {
     const float ts = t.tsum[v];
-    const float ub = lp_bound<NB>(b, t.tmax + (uint64_t)v * t.d, t.tmin + (uint64_t)v * t.d, ts, t.tminsum[v]);
+    const float ub = lp_bound<NB>(b, t.tmax + v, t.tmin + v, ts, t.tminsum[v], 2 * (uint64_t)t.p2);
     return ts == -INFINITY ? -INFINITY : ub;
 }
 
@@ -519,8 +520,9 @@ struct ScoreArgs {
     const BatchQuery *q;
     uint64_t *cand;
     unsigned int *ccount;
-    const float *cstat;          // chunk statistics rows (crow floats each)
-    uint32_t crow;
+    const float *cstat;          // chunk statistics, element e of chunk c at [e * nchunks + c]
+    uint32_t crow;               // elements per chunk
+    uint64_t nchunks;
     unsigned long long *scored;  // per workgroup: (object, query) pairs scored
 };
 
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 if constexpr (LANE_W) {
                     const uint32_t dd = a.d;
                     const uint32_t c0 = (start + wb) / PTA_CHUNK;
-                    const float cs = lane < a.crow ? a.cstat[(uint64_t)c0 * a.crow + lane] : 0.0f;  // lane e: element e
+                    const float cs = lane < a.crow ? a.cstat[(uint64_t)lane * a.nchunks + c0] : 0.0f;  // lane e: element e
                     const float csum = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd)));
                     const float cminsum =
                         __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs), (int)(2 * dd + 1)));
@@ -778,6 +780,7 @@ struct FinalArgs {
     int seed2;               // the second pass runs a seed threshold pass
     unsigned int *pcnt;      // first pass: cleared here for the second
     uint32_t L;
+    unsigned int *flist;     // first pass: queries flagged for the second ([0] count, then ids); may be null
 };
 
 template <int DB>
@@ -829,6 +832,7 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
             a.q[q].tau = rkey_score(kth);
             a.q[q].tau_id = rkey_id(kth);
             a.qflag[q] = QF_GIVEN;
+            if (a.flist) a.flist[1 + atomicAdd(&a.flist[0], 1u)] = q;
         }
         return;
     }
@@ -848,6 +852,7 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
                     // host's exact path (status RETRY stays)
                     a.qflag[q] = a.seed2 ? QF_SEED : QF_DONE;
                 }
+                if (a.flist && a.qflag[q] != QF_DONE) a.flist[1 + atomicAdd(&a.flist[0], 1u)] = q;
             }
         }
         return;
@@ -859,21 +864,460 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// host
+// query-major pass
 // ---------------------------------------------------------------------------
-struct BatchVariant {
-    int vpt, bs, qpi, digit, grid;
-    int item;  // objects per work item (a multiple of PTA_CHUNK)
+// Three launches replace the mask, the compaction and the item scoring when
+// thresholds are tight (a trained predictor): each query then reaches a few
+// dozen partitions and a few chunks in each, and the time of the item path
+// goes to list building, contended partition counters and dependent loads.
+//
+//  1. batch_cut_kernel, one workgroup per (query, slice s of S): the bound of
+//     every group of 2^H partitions of the slice (groups s, s + S, ...);
+//     the bounds of every partition of a group that reaches tau (BQ_U in
+//     flight per thread); then, for the partitions that reach tau, the
+//     suffix-row bound of every chunk (one thread per chunk over an
+//     exclusive scan of their chunk counts): the first failing chunk cuts
+//     the partition (every later chunk fails too).  The workgroup's unit
+//     u = q * S + s lists (partition, first chunk index) pairs in
+//     tasks[u * ucap ...], chunk indices as an exclusive prefix over the
+//     list, and ucnt[u] = (entries, chunks).
+//  2. batch_unit_scan_kernel (one workgroup): exclusive prefix of the units'
+//     chunk counts, uoff[0..U].
+//  3. batch_chunk_kernel: the uoff[U] chunks of all units are split into
+//     equal contiguous ranges, one per wave; a wave walks its range (its
+//     first unit and list entry found by a 64-way search per level), scores
+//     each chunk (4 objects per lane, the query's own term order) and
+//     appends the keys that reach (tau, tau_id) to the query's candidate
+//     list (one atomic per wave and chunk on the query's counter).  A query
+//     with many chunks is spread over many waves, so no query's chain sets
+//     the pass's time.
+constexpr int BQ_BS = 256;
+constexpr uint32_t BQ_PCAP = 1024;       // listed partitions per flush (4 per thread in the scans)
+constexpr uint32_t BQ_EPT = BQ_PCAP / BQ_BS;
+constexpr uint32_t BQ_MAX_S = 64;        // slices per query
+// A full batch of 1024 queries keeps every workgroup resident at 4 per CU
+// when a wave holds at most 128 VGPRs (4 waves per SIMD).
+constexpr int bq_waves_per_simd(int nb) { return nb <= 16 ? 4 : 2; }
+// partition bounds in flight per thread
+constexpr int bq_unroll(int nb) { return nb <= 8 ? 2 : 1; }
+constexpr int BC_BS = 256;               // chunk-scoring workgroup
+
+struct CutArgs {
+    const BatchQuery *q;
+    const BoundArgs *qb;
+    TreeArgs t;
+    uint32_t G, H, L;
+    uint32_t d;
+    const uint32_t *poff, *psz;
+    const float *cstat;  // element e of chunk c at [e * nchunks + c]
+    uint64_t nchunks;
+    uint2 *tasks;        // unit u: tasks[u * ucap ...] = (partition, first chunk index within the unit)
+    uint64_t ucap;
+    uint2 *ucnt;         // unit u: (entries, chunks)
+    unsigned int *ctr;
+    const uint32_t *qflag;  // second pass: settled queries are skipped
+    uint32_t S;             // slices per query (gridDim.y)
+    const uint32_t *flist;  // second pass: the flagged queries ([0] count); workgroup x takes entries x, x + gridDim.x, ...
 };
 
-using GraphKey = std::array<uint64_t, 15>;
+// Index i < n of the last entry with off[i] <= t (off non-decreasing, off[0] = 0).
+__device__ __forceinline__ uint32_t last_le(const uint32_t *off, uint32_t n, uint32_t t) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Exclusive scan of cnt[0..n) (n <= BQ_PCAP) into off[0..n], BQ_EPT
+// consecutive entries per thread.  All threads call it.
+__device__ __forceinline__ void bq_scan(const uint32_t *cnt, uint32_t *off, uint32_t n, uint32_t *wsum,
+                                        uint32_t *tot) {
+    const uint32_t i0 = BQ_EPT * threadIdx.x;
+    uint32_t c[BQ_EPT], mine = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < BQ_EPT; e++) {
+        c[e] = i0 + e < n ? cnt[i0 + e] : 0u;
+        mine += c[e];
+    }
+    uint32_t run = block_excl_scan<BQ_BS>(mine, wsum, tot);
+#pragma unroll
+    for (uint32_t e = 0; e < BQ_EPT; e++) {
+        if (i0 + e < n) off[i0 + e] = run;
+        run += c[e];
+    }
+    if (threadIdx.x == 0) off[n] = *tot;
+}
+
+// Cuts of the np listed partitions lp[0..np) (positions lpos) and their
+// task entries at out[nent ...], chunk indices from nch0 on; all threads
+// call it.  Returns the chunks listed.
+template <int NB>
+__device__ uint32_t bq_cut_listed(const CutArgs &a, const BoundArgs &b, float tau, const uint32_t *lp,
+                                  const uint32_t *lpos, uint32_t np, uint32_t *lcnt, uint32_t *loff, uint32_t *wsum,
+                                  uint32_t *tot, uint2 *out, uint32_t nent, uint32_t nch0) {
+    const uint32_t tid = threadIdx.x;
+    bq_scan(lcnt, loff, np, wsum, tot);
+    __syncthreads();
+    // suffix-row bounds, R rows per thread in flight
+    constexpr int R = bq_unroll(NB);
+    const uint32_t nch = loff[np];
+    const uint64_t S = a.nchunks;
+    for (uint32_t t0 = tid; t0 < nch; t0 += R * BQ_BS) {
+        float ub[R];
+        uint32_t ii[R], cc[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t t = t0 + r * BQ_BS;
+            ub[r] = INFINITY;
+            ii[r] = 0;
+            cc[r] = 0;
+            if (t < nch) {
+                ii[r] = last_le(loff, np, t);
+                cc[r] = t - loff[ii[r]];
+                const float *row = a.cstat + (lpos[ii[r]] / PTA_CHUNK + cc[r]);  // element e at row[e * S]
+                ub[r] = lp_bound<NB>(b, row, row + a.d * S, row[2 * a.d * S], row[(2 * a.d + 1) * S], S);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (ub[r] < tau) atomicMin(&lcnt[ii[r]], cc[r]);
+    }
+    __syncthreads();
+    bq_scan(lcnt, loff, np, wsum, tot);
+    __syncthreads();
+    for (uint32_t i = tid; i < np; i += BQ_BS) out[nent + i] = make_uint2(lp[i], nch0 + loff[i]);
+    const uint32_t listed = loff[np];
+    __syncthreads();  // the caller reuses the lists
+    return listed;
+}
+
+template <int NB>
+__global__ __launch_bounds__(BQ_BS, bq_waves_per_simd(NB)) void batch_cut_kernel(CutArgs a) {
+    constexpr int U = bq_unroll(NB);
+    __shared__ uint16_t glist[BMASK_PAIRS];
+    __shared__ uint32_t lp[BQ_PCAP], lpos[BQ_PCAP], lcnt[BQ_PCAP], loff[BQ_PCAP + 1];
+    __shared__ uint32_t wsum[BQ_BS / 64];
+    __shared__ uint32_t ng, np, tot;
+    const uint32_t sl = blockIdx.y, tid = threadIdx.x;
+    const uint32_t nent_list = a.flist ? a.flist[0] : gridDim.x;
+    for (uint32_t f = blockIdx.x; f < nent_list; f += gridDim.x) {
+    const uint32_t q = a.flist ? a.flist[1 + f] : f;
+    const uint64_t u = (uint64_t)f * a.S + sl;
+    if (a.qflag && a.qflag[q] == QF_DONE) {
+        if (tid == 0) a.ucnt[u] = make_uint2(0u, 0u);
+        continue;
+    }
+    const BoundArgs &b = a.qb[q];
+    const float tau = a.q[q].tau;
+    if (tid == 0) {
+        ng = 0;
+        np = 0;
+    }
+    __syncthreads();
+    // groups of this slice
+    for (uint32_t g = sl + a.S * tid; g < a.G; g += a.S * BQ_BS) {
+        const float ub = node_bound<NB>(b, a.t, a.G + g);
+        if (ub > -INFINITY && ub >= tau) glist[atomicAdd(&ng, 1u)] = (uint16_t)g;
+    }
+    __syncthreads();
+    const uint32_t H = a.H, per = 1u << H, total = ng << H;
+    constexpr uint32_t STEP = BQ_BS * U;  // partition bounds per step
+    uint2 *out = a.tasks + u * a.ucap;
+    uint32_t parts = 0, nent = 0, nchk = 0;
+    auto flush = [&]() {
+        const uint32_t n = np;
+        for (uint32_t i = tid; i < n; i += BQ_BS) {
+            const uint32_t p = lp[i];
+            lpos[i] = a.poff[p];
+            lcnt[i] = (a.psz[p] + PTA_CHUNK - 1) / PTA_CHUNK;
+        }
+        __syncthreads();
+        nchk += bq_cut_listed<NB>(a, b, tau, lp, lpos, n, lcnt, loff, wsum, &tot, out, nent, nchk);
+        nent += n;
+        parts += n;
+        if (tid == 0) np = 0;
+        __syncthreads();
+    };
+    for (uint32_t i0 = 0; i0 < total; i0 += STEP) {
+        if (np > BQ_PCAP - STEP) flush();  // uniform: np is read after a barrier
+        uint32_t p[U];
+        bool in[U];
+#pragma unroll
+        for (int v = 0; v < U; v++) {
+            const uint32_t i = i0 + v * BQ_BS + tid;
+            p[v] = i < total ? ((uint32_t)glist[i >> H] << H) + (i & (per - 1)) : a.L;
+            const float ub = p[v] < a.L ? node_bound<NB>(b, a.t, a.t.p2 + p[v]) : -INFINITY;
+            in[v] = ub > -INFINITY && ub >= tau;
+        }
+#pragma unroll
+        for (int v = 0; v < U; v++)
+            if (in[v]) lp[atomicAdd(&np, 1u)] = p[v];
+        __syncthreads();
+    }
+    if (np) flush();
+    if (tid == 0) {
+        a.ucnt[u] = make_uint2(nent, nchk);
+        if (parts) atomicAdd(reinterpret_cast<unsigned long long *>(a.ctr + 2), (unsigned long long)parts);
+    }
+    __syncthreads();  // the next entry reuses the lists
+    }
+}
+
+// Exclusive prefix of the units' chunk counts into uoff[0..nu] (nu =
+// nu_host, or the flagged-list count x S); one workgroup of 1024 threads,
+// contiguous ranges per thread, 16 loads in flight.
+__global__ __launch_bounds__(1024) void batch_unit_scan_kernel(const uint2 *__restrict__ ucnt, uint32_t nu_host,
+                                                               const uint32_t *__restrict__ flist, uint32_t S,
+                                                               uint32_t *__restrict__ uoff) {
+    __shared__ uint32_t wsum[1024 / 64];
+    __shared__ uint32_t tot;
+    const uint32_t nu = flist ? flist[0] * S : nu_host;
+    const uint32_t per = (nu + 1023) / 1024, b0 = threadIdx.x * per, b1 = b0 + per < nu ? b0 + per : nu;
+    uint32_t mine = 0;
+    for (uint32_t i0 = b0; i0 < b1; i0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = i0 + r < b1 ? ucnt[i0 + r].y : 0u;
+#pragma unroll
+        for (int r = 0; r < 16; r++) mine += v[r];
+    }
+    uint32_t run = block_excl_scan<1024>(mine, wsum, &tot);
+    for (uint32_t i0 = b0; i0 < b1; i0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = i0 + r < b1 ? ucnt[i0 + r].y : 0u;
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            if (i0 + r < b1) {
+                uoff[i0 + r] = run;
+                run += v[r];
+            }
+    }
+    if (threadIdx.x == 0) uoff[nu] = tot;
+}
+
+struct ChunkArgs {
+    const BatchQuery *q;
+    const float *pcols;
+    uint64_t ld;
+    const uint32_t *poff, *psz, *pid;
+    const uint2 *tasks;
+    uint64_t ucap;
+    const uint2 *ucnt;
+    const uint32_t *uoff;
+    uint32_t nu, S;            // units (nu_host) and slices per query
+    const uint32_t *flist;     // second pass: unit u belongs to query flist[1 + u / S] ([0] count)
+    uint64_t *cand;
+    unsigned int *ccount;
+    unsigned long long *scored;  // per workgroup
+};
+
+// The last index i in [lo, hi) with key(i) <= t (key non-decreasing, key(lo)
+// <= t), found by the wave: 64 probes per round narrow the range 64-fold.
+template <class Key>
+__device__ __forceinline__ uint32_t wave_last_le(Key key, uint32_t lo, uint32_t hi, uint32_t t) {
+    const uint32_t lane = lane_id();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t i = lo + lane * step;
+        const bool le = i < hi && key(i) <= t;
+        const uint64_t m = __ballot(le);  // lanes 0..j (keys are non-decreasing)
+        const uint32_t j = 63u - (uint32_t)__builtin_clzll(m);
+        lo = lo + j * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    return lo;
+}
+
+constexpr uint32_t BC_STAGE = 256;  // candidate keys a wave stages in LDS before one append
+
+__global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
+    __shared__ unsigned long long wpairs[BC_BS / 64];
+    __shared__ uint64_t stage[BC_BS / 64][BC_STAGE];
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    const uint32_t nwaves = gridDim.x * (BC_BS / 64), gw = blockIdx.x * (BC_BS / 64) + wave;
+    const uint32_t nu = a.flist ? a.flist[0] * a.S : a.nu;
+    const uint32_t total = a.uoff[nu];
+    const uint32_t per = (total + nwaves - 1) / nwaves;
+    uint32_t t = gw * per;
+    const uint32_t t_end = t + per < total ? t + per : total;
+    uint32_t pairs = 0;
+    uint64_t *wst = stage[wave];
+    if (t < t_end) {
+        const uint32_t *uoff = a.uoff;
+        uint32_t u = wave_last_le([&](uint32_t i) { return uoff[i]; }, 0u, nu, t);
+        uint32_t ub = uoff[u], ue = uoff[u + 1];
+        const uint2 *tl = a.tasks + (uint64_t)u * a.ucap;
+        uint32_t ne = a.ucnt[u].x;
+        uint32_t j = wave_last_le([&](uint32_t i) { return tl[i].y; }, 0u, ne, t - ub);
+        auto query_of = [&](uint32_t uu) { return a.flist ? a.flist[1 + uu / a.S] : uu / a.S; };
+        uint32_t q = query_of(u);
+        // the query's terms, lane j holding term j
+        float lw = 0.0f, tau = 0.0f;
+        uint32_t la = 0, tau_id = 0, m = 0;
+        auto load_query = [&]() {
+            const BatchQuery &Q = a.q[q];
+            lw = lane < TOPK_MAX_DIMS ? Q.w[lane] : 0.0f;
+            la = lane < TOPK_MAX_DIMS ? Q.attr[lane] : 0u;
+            tau = Q.tau;
+            tau_id = Q.tau_id;
+            m = Q.m;
+        };
+        load_query();
+        // staged candidates of query q (wave-uniform count); one append each flush
+        uint32_t nst = 0;
+        auto flush = [&]() {
+            if (!nst) return;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&a.ccount[q], nst);
+            base = __shfl(base, 0, 64);
+            uint64_t *out = a.cand + (uint64_t)q * BATCH_CAP;
+            for (uint32_t i = lane; i < nst; i += 64)
+                if (base + i < BATCH_CAP) out[base + i] = wst[i];
+            nst = 0;
+            __builtin_amdgcn_wave_barrier();  // the stage is rewritten after every lane read it
+        };
+        uint32_t jb = tl[j].y, je = j + 1 < ne ? tl[j + 1].y : ue - ub;
+        uint32_t p = tl[j].x;
+        for (; t < t_end; t++) {
+            while (t >= ue) {  // next unit with chunks
+                u++;
+                ub = ue;
+                ue = uoff[u + 1];
+                if (t < ue) {
+                    tl = a.tasks + (uint64_t)u * a.ucap;
+                    ne = a.ucnt[u].x;
+                    j = 0;
+                    jb = 0;
+                    je = ne > 1 ? tl[1].y : ue - ub;
+                    p = tl[0].x;
+                    const uint32_t qn = query_of(u);
+                    if (qn != q) {
+                        flush();
+                        q = qn;
+                        load_query();
+                    }
+                }
+            }
+            const uint32_t lt = t - ub;
+            while (lt >= je) {  // next entry with chunks
+                j++;
+                jb = je;
+                je = j + 1 < ne ? tl[j + 1].y : ue - ub;
+                p = tl[j].x;
+            }
+            const uint32_t c = lt - jb, sz = a.psz[p];
+            const uint32_t first = c * PTA_CHUNK + 4 * lane;
+            const uint64_t pos0 = (uint64_t)a.poff[p] + first;
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (uint32_t j0 = 0; j0 < m; j0 += 4) {
+                f4 x[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (j0 + v < m) {
+                        const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)la, (int)(j0 + v));
+                        x[v] = *reinterpret_cast<const f4 *>(a.pcols + (uint64_t)at * a.ld + pos0);
+                    }
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (j0 + v < m) {
+                        const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lw), (int)(j0 + v)));
+                        f4 pr = x[v] * w;
+                        acc = acc + pr;
+                    }
+            }
+            const uint32_t nvalid = first < sz ? (sz - first < 4u ? sz - first : 4u) : 0u;
+            pairs += nvalid;
+            uint64_t kk[4];
+            uint32_t nin = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                bool in = (uint32_t)e < nvalid && acc[e] >= tau;
+                uint32_t id = 0;
+                if (in) {
+                    id = a.pid[pos0 + e];
+                    if (acc[e] == tau && id > tau_id) in = false;
+                }
+                kk[e] = in ? rkey(acc[e], id) : 0ull;
+                nin += in ? 1u : 0u;
+            }
+            if (__ballot(nin != 0)) {
+                const uint32_t incl = wave_incl_scan(nin);
+                const uint32_t tot = __shfl(incl, 63, 64);
+                if (nst + tot > BC_STAGE) flush();
+                uint32_t pos = nst + incl - nin;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (kk[e]) {
+                        wst[pos] = kk[e];
+                        pos++;
+                    }
+                }
+                nst += tot;
+            }
+        }
+        flush();
+    }
+    unsigned long long s = pairs;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) wpairs[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BC_BS / 64; w++) s += wpairs[w];
+        a.scored[blockIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+// Slices per query of the query-major pass: 1 in a first pass (every
+// query its own workgroup), up to BQ_MAX_S in a second (few queries).
+static uint32_t bq_slices(uint32_t nq, bool second) {
+    return second ? std::max(1u, std::min(BQ_MAX_S, BATCH_MAX_GRID / nq)) : 1u;
+}
+
+// Task-list entries both passes need (a unit lists at most its groups'
+// partitions); 0 when they exceed the memory budget.
+static uint64_t bq_task_entries(uint32_t nq, uint32_t p2, uint32_t H) {
+    const uint32_t G = p2 >> H;
+    uint64_t need = 0;
+    for (int second = 0; second < 2; second++) {
+        const uint32_t S = bq_slices(nq, second != 0);
+        need = std::max(need, (uint64_t)nq * S * ((uint64_t)((G + S - 1) / S) << H));
+    }
+    constexpr uint64_t BUDGET = 4ull << 30;  // bytes
+    return need * sizeof(uint2) <= BUDGET ? need : 0;
+}
+
+struct BatchVariant {
+    int vpt, bs, qpi, digit, grid;
+    int item;   // objects per work item (a multiple of PTA_CHUNK)
+    int fused;  // 1: the query-major fused pass replaces mask, compaction and item scoring
+};
+
+using GraphKey = std::array<uint64_t, 16>;
 
 // Pinned staging of one batch in flight, and where its results go.
+// One batch in flight: its pinned staging, its device inputs and outputs,
+// and the events that order its upload, kernels and result copy.
 struct HostSlot {
     unsigned char *h_in = nullptr, *h_out = nullptr;
     size_t in_cap = 0, out_cap = 0;
-    hipEvent_t done = nullptr;  // recorded after the batch's result copy
+    unsigned char *d_in = nullptr;   // raw queries | taus | BatchQuery[nq] | BoundArgs[nq]
+    unsigned char *d_out = nullptr;  // ids | scores | thr | status | cnt | flags | ctr
+    size_t din_cap = 0, dout_cap = 0;
+    hipEvent_t in_ready = nullptr;   // recorded after the upload (upload stream)
+    hipEvent_t kdone = nullptr;      // recorded after the kernels (compute stream)
+    hipEvent_t done = nullptr;       // recorded after the result copy (download stream)
     bool busy = false;
+    bool direct = false;             // ids / scores / thresholds were copied straight into the caller's arrays
     uint64_t ticket = 0;
     uint32_t nq = 0, ks = 0;
     uint64_t *out_ids = nullptr;
@@ -881,17 +1325,39 @@ struct HostSlot {
     double t0 = 0;
 };
 
+// True when p is pinned (page-locked) host memory the GPU can copy into.
+static bool pinned_host(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t at;
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory: an expected miss, not an error to report later
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
+
+// Waits for an event by polling (a blocking wait can add tens of
+// microseconds of wake-up latency per batch), then confirms it.
+static hipError_t wait_event(hipEvent_t e) {
+    for (int i = 0; i < (1 << 22); i++) {
+        const hipError_t q = hipEventQuery(e);
+        if (q != hipErrorNotReady) return q;
+    }
+    return hipEventSynchronize(e);
+}
+
 struct BatchState {
     uint64_t gen = ~0ull;   // layout generation the buffers and graphs belong to
     uint32_t nq_cap = 0;
-    size_t din_cap = 0, dout_cap = 0;
     uint64_t plist_cap = 0, pcnt_cap = 0, item_cap = 0;
-    unsigned char *d_in = nullptr;   // raw queries | taus | BatchQuery[nq] | BoundArgs[nq]
-    unsigned char *d_out = nullptr;  // ids | scores | thr | status | cnt | flags | ctr
-    // Two batches may be in flight (topk_batch_submit): each stages through
-    // its own pinned slot; the device buffers are shared because the batches
-    // run one after the other on one stream.
+    // Two batches may be in flight (topk_batch_submit): each has its own slot
+    // (staging, inputs, outputs), so one batch's upload and result copy run
+    // on the copy streams while the other's kernels run; the scratch below is
+    // shared because the kernels of the batches run one after the other on
+    // the compute stream.
     HostSlot slot[2];
+    hipStream_t up = nullptr, down = nullptr;  // upload / result-copy streams (default priority)
     int cur = 0;                     // slot of the batch being enqueued
     int next = 0;                    // slot the next submit uses
     uint64_t next_ticket = 1;
@@ -900,6 +1366,14 @@ struct BatchState {
     BatchItem *d_items = nullptr;
     unsigned int *d_ctr = nullptr;
     unsigned long long *d_scored = nullptr;  // per scoring workgroup: pairs scored
+    // query-major pass: task lists per (query, slice) unit, their counts and
+    // the prefix of their chunk counts (null when over the memory budget)
+    uint2 *d_tasks = nullptr;
+    uint64_t tasks_cap = 0;
+    uint2 *d_ucnt = nullptr;
+    uint32_t *d_uoff = nullptr;
+    uint32_t unit_cap = 0;
+    unsigned int *d_flist = nullptr;  // queries the first pass flagged for the second: [0] count, then ids
     unsigned int *d_ccount = nullptr;
     uint64_t *d_cand = nullptr;
     int cus = 0;
@@ -950,10 +1424,14 @@ void batch_free(topk_ctx *ctx) {
     BatchState *B = static_cast<BatchState *>(ctx->batch);
     if (!B) return;
     hipStreamSynchronize(ctx->lane[0].stream);  // batches still in flight
+    if (B->up) hipStreamSynchronize(B->up);
+    if (B->down) hipStreamSynchronize(B->down);
     drop_graphs(*B);
-    hipFree(B->d_in);
-    hipFree(B->d_out);
     hipFree(B->d_plist);
+    hipFree(B->d_tasks);
+    hipFree(B->d_ucnt);
+    hipFree(B->d_uoff);
+    hipFree(B->d_flist);
     hipFree(B->d_pcnt);
     hipFree(B->d_items);
     hipFree(B->d_ctr);
@@ -963,8 +1441,14 @@ void batch_free(topk_ctx *ctx) {
     for (HostSlot &h : B->slot) {
         if (h.h_in) hipHostFree(h.h_in);
         if (h.h_out) hipHostFree(h.h_out);
+        hipFree(h.d_in);
+        hipFree(h.d_out);
+        if (h.in_ready) hipEventDestroy(h.in_ready);
+        if (h.kdone) hipEventDestroy(h.kdone);
         if (h.done) hipEventDestroy(h.done);
     }
+    if (B->up) hipStreamDestroy(B->up);
+    if (B->down) hipStreamDestroy(B->down);
     delete B;
     ctx->batch = nullptr;
 }
@@ -1000,9 +1484,11 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
     const size_t in_need = InLayout::make(nq).bytes, out_need = OutLayout::make(nq, ks).bytes;
     const uint64_t items_need =
         (uint64_t)L * (4 * nw + 1) * ((P.maxsz + ITEM_MIN_OBJECTS - 1) / ITEM_MIN_OBJECTS + 1);
-    const bool grow = B.gen != P.generation || nq > B.nq_cap || in_need > B.din_cap || out_need > B.dout_cap ||
+    const uint64_t te = bq_task_entries(nq, P.p2, group_height(P.p2));
+    const bool grow = B.gen != P.generation || nq > B.nq_cap || in_need > H.din_cap || out_need > H.dout_cap ||
                       in_need > H.in_cap || out_need > H.out_cap || (uint64_t)L * nq > B.plist_cap ||
-                      (uint64_t)L > B.pcnt_cap || items_need > B.item_cap || !B.d_ctr || !H.done;
+                      (uint64_t)L > B.pcnt_cap || items_need > B.item_cap || !B.d_ctr || !H.done || !B.up ||
+                      te > B.tasks_cap || (te && nq * BQ_MAX_S > B.unit_cap) || (!te && B.d_tasks);
     if (!grow) return TOPK_OK;
     int rc = batch_drain(ctx, B);
     if (rc) return rc;
@@ -1016,29 +1502,32 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
         while (cap < nq) cap *= 2;
         hipFree(B.d_ccount);
         hipFree(B.d_cand);
+        hipFree(B.d_flist);
         B.d_ccount = nullptr;
         B.d_cand = nullptr;
+        B.d_flist = nullptr;
         B.nq_cap = 0;
         changed = true;
+        TK_HIP(hipMalloc(&B.d_flist, sizeof(unsigned int) * (cap + 1)));
         TK_HIP(hipMalloc(&B.d_ccount, sizeof(unsigned int) * cap));
         TK_HIP(hipMalloc(&B.d_cand, sizeof(uint64_t) * BATCH_CAP * cap));
         B.nq_cap = cap;
     }
-    if (in_need > B.din_cap) {
-        hipFree(B.d_in);
-        B.d_in = nullptr;
-        B.din_cap = 0;
+    if (in_need > H.din_cap) {
+        hipFree(H.d_in);
+        H.d_in = nullptr;
+        H.din_cap = 0;
         changed = true;
-        TK_HIP(hipMalloc(&B.d_in, in_need));
-        B.din_cap = in_need;
+        TK_HIP(hipMalloc(&H.d_in, in_need));
+        H.din_cap = in_need;
     }
-    if (out_need > B.dout_cap) {
-        hipFree(B.d_out);
-        B.d_out = nullptr;
-        B.dout_cap = 0;
+    if (out_need > H.dout_cap) {
+        hipFree(H.d_out);
+        H.d_out = nullptr;
+        H.dout_cap = 0;
         changed = true;
-        TK_HIP(hipMalloc(&B.d_out, out_need));
-        B.dout_cap = out_need;
+        TK_HIP(hipMalloc(&H.d_out, out_need));
+        H.dout_cap = out_need;
     }
     if (in_need > H.in_cap) {
         if (H.h_in) hipHostFree(H.h_in);
@@ -1056,12 +1545,44 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
         TK_HIP(hipHostMalloc(&H.h_out, out_need, hipHostMallocDefault));
         H.out_cap = out_need;
     }
-    if (!H.done) TK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
+    if (!H.done) {
+        TK_HIP(hipEventCreateWithFlags(&H.in_ready, hipEventDisableTiming));
+        TK_HIP(hipEventCreateWithFlags(&H.kdone, hipEventDisableTiming));
+        TK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
+    }
+    if (!B.up) {
+        TK_HIP(hipStreamCreateWithFlags(&B.up, hipStreamNonBlocking));
+        TK_HIP(hipStreamCreateWithFlags(&B.down, hipStreamNonBlocking));
+    }
     rc = grow_dev(B.d_plist, B.plist_cap, (uint64_t)L * nq, changed);
     if (!rc) rc = grow_dev(B.d_pcnt, B.pcnt_cap, (uint64_t)L, changed);
     // budget >= 16 queries, chunks >= ITEM_MIN_OBJECTS objects
     if (!rc) rc = grow_dev(B.d_items, B.item_cap, items_need, changed);
     if (rc) return rc;
+    {
+        const uint32_t units = nq * BQ_MAX_S;
+        if (!te && B.d_tasks) {  // over the budget for this batch size: the item path runs
+            hipFree(B.d_tasks);
+            B.d_tasks = nullptr;
+            B.tasks_cap = 0;
+            changed = true;
+        }
+        if (te) {
+            rc = grow_dev(B.d_tasks, B.tasks_cap, te, changed);
+            if (!rc && units > B.unit_cap) {
+                hipFree(B.d_ucnt);
+                hipFree(B.d_uoff);
+                B.d_ucnt = nullptr;
+                B.d_uoff = nullptr;
+                B.unit_cap = 0;
+                changed = true;
+                TK_HIP(hipMalloc(&B.d_ucnt, sizeof(uint2) * units));
+                TK_HIP(hipMalloc(&B.d_uoff, sizeof(uint32_t) * (units + 1)));
+                B.unit_cap = units;
+            }
+            if (rc) return rc;
+        }
+    }
     if (!B.d_ctr) {
         changed = true;
         TK_HIP(hipMalloc(&B.d_ctr, sizeof(unsigned int) * BCTR_WORDS));
@@ -1094,6 +1615,7 @@ static BatchVariant pick_variant(const topk_ctx *ctx, const BatchState &B) {
     v.digit = ctx->tune.batch_digit ? ctx->tune.batch_digit : 11;
     v.grid = ctx->tune.batch_grid ? ctx->tune.batch_grid : B.cus * (v.bs == 256 ? 4 : 2);
     v.item = ctx->tune.batch_item ? ctx->tune.batch_item : 4096;
+    v.fused = ctx->tune.batch_fused == 2 ? 1 : 0;
     return v;
 }
 
@@ -1128,9 +1650,10 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     const uint32_t L = P.nparts;
     const InLayout il = InLayout::make(nq);
     const OutLayout ol = OutLayout::make(nq, ks);
-    BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
-    BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
-    uint32_t *qflag = reinterpret_cast<uint32_t *>(B.d_out + ol.fl_off);
+    unsigned char *d_in = B.slot[B.cur].d_in, *d_out = B.slot[B.cur].d_out;
+    BatchQuery *dq = reinterpret_cast<BatchQuery *>(d_in + il.q_off);
+    BoundArgs *db = reinterpret_cast<BoundArgs *>(d_in + il.b_off);
+    uint32_t *qflag = reinterpret_cast<uint32_t *>(d_out + ol.fl_off);
     const uint32_t *pass_flags = second ? qflag : nullptr;
     // counters: cleared by the prep kernel for the first pass, by the first
     // pass's final kernel for the second
@@ -1161,6 +1684,53 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     // a query's group list and a group's partition counters live in LDS
     if (G > BMASK_PAIRS) return TOPK_ERR_UNSUPPORTED;  // over 262144 partitions: use larger partitions
     const int NB = dims_class(ctx->d);
+    FinalArgs fa{dq,
+                 B.d_cand,
+                 B.d_ccount,
+                 B.d_ctr,
+                 ks,
+                 reinterpret_cast<uint64_t *>(d_out + ol.ids_off),
+                 reinterpret_cast<float *>(d_out + ol.sc_off),
+                 reinterpret_cast<float *>(d_out + ol.thr_off),
+                 reinterpret_cast<uint32_t *>(d_out + ol.st_off),
+                 reinterpret_cast<uint32_t *>(d_out + ol.cnt_off),
+                 reinterpret_cast<uint32_t *>(d_out + (second ? ol.ctr2_off : ol.ctr_off)),
+                 qflag,
+                 second ? 1 : 0,
+                 B.d_scored,
+                 0u,
+                 seed2 ? 1 : 0,
+                 B.d_pcnt,
+                 L,
+                 second ? nullptr : B.d_flist};
+    if (v.fused && B.d_tasks) {
+        // a second pass re-runs few queries: each is spread over S slices
+        const uint32_t S = bq_slices(nq, second);
+        const uint64_t ucap = (uint64_t)((G + S - 1) / S) << H;
+        // the second pass works through the list of flagged queries the
+        // first final select wrote: a few workgroups per slice, not one per query
+        const uint32_t *fl = second ? B.d_flist : nullptr;
+        const uint32_t gx = second ? std::min(nq, 16u) : nq;
+        const CutArgs ca{dq,       db,    ta,     G,       H,       L,         ctx->d,  P.poff, P.psz,
+                         P.cstat,  (uint64_t)P.nchunks, B.d_tasks, ucap, B.d_ucnt, B.d_ctr, pass_flags, S, fl};
+        auto ck = NB == 8 ? batch_cut_kernel<8> : (NB == 16 ? batch_cut_kernel<16> : batch_cut_kernel<32>);
+        hipLaunchKernelGGL(ck, dim3(gx, S), dim3(BQ_BS), 0, s, ca);
+        hipLaunchKernelGGL(batch_unit_scan_kernel, dim3(1), dim3(1024), 0, s, B.d_ucnt, nq * S, fl, S, B.d_uoff);
+        const uint32_t g2 = (uint32_t)B.cus * (second ? 2 : 7);  // 7 resident 4-wave workgroups per CU
+        const ChunkArgs xa{dq, P.pcols, P.ld, P.poff, P.psz, P.pid, B.d_tasks, ucap, B.d_ucnt, B.d_uoff, nq * S, S,
+                           fl, B.d_cand, B.d_ccount, B.d_scored};
+        hipEvent_t ev0 = nullptr;
+        timing_begin(ctx, s, ev0);
+        hipLaunchKernelGGL(batch_chunk_kernel, dim3(g2), dim3(BC_BS), 0, s, xa);
+        timing_end(ctx, s, second ? "batch_score_second" : "batch_score", ev0);
+        fa.nscored = g2;
+        if (v.digit == 8)
+            hipLaunchKernelGGL(batch_final_kernel<8>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
+        else
+            hipLaunchKernelGGL(batch_final_kernel<11>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
+        TK_HIP(hipGetLastError());
+        return TOPK_OK;
+    }
     // mask layout: query-major (one workgroup per query) spreads a full
     // batch over the CUs; group-major (one per group and query slice) keeps
     // the few queries of a second pass from running one long chain each
@@ -1184,31 +1754,15 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_pcnt, L,
                        (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
     ScoreArgs sa{P.pcols, P.ld, ctx->d,    P.poff,     P.psz,  P.pid,  B.d_items, B.item_cap,
-                 ochunk,  B.d_plist, nq, B.d_ctr, dq, B.d_cand, B.d_ccount, P.cstat,  P.crow, B.d_scored};
+                 ochunk,  B.d_plist, nq, B.d_ctr, dq, B.d_cand, B.d_ccount, P.cstat,  P.crow, (uint64_t)P.nchunks,
+                 B.d_scored};
     BatchVariant vs = v;
     if (second) vs.grid = v.grid / 8 > 32 ? v.grid / 8 : 32;  // the second pass has few items
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
     launch_score(dims_class(ctx->d), vs, s, sa);
     timing_end(ctx, s, second ? "batch_score_second" : "batch_score", ev0);
-    FinalArgs fa{dq,
-                 B.d_cand,
-                 B.d_ccount,
-                 B.d_ctr,
-                 ks,
-                 reinterpret_cast<uint64_t *>(B.d_out + ol.ids_off),
-                 reinterpret_cast<float *>(B.d_out + ol.sc_off),
-                 reinterpret_cast<float *>(B.d_out + ol.thr_off),
-                 reinterpret_cast<uint32_t *>(B.d_out + ol.st_off),
-                 reinterpret_cast<uint32_t *>(B.d_out + ol.cnt_off),
-                 reinterpret_cast<uint32_t *>(B.d_out + (second ? ol.ctr2_off : ol.ctr_off)),
-                 qflag,
-                 second ? 1 : 0,
-                 B.d_scored,
-                 (uint32_t)vs.grid,
-                 seed2 ? 1 : 0,
-                 B.d_pcnt,
-                 L};
+    fa.nscored = (uint32_t)vs.grid;
     if (v.digit == 8)
         hipLaunchKernelGGL(batch_final_kernel<8>, dim3(nq), dim3(BFIN_BS), 0, s, fa);
     else
@@ -1217,28 +1771,25 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     return TOPK_OK;
 }
 
-// Enqueues a whole batch on s: staged inputs to the device, the first pass,
-// the second pass for flagged queries, results to the pinned mirror.
+// Enqueues the kernels of a whole batch on s (its inputs are on the
+// device): prep, the first pass, the second pass for flagged queries.
 static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
                          const BatchVariant &v) {
     const InLayout il = InLayout::make(nq);
-    const OutLayout ol = OutLayout::make(nq, ks);
-    BatchQuery *dq = reinterpret_cast<BatchQuery *>(B.d_in + il.q_off);
-    BoundArgs *db = reinterpret_cast<BoundArgs *>(B.d_in + il.b_off);
-    TK_HIP(hipMemcpyAsync(B.d_in, B.slot[B.cur].h_in, il.host, hipMemcpyHostToDevice, s));
+    unsigned char *d_in = B.slot[B.cur].d_in;
+    BatchQuery *dq = reinterpret_cast<BatchQuery *>(d_in + il.q_off);
+    BoundArgs *db = reinterpret_cast<BoundArgs *>(d_in + il.b_off);
     const uint32_t L = ctx->pta.nparts;
     uint32_t pg = (L + 63) / 64;
     if (pg > 1024) pg = 1024;
     if (pg < nq) pg = nq;
     hipLaunchKernelGGL(batch_prep_kernel, dim3(pg), dim3(64), 0, s,
-                       reinterpret_cast<const topk_query *>(B.d_in + il.raw_off),
-                       mode == BMODE_GIVEN ? reinterpret_cast<const float *>(B.d_in + il.tau_off) : nullptr, nq, ctx->n,
-                       ctx->d, dq, db, B.d_pcnt, L, B.d_ccount, B.d_ctr);
+                       reinterpret_cast<const topk_query *>(d_in + il.raw_off),
+                       mode == BMODE_GIVEN ? reinterpret_cast<const float *>(d_in + il.tau_off) : nullptr, nq, ctx->n,
+                       ctx->d, dq, db, B.d_pcnt, L, B.d_ccount, B.d_ctr, B.d_flist);
     int rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, false);
     if (!rc) rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, true);
-    if (rc) return rc;
-    TK_HIP(hipMemcpyAsync(B.slot[B.cur].h_out, B.d_out, ol.bytes, hipMemcpyDeviceToHost, s));
-    return TOPK_OK;
+    return rc;
 }
 
 static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
@@ -1247,7 +1798,7 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
                        (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
                        (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur, (uint64_t)v.item,
-                       (uint64_t)ctx->tune.batch_retry, (uint64_t)ctx->tune.batch_retry_width};
+                       (uint64_t)ctx->tune.batch_retry, (uint64_t)ctx->tune.batch_retry_width, (uint64_t)v.fused};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1328,7 +1879,7 @@ static int batch_complete(topk_ctx *ctx, BatchState &B, int si) {
     HostSlot &H = B.slot[si];
     if (!H.busy) return TOPK_OK;
     H.busy = false;
-    TK_HIP(hipEventSynchronize(H.done));
+    TK_HIP(wait_event(H.done));
     const uint32_t nq = H.nq, ks = H.ks;
     const topk_query *qs = reinterpret_cast<const topk_query *>(H.h_in + InLayout::make(nq).raw_off);
     const OutLayout ol = OutLayout::make(nq, ks);
@@ -1355,7 +1906,9 @@ static int batch_complete(topk_ctx *ctx, BatchState &B, int si) {
     // first k_i entries (min(k_i, n) results, then padding)
     bool full = ctx->n >= ks;
     for (uint32_t i = 0; full && i < nq; i++) full = qs[i].k == ks;
-    if (full) {  // every query fills its stride: one copy per array
+    if (H.direct) {
+        // the result copy wrote the caller's arrays
+    } else if (full) {  // every query fills its stride: one copy per array
         if (H.out_ids) memcpy(H.out_ids, ids, sizeof(uint64_t) * nq * ks);
         if (H.out_scores) memcpy(H.out_scores, sc, sizeof(float) * nq * ks);
     } else {
@@ -1372,7 +1925,7 @@ static int batch_complete(topk_ctx *ctx, BatchState &B, int si) {
             }
         }
     }
-    if (H.out_thr) memcpy(H.out_thr, thr, sizeof(float) * nq);
+    if (H.out_thr && !H.direct) memcpy(H.out_thr, thr, sizeof(float) * nq);
     for (uint32_t i : fte) {
         const topk_query &Q = qs[i];
         float t = 0;
@@ -1481,10 +2034,33 @@ static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     rc = stage_queries(ctx, B, nq, qs, ks);
     if (rc) return rc;
     hipStream_t s = ctx->lane[0].stream;
+    HostSlot &H = B.slot[si];
+    // upload (copy stream) -> kernels (compute stream) -> result copy (copy
+    // stream): the other batch in flight runs its kernels meanwhile
+    TK_HIP(hipMemcpyAsync(H.d_in, H.h_in, InLayout::make(nq).host, hipMemcpyHostToDevice, B.up));
+    TK_HIP(hipEventRecord(H.in_ready, B.up));
+    TK_HIP(hipStreamWaitEvent(s, H.in_ready, 0));
     rc = batch_launch(ctx, B, s, nq, ks, mode, v);
     if (rc) return rc;
-    HostSlot &H = B.slot[si];
-    TK_HIP(hipEventRecord(H.done, s));
+    TK_HIP(hipEventRecord(H.kdone, s));
+    TK_HIP(hipStreamWaitEvent(B.down, H.kdone, 0));
+    const OutLayout ol = OutLayout::make(nq, ks);
+    // Every query filling its stride (k = k_stride <= n) into pinned caller
+    // arrays: the result copy writes them directly and only the status
+    // words go through the staging buffer.
+    bool full = ctx->n >= ks;
+    for (uint32_t i = 0; full && i < nq; i++) full = qs[i].k == ks;
+    H.direct = full && pinned_host(out_ids) && pinned_host(out_scores) && pinned_host(out_thr);
+    if (H.direct) {
+        TK_HIP(hipMemcpyAsync(out_ids, H.d_out + ol.ids_off, sizeof(uint64_t) * nq * ks, hipMemcpyDeviceToHost, B.down));
+        TK_HIP(hipMemcpyAsync(out_scores, H.d_out + ol.sc_off, sizeof(float) * nq * ks, hipMemcpyDeviceToHost, B.down));
+        TK_HIP(hipMemcpyAsync(out_thr, H.d_out + ol.thr_off, sizeof(float) * nq, hipMemcpyDeviceToHost, B.down));
+        TK_HIP(hipMemcpyAsync(H.h_out + ol.st_off, H.d_out + ol.st_off, ol.bytes - ol.st_off, hipMemcpyDeviceToHost,
+                              B.down));
+    } else {
+        TK_HIP(hipMemcpyAsync(H.h_out, H.d_out, ol.bytes, hipMemcpyDeviceToHost, B.down));
+    }
+    TK_HIP(hipEventRecord(H.done, B.down));
     H.busy = true;
     H.ticket = B.next_ticket++;
     H.nq = nq;
@@ -1553,6 +2129,8 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     B.cur = 0;
     rc = stage_queries(ctx, B, nq, qs, k_stride);
     if (rc) return rc;
+    hipStream_t s = ctx->lane[0].stream;
+    TK_HIP(hipMemcpyAsync(B.slot[0].d_in, B.slot[0].h_in, InLayout::make(nq).host, hipMemcpyHostToDevice, s));
     const int D = dims_class(ctx->d);
     const BatchVariant def = pick_variant(ctx, B);
     std::vector<BatchVariant> cand;
@@ -1562,9 +2140,8 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
             if (!variant_ok(D, vpt, bs)) continue;
             for (int qpi : qpis)
                 for (int pc : per_cu)
-                    cand.push_back(BatchVariant{vpt, bs, qpi, def.digit, B.cus * pc * 256 / bs, def.item});
+                    cand.push_back(BatchVariant{vpt, bs, qpi, def.digit, B.cus * pc * 256 / bs, def.item, 0});
         }
-    hipStream_t s = ctx->lane[0].stream;
     hipEvent_t e0, e1;
     TK_HIP(hipEventCreate(&e0));
     TK_HIP(hipEventCreate(&e1));
@@ -1582,6 +2159,11 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
         }
         return TOPK_OK;
     };
+    {
+        BatchVariant f = def;  // the query-major fused pass (its shape knobs do not apply)
+        f.fused = 1;
+        cand.push_back(f);
+    }
     double ms_def = 0, best_ms = 1e30;
     BatchVariant best = def;
     rc = time_one(def, ms_def);
@@ -1608,7 +2190,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
         }
     }
     for (int it : {2048, 4096, 8192, 16384}) {  // objects per work item, with the best shape
-        if (rc || it == best.item) continue;
+        if (rc || it == best.item || best.fused) continue;
         BatchVariant v = best;
         v.item = it;
         double ms;
@@ -1628,6 +2210,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     ctx->tune.batch_digit = best.digit;
     ctx->tune.batch_grid = best.grid;
     ctx->tune.batch_item = best.item;
+    ctx->tune.batch_fused = best.fused ? 2 : 1;
     if (out) {
         out->vpt = (uint32_t)best.vpt;
         out->bs = (uint32_t)best.bs;
@@ -1638,6 +2221,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
         out->variants = (uint32_t)cand.size() + 1 + extra;
         out->ms_best = best_ms;
         out->ms_default = ms_def;
+        out->fused = (uint32_t)best.fused;
     }
     return TOPK_OK;
 }

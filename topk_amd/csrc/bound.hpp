@@ -95,13 +95,17 @@ __device__ __forceinline__ float lp_bound_regs(const double (&wv)[NMAX], const f
     return f32_round_up(lp + absw * 1.52587890625e-05 + 1e-30);  // margin 2^-16 * sum |W x|
 }
 
-// pmax / pmin point at the partition's d per-attribute maxima / minima.
-// NMAX bounds b.nd at compile time.  Every operand (weight, maximum,
-// minimum) is read with independent loads into registers before any
-// arithmetic, so a bound costs one round of memory latency.
+// pmax / pmin point at the partition's per-attribute maxima / minima,
+// attribute a at pmax[a * stride] (stride 1: a row; the number of nodes or
+// chunks: an attribute-major array, where the lanes of a wave bounding
+// consecutive nodes read consecutive words).  NMAX bounds b.nd at compile
+// time.  Every operand (weight, maximum, minimum) is read with independent
+// loads into registers before any arithmetic, so a bound costs one round
+// of memory latency.
 template <int NMAX = TOPK_MAX_DIMS>
 __device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__restrict__ pmax,
-                                          const float *__restrict__ pmin, float psum, float pminsum) {
+                                          const float *__restrict__ pmin, float psum, float pminsum,
+                                          uint64_t stride = 1) {
     const uint32_t nd = b.nd;
     float mxv[NMAX], mnv[NMAX];
     double wv[NMAX];
@@ -110,8 +114,8 @@ __device__ __forceinline__ float lp_bound(const BoundArgs &b, const float *__res
         const bool on = (uint32_t)i < nd;
         const uint32_t ai = on ? b.attr[i] : 0u;  // entries past nd may be stale
         wv[i] = on ? b.w[i] : 0.0;
-        mxv[i] = on ? pmax[ai] : 0.0f;
-        mnv[i] = on ? pmin[ai] : 0.0f;
+        mxv[i] = on ? pmax[ai * stride] : 0.0f;
+        mnv[i] = on ? pmin[ai * stride] : 0.0f;
     }
     return lp_bound_regs<NMAX>(wv, mxv, mnv, psum, pminsum);
 }

@@ -82,6 +82,7 @@ struct Tuning {
     int batch_graph = 1;        // replay captured hipGraphs
     int batch_mode = 0;         // 0 auto (learned when trained), 1 seed threshold, 2 learned threshold
     int batch_mask = 0;         // mask layout: 0 auto, 1 query-major, 2 group-major
+    int batch_fused = 0;        // 0 default (work items), 1 work items, 2 query-major fused pass
 };
 
 // One query of a batch as the device kernels read it (prepared on the host;
@@ -209,18 +210,22 @@ struct PtaLayout {
     // spatially grouped range of partitions (exactly a k-d subtree when the
     // partition count is a power of two).  Per node: the union box, the
     // largest attribute sum, and the sum of the box minima; empty nodes have
-    // tsum = -inf.
+    // tsum = -inf.  tmax / tmin are attribute-major: attribute a of node v
+    // at [a * 2*p2 + v], so a wave bounding consecutive nodes reads
+    // consecutive words.
     uint32_t p2 = 0;
-    float *tmax = nullptr;       // 2*p2 x d
-    float *tmin = nullptr;       // 2*p2 x d
+    float *tmax = nullptr;       // d x 2*p2
+    float *tmin = nullptr;       // d x 2*p2
     float *tsum = nullptr;       // 2*p2
     float *tminsum = nullptr;    // 2*p2
     // Chunk statistics: positions [c * PTA_CHUNK, (c + 1) * PTA_CHUNK) form
-    // chunk c (partitions start on chunk boundaries); row c of cstat (crow =
-    // 2d + 2 floats) holds the maxima, minima, largest attribute sum (rounded
+    // chunk c (partitions start on chunk boundaries); the crow = 2d + 2
+    // statistics of chunk c -- maxima, minima, largest attribute sum (rounded
     // up; -inf: no objects) and sum of minima (rounded down) of chunk c and
     // every later chunk of its partition (suffix statistics: the bounds never
-    // increase along a partition).  Objects are ordered by attribute sum
+    // increase along a partition) -- are stored attribute-major, element e
+    // at cstat[e * nchunks + c], so the lanes of a wave bounding consecutive
+    // chunks read consecutive words.  Objects are ordered by attribute sum
     // inside a partition, so the leading chunks hold the large sums.
     uint64_t nchunks = 0;
     uint32_t crow = 0;

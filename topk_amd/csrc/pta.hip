@@ -194,11 +194,11 @@ __global__ __launch_bounds__(PTA_BS) void pta_bounds_kernel(const float *__restr
     }
 }
 
-// One wave per chunk of PTA_CHUNK positions: row c of cstat holds the
-// chunk's per-attribute maxima [0, d) and minima [d, 2d), the largest
-// attribute sum of its objects (FP64, rounded up) at 2d and the sum of its
-// minima (FP64, rounded down) at 2d + 1.  A chunk without objects has sum
-// -inf.
+// One wave per chunk of PTA_CHUNK positions: element e of chunk c (at
+// cstat[e * nchunks + c]) holds the chunk's per-attribute maxima (e in
+// [0, d)) and minima ([d, 2d)), the largest attribute sum of its objects
+// (FP64, rounded up) at 2d and the sum of its minima (FP64, rounded down)
+// at 2d + 1.  A chunk without objects has sum -inf.
 __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict__ pcols, uint64_t ldp, uint32_t d,
                                                         const uint32_t *__restrict__ pid, uint64_t nchunks,
                                                         float *__restrict__ cstat) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict_
         any |= ok[j];
     }
     const bool nonempty = __ballot(any) != 0;
-    float *row = cstat + c * (2 * d + 2);
+    float *row = cstat + c;  // element e at row[e * nchunks]
     double best = -INFINITY, minsum = 0.0;
     double sums[PER];
 #pragma unroll
@@ -237,8 +237,8 @@ __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict_
         }
         minsum += (double)mn;
         if (lane == 0) {
-            row[m] = nonempty ? mx : 0.0f;
-            row[d + m] = nonempty ? mn : 0.0f;
+            row[m * nchunks] = nonempty ? mx : 0.0f;
+            row[(d + m) * nchunks] = nonempty ? mn : 0.0f;
         }
     }
 #pragma unroll
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict_
         best = t > best ? t : best;
     }
     if (lane == 0) {
-        row[2 * d] = nonempty ? f32_round_up(best) : -INFINITY;
-        row[2 * d + 1] = nonempty ? f32_round_down(minsum) : 0.0f;
+        row[2 * d * nchunks] = nonempty ? f32_round_up(best) : -INFINITY;
+        row[(2 * d + 1) * nchunks] = nonempty ? f32_round_down(minsum) : 0.0f;
     }
 }
 
@@ -259,22 +259,23 @@ __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict_
 // FP64 rounded down), so a partition's row bounds never increase with c.
 __global__ __launch_bounds__(256) void pta_chunk_suffix_kernel(uint32_t L, uint32_t d,
                                                                const uint32_t *__restrict__ poff,
-                                                               const uint32_t *__restrict__ psz,
+                                                               const uint32_t *__restrict__ psz, uint64_t nchunks,
                                                                float *__restrict__ cstat) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= L || psz[p] == 0) return;
-    const uint32_t crow = 2 * d + 2, c0 = poff[p] / PTA_CHUNK, n = (psz[p] + PTA_CHUNK - 1) / PTA_CHUNK;
+    const uint32_t c0 = poff[p] / PTA_CHUNK, n = (psz[p] + PTA_CHUNK - 1) / PTA_CHUNK;
+    const uint64_t S = nchunks;  // element stride
     for (uint32_t i = n - 1; i > 0; i--) {
-        const float *nx = cstat + (uint64_t)(c0 + i) * crow;
-        float *row = cstat + (uint64_t)(c0 + i - 1) * crow;
+        const float *nx = cstat + (c0 + i);
+        float *row = cstat + (c0 + i - 1);
         double ms = 0.0;
         for (uint32_t m = 0; m < d; m++) {
-            row[m] = fmaxf(row[m], nx[m]);
-            row[d + m] = fminf(row[d + m], nx[d + m]);
-            ms += (double)row[d + m];
+            row[m * S] = fmaxf(row[m * S], nx[m * S]);
+            row[(d + m) * S] = fminf(row[(d + m) * S], nx[(d + m) * S]);
+            ms += (double)row[(d + m) * S];
         }
-        row[2 * d] = fmaxf(row[2 * d], nx[2 * d]);
-        row[2 * d + 1] = f32_round_down(ms);
+        row[2 * d * S] = fmaxf(row[2 * d * S], nx[2 * d * S]);
+        row[(2 * d + 1) * S] = f32_round_down(ms);
     }
 }
 
@@ -380,12 +381,19 @@ static int build_bound_tree(PtaLayout &P, uint32_t d, hipStream_t s) {
         ms[v] = any ? round_down_f32(t) : 0.0f;
     }
     P.p2 = p2;
+    // attribute-major copies for the device: attribute m of node v at [m * nn + v]
+    std::vector<float> mxt(nn * d), mnt(nn * d);
+    for (size_t v = 0; v < nn; v++)
+        for (uint32_t m = 0; m < d; m++) {
+            mxt[m * nn + v] = mx[v * d + m];
+            mnt[m * nn + v] = mn[v * d + m];
+        }
     TK_HIP(hipMalloc(&P.tmax, sizeof(float) * nn * d));
     TK_HIP(hipMalloc(&P.tmin, sizeof(float) * nn * d));
     TK_HIP(hipMalloc(&P.tsum, sizeof(float) * nn));
     TK_HIP(hipMalloc(&P.tminsum, sizeof(float) * nn));
-    TK_HIP(hipMemcpyAsync(P.tmax, mx.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
-    TK_HIP(hipMemcpyAsync(P.tmin, mn.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemcpyAsync(P.tmax, mxt.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
+    TK_HIP(hipMemcpyAsync(P.tmin, mnt.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
     TK_HIP(hipMemcpyAsync(P.tsum, sm.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
     TK_HIP(hipMemcpyAsync(P.tminsum, ms.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
     TK_HIP(hipStreamSynchronize(s));
@@ -526,7 +534,7 @@ int pta_build(topk_ctx *ctx) {
                                d, P.pid, (uint64_t)P.nchunks, P.cstat);
         if (P.nchunks)
             hipLaunchKernelGGL(pta_chunk_suffix_kernel, dim3((L + 255) / 256), dim3(256), 0, s, L, d, P.poff, P.psz,
-                               P.cstat);
+                               (uint64_t)P.nchunks, P.cstat);
         TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
         TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
         TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
@@ -586,7 +594,7 @@ template <int NB>
 __global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L, const uint64_t *__restrict__ pbound,
                                                          const uint32_t *__restrict__ poff,
                                                          const uint32_t *__restrict__ psz,
-                                                         const float *__restrict__ cstat, uint32_t crow,
+                                                         const float *__restrict__ cstat, uint64_t nchunks,
                                                          const FteState *__restrict__ st, uint32_t *__restrict__ cut) {
     const uint32_t p = blockIdx.x * (PTA_BS / 64) + threadIdx.x / 64;
     if (p >= L) return;
@@ -603,8 +611,9 @@ __global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L
         const uint32_t c = cb + lane;
         bool fail = false;
         if (c < nc) {
-            const float *row = cstat + (uint64_t)(c0 + c) * crow;
-            fail = okey(lp_bound<NB>(b, row, row + d, row[2 * d], row[2 * d + 1])) < tau_hi;
+            const float *row = cstat + (c0 + c);  // element e at row[e * nchunks]
+            fail = okey(lp_bound<NB>(b, row, row + d * nchunks, row[2 * d * nchunks], row[(2 * d + 1) * nchunks],
+                                     nchunks)) < tau_hi;
         }
         const uint64_t m = __ballot(fail);
         if (m) {
@@ -850,13 +859,13 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     const uint32_t nw = (L + PTA_BS / 64 - 1) / (PTA_BS / 64);  // one wave per partition
     if (ctx->d <= 8)
         hipLaunchKernelGGL(pta_cut_kernel<8>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff, P.psz, P.cstat,
-                           P.crow, Ln.st, P.pcut);
+                           (uint64_t)P.nchunks, Ln.st, P.pcut);
     else if (ctx->d <= 16)
         hipLaunchKernelGGL(pta_cut_kernel<16>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff, P.psz,
-                           P.cstat, P.crow, Ln.st, P.pcut);
+                           P.cstat, (uint64_t)P.nchunks, Ln.st, P.pcut);
     else
         hipLaunchKernelGGL(pta_cut_kernel<TOPK_MAX_DIMS>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff,
-                           P.psz, P.cstat, P.crow, Ln.st, P.pcut);
+                           P.psz, P.cstat, (uint64_t)P.nchunks, Ln.st, P.pcut);
 #define TK_STREAM(MM)                                                                                      \
     hipLaunchKernelGGL(pta_stream_kernel<MM>, dim3(g), dim3(PTA_BS), 0, s, q, L, P.pcut, P.poff, P.psz, \
                        P.pid, Ln.st, cs, P.pscored)

@@ -110,6 +110,21 @@ def query_array(queries):
     return qs
 
 
+def _host_array(count, dtype):
+    """An output array in pinned host memory when a GPU is present (the
+    library then copies results into it directly), else pageable."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            tdt = {np.uint64: torch.int64, np.float32: torch.float32}[dtype]
+            t = torch.empty(count, dtype=tdt, pin_memory=True)
+            a = t.numpy().view(dtype)
+            return a, t
+    except (ImportError, RuntimeError):
+        pass
+    return np.empty(count, dtype=dtype), None
+
+
 class PreparedBatch:
     """A query list staged once as topk_query structs plus output arrays."""
 
@@ -118,9 +133,9 @@ class PreparedBatch:
         self.nq = len(queries)
         self.qs = query_array(queries)
         self.kmax = max(1, max(int(q[0]) for q in queries))
-        self.ids = np.empty(self.nq * self.kmax, dtype=np.uint64)
-        self.scores = np.empty(self.nq * self.kmax, dtype=np.float32)
-        self.thr = np.empty(self.nq, dtype=np.float32)
+        self.ids, self._tids = _host_array(self.nq * self.kmax, np.uint64)
+        self.scores, self._tsc = _host_array(self.nq * self.kmax, np.float32)
+        self.thr, self._tthr = _host_array(self.nq, np.float32)
 
     def result(self, i, n):
         c = min(int(self.queries[i][0]), n)
