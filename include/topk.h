@@ -272,7 +272,7 @@ typedef struct topk_learned_config {
     uint32_t epochs;          /* passes over the training queries (default 60) */
     uint32_t batch;           /* mini-batch, a multiple of 64 (default 512) */
     float lr;                 /* Adam step size (default 2e-3, decayed x0.3 / x0.1 late) */
-    float target_fallback;    /* fraction of calibration queries allowed to fall back (0.002) */
+    float target_fallback;    /* fraction of calibration queries allowed to fall back (0.0005) */
     float calib_fraction;     /* held-out fraction used to calibrate the margin (0.2) */
     uint64_t seed;            /* split / init / shuffle seed (1) */
 } topk_learned_config;

@@ -761,6 +761,15 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
 }
 #undef TK_TERM
 
+// Where a batch's final select writes ids / scores / thresholds: the
+// caller's own arrays when they are pinned host memory the device can write
+// (direct), else the device output block.  Staged with the queries, so a
+// captured graph reads it from memory.
+struct OutDesc {
+    uint64_t ids, scores, thr;
+    uint32_t direct, pad;
+};
+
 struct FinalArgs {
     BatchQuery *q;
     const uint64_t *cand;
@@ -781,6 +790,7 @@ struct FinalArgs {
     unsigned int *pcnt;      // first pass: cleared here for the second
     uint32_t L;
     unsigned int *flist;     // first pass: queries flagged for the second ([0] count, then ids); may be null
+    const OutDesc *od;       // result destination of this batch (null: the device output block)
 };
 
 template <int DB>
@@ -858,7 +868,14 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
         return;
     }
     if (tid == 0 && !a.second) a.qflag[q] = QF_DONE;
-    const FinalOut o{a.ids + (uint64_t)q * a.kstride, a.scores + (uint64_t)q * a.kstride, a.thr + q, a.status + q};
+    uint64_t *ids = a.ids;
+    float *scores = a.scores, *thr = a.thr;
+    if (a.od && a.od->direct) {
+        ids = reinterpret_cast<uint64_t *>(a.od->ids);
+        scores = reinterpret_cast<float *>(a.od->scores);
+        thr = reinterpret_cast<float *>(a.od->thr);
+    }
+    const FinalOut o{ids + (uint64_t)q * a.kstride, scores + (uint64_t)q * a.kstride, thr + q, a.status + q};
     select_and_emit<BFIN_BS, BFIN_KPT, FlatView, DB>(FlatView{a.cand + (uint64_t)q * BATCH_CAP}, c, k, o, top, sorted,
                                                      ss);
 }
@@ -1313,9 +1330,7 @@ struct HostSlot {
     unsigned char *d_in = nullptr;   // raw queries | taus | BatchQuery[nq] | BoundArgs[nq]
     unsigned char *d_out = nullptr;  // ids | scores | thr | status | cnt | flags | ctr
     size_t din_cap = 0, dout_cap = 0;
-    hipEvent_t in_ready = nullptr;   // recorded after the upload (upload stream)
-    hipEvent_t kdone = nullptr;      // recorded after the kernels (compute stream)
-    hipEvent_t done = nullptr;       // recorded after the result copy (download stream)
+    hipEvent_t done = nullptr;       // recorded after the batch's status copy
     bool busy = false;
     bool direct = false;             // ids / scores / thresholds were copied straight into the caller's arrays
     uint64_t ticket = 0;
@@ -1325,8 +1340,8 @@ struct HostSlot {
     double t0 = 0;
 };
 
-// True when p is pinned (page-locked) host memory the GPU can copy into.
-static bool pinned_host(const void *p) {
+// True when p is pinned (page-locked) host memory the GPU can write.
+static bool pinned_host_query(const void *p) {
     if (!p) return false;
     hipPointerAttribute_t at;
     const hipError_t e = hipPointerGetAttributes(&at, p);
@@ -1352,12 +1367,9 @@ struct BatchState {
     uint32_t nq_cap = 0;
     uint64_t plist_cap = 0, pcnt_cap = 0, item_cap = 0;
     // Two batches may be in flight (topk_batch_submit): each has its own slot
-    // (staging, inputs, outputs), so one batch's upload and result copy run
-    // on the copy streams while the other's kernels run; the scratch below is
-    // shared because the kernels of the batches run one after the other on
-    // the compute stream.
+    // (staging, inputs, outputs); the scratch below is shared because the
+    // batches run one after the other on the context's stream.
     HostSlot slot[2];
-    hipStream_t up = nullptr, down = nullptr;  // upload / result-copy streams (default priority)
     int cur = 0;                     // slot of the batch being enqueued
     int next = 0;                    // slot the next submit uses
     uint64_t next_ticket = 1;
@@ -1378,19 +1390,31 @@ struct BatchState {
     uint64_t *d_cand = nullptr;
     int cus = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
+    // pinned-memory answers for recently seen output arrays (callers reuse them)
+    std::map<const void *, bool> pinned;
 };
+
+static bool pinned_host(BatchState &B, const void *p) {
+    auto it = B.pinned.find(p);
+    if (it != B.pinned.end()) return it->second;
+    if (B.pinned.size() > 64) B.pinned.clear();
+    const bool r = pinned_host_query(p);
+    B.pinned[p] = r;
+    return r;
+}
 
 // Staged inputs: the caller's topk_query records and given thresholds are
 // copied from pinned memory (the first `host` bytes); the device builds the
 // BatchQuery / BoundArgs forms behind them.
 struct InLayout {
-    size_t raw_off, tau_off, host, q_off, b_off, bytes;
+    size_t raw_off, tau_off, od_off, host, q_off, b_off, bytes;
     static InLayout make(uint32_t nq) {
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
         InLayout l;
         l.raw_off = 0;
         l.tau_off = al(sizeof(topk_query) * nq);
-        l.host = l.tau_off + sizeof(float) * nq;
+        l.od_off = al(l.tau_off + sizeof(float) * nq);
+        l.host = l.od_off + sizeof(OutDesc);
         l.q_off = al(l.host);
         l.b_off = al(l.q_off + sizeof(BatchQuery) * nq);
         l.bytes = l.b_off + sizeof(BoundArgs) * nq;
@@ -1424,8 +1448,6 @@ void batch_free(topk_ctx *ctx) {
     BatchState *B = static_cast<BatchState *>(ctx->batch);
     if (!B) return;
     hipStreamSynchronize(ctx->lane[0].stream);  // batches still in flight
-    if (B->up) hipStreamSynchronize(B->up);
-    if (B->down) hipStreamSynchronize(B->down);
     drop_graphs(*B);
     hipFree(B->d_plist);
     hipFree(B->d_tasks);
@@ -1443,12 +1465,8 @@ void batch_free(topk_ctx *ctx) {
         if (h.h_out) hipHostFree(h.h_out);
         hipFree(h.d_in);
         hipFree(h.d_out);
-        if (h.in_ready) hipEventDestroy(h.in_ready);
-        if (h.kdone) hipEventDestroy(h.kdone);
         if (h.done) hipEventDestroy(h.done);
     }
-    if (B->up) hipStreamDestroy(B->up);
-    if (B->down) hipStreamDestroy(B->down);
     delete B;
     ctx->batch = nullptr;
 }
@@ -1487,7 +1505,7 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
     const uint64_t te = bq_task_entries(nq, P.p2, group_height(P.p2));
     const bool grow = B.gen != P.generation || nq > B.nq_cap || in_need > H.din_cap || out_need > H.dout_cap ||
                       in_need > H.in_cap || out_need > H.out_cap || (uint64_t)L * nq > B.plist_cap ||
-                      (uint64_t)L > B.pcnt_cap || items_need > B.item_cap || !B.d_ctr || !H.done || !B.up ||
+                      (uint64_t)L > B.pcnt_cap || items_need > B.item_cap || !B.d_ctr || !H.done ||
                       te > B.tasks_cap || (te && nq * BQ_MAX_S > B.unit_cap) || (!te && B.d_tasks);
     if (!grow) return TOPK_OK;
     int rc = batch_drain(ctx, B);
@@ -1545,15 +1563,7 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
         TK_HIP(hipHostMalloc(&H.h_out, out_need, hipHostMallocDefault));
         H.out_cap = out_need;
     }
-    if (!H.done) {
-        TK_HIP(hipEventCreateWithFlags(&H.in_ready, hipEventDisableTiming));
-        TK_HIP(hipEventCreateWithFlags(&H.kdone, hipEventDisableTiming));
-        TK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
-    }
-    if (!B.up) {
-        TK_HIP(hipStreamCreateWithFlags(&B.up, hipStreamNonBlocking));
-        TK_HIP(hipStreamCreateWithFlags(&B.down, hipStreamNonBlocking));
-    }
+    if (!H.done) TK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
     rc = grow_dev(B.d_plist, B.plist_cap, (uint64_t)L * nq, changed);
     if (!rc) rc = grow_dev(B.d_pcnt, B.pcnt_cap, (uint64_t)L, changed);
     // budget >= 16 queries, chunks >= ITEM_MIN_OBJECTS objects
@@ -1702,7 +1712,8 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                  seed2 ? 1 : 0,
                  B.d_pcnt,
                  L,
-                 second ? nullptr : B.d_flist};
+                 second ? nullptr : B.d_flist,
+                 reinterpret_cast<const OutDesc *>(d_in + il.od_off)};
     if (v.fused && B.d_tasks) {
         // a second pass re-runs few queries: each is spread over S slices
         const uint32_t S = bq_slices(nq, second);
@@ -2035,32 +2046,33 @@ static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     if (rc) return rc;
     hipStream_t s = ctx->lane[0].stream;
     HostSlot &H = B.slot[si];
-    // upload (copy stream) -> kernels (compute stream) -> result copy (copy
-    // stream): the other batch in flight runs its kernels meanwhile
-    TK_HIP(hipMemcpyAsync(H.d_in, H.h_in, InLayout::make(nq).host, hipMemcpyHostToDevice, B.up));
-    TK_HIP(hipEventRecord(H.in_ready, B.up));
-    TK_HIP(hipStreamWaitEvent(s, H.in_ready, 0));
-    rc = batch_launch(ctx, B, s, nq, ks, mode, v);
-    if (rc) return rc;
-    TK_HIP(hipEventRecord(H.kdone, s));
-    TK_HIP(hipStreamWaitEvent(B.down, H.kdone, 0));
+    const InLayout il = InLayout::make(nq);
     const OutLayout ol = OutLayout::make(nq, ks);
     // Every query filling its stride (k = k_stride <= n) into pinned caller
-    // arrays: the result copy writes them directly and only the status
-    // words go through the staging buffer.
+    // arrays: the final select writes them directly (the device reaches
+    // pinned host memory) and only the status words are copied back.
     bool full = ctx->n >= ks;
     for (uint32_t i = 0; full && i < nq; i++) full = qs[i].k == ks;
-    H.direct = full && pinned_host(out_ids) && pinned_host(out_scores) && pinned_host(out_thr);
+    H.direct = full && pinned_host(B, out_ids) && pinned_host(B, out_scores) && pinned_host(B, out_thr);
+    OutDesc od{};
     if (H.direct) {
-        TK_HIP(hipMemcpyAsync(out_ids, H.d_out + ol.ids_off, sizeof(uint64_t) * nq * ks, hipMemcpyDeviceToHost, B.down));
-        TK_HIP(hipMemcpyAsync(out_scores, H.d_out + ol.sc_off, sizeof(float) * nq * ks, hipMemcpyDeviceToHost, B.down));
-        TK_HIP(hipMemcpyAsync(out_thr, H.d_out + ol.thr_off, sizeof(float) * nq, hipMemcpyDeviceToHost, B.down));
-        TK_HIP(hipMemcpyAsync(H.h_out + ol.st_off, H.d_out + ol.st_off, ol.bytes - ol.st_off, hipMemcpyDeviceToHost,
-                              B.down));
-    } else {
-        TK_HIP(hipMemcpyAsync(H.h_out, H.d_out, ol.bytes, hipMemcpyDeviceToHost, B.down));
+        od.ids = reinterpret_cast<uint64_t>(out_ids);
+        od.scores = reinterpret_cast<uint64_t>(out_scores);
+        od.thr = reinterpret_cast<uint64_t>(out_thr);
+        od.direct = 1;
     }
-    TK_HIP(hipEventRecord(H.done, B.down));
+    memcpy(H.h_in + il.od_off, &od, sizeof(od));
+    // upload -> kernels -> status copy, in order on the context's stream:
+    // three calls per batch keep the host ahead of the device
+    TK_HIP(hipMemcpyAsync(H.d_in, H.h_in, il.host, hipMemcpyHostToDevice, s));
+    rc = batch_launch(ctx, B, s, nq, ks, mode, v);
+    if (rc) return rc;
+    if (H.direct)
+        TK_HIP(hipMemcpyAsync(H.h_out + ol.st_off, H.d_out + ol.st_off, ol.bytes - ol.st_off, hipMemcpyDeviceToHost,
+                              s));
+    else
+        TK_HIP(hipMemcpyAsync(H.h_out, H.d_out, ol.bytes, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipEventRecord(H.done, s));
     H.busy = true;
     H.ticket = B.next_ticket++;
     H.nq = nq;
@@ -2130,6 +2142,10 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     rc = stage_queries(ctx, B, nq, qs, k_stride);
     if (rc) return rc;
     hipStream_t s = ctx->lane[0].stream;
+    {
+        const OutDesc od{};  // results to the device output block
+        memcpy(B.slot[0].h_in + InLayout::make(nq).od_off, &od, sizeof(od));
+    }
     TK_HIP(hipMemcpyAsync(B.slot[0].d_in, B.slot[0].h_in, InLayout::make(nq).host, hipMemcpyHostToDevice, s));
     const int D = dims_class(ctx->d);
     const BatchVariant def = pick_variant(ctx, B);
