@@ -197,10 +197,10 @@ def learned_batched(eng, n, k, args):
     eng.timing(True)
     eng.timing_reset()
     tsteps = min(steps, 10)
-    tobjs = 0
+    tterms = 0
     for i in range(tsteps):
         eng.batch_run(held[i % len(held)])
-        tobjs += eng.stats()["tuple_count"]
+        tterms += eng.stats()["terms"]
     score_ms, launches = eng.timing_get("batch_score")
     score2_ms, launches2 = eng.timing_get("batch_score_second")
     score_ms, launches = score_ms + score2_ms, launches + launches2  # both passes of every batch
@@ -208,11 +208,11 @@ def learned_batched(eng, n, k, args):
     lrn_ms, _ = eng.timing_get("batch_tau_learned")
     eng.timing(False)
     avg_ms = score_ms / max(1, launches)
-    # mul + add per term (m = 8 terms) for every (object, query) pair the
-    # scoring kernel scored (counted on the device, chunk-pruned pairs not
-    # included), over all its launches (first and second passes)
-    flops = 2.0 * D * tobjs / max(1, launches)
-    tflops = 2.0 * D * tobjs / (score_ms * 1e-3) / 1e12 if score_ms > 0 else 0.0
+    # mul + add per (object, query, attribute) term the scoring kernel
+    # scored (counted on the device; chunk-pruned objects not included),
+    # over all its launches (first and second passes)
+    flops = 2.0 * tterms / max(1, launches)
+    tflops = 2.0 * tterms / (score_ms * 1e-3) / 1e12 if score_ms > 0 else 0.0
 
     # every held-out answer judged by the plain GPU FTE path
     mism = 0
@@ -364,123 +364,237 @@ def pipelined(eng, pbs, steps):
     return nq / el, fb / nq, ff / nq
 
 
-def pta_bench(threads, cpu=True):
-    """PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
-    weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128; the plain
-    FTE path and the oracle's threaded TPAc on the same store beside it."""
+def pta_bench(threads, cpu=True, queries=20):
+    """PTA (C3): N = 2^28, d = 8 anticorrelated (seed 3), 4096-object
+    partitions, weights U(0.1, 1) on all 8 attributes (query seed 100), k =
+    128, one synchronous topk_pta call per query.
+
+    Touched bytes per query = the bound pass over every partition ((8d + 12)
+    bytes: per-attribute max and min, attribute-sum bounds, size) + 4*m bytes
+    per object the stream scored (counted on the device) -> roofline against
+    HBM; the FTE-equivalent rate is 4*N*m bytes / query time.  CPU baseline:
+    the oracle's threaded PTA (cpu/PTA.h:569-671 restated: partitions in
+    descending bound order, strict skip, per-thread heaps) over the engine's
+    own layout, 1 warm-up + 10 timed calls on the first 10 queries, each
+    compared with the GPU answer."""
     import oracle
     import topk_amd
-    n3, d = 1 << 28, 8
+    n3, d, k = 1 << 28, 8, 128
     rng = np.random.default_rng(100)
-    ws = [rng.uniform(0.1, 1.0, d).astype(np.float32) for _ in range(20)]
+    ws = [rng.uniform(0.1, 1.0, d).astype(np.float32) for _ in range(queries)]
     attr = np.arange(d, dtype=np.uint8)
     with topk_amd.Engine.generated(n3, d, topk_amd.ANTICORRELATED, 3) as eng:
         eng.set_partition_size(4096)
         t0 = time.perf_counter()
         eng.init(topk_amd.TOPK_INIT_PTA)
         init_s = time.perf_counter() - t0
-        eng.pta(128, ws[0], attr)
-        t0 = time.perf_counter()
-        scored = 0
+        for w in ws[:3]:
+            eng.pta(k, w, attr)
+        lat, parts, objs, gpu = [], [], [], []
         for w in ws:
-            eng.pta(128, w, attr)
+            t0 = time.perf_counter()
+            got = eng.pta(k, w, attr)
+            lat.append(time.perf_counter() - t0)
             st = eng.stats()
-            scored += st["partitions_scored"] / max(1, st["partitions_total"])
-        pta_s = (time.perf_counter() - t0) / len(ws)
+            parts.append(st["partitions_scored"])
+            objs.append(st["tuple_count"])
+            gpu.append(got)
+        P = st["partitions_total"]
         t0 = time.perf_counter()
         for w in ws[:5]:
-            eng.fte(128, w, attr)
+            eng.fte(k, w, attr)
         fte_s = (time.perf_counter() - t0) / 5
-        got = eng.pta(128, ws[1], attr)
-    cpu_s, match = float("nan"), None
+        layout = eng.pta_layout() if cpu else None
+    t_q = float(np.mean(lat))
+    bound_bytes = (8 * d + 12) * P
+    touched = bound_bytes + 4.0 * d * float(np.mean(objs))
+    achieved = touched / t_q / 1e9
+    out = {
+        "pta_qps": 1.0 / t_q, "ms_per_query_mean": t_q * 1e3, "ms_per_query_median": float(np.median(lat)) * 1e3,
+        "fte_qps": 1.0 / fte_s, "partitions": P, "partitions_scored_mean": float(np.mean(parts)),
+        "partitions_scored_frac": float(np.mean(parts)) / P, "objects_scored_mean": float(np.mean(objs)),
+        "objects_scored_frac": float(np.mean(objs)) / n3, "layout_build_s": init_s,
+        "fte_equivalent_GBps": 4.0 * n3 * d / t_q / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "touched_bytes_per_query": touched,
+                     "bound_pass_bytes": bound_bytes, "scored_bytes": 4.0 * d * float(np.mean(objs)),
+                     "time": "whole synchronous topk_pta call"},
+    }
     if cpu:
+        pid, poff, psz, pmax, pmin = layout
         x = topk_amd.generate(n3, d, topk_amd.ANTICORRELATED, 3, threads=threads)
-        t0 = time.perf_counter()
-        ref = oracle.tpac_threads(x, n3, 128, ws[1], attr, threads)
-        cpu_s = time.perf_counter() - t0
-        del x
-        match = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
-    return {
-        "pta_qps": 1.0 / pta_s, "fte_qps": 1.0 / fte_s, "partitions_scored_frac": scored / len(ws),
-        "layout_build_s": init_s, "cpu_tpac_threads_qps": 1.0 / cpu_s, "cpu_threads": threads,
-        "matches_oracle": match}
+        valid = pid != 0xFFFFFFFF
+        pidv = np.where(valid, pid, 0).astype(np.int64)
+        pdata = np.empty((d, pid.size), np.float32)
+        for m in range(d):
+            np.take(x[m], pidv, out=pdata[m])
+        del x, pidv
+        oracle.pta_topk(pdata, pid, poff, psz, pmax, pmin, k, ws[0], attr, threads)  # warm-up
+        ts, mism, cscored = [], 0, []
+        for i in range(10):
+            t0 = time.perf_counter()
+            ri, rs, rt, sc = oracle.pta_topk(pdata, pid, poff, psz, pmax, pmin, k, ws[i], attr, threads)
+            ts.append(time.perf_counter() - t0)
+            cscored.append(sc)
+            gi, gs, gt = gpu[i]
+            mism += int(not (np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
+                             and gt == rt))
+        del pdata
+        out["cpu_baseline"] = {
+            "kind": "port", "cores": threads, "value": 1.0 / float(np.mean(ts)), "unit": "queries/s",
+            "ms_mean": float(np.mean(ts)) * 1e3, "ms_median": float(np.median(ts)) * 1e3,
+            "partitions_scored_mean": float(np.mean(cscored)), "host": host_info(),
+            "sample": "1 warm-up + 10 timed calls on the first 10 queries; oracle or_pta_threads (partition "
+                      "box bounds in descending order, strict skip, per-thread heaps) over the engine's own "
+                      "partition layout"}
+        out["mismatches_vs_cpu_pta"] = mism
+    return out
 
 
-def other_configs(threads):
-    """The other workloads of the scope table, measured once each (one JSON
-    line; the headline line is the default mode):
-    * C1: the reference's CPU-runnable case (c1_bench);
-    * PTA: N = 2^28, d = 8 anticorrelated (seed 3), 4096-object partitions,
-      weights U(0.1, 1) on all 8 attributes (query seed 100), k = 128;
-    * batched: N = 2^25, d = 16 mixed (seed 4), 1024 queries per launch on
-      random subsets of 2-16 attributes, k from {16..256} (query seed 200);
-    * learned: N = 2^26, d = 16 mixed (seed 5), 100k training queries (seed
-      300), 4096 held-out queries (seed 301) in launches of 1024;
-    * ingest: N = 2^24, d = 8 text and column files (ingest_bench);
-    * host-memory store: N = 2^28, d = 8 streamed through HBM (stream_bench)."""
+def batch_kernel_view(eng, pbs, steps=10):
+    """Eager batches with HIP-event timing of the scoring kernel (both
+    passes): device-counted (object, query) pairs and terms per batch, the
+    scoring kernel's time per batch and its FP32 rate (2 flops per term)."""
+    eng.timing(True)
+    eng.timing_reset()
+    pairs = terms = 0
+    for i in range(steps):
+        eng.batch_run(pbs[i % len(pbs)])
+        st = eng.stats()
+        pairs += st["tuple_count"]
+        terms += st["terms"]
+    s1, l1 = eng.timing_get("batch_score")
+    s2, l2 = eng.timing_get("batch_score_second")
+    seed_ms, _ = eng.timing_get("batch_tau_seed")
+    lrn_ms, _ = eng.timing_get("batch_tau_learned")
+    eng.timing(False)
+    score_ms = s1 + s2
+    tflops = 2.0 * terms / (score_ms * 1e-3) / 1e12 if score_ms > 0 else 0.0
+    return {"pairs_per_batch": pairs / steps, "terms_per_batch": terms / steps,
+            "score_ms_per_batch": score_ms / steps, "score_launches_per_batch": (l1 + l2) / steps,
+            "avg_launch_ms": score_ms / max(1, l1 + l2), "flops_per_launch": 2.0 * terms / max(1, l1 + l2),
+            "tflops": tflops, "tau_ms_per_batch": (seed_ms + lrn_ms) / steps}
+
+
+def valu_roofline(view, kernel):
+    t = view["tflops"]
+    return {"bound": "valu", "achieved": t, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": t / FP32_PEAK_TFLOPS,
+            "issue_peak": FP32_ISSUE_TFLOPS, "issue_frac": t / FP32_ISSUE_TFLOPS, "kernel": kernel,
+            "avg_launch_ms": view["avg_launch_ms"], "flops_per_launch": view["flops_per_launch"],
+            "flops": "2 per (object, query, attribute) term the scoring kernel scored, counted on the device"}
+
+
+def batched_bench(threads, steps=20, cpu=True):
+    """Batched (C4): N = 2^25, d = 16 mixed (seed 4), 1024 queries per batch
+    on random subsets of 2-16 attributes, weights U(0.1, 1), k from
+    {16..256} (query seed 200), seed thresholds, two batches in flight.
+    Full-scan flops = 2*N*sum(m) per batch (what one scan per query
+    computes); executed flops = 2 per device-counted term of the scoring
+    kernel.  CPU baseline: the oracle's findTopKsimdMQ restatement (one scan
+    per query, queries over the host threads) on the first 32 queries."""
+    import oracle
     import topk_amd
+    n4, d = 1 << 25, 16
     ks = [16, 32, 64, 128, 256]
-    res = {}
-    res["c1_n2e20_d8_m4_k16"] = c1_bench(threads)
-    res["pta_n2e28_anticorrelated_k128"] = pta_bench(threads)
-
-    n4 = 1 << 25
-    with topk_amd.Engine.generated(n4, 16, topk_amd.MIXED, 4) as eng:
-        qlists = [subset_queries(np.random.default_rng(200 + i), 1024, 16, ks) for i in range(4)]
+    with topk_amd.Engine.generated(n4, d, topk_amd.MIXED, 4) as eng:
+        eng.init(topk_amd.TOPK_INIT_PTA)
+        qlists = [subset_queries(np.random.default_rng(200 + i), 1024, d, ks) for i in range(4)]
         pbs = [eng.prepare_batch(ql) for ql in qlists]
+        eng.set_tuning("batch_mode", 1)
         tune = eng.batch_autotune(pbs[0])
-        qps, fb, ff = pipelined(eng, pbs, 20)
+        qps, fb, ff = pipelined(eng, pbs, steps)
+        view = batch_kernel_view(eng, pbs)
         sample = qlists[0][:32]
         gpu_sample = eng.batch(sample)
-    # CPU baseline: the oracle's findTopKsimdMQ restatement (one scan per
-    # query, queries over the host threads) on the first 32 queries
-    import oracle
-    x = topk_amd.generate(n4, 16, topk_amd.MIXED, 4, threads=threads)
-    amax = max(len(q[2]) for q in sample)
-    attrs = np.zeros((len(sample), amax), np.uint8)
-    for i, q in enumerate(sample):
-        attrs[i, :len(q[2])] = q[2]
-    t0 = time.perf_counter()
-    ref = oracle.mq_threads(x, n4, [q[0] for q in sample], [len(q[2]) for q in sample],
-                            np.stack([q[1] for q in sample]), attrs, threads)
-    cpu_s = time.perf_counter() - t0
-    del x
-    mism = sum(int(not (np.array_equal(g[0], r[0]) and np.array_equal(g[1], r[1]))) for g, r in zip(gpu_sample, ref))
-    res["batch_n2e25_d16_mixed"] = {"qps": qps, "fte_fallback_rate": ff, "autotune": tune,
-                                    "threshold": "seed (no predictor)",
-                                    "cpu_baseline": {"kind": "port", "cores": threads, "qps": len(sample) / cpu_s,
-                                                     "sample": "first 32 queries of the first batch"},
-                                    "mismatches_vs_oracle_sample": mism}
+    sum_m = float(np.mean([sum(len(q[2]) for q in ql) for ql in qlists]))
+    t_batch = 1024.0 / qps
+    full_flops = 2.0 * n4 * sum_m
+    out = {"qps": qps, "ms_per_batch": t_batch * 1e3, "fallback_rate": fb, "fte_fallback_rate": ff,
+           "autotune": tune, "threshold": "seed (no predictor)",
+           "full_scan_flops_per_batch": full_flops, "full_scan_equivalent_tflops": full_flops / t_batch / 1e12,
+           "executed_flops_per_batch": 2.0 * view["terms_per_batch"],
+           "executed_tflops_whole_batch": 2.0 * view["terms_per_batch"] / t_batch / 1e12,
+           "scored_pair_frac": view["pairs_per_batch"] / (1024.0 * n4), "kernel_view": view,
+           "roofline": valu_roofline(view, "batch scoring kernel (batch_score timing)")}
+    if cpu:
+        x = topk_amd.generate(n4, d, topk_amd.MIXED, 4, threads=threads)
+        amax = max(len(q[2]) for q in sample)
+        attrs = np.zeros((len(sample), amax), np.uint8)
+        for i, q in enumerate(sample):
+            attrs[i, :len(q[2])] = q[2]
+        t0 = time.perf_counter()
+        ref = oracle.mq_threads(x, n4, [q[0] for q in sample], [len(q[2]) for q in sample],
+                                np.stack([q[1] for q in sample]), attrs, threads)
+        cpu_s = time.perf_counter() - t0
+        del x
+        out["mismatches_vs_oracle_sample"] = sum(
+            int(not (np.array_equal(g[0], r[0]) and np.array_equal(g[1], r[1]))) for g, r in zip(gpu_sample, ref))
+        out["cpu_baseline"] = {"kind": "port", "cores": threads, "value": len(sample) / cpu_s, "unit": "queries/s",
+                               "host": host_info(), "sample": "first 32 queries of the first batch, one call"}
+    return out
 
-    with topk_amd.Engine.generated(1 << 26, 16, topk_amd.MIXED, 5) as eng:
-        train = subset_queries(np.random.default_rng(300), 100_000, 16, ks)
+
+def learned16_bench(steps=20, train_queries=100_000):
+    """Learned (C5): N = 2^26, d = 16 mixed (seed 5), 100k training queries
+    (seed 300), 4096 held-out queries (seed 301) in batches of 1024; learned
+    and seed thresholds on the same batches (threshold-stage time and
+    device-counted scored pairs of each), every held-out answer judged
+    against the plain GPU FTE path."""
+    import topk_amd
+    ks = [16, 32, 64, 128, 256]
+    n5, d = 1 << 26, 16
+    with topk_amd.Engine.generated(n5, d, topk_amd.MIXED, 5) as eng:
+        train = subset_queries(np.random.default_rng(300), train_queries, d, ks)
         t0 = time.perf_counter()
         rep = eng.learned_train(train)
         train_s = time.perf_counter() - t0
-        held = subset_queries(np.random.default_rng(301), 4096, 16, ks)
+        held = subset_queries(np.random.default_rng(301), 4096, d, ks)
         pbs = [eng.prepare_batch(held[i:i + 1024]) for i in range(0, 4096, 1024)]
         eng.set_tuning("batch_mode", 2)
         tune = eng.batch_autotune(pbs[0])
-        qps, fb, ff = pipelined(eng, pbs, 20)
+        qps, fb, ff = pipelined(eng, pbs, steps)
+        lview = batch_kernel_view(eng, pbs)
         eng.set_tuning("batch_mode", 1)
-        sqps, _, _ = pipelined(eng, pbs, 8)
+        sqps, _, _ = pipelined(eng, pbs, steps)
+        sview = batch_kernel_view(eng, pbs)
         eng.set_tuning("batch_mode", 2)
         mism = 0
         for pb in pbs:
             eng.batch_run(pb)
             for i, q in enumerate(pb.queries):
-                gi, gs, gt = pb.result(i, 1 << 26)
+                gi, gs, gt = pb.result(i, n5)
                 ri, rs, rt = eng.fte(*q)
-                mism += int(not (np.array_equal(gi, ri) and np.array_equal(gs, rs) and gt == rt))
+                mism += int(not (np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
+                                 and gt == rt))
         t0 = time.perf_counter()
         for q in held[:50]:
             eng.fte(*q)
         fte_qps = 50 / (time.perf_counter() - t0)
-        res["learned_n2e26_d16_mixed"] = {
-            "learned_qps": qps, "seed_qps": sqps, "plain_fte_qps": fte_qps, "speedup_vs_plain_fte": qps / fte_qps,
-            "fallback_rate": fb, "fte_fallback_rate": ff, "mismatches_vs_plain_fte": mism, "judged_queries": 4096,
-            "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
-                         "train_mse": rep["train_mse"], "wall_s": train_s}, "autotune": tune}
+    return {
+        "learned_qps": qps, "seed_qps": sqps, "learned_over_seed": qps / sqps, "plain_fte_qps": fte_qps,
+        "speedup_vs_plain_fte": qps / fte_qps, "fallback_rate": fb, "fte_fallback_rate": ff,
+        "mismatches_vs_plain_fte": mism, "judged_queries": 4096,
+        "threshold_stage_ms_per_batch": {"learned": lview["tau_ms_per_batch"], "seed": sview["tau_ms_per_batch"]},
+        "scored_pairs_per_batch": {"learned": lview["pairs_per_batch"], "seed": sview["pairs_per_batch"]},
+        "kernel_view": {"learned": lview, "seed": sview},
+        "roofline": valu_roofline(lview, "batch scoring kernel (batch_score timing), learned thresholds"),
+        "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
+                     "saturated": rep.get("saturated"), "train_mse": rep["train_mse"], "wall_s": train_s},
+        "autotune": tune}
+
+
+def other_configs(threads):
+    """The other workloads of the scope table, measured once each (one JSON
+    line; the headline line is the default mode): C1 (c1_bench), PTA at
+    N = 2^28 (pta_bench), batched d = 16 (batched_bench), learned d = 16
+    (learned16_bench), BTA / GPA / radix select / normalisation
+    (kernels_bench), ingest at N = 2^24 (ingest_bench) and a host-memory
+    store at N = 2^28 (stream_bench)."""
+    res = {}
+    res["c1_n2e20_d8_m4_k16"] = c1_bench(threads)
+    res["pta_n2e28_anticorrelated_k128"] = pta_bench(threads)
+    res["batch_n2e25_d16_mixed"] = batched_bench(threads)
+    res["learned_n2e26_d16_mixed"] = learned16_bench()
     res["kernels_n2e26_d8"] = kernels_bench(threads)
     res["ingest_n2e24_d8"] = ingest_bench(1 << 24, 8, 3)
     res["stream_n2e28_d8_host"] = stream_bench(1 << 28, 8, 1 << 26, 64)
@@ -688,7 +802,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels", "c1"],
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels", "c1",
+                                       "batched", "learned16"],
                     default="learned",
                     help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
                          "ingest: data ingest alone; stream: a host-memory store streamed through HBM")
@@ -696,7 +811,8 @@ def main():
     ap.add_argument("--m", type=int, default=D, help="--mode fte: query width")
     ap.add_argument("--k", type=int, default=K)
     ap.add_argument("--batch", type=int, default=1024)
-    ap.add_argument("--train-queries", type=int, default=16384)
+    ap.add_argument("--train-queries", type=int, default=None,
+                    help="training queries (default 16384; 100000 with --mode learned16)")
     ap.add_argument("--held-batches", type=int, default=4)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", metavar="VPT,BS,QPI,DIGIT,GRID[,ITEM]",
@@ -707,6 +823,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=2)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
+    if args.train_queries is None and args.mode != "learned16":
+        args.train_queries = 16384
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -731,6 +849,13 @@ def main():
         return
     if args.mode == "pta":
         print(json.dumps({"pta_n2e28_anticorrelated_k128": pta_bench(16, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.mode == "batched":
+        print(json.dumps({"batch_n2e25_d16_mixed": batched_bench(16, args.steps, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.mode == "learned16":
+        print(json.dumps({"learned_n2e26_d16_mixed": learned16_bench(args.steps, args.train_queries or 100_000)}),
+              flush=True)
         return
     if args.mode == "stream":
         n = args.n if args.n != N_DEFAULT else 1 << 28

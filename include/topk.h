@@ -423,6 +423,8 @@ typedef struct topk_stats {
     uint32_t retried;            /* batch: queries re-run with a tightened threshold */
     uint32_t items;              /* batch: work items (partition loads) of the first pass */
     uint32_t fte_fallback;       /* batch: queries neither device pass settled, answered by topk_fte */
+    uint64_t terms;              /* (object, query, attribute) products scored, counted on the device
+                                    (2 flops each: a multiply and an add) */
 } topk_stats;
 
 int topk_get_stats(topk_ctx *ctx, topk_stats *out);

@@ -40,7 +40,8 @@ class TopkStats(ctypes.Structure):
                 ("partitions_scored", _c.c_uint64), ("partitions_total", _c.c_uint64),
                 ("fallback", _c.c_uint32), ("queries", _c.c_uint32),
                 ("ms_last", _c.c_double), ("tt_init", _c.c_double),
-                ("retried", _c.c_uint32), ("items", _c.c_uint32), ("fte_fallback", _c.c_uint32)]
+                ("retried", _c.c_uint32), ("items", _c.c_uint32), ("fte_fallback", _c.c_uint32),
+                ("terms", _c.c_uint64)]
 
 
 class TopkQuery(ctypes.Structure):

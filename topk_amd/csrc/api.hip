@@ -378,6 +378,7 @@ int topk_fte(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     emit_result(ctx, k, out_ids, out_scores, out_threshold);
     ctx->stats.ms_last = now_ms() - t0;
     ctx->stats.tuple_count = ctx->n;
+    ctx->stats.terms = ctx->n * qq;
     ctx->stats.queries = 1;
     return TOPK_OK;
 }

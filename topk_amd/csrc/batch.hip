@@ -77,8 +77,9 @@ enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
 // Device counters of one batch: [0] heavy work items (front of the list),
 // [1] light work items (back of the list), [2..3] scored (partition, query)
-// pairs, [4..5] scored (object, query) pairs, [6] work ticket, [7] unused.
-constexpr uint32_t BCTR_WORDS = 8;
+// pairs, [4..5] scored (object, query) pairs, [6] work ticket, [7] unused,
+// [8..9] scored (object, query, attribute) terms.
+constexpr uint32_t BCTR_WORDS = 10;
 
 // Work item: objects [chunk * ochunk, (chunk + 1) * ochunk) of partition p,
 // scored for the qcnt queries at plist[p * nq + qbeg ...].
@@ -585,6 +586,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
     uint32_t it = __builtin_amdgcn_readfirstlane(s_it[0]);
     uint32_t slot = 1;
     uint32_t npairs = 0;
+    unsigned long long nterms = 0;
     while (it < nitems) {
         const BatchItem I = it < nheavy ? a.items[it] : a.items[a.cap - 1 - (it - nheavy)];
         const uint32_t first = I.chunk * a.ochunk, psz_ = a.psz[I.p];
@@ -675,6 +677,8 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 for (uint64_t mm = qmask; mm; mm &= mm - 1) {
                     const uint32_t qi = (uint32_t)__ffsll((long long)mm) - 1u;
                     npairs += nvalid;
+                    nterms += (unsigned long long)nvalid *
+                              (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[3]), (int)qi);
                     const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)lq, (int)qi);
                     const uint32_t asc = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(lt[2]), (int)qi);
                         float s[VPT];
@@ -748,15 +752,25 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
         it = __builtin_amdgcn_readfirstlane(s_it[slot]);
         slot ^= 1u;
     }
-    // scored (object, query) pairs of this workgroup
-    __shared__ unsigned long long wpairs[BS / 64];
-    unsigned long long t = npairs;
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if (lane == 0) wpairs[tid / 64] = t;
+    // scored (object, query) pairs and terms of this workgroup
+    __shared__ unsigned long long wpairs[BS / 64][2];
+    unsigned long long t = npairs, tt = nterms;
+    for (int o = 32; o > 0; o >>= 1) {
+        t += __shfl_xor(t, o, 64);
+        tt += __shfl_xor(tt, o, 64);
+    }
+    if (lane == 0) {
+        wpairs[tid / 64][0] = t;
+        wpairs[tid / 64][1] = tt;
+    }
     __syncthreads();
     if (tid == 0) {
-        for (int w = 1; w < BS / 64; w++) t += wpairs[w];
+        for (int w = 1; w < BS / 64; w++) {
+            t += wpairs[w][0];
+            tt += wpairs[w][1];
+        }
         a.scored[blockIdx.x] = t;
+        a.scored[BATCH_MAX_GRID + blockIdx.x] = tt;
     }
 }
 #undef TK_TERM
@@ -800,20 +814,33 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
     __shared__ SelectScratch ss;
     const uint32_t q = blockIdx.x, tid = threadIdx.x;
     if (q == 0) {
-        // counters of this pass; words 4..5 become the (object, query) pairs
-        // the scoring kernel actually scored
+        // counters of this pass; words 4..5 and 8..9 become the (object,
+        // query) pairs and terms the scoring kernel actually scored
         if (tid < BCTR_WORDS) {
-            if (tid < 4 || tid > 5) a.ctr_out[tid] = a.ctr[tid];
+            if (tid < 4 || tid == 6 || tid == 7) a.ctr_out[tid] = a.ctr[tid];
             if (!a.second) a.ctr[tid] = 0u;  // counters of the second pass
         }
-        unsigned long long t = 0;
-        for (uint32_t i = tid; i < a.nscored; i += BFIN_BS) t += a.scored[i];
-        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-        if (lane_id() == 0) top[tid / 64] = t;
+        unsigned long long t = 0, tt = 0;
+        for (uint32_t i = tid; i < a.nscored; i += BFIN_BS) {
+            t += a.scored[i];
+            tt += a.scored[BATCH_MAX_GRID + i];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            t += __shfl_xor(t, o, 64);
+            tt += __shfl_xor(tt, o, 64);
+        }
+        if (lane_id() == 0) {
+            top[tid / 64] = t;
+            sorted[tid / 64] = tt;
+        }
         __syncthreads();
         if (tid == 0) {
-            for (int w = 1; w < BFIN_BS / 64; w++) t += top[w];
+            for (int w = 1; w < BFIN_BS / 64; w++) {
+                t += top[w];
+                tt += sorted[w];
+            }
             *reinterpret_cast<unsigned long long *>(a.ctr_out + 4) = t;
+            *reinterpret_cast<unsigned long long *>(a.ctr_out + 8) = tt;
         }
         __syncthreads();
     }
@@ -1152,7 +1179,7 @@ __device__ __forceinline__ uint32_t wave_last_le(Key key, uint32_t lo, uint32_t 
 constexpr uint32_t BC_STAGE = 256;  // candidate keys a wave stages in LDS before one append
 
 __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
-    __shared__ unsigned long long wpairs[BC_BS / 64];
+    __shared__ unsigned long long wpairs[BC_BS / 64][2];
     __shared__ uint64_t stage[BC_BS / 64][BC_STAGE];
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
     const uint32_t nwaves = gridDim.x * (BC_BS / 64), gw = blockIdx.x * (BC_BS / 64) + wave;
@@ -1162,6 +1189,7 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
     uint32_t t = gw * per;
     const uint32_t t_end = t + per < total ? t + per : total;
     uint32_t pairs = 0;
+    unsigned long long terms = 0;
     uint64_t *wst = stage[wave];
     if (t < t_end) {
         const uint32_t *uoff = a.uoff;
@@ -1251,6 +1279,7 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
             }
             const uint32_t nvalid = first < sz ? (sz - first < 4u ? sz - first : 4u) : 0u;
             pairs += nvalid;
+            terms += (unsigned long long)nvalid * m;
             uint64_t kk[4];
             uint32_t nin = 0;
 #pragma unroll
@@ -1281,13 +1310,23 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
         }
         flush();
     }
-    unsigned long long s = pairs;
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) wpairs[wave] = s;
+    unsigned long long s = pairs, ts = terms;
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        ts += __shfl_xor(ts, o, 64);
+    }
+    if (lane == 0) {
+        wpairs[wave][0] = s;
+        wpairs[wave][1] = ts;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < BC_BS / 64; w++) s += wpairs[w];
+        for (int w = 1; w < BC_BS / 64; w++) {
+            s += wpairs[w][0];
+            ts += wpairs[w][1];
+        }
         a.scored[blockIdx.x] = s;
+        a.scored[BATCH_MAX_GRID + blockIdx.x] = ts;
     }
 }
 
@@ -1596,7 +1635,7 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
     if (!B.d_ctr) {
         changed = true;
         TK_HIP(hipMalloc(&B.d_ctr, sizeof(unsigned int) * BCTR_WORDS));
-        TK_HIP(hipMalloc(&B.d_scored, sizeof(unsigned long long) * BATCH_MAX_GRID));
+        TK_HIP(hipMalloc(&B.d_scored, sizeof(unsigned long long) * 2 * BATCH_MAX_GRID));
     }
     if (!B.cus) {
         TK_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -1956,6 +1995,7 @@ static int batch_complete(topk_ctx *ctx, BatchState &B, int si) {
     S.partitions_scored = ((uint64_t)c1[2] | ((uint64_t)c1[3] << 32)) + ((uint64_t)c2[2] | ((uint64_t)c2[3] << 32));
     S.partitions_total = (uint64_t)ctx->pta.nonempty * nq;
     S.tuple_count = ((uint64_t)c1[4] | ((uint64_t)c1[5] << 32)) + ((uint64_t)c2[4] | ((uint64_t)c2[5] << 32));
+    S.terms = ((uint64_t)c1[8] | ((uint64_t)c1[9] << 32)) + ((uint64_t)c2[8] | ((uint64_t)c2[9] << 32));
     S.ms_last = now_ms() - H.t0;
     return TOPK_OK;
 }
@@ -2009,6 +2049,7 @@ static int batch_submit_divided(topk_ctx *ctx, uint32_t nq, const topk_query *qs
     S.fte_fallback = nq;
     S.candidates = candidates;
     S.tuple_count = (uint64_t)nq * ctx->n;
+    for (uint32_t i = 0; i < nq; i++) S.terms += ctx->n * qs[i].qq;
     S.ms_last = now_ms() - t0;
     if (ticket) *ticket = B.next_ticket;
     B.next_ticket++;

@@ -587,6 +587,7 @@ int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     S.retried = 0;
     S.fte_fallback = 0;
     S.tuple_count = ctx->n;
+    S.terms = ctx->n * qq;
     S.candidates = (uint64_t)nblk * K2;
     S.partitions_scored = nblk;
     S.partitions_total = nblk;
@@ -683,6 +684,7 @@ int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     S.fte_fallback = 0;
     S.candidates = last;
     S.tuple_count = n;
+    S.terms = 0;  // attributes are scored progressively over shrinking sets: not counted
     S.partitions_scored = 0;
     S.partitions_total = 0;
     S.ms_last = now_ms() - t0;

@@ -263,8 +263,11 @@ int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
     TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
     TK_HIP(hipStreamSynchronize(L.stream));
-    uint64_t scored = 0;
-    for (uint32_t i = 0; i < ctx->pta.nscored; i++) scored += ctx->pta.h_scored[i];
+    uint64_t scored = 0, objs = 0;
+    for (uint32_t i = 0; i < ctx->pta.nscored; i++) {
+        scored += ctx->pta.h_scored[2 * i];
+        objs += ctx->pta.h_scored[2 * i + 1];
+    }
     const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
     if (status) {
         // candidate storage overflowed (pathological data): the exact FTE path answers
@@ -288,7 +291,8 @@ int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     ctx->stats.fte_fallback = 0;
     ctx->stats.queries = 1;
     ctx->stats.ms_last = now_ms() - t0;
-    ctx->stats.tuple_count = 0;
+    ctx->stats.tuple_count = objs;
+    ctx->stats.terms = objs * qq;
     return TOPK_OK;
 }
 

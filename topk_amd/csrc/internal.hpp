@@ -200,10 +200,10 @@ struct PtaLayout {
     float *pminsum = nullptr;    // nparts: sum of the d minima (rounded down)
     uint64_t *pbound = nullptr;  // query scratch: bound key per partition
     uint64_t *bmax = nullptr;    // query scratch: max bound key per 256 partitions
-    uint32_t *pscored = nullptr; // query scratch: partitions scored per stream block
+    uint32_t *pscored = nullptr; // query scratch: per stream block, partitions then objects scored
     uint32_t *pcut = nullptr;    // query scratch: chunks to score per partition (0 = ruled out)
     uint32_t *h_scored = nullptr; // pinned copy of pscored (read after the query's one synchronisation)
-    uint32_t nscored = 0;        // entries of h_scored the last query wrote
+    uint32_t nscored = 0;        // stream blocks of the last query (2 words each in h_scored)
     // Bound tree: an implicit binary tree over partition ids padded to p2 (a
     // power of two); node v has children 2v, 2v+1 and leaf p is node p2 + p.
     // Leaves are numbered in k-d tree order, so a node is a contiguous,

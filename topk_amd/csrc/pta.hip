@@ -537,9 +537,9 @@ int pta_build(topk_ctx *ctx) {
                                (uint64_t)P.nchunks, P.cstat);
         TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
         TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
-        TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * MAX_STREAM_GRID));
+        TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * 2 * MAX_STREAM_GRID));
         TK_TRY(hipMalloc(&P.pcut, sizeof(uint32_t) * L));
-        TK_TRY(hipHostMalloc(&P.h_scored, sizeof(uint32_t) * MAX_STREAM_GRID, hipHostMallocDefault));
+        TK_TRY(hipHostMalloc(&P.h_scored, sizeof(uint32_t) * 2 * MAX_STREAM_GRID, hipHostMallocDefault));
         TK_TRY(hipGetLastError());
         TK_TRY(hipStreamSynchronize(s));
         rc = build_bound_tree(P, d, s);
@@ -738,12 +738,13 @@ __global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L,
     const uint32_t tau_hi = (uint32_t)(tau >> 32);
     const uint32_t lane = lane_id();
     uint64_t *my = cs.keys + (uint64_t)blockIdx.x * cs.slot;
-    uint32_t scored = 0;
+    uint32_t scored = 0, objs = 0;
     for (uint32_t p = blockIdx.x; p < L; p += gridDim.x) {
         const uint32_t nc = cut[p];
         if (!nc) continue;
         scored++;
         const uint32_t start = poff[p], end = start + min(psz[p], nc * PTA_CHUNK);
+        objs += end - start;
         for (uint32_t t0 = start; t0 < end; t0 += PTA_TILE) {
             const uint64_t v = t0 / 4 + threadIdx.x;
             f4 sc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -808,7 +809,8 @@ __global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L,
     __syncthreads();
     if (threadIdx.x == 0) {
         cs.counts[blockIdx.x] = lcount < cs.slot ? lcount : cs.slot;
-        pscored[blockIdx.x] = scored;
+        pscored[2 * blockIdx.x] = scored;
+        pscored[2 * blockIdx.x + 1] = objs;
     }
 }
 
@@ -875,8 +877,8 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     const uint32_t kreq = (uint32_t)(ctx->n < k ? ctx->n : k);
     final_enqueue(ctx, Ln, cs, k, kreq, FinalOut{d_ids, d_scores, d_thr, d_status}, false);
     TK_HIP(hipGetLastError());
-    // partitions scored per stream block: read by the caller after its synchronisation
-    TK_HIP(hipMemcpyAsync(P.h_scored, P.pscored, sizeof(uint32_t) * g, hipMemcpyDeviceToHost, s));
+    // partitions and objects scored per stream block: read by the caller after its synchronisation
+    TK_HIP(hipMemcpyAsync(P.h_scored, P.pscored, sizeof(uint32_t) * 2 * g, hipMemcpyDeviceToHost, s));
     P.nscored = g;
     return TOPK_OK;
 }
