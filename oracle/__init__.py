@@ -45,6 +45,8 @@ def lib():
                                           _f32p, _u8p, _u64p, _f32p, _f32p]),
         "or_tpac_threads": (_c.c_uint32, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _c.c_uint32,
                                           _f32p, _u8p, _c.c_int, _u64p, _f32p, _f32p]),
+        "or_fte_topk_wide": (_c.c_uint32, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _c.c_uint32,
+                                           _f32p, _u8p, _c.c_uint64, _c.c_int, _u64p, _f32p, _f32p]),
         "or_mq_threads": (_c.c_int, [_f32p, _c.c_uint64, _c.c_uint64, _c.c_uint32, _u32p, _u32p,
                                      _f32p, _c.c_uint32, _u8p, _c.c_uint32, _c.c_int,
                                      _u64p, _f32p, _f32p, _c.c_uint32]),
@@ -132,6 +134,18 @@ def tpac_threads(data, n, k, weights, attr, threads):
     sc = np.zeros(k, dtype=np.float32)
     thr = np.zeros(1, dtype=np.float32)
     c = lib().or_tpac_threads(a.ravel(), n, ld, k, at.size, w, at, threads, ids, sc, thr)
+    return ids[:c], sc[:c], float(thr[0])
+
+
+def fte_topk_wide(data, n, k, weights, attr, threads=1, id_base=0):
+    """FTE top-k with 64-bit ids (object i has id id_base + i; ties by the
+    lower 64-bit id), over threads.  Returns (ids, scores, thr)."""
+    a, d, ld = _cols(data)
+    w, at = _query(d, weights, attr)
+    ids = np.zeros(k, dtype=np.uint64)
+    sc = np.zeros(k, dtype=np.float32)
+    thr = np.zeros(1, dtype=np.float32)
+    c = lib().or_fte_topk_wide(a.ravel(), n, ld, k, at.size, w, at, id_base, threads, ids, sc, thr)
     return ids[:c], sc[:c], float(thr[0])
 
 

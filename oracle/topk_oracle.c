@@ -178,6 +178,86 @@ uint32_t or_fte_topk(const float *cols, uint64_t n, uint64_t ld, uint32_t k,
     return c;
 }
 
+/*
+ * 64-bit ids (stores of 2^32 - 1 objects or more; the reference's tuples
+ * carry uint64_t tids, cpu/AA.h:29-35): the same ranking with the id in 64
+ * bits, key = okey(score) << 64 | (2^64 - 1 - id) in a 128-bit integer.
+ * Object i of the columns has id id_base + i.  Threads own contiguous
+ * ranges with private heaps, merged at the end, as or_tpac_threads.
+ * Parity pinned by agreement with or_tpac_threads below 2^32 and by a
+ * brute-force lexicographic sort of ranges that cross 2^32.
+ */
+typedef unsigned __int128 u128;
+typedef struct { u128 *a; uint32_t n, cap; } wheap;
+
+static void wheap_push(wheap *h, u128 key)
+{
+    if (h->n < h->cap) {
+        uint32_t i = h->n++;
+        h->a[i] = key;
+        while (i > 0) {
+            uint32_t p = (i - 1) / 2;
+            if (h->a[p] <= h->a[i]) break;
+            u128 t = h->a[p]; h->a[p] = h->a[i]; h->a[i] = t; i = p;
+        }
+    } else if (key > h->a[0]) {
+        h->a[0] = key;
+        for (uint32_t i = 0;;) {
+            uint32_t l = 2 * i + 1, r = l + 1, m = i;
+            if (l < h->n && h->a[l] < h->a[m]) m = l;
+            if (r < h->n && h->a[r] < h->a[m]) m = r;
+            if (m == i) break;
+            u128 t = h->a[i]; h->a[i] = h->a[m]; h->a[m] = t; i = m;
+        }
+    }
+}
+
+static inline u128 wkey(float s, uint64_t id)
+{
+    return ((u128)or_okey(s) << 64) | (u128)(UINT64_MAX - id);
+}
+
+static int cmp_u128_desc(const void *x, const void *y)
+{
+    u128 a = *(const u128 *)x, b = *(const u128 *)y;
+    return (a < b) - (a > b);
+}
+
+uint32_t or_fte_topk_wide(const float *cols, uint64_t n, uint64_t ld, uint32_t k,
+                          uint32_t qq, const float *w, const uint8_t *attr, uint64_t id_base,
+                          int threads, uint64_t *ids, float *scores, float *thr)
+{
+    if (threads < 1) threads = 1;
+    u128 *all = (u128 *)malloc(sizeof(u128) * (uint64_t)(k ? k : 1) * threads);
+    uint32_t *cnt = (uint32_t *)calloc(threads, sizeof(uint32_t));
+#pragma omp parallel num_threads(threads)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+#else
+        int t = 0;
+#endif
+        wheap h = { all + (uint64_t)k * t, 0, k };
+        uint64_t lo = (uint64_t)t * (n / threads) + ((uint64_t)t < n % threads ? (uint64_t)t : n % threads);
+        uint64_t hi = lo + n / threads + ((uint64_t)t < n % threads ? 1 : 0);
+        for (uint64_t i = lo; i < hi; i++)
+            wheap_push(&h, wkey(score_one(cols, ld, i, qq, w, attr), id_base + i));
+        cnt[t] = h.n;
+    }
+    wheap g = { (u128 *)malloc(sizeof(u128) * (k ? k : 1)), 0, k };
+    for (int t = 0; t < threads; t++)
+        for (uint32_t j = 0; j < cnt[t]; j++) wheap_push(&g, all[(uint64_t)k * t + j]);
+    qsort(g.a, g.n, sizeof(u128), cmp_u128_desc);
+    for (uint32_t j = 0; j < g.n; j++) {
+        ids[j] = UINT64_MAX - (uint64_t)g.a[j];
+        scores[j] = or_okey_inv((uint32_t)(g.a[j] >> 64));
+    }
+    if (thr) *thr = g.n ? scores[g.n - 1] : -INFINITY;
+    uint32_t c = g.n;
+    free(g.a); free(all); free(cnt);
+    return c;
+}
+
 /* or_fte_topk over real-data scores (score_one_div). */
 uint32_t or_fte_topk_div(const float *cols, uint64_t n, uint64_t ld, uint32_t k,
                          uint32_t qq, const float *w, const uint8_t *attr,

@@ -142,6 +142,44 @@ def test_tpac_threads_matches_scalar(threads):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("threads", [1, 5])
+def test_fte_topk_wide_matches_32bit_below_2e32(threads):
+    """Below 2^32 the 128-bit-key restatement ranks as the 64-bit-key one."""
+    n = 30_011
+    x = (RNG.integers(0, 64, size=(4, n)) / 64.0).astype(np.float32)  # many ties
+    w = RNG.uniform(0.1, 1.0, 4).astype(np.float32)
+    for attr, k in (([0, 1, 2, 3], 50), ([2], 300), ([3, 1], 1)):
+        a = oracle.fte_topk(x, n, k, w, attr)
+        b = oracle.fte_topk_wide(x, n, k, w, attr, threads)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_fte_topk_wide_ids_across_2e32_bruteforce(threads):
+    """Ids id_base + i crossing 2^32 (and 2^33): ranked by a lexicographic
+    sort on (-score, id) of numpy-computed scores; quantised values make
+    ties straddle the boundary, where the lower 64-bit id must win."""
+    n, d = 4_000, 2
+    for base in ((1 << 32) - 1_500, (1 << 33) - 7):
+        x = (RNG.integers(0, 8, size=(d, n)) / 8.0).astype(np.float32)
+        w = np.array([1.0, 0.5], np.float32)
+        sc = (np.float32(0.0) + x[0] * w[0]).astype(np.float32) + (x[1] * w[1]).astype(np.float32)
+        ids = base + np.arange(n, dtype=np.uint64)
+        order = np.lexsort((ids, -sc.astype(np.float64)))
+        for k in (1, 37, 600):
+            gi, gs, gt = oracle.fte_topk_wide(x, n, k, w, [0, 1], threads, id_base=base)
+            assert np.array_equal(gi, ids[order[:k]])
+            assert np.array_equal(gs, sc[order[:k]]) and gt == sc[order[k - 1]]
+        # a tie across 2^32: the object below the boundary ranks first
+        if base < (1 << 32):
+            lo, hi = (1 << 32) - 1 - base, (1 << 32) + 5 - base
+            x2 = np.zeros((d, n), np.float32)
+            x2[0, hi] = 1.0
+            x2[0, lo] = 1.0
+            gi, gs, _ = oracle.fte_topk_wide(x2, n, 2, w, [0, 1], threads, id_base=base)
+            assert gi.tolist() == [(1 << 32) - 1, (1 << 32) + 5] and gs.tolist() == [1.0, 1.0]
+
+
 def test_mq_threads_matches_single():
     n = 20_000
     x = RNG.random((6, n), dtype=np.float32)

@@ -81,20 +81,20 @@ int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq,
 // store kernels
 // ---------------------------------------------------------------------------
 
+// Grid-stride over objects: a dispatch holds fewer than 2^32 work-items.
 __global__ void transpose_rows_kernel(const float *__restrict__ rows, uint64_t n, uint32_t d, uint64_t ld,
                                       float *__restrict__ cols) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = rows[i * d + m];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = rows[i * d + m];
 }
 
 __global__ void generate_kernel(uint64_t n, uint32_t d, uint64_t ld, int distr, uint64_t seed,
                                 float *__restrict__ cols) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float v[TOPK_MAX_DIMS];
-    gen_object(v, n, d, distr, seed, i);
-    for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = v[m];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float v[TOPK_MAX_DIMS];
+        gen_object(v, n, d, distr, seed, i);
+        for (uint32_t m = 0; m < d; m++) cols[(uint64_t)m * ld + i] = v[m];
+    }
 }
 
 // Sets *flag if any of the n values of column blockIdx.y is NaN or
@@ -118,7 +118,11 @@ void enqueue_finite_check(const topk_ctx *ctx, hipStream_t s, unsigned int *flag
     hipLaunchKernelGGL(nonfinite_kernel, dim3(512, ctx->d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, flag);
 }
 
-static uint32_t grid_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+// Workgroups for one object per work-item, capped for the grid-stride kernels.
+static uint32_t grid_for(uint64_t n, uint32_t bs) {
+    const uint64_t g = (n + bs - 1) / bs;
+    return (uint32_t)(g < (1u << 20) ? g : (1u << 20));
+}
 
 // Allocates the column store and per-query scratch of a fresh context.
 int ctx_alloc(topk_ctx *ctx) {
@@ -140,6 +144,11 @@ int ctx_alloc(topk_ctx *ctx) {
         TK_HIP(hipMalloc(&L.counts, sizeof(uint32_t) * MAX_STREAM_GRID));
         TK_HIP(hipMalloc(&L.cand, sizeof(uint64_t) * ctx->cand_cap));
         TK_HIP(hipMalloc(&L.spill, sizeof(uint64_t) * ctx->spill_cap));
+        const uint32_t S = id_shards(ctx->n);
+        if (S > 1) {
+            TK_HIP(hipMalloc(&L.sh_ids, sizeof(uint64_t) * S * TOPK_MAX_K));
+            TK_HIP(hipMalloc(&L.sh_scores, sizeof(float) * S * TOPK_MAX_K));
+        }
         TK_HIP(hipStreamSynchronize(L.stream));
     }
     TK_HIP(hipMalloc(&ctx->offsets, sizeof(uint32_t) * (MAX_STREAM_GRID + 1)));
@@ -211,6 +220,8 @@ int topk_ctx_destroy(topk_ctx *ctx) {
         hipFree(L.counts);
         hipFree(L.cand);
         hipFree(L.spill);
+        hipFree(L.sh_ids);
+        hipFree(L.sh_scores);
         if (i > 0 && L.stream) hipStreamDestroy(L.stream);
     }
     hipFree(ctx->offsets);
@@ -228,7 +239,8 @@ int topk_ctx_destroy(topk_ctx *ctx) {
 }
 
 int topk_ctx_create(topk_ctx **out, const float *data, uint64_t n, uint32_t d, uint32_t flags) {
-    if (!out || !data || n < 1 || n >= 0xFFFFFFFFull || d < 1 || d > TOPK_MAX_DIMS) return TOPK_ERR_INVALID;
+    if (!out || !data || n < 1 || d < 1 || d > TOPK_MAX_DIMS) return TOPK_ERR_INVALID;
+    if (n > MAX_OBJECTS) return TOPK_ERR_UNSUPPORTED;
     *out = nullptr;
     int rc = check_device();
     if (rc) return rc;
@@ -263,8 +275,8 @@ int topk_ctx_create(topk_ctx **out, const float *data, uint64_t n, uint32_t d, u
 }
 
 int topk_ctx_create_generated(topk_ctx **out, uint64_t n, uint32_t d, int distr, uint64_t seed) {
-    if (!out || n < 1 || n >= 0xFFFFFFFFull || d < 1 || d > TOPK_MAX_DIMS || distr < 0 || distr > 3)
-        return TOPK_ERR_INVALID;
+    if (!out || n < 1 || d < 1 || d > TOPK_MAX_DIMS || distr < 0 || distr > 3) return TOPK_ERR_INVALID;
+    if (n > MAX_OBJECTS) return TOPK_ERR_UNSUPPORTED;
     *out = nullptr;
     int rc = check_device();
     if (rc) return rc;
@@ -315,8 +327,8 @@ static void emit_result(topk_ctx *ctx, uint32_t k, uint64_t *out_ids, float *out
 
 // Fallback buffers: slots large enough that no stream block can overflow
 // (every block's objects fit in its slot) and a flat gather buffer.
-static int ensure_big(topk_ctx *ctx) {
-    const uint64_t need = ctx->n + (uint64_t)MAX_STREAM_GRID * 8192 + TOPK_MAX_K;
+static int ensure_big(topk_ctx *ctx, uint64_t n) {
+    const uint64_t need = n + (uint64_t)MAX_STREAM_GRID * 8192 + TOPK_MAX_K;
     if (ctx->cand_big && ctx->cand_big_cap >= need) return TOPK_OK;
     hipFree(ctx->cand_big);
     hipFree(ctx->flat);
@@ -324,16 +336,56 @@ static int ensure_big(topk_ctx *ctx) {
     ctx->flat = nullptr;
     TK_HIP(hipMalloc(&ctx->cand_big, sizeof(uint64_t) * need));
     ctx->cand_big_cap = need;
-    TK_HIP(hipMalloc(&ctx->flat, sizeof(uint64_t) * (ctx->n + TOPK_MAX_K)));
-    ctx->flat_cap = ctx->n + TOPK_MAX_K;
+    TK_HIP(hipMalloc(&ctx->flat, sizeof(uint64_t) * (n + TOPK_MAX_K)));
+    ctx->flat_cap = n + TOPK_MAX_K;
     return TOPK_OK;
 }
 
+// One FTE query over objects [0, n) of the columns q points at, results to
+// device pointers (status in the context's result block).  The
+// sample-seeded pipeline runs first; if its guess left too few candidates
+// (STATUS_RETRY) or too many for the buffer (STATUS_OVERFLOW), the
+// exact-prefix pipeline runs, and after an overflow there an n-entry
+// candidate buffer.
+static int fte_run_range(topk_ctx *ctx, uint32_t k, const QCols &q, uint64_t n, uint64_t *d_ids, float *d_sc,
+                         float *d_thr) {
+    uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
+    Lane &L = ctx->lane[0];
+    int mode = ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if (mode == FTE_EXACT_BIG) {
+            int rc = ensure_big(ctx, n);
+            if (rc) return rc;
+        }
+        int rc = fte_enqueue(ctx, L, k, q, d_ids, d_sc, d_thr, d_status, mode, n);
+        if (rc) return rc;
+        TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
+        TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
+        TK_HIP(hipStreamSynchronize(L.stream));
+        const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
+        if (!status) {
+            ctx->stats.candidates += ctx->h_state->cand_last;
+            return TOPK_OK;
+        }
+        TK_HIP(hipMemsetAsync(&L.st->status, 0, sizeof(uint32_t), L.stream));
+        ctx->stats.fallback = 1;
+        // a sample guess that overflowed is re-run with the exact prefix
+        // seed first; the n-entry buffer only serves an exact run's overflow
+        mode = (status & STATUS_OVERFLOW) && mode != FTE_SAMPLE ? FTE_EXACT_BIG : FTE_EXACT;
+    }
+    return TOPK_ERR_OVERFLOW;
+}
+
+// The columns of q shifted to the first object of id shard s.
+static QCols shard_cols(const QCols &q, uint32_t s) {
+    QCols r = q;
+    for (uint32_t j = 0; j < q.m; j++) r.col[j] = q.col[j] + (uint64_t)s * ID_SHARD;
+    return r;
+}
+
 // Runs one FTE query into the context's result block and copies it to the
-// pinned host mirror.  The sample-seeded pipeline runs first; if its guess
-// left too few candidates (STATUS_RETRY) or too many for the buffer
-// (STATUS_OVERFLOW), the exact-prefix pipeline runs, with an n-entry
-// candidate buffer after an overflow.
+// pinned host mirror: one run over the store, or one per id shard and a
+// device merge of their top-k lists.
 static int fte_run_sync(topk_ctx *ctx, uint32_t k, const QCols &q) {
     uint64_t *d_ids = reinterpret_cast<uint64_t *>(ctx->d_res + ctx->res.ids_off);
     float *d_sc = reinterpret_cast<float *>(ctx->d_res + ctx->res.scores_off);
@@ -342,26 +394,21 @@ static int fte_run_sync(topk_ctx *ctx, uint32_t k, const QCols &q) {
     ctx->stats.fallback = 0;
     ctx->stats.retried = 0;
     ctx->stats.fte_fallback = 0;
+    ctx->stats.candidates = 0;
+    const uint32_t S = id_shards(ctx->n);
+    if (S == 1) return fte_run_range(ctx, k, q, ctx->n, d_ids, d_sc, d_thr);
     Lane &L = ctx->lane[0];
-    int mode = ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE;
-    for (int attempt = 0; attempt < 3; attempt++) {
-        if (mode == FTE_EXACT_BIG) {
-            int rc = ensure_big(ctx);
-            if (rc) return rc;
-        }
-        int rc = fte_enqueue(ctx, L, k, q, d_ids, d_sc, d_thr, d_status, mode);
+    for (uint32_t s = 0; s < S; s++) {
+        const uint64_t n0 = (uint64_t)s * ID_SHARD, ns = ctx->n - n0 < ID_SHARD ? ctx->n - n0 : ID_SHARD;
+        const int rc = fte_run_range(ctx, k, shard_cols(q, s), ns, L.sh_ids + (uint64_t)s * TOPK_MAX_K,
+                                     L.sh_scores + (uint64_t)s * TOPK_MAX_K, nullptr);
         if (rc) return rc;
-        TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
-        TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
-        TK_HIP(hipStreamSynchronize(L.stream));
-        const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
-        ctx->stats.candidates = ctx->h_state->cand_last;
-        if (!status) return TOPK_OK;
-        TK_HIP(hipMemsetAsync(&L.st->status, 0, sizeof(uint32_t), L.stream));
-        ctx->stats.fallback = 1;
-        mode = (status & STATUS_OVERFLOW) ? FTE_EXACT_BIG : FTE_EXACT;
     }
-    return TOPK_ERR_OVERFLOW;
+    shard_merge_enqueue(L.stream, L.sh_ids, L.sh_scores, ctx->n, S, k, FinalOut{d_ids, d_sc, d_thr, d_status});
+    TK_HIP(hipGetLastError());
+    TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
+    TK_HIP(hipStreamSynchronize(L.stream));
+    return TOPK_OK;
 }
 
 int topk_fte(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *out_ids,
@@ -396,8 +443,19 @@ int topk_fte_async(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, 
     const uint32_t lanes = ctx->timing_on ? 1u : (uint32_t)ctx->tune.lanes;
     Lane &L = ctx->lane[ctx->next_lane % lanes];
     ctx->next_lane = (ctx->next_lane + 1) % lanes;
-    return fte_enqueue(ctx, L, k, q, d_ids, d_scores, nullptr, nullptr,
-                       ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE);
+    const int mode = ctx->tune.fte_force_exact ? FTE_EXACT : FTE_SAMPLE;
+    const uint32_t S = id_shards(ctx->n);
+    if (S == 1) return fte_enqueue(ctx, L, k, q, d_ids, d_scores, nullptr, nullptr, mode);
+    // per id shard into the lane's shard lists, then the merge
+    for (uint32_t s = 0; s < S; s++) {
+        const uint64_t n0 = (uint64_t)s * ID_SHARD, ns = ctx->n - n0 < ID_SHARD ? ctx->n - n0 : ID_SHARD;
+        rc = fte_enqueue(ctx, L, k, shard_cols(q, s), L.sh_ids + (uint64_t)s * TOPK_MAX_K,
+                         L.sh_scores + (uint64_t)s * TOPK_MAX_K, nullptr, nullptr, mode, ns);
+        if (rc) return rc;
+    }
+    shard_merge_enqueue(L.stream, L.sh_ids, L.sh_scores, ctx->n, S, k, FinalOut{d_ids, d_scores, nullptr, nullptr});
+    TK_HIP(hipGetLastError());
+    return TOPK_OK;
 }
 
 int topk_sync(topk_ctx *ctx) {

@@ -2060,7 +2060,9 @@ static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
                         float *out_scores, float *out_thr, const BatchVariant *force, uint64_t *ticket) {
     if (!ctx || !qs || nq == 0 || ks == 0) return TOPK_ERR_INVALID;
     if (nq > BATCH_MAX_Q || ks > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
-    if (ctx->score_div) {
+    // real-data scores, and stores with 64-bit ids (no partition layout):
+    // every query through the FTE path
+    if (ctx->score_div || ctx->n >= ID_LIMIT) {
         if (force) return TOPK_ERR_UNSUPPORTED;  // no launch shapes to time
         return batch_submit_divided(ctx, nq, qs, ks, out_ids, out_scores, out_thr, ticket);
     }
@@ -2167,7 +2169,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
                         topk_batch_tuning *out) {
     if (!ctx || !qs || nq == 0 || k_stride == 0) return TOPK_ERR_INVALID;
     if (nq > BATCH_MAX_Q || k_stride > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
-    if (ctx->score_div) return TOPK_ERR_UNSUPPORTED;  // real-data batches run the FTE path
+    if (ctx->score_div || ctx->n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;  // those batches run the FTE path
     if (!ctx->pta.built) {
         const int rc = pta_build(ctx);
         if (rc) return rc;

@@ -541,7 +541,7 @@ int topk_radix_select_u32(topk_ctx *ctx, const uint32_t *d_keys, uint64_t n, uin
 int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *out_ids,
              float *out_scores, float *out_threshold) {
     if (!ctx || k < 1) return TOPK_ERR_INVALID;
-    if (k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
+    if (k > TOPK_MAX_K || ctx->n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;  // run keys carry 32-bit ids
     QCols q;
     int rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
     if (rc) return rc;
@@ -602,7 +602,7 @@ int topk_gpa(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     QCols q;
     int rc = build_qcols(ctx, ctx->cols, ctx->ld, qq, weights, attr, q);
     if (rc) return rc;
-    if (ctx->n > 0xFFFFFFFFull) return TOPK_ERR_UNSUPPORTED;
+    if (ctx->n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;  // survivor ids are 32-bit
     const double t0 = now_ms();
     AuxState &A = aux(ctx);
     rc = ensure_dsel(A);

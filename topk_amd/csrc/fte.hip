@@ -568,10 +568,12 @@ static CandSet fte_launch(const FteRun &r, uint32_t k, const QCols &q, const Fin
 }
 
 int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
-                float *d_thr, uint32_t *d_status, int mode) {
+                float *d_thr, uint32_t *d_status, int mode, uint64_t n) {
+    if (!n) n = ctx->n;
+    if (n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;  // a run's ids are 32-bit: callers split larger stores
     FteRun r{};
     r.s = L.stream;
-    r.n = ctx->n;
+    r.n = n;
     r.st = L.st;
     r.blk = L.blk;
     r.pre = L.pre;
@@ -584,12 +586,74 @@ int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_
     r.vpt = ctx->tune.stream_vpt;
     r.grid_cap = ctx->tune.stream_grid;
     r.timed = ctx;
-    const uint32_t kreq = (uint32_t)(ctx->n < k ? ctx->n : k);
+    const uint32_t kreq = (uint32_t)(n < k ? n : k);
     const FinalOut fo{d_ids, d_scores, d_thr, d_status};
     const CandSet cs = fte_launch(r, k, q, fo, mode);
     if (r.big) final_enqueue(ctx, L, cs, k, kreq, fo, true);
     TK_HIP(hipGetLastError());
     return TOPK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// id shards: top k of the union of per-shard top-k lists
+// ---------------------------------------------------------------------------
+// Entry j of shard s's list (best first) lands at output position j plus,
+// for every other shard, the number of its entries that rank above it -- a
+// binary search, since each list is sorted.  Ranking is by score (monotone
+// key) and then global id; shard t < s holds only lower ids, so on equal
+// scores shard t's entries rank above shard s's.  Positions past k are
+// dropped; the k-th entry's score is the threshold.
+constexpr int MERGE_BS = 1024;
+struct ShardCounts {
+    uint32_t shards;
+    uint32_t cnt[MAX_ID_SHARDS];  // entries of each shard's list: min(k, objects of the shard)
+};
+
+__global__ __launch_bounds__(MERGE_BS) void shard_merge_kernel(const uint64_t *__restrict__ sh_ids,
+                                                               const float *__restrict__ sh_scores, ShardCounts c,
+                                                               uint32_t k, uint32_t kreq, FinalOut fo) {
+    const uint32_t S = c.shards;
+    for (uint32_t e = threadIdx.x; e < S * k; e += MERGE_BS) {
+        const uint32_t s = e / k, j = e % k;
+        if (j >= c.cnt[s]) continue;
+        const float sc = sh_scores[(uint64_t)s * TOPK_MAX_K + j];
+        const uint32_t ok = okey(sc);
+        uint32_t r = j;
+        for (uint32_t t = 0; t < S && r < k; t++) {
+            if (t == s) continue;
+            const float *ts = sh_scores + (uint64_t)t * TOPK_MAX_K;
+            uint32_t lo = 0, hi = c.cnt[t];  // entries [0, lo) rank above, [hi, cnt) below
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                const uint32_t om = okey(ts[mid]);
+                if (om > ok || (om == ok && t < s)) lo = mid + 1;
+                else hi = mid;
+            }
+            r += lo;
+        }
+        if (r < k) {  // positions are distinct: global ids are
+            fo.ids[r] = (uint64_t)s * ID_SHARD + sh_ids[(uint64_t)s * TOPK_MAX_K + j];
+            fo.scores[r] = sc;
+            if (r == kreq - 1 && fo.thr) *fo.thr = sc;
+        }
+    }
+    for (uint32_t i = kreq + threadIdx.x; i < k; i += MERGE_BS) {
+        fo.ids[i] = ~0ull;
+        fo.scores[i] = -INFINITY;
+    }
+    if (threadIdx.x == 0 && fo.status) *fo.status = 0;
+}
+
+void shard_merge_enqueue(hipStream_t s, const uint64_t *sh_ids, const float *sh_scores, uint64_t n, uint32_t shards,
+                         uint32_t k, const FinalOut &fo) {
+    ShardCounts c{};
+    c.shards = shards;
+    for (uint32_t i = 0; i < shards; i++) {
+        const uint64_t ns = n - (uint64_t)i * ID_SHARD < ID_SHARD ? n - (uint64_t)i * ID_SHARD : ID_SHARD;
+        c.cnt[i] = (uint32_t)(ns < k ? ns : k);
+    }
+    const uint32_t kreq = (uint32_t)(n < k ? n : k);
+    hipLaunchKernelGGL(shard_merge_kernel, dim3(1), dim3(MERGE_BS), 0, s, sh_ids, sh_scores, c, k, kreq, fo);
 }
 
 void final_enqueue(topk_ctx *ctx, Lane &L, const CandSet &cs, uint32_t k, uint32_t kreq, const FinalOut &fo,

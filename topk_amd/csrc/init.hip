@@ -113,24 +113,24 @@ __global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t 
 template <int D>
 __global__ __launch_bounds__(256) void reorder_attr_kernel(float *__restrict__ cols, uint64_t n, uint32_t d,
                                                            uint64_t ld) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    float v[D];
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        float v[D];
 #pragma unroll
-    for (int m = 0; m < D; m++) v[m] = (uint32_t)m < d ? cols[(uint64_t)m * ld + i] : -INFINITY;
+        for (int m = 0; m < D; m++) v[m] = (uint32_t)m < d ? cols[(uint64_t)m * ld + i] : -INFINITY;
 #pragma unroll
-    for (int r = 0; r < D; r++) {
+        for (int r = 0; r < D; r++) {
 #pragma unroll
-        for (int a = r & 1; a + 1 < D; a += 2) {
-            const float x = v[a], y = v[a + 1];
-            const bool sw = x < y;
-            v[a] = sw ? y : x;
-            v[a + 1] = sw ? x : y;
+            for (int a = r & 1; a + 1 < D; a += 2) {
+                const float x = v[a], y = v[a + 1];
+                const bool sw = x < y;
+                v[a] = sw ? y : x;
+                v[a + 1] = sw ? x : y;
+            }
         }
-    }
 #pragma unroll
-    for (int m = 0; m < D; m++)
-        if ((uint32_t)m < d) cols[(uint64_t)m * ld + i] = v[m];
+        for (int m = 0; m < D; m++)
+            if ((uint32_t)m < d) cols[(uint64_t)m * ld + i] = v[m];
+    }
 }
 
 // Column maxima start at 0 (common/common.h:36), minima at FLT_MAX, first
@@ -232,7 +232,9 @@ int topk_reorder_attr(topk_ctx *ctx) {
     ctx->store_gen++;
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, ctx->stream, ev0);
-    const dim3 grid((uint32_t)((ctx->n + 255) / 256));
+    // one object per work-item, grid-stride past 2^28 workgroups
+    const uint64_t nb = (ctx->n + 255) / 256;
+    const dim3 grid((uint32_t)(nb < (1u << 20) ? nb : (1u << 20)));
     if (ctx->d <= 8)
         hipLaunchKernelGGL(reorder_attr_kernel<8>, grid, dim3(256), 0, ctx->stream, ctx->cols, ctx->n, ctx->d, ctx->ld);
     else if (ctx->d <= 16)

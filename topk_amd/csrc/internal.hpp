@@ -64,6 +64,16 @@ enum { FTE_SAMPLE = 0, FTE_EXACT = 1, FTE_EXACT_BIG = 2 };
 
 constexpr uint32_t MAX_STREAM_GRID = 4096;
 
+// Object ids.  Ranking keys carry a 32-bit id (0xFFFFFFFF is reserved), so a
+// store of 2^32 - 1 objects or more runs the FTE path per id shard of
+// ID_SHARD objects (local ids) and merges the shards' top-k lists by score
+// and 64-bit global id.  Stores hold at most MAX_OBJECTS objects.
+constexpr uint64_t ID_LIMIT = 0xFFFFFFFFull;
+constexpr uint64_t ID_SHARD = 1ull << 31;
+constexpr uint64_t MAX_OBJECTS = 1ull << 38;
+constexpr uint32_t MAX_ID_SHARDS = (uint32_t)(MAX_OBJECTS / ID_SHARD);
+inline uint32_t id_shards(uint64_t n) { return n >= ID_LIMIT ? (uint32_t)((n + ID_SHARD - 1) / ID_SHARD) : 1u; }
+
 // Launch-shape knobs (set by topk_set_tuning / the autotuner).
 struct Tuning {
     uint32_t stream_grid = 0;   // 0: default grid cap
@@ -168,6 +178,8 @@ struct Lane {
     uint32_t *counts = nullptr;  // per-stream-block candidate counts (MAX_STREAM_GRID)
     uint64_t *cand = nullptr;    // candidate slots (cand_cap keys)
     uint64_t *spill = nullptr;   // shared spill region (spill_cap keys)
+    uint64_t *sh_ids = nullptr;  // stores of 2^32 - 1 objects or more: per id shard, its top-k
+    float *sh_scores = nullptr;  //   (local ids, TOPK_MAX_K entries per shard)
 };
 constexpr int NUM_LANES = 2;
 
@@ -292,8 +304,15 @@ struct FinalOut {
 // Enqueues the final select over a candidate set on lane L (fte.hip).
 void final_enqueue(topk_ctx *ctx, Lane &L, const CandSet &cs, uint32_t k, uint32_t kreq, const FinalOut &fo,
                    bool flat_fallback);
+// One FTE pipeline run over objects [0, n) of the columns q points at (n = 0:
+// the whole store; ids are local to those columns).
 int fte_enqueue(topk_ctx *ctx, Lane &L, uint32_t k, const QCols &q, uint64_t *d_ids, float *d_scores,
-                float *d_thr, uint32_t *d_status, int mode);
+                float *d_thr, uint32_t *d_status, int mode, uint64_t n = 0);
+// Merges the top-k lists of `shards` id shards (shard s: sh_ids / sh_scores
+// + s * TOPK_MAX_K, min(k, objects of the shard) entries, best first) into
+// fo with global ids, on stream s (fte.hip).
+void shard_merge_enqueue(hipStream_t s, const uint64_t *sh_ids, const float *sh_scores, uint64_t n, uint32_t shards,
+                         uint32_t k, const FinalOut &fo);
 int build_qcols(const topk_ctx *ctx, const float *cols, uint64_t ld, uint8_t qq, const float *w,
                 const uint8_t *attr, QCols &q);
 

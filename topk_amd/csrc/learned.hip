@@ -504,7 +504,8 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
                        topk_learned_report *rep) {
     if (!ctx || !qs || nq < 16) return TOPK_ERR_INVALID;
     if (ctx->d > LRN_MAX_D) return TOPK_ERR_UNSUPPORTED;
-    if (ctx->score_div) return TOPK_ERR_UNSUPPORTED;  // real-data batches run the FTE path, no threshold to learn
+    // real-data batches and stores with 64-bit ids run the FTE path: no threshold to learn
+    if (ctx->score_div || ctx->n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;
     topk_learned_config cfg;
     topk_learned_default_config(&cfg);
     if (cfg_in) cfg = *cfg_in;

@@ -438,6 +438,7 @@ int pta_build(topk_ctx *ctx) {
     const uint64_t n = ctx->n;
     const uint32_t d = ctx->d;
     hipStream_t s = ctx->stream;
+    if (n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;  // positions and partition ids are 32-bit
     const uint32_t L = (uint32_t)((n + P.psize - 1) / P.psize);
 
     // 1. deterministic sample -> host k-d tree with L leaves
