@@ -100,3 +100,24 @@ def test_wide_generated_async_and_sync_match_oracle(gpu):
     ri, rs, rt = oracle.fte_topk_wide(x, N, 64, w, [0], threads=16)
     gi, gs, gt = sync[64]
     assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)) and gt == rt
+
+
+def test_large_store_layout_caps_partitions_for_batches(gpu):
+    """2^30 + 12345 objects at the default 4096-object partitions would need
+    262,147 partitions: the layout stops at 2^18 (larger partitions) and
+    batches, PTA and FTE agree with the oracle."""
+    n, d = (1 << 30) + 12345, 2
+    rng = np.random.default_rng(5)
+    qs = [(k, rng.uniform(0.1, 1.0, d).astype(np.float32), np.array(a, np.uint8))
+          for k, a in ((64, [0, 1]), (16, [1]), (256, [1, 0]))]
+    with topk_amd.Engine.generated(n, d, topk_amd.ANTICORRELATED, 9) as eng:
+        eng.init(topk_amd.TOPK_INIT_PTA)
+        pid, poff, psz, pmax, pmin = eng.pta_layout()
+        assert psz.size == 1 << 18 and int(psz.sum()) == n
+        got = eng.batch(qs)
+        pta = [eng.pta(*q) for q in qs]
+    x = topk_amd.generate(n, d, topk_amd.ANTICORRELATED, 9, threads=16)
+    for q, (gi, gs, gt), (pi, ps, pt) in zip(qs, got, pta):
+        ri, rs, rt = oracle.tpac_threads(x, n, q[0], q[1], q[2], 16)
+        assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)) and gt == rt
+        assert np.array_equal(pi, ri) and np.array_equal(ps, rs) and pt == rt

@@ -72,6 +72,7 @@ constexpr uint32_t ITEM_MIN_OBJECTS = 2048;              // smallest object chun
 constexpr int BMASK_BS = 256;                            // mask workgroup
 constexpr uint32_t BMASK_MAX_PER = 64;                   // partitions per mask group (2^H)
 constexpr uint32_t BMASK_PAIRS = 4096;                   // (query, partition) pairs a mask workgroup holds
+static_assert(BMASK_PAIRS * BMASK_MAX_PER == PTA_MAX_PARTS, "every layout fits the group lists");
 enum { BMODE_SEED = 1, BMODE_LEARNED = 2, BMODE_GIVEN = 3 };
 // Per-query flags the first pass leaves for the second (in-graph) pass.
 enum : uint32_t { QF_DONE = 0, QF_SEED = 1, QF_GIVEN = 2, QF_LOWER = 3 };
@@ -1159,23 +1160,6 @@ struct ChunkArgs {
     unsigned long long *scored;  // per workgroup
 };
 
-// The last index i in [lo, hi) with key(i) <= t (key non-decreasing, key(lo)
-// <= t), found by the wave: 64 probes per round narrow the range 64-fold.
-template <class Key>
-__device__ __forceinline__ uint32_t wave_last_le(Key key, uint32_t lo, uint32_t hi, uint32_t t) {
-    const uint32_t lane = lane_id();
-    while (hi - lo > 1) {
-        const uint32_t step = (hi - lo + 63) / 64;
-        const uint32_t i = lo + lane * step;
-        const bool le = i < hi && key(i) <= t;
-        const uint64_t m = __ballot(le);  // lanes 0..j (keys are non-decreasing)
-        const uint32_t j = 63u - (uint32_t)__builtin_clzll(m);
-        lo = lo + j * step;
-        hi = lo + step < hi ? lo + step : hi;
-    }
-    return lo;
-}
-
 constexpr uint32_t BC_STAGE = 256;  // candidate keys a wave stages in LDS before one append
 
 __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
@@ -1731,7 +1715,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                evt);
     const uint32_t H = group_height(P.p2), G = P.p2 >> H;
     // a query's group list and a group's partition counters live in LDS
-    if (G > BMASK_PAIRS) return TOPK_ERR_UNSUPPORTED;  // over 262144 partitions: use larger partitions
+    if (G > BMASK_PAIRS) return TOPK_ERR_UNSUPPORTED;  // layouts stop at PTA_MAX_PARTS partitions
     const int NB = dims_class(ctx->d);
     FinalArgs fa{dq,
                  B.d_cand,

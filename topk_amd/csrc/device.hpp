@@ -34,6 +34,23 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     return v;
 }
 
+// The last index i in [lo, hi) with key(i) <= t (key non-decreasing, key(lo)
+// <= t), found by the wave: 64 probes per round narrow the range 64-fold.
+template <class Key>
+__device__ __forceinline__ uint32_t wave_last_le(Key key, uint32_t lo, uint32_t hi, uint32_t t) {
+    const uint32_t lane = lane_id();
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t i = lo + lane * step;
+        const bool le = i < hi && key(i) <= t;
+        const uint64_t m = __ballot(le);  // lanes 0..j (keys are non-decreasing)
+        const uint32_t j = 63u - (uint32_t)__builtin_clzll(m);
+        lo = lo + j * step;
+        hi = lo + step < hi ? lo + step : hi;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -60,6 +77,29 @@ struct SelectScratch {
     uint32_t digit, need, bcnt, counter;
     unsigned long long kmin, kmax;
 };
+
+// How many of the n keys a[0..n) (LDS, every lane reading the same word)
+// are above x and above y: eight reads in flight per step, so the loop
+// runs at LDS throughput rather than latency.
+__device__ __forceinline__ void rank2_lds(const uint64_t *a, uint32_t n, uint64_t x, uint64_t y, uint32_t &rx,
+                                          uint32_t &ry) {
+    uint32_t j = 0;
+    for (; j + 8 <= n; j += 8) {
+        uint64_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = a[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            rx += v[u] > x ? 1u : 0u;
+            ry += v[u] > y ? 1u : 0u;
+        }
+    }
+    for (; j < n; j++) {
+        const uint64_t v = a[j];
+        rx += v > x ? 1u : 0u;
+        ry += v > y ? 1u : 0u;
+    }
+}
 
 // Key sources: the keys a thread owns, visited by for_each.  Key 0 marks an
 // empty slot and is never selected.
@@ -284,8 +324,8 @@ template <int BS>
 __device__ void block_rank_sort_desc(const uint64_t *in, uint32_t got, uint64_t *out) {
     for (uint32_t i = threadIdx.x; i < got; i += BS) {
         const uint64_t x = in[i];
-        uint32_t r = 0;
-        for (uint32_t j = 0; j < got; j++) r += in[j] > x ? 1u : 0u;
+        uint32_t r = 0, unused = 0;
+        rank2_lds(in, got, x, x, r, unused);
         out[r] = x;
     }
     __syncthreads();

@@ -20,13 +20,17 @@ struct FlatView {
 // registers first).  Up to 2*BS candidates take one rank pass in LDS; more
 // take a radix select (DB-bit digits in registers, or over memory) and a
 // rank sort of the k selected.
+constexpr uint32_t SMALL_RANK = 384;  // candidates ranked directly (no radix select)
+
 template <int BS, int KPT, class View, int DB = 11>
 __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const FinalOut &o, uint64_t *top,
                                 uint64_t *sorted, SelectScratch &ss) {
     const uint32_t tid = threadIdx.x;
-    if (c <= 2u * BS && c <= (uint32_t)TOPK_MAX_K) {
-        // Few candidates: one rank pass.  Keys are distinct (ids are), so the
-        // number of larger keys is each key's output position.
+    if (c <= SMALL_RANK && c <= 2u * BS) {
+        // Few candidates: one rank pass (c compares per key, VALU-bound:
+        // beyond a few hundred keys the radix select below is faster).
+        // Keys are distinct (ids are), so the number of larger keys is each
+        // key's output position.
         uint64_t mk[2];
         uint32_t r[2] = {0u, 0u};
 #pragma unroll
@@ -39,11 +43,7 @@ __device__ void select_and_emit(const View &v, uint32_t c, uint32_t k, const Fin
         for (int u = 0; u < 2; u++)
             if (tid + u * BS < c) sorted[tid + u * BS] = mk[u];
         __syncthreads();
-        for (uint32_t j = 0; j < c; j++) {
-            const uint64_t x = sorted[j];
-            r[0] += x > mk[0] ? 1u : 0u;
-            r[1] += x > mk[1] ? 1u : 0u;
-        }
+        rank2_lds(sorted, c, mk[0], mk[1], r[0], r[1]);
         const uint32_t got = c < k ? c : k;
 #pragma unroll
         for (int u = 0; u < 2; u++) {

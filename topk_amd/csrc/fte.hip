@@ -354,7 +354,10 @@ __global__ __launch_bounds__(FINAL_BS) void fte_final_kernel(FteState *__restric
         }
         return;
     }
-    if (tid == 0) st->cand_last = c;
+    if (tid == 0) {
+        st->cand_last = c;
+        if (fo.cand) *fo.cand = c;
+    }
     const SlotView v{cs.pre, npre, cs.keys, cs.slot, off, cs.nblk, cs.spill};
     select_and_emit<FINAL_BS, FINAL_KPT>(v, c, k, fo, top, sorted, ss);
 }

@@ -19,8 +19,7 @@ namespace topk {
 
 int pta_build(topk_ctx *ctx);
 void pta_release(topk_ctx *ctx);
-int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *d_ids,
-              float *d_scores, float *d_thr, uint32_t *d_status);
+int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, const FinalOut &fo);
 
 // Per column max (starting at 0) and min (starting at FLT_MAX) as monotone
 // keys, so atomicMax / atomicMin on uint32 implement float max / min, and
@@ -255,21 +254,17 @@ int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     if (k > TOPK_MAX_K) return TOPK_ERR_UNSUPPORTED;
     if (!ctx->pta.built) return TOPK_ERR_STATE;
     const double t0 = now_ms();
-    uint64_t *d_ids = reinterpret_cast<uint64_t *>(ctx->d_res + ctx->res.ids_off);
-    float *d_sc = reinterpret_cast<float *>(ctx->d_res + ctx->res.scores_off);
-    float *d_thr = reinterpret_cast<float *>(ctx->d_res + ctx->res.thr_off);
-    uint32_t *d_status = reinterpret_cast<uint32_t *>(ctx->d_res + ctx->res.status_off);
-    int rc = pta_query(ctx, k, qq, weights, attr, d_ids, d_sc, d_thr, d_status);
+    // the final select writes the pinned result block and report directly
+    uint64_t *h_ids = reinterpret_cast<uint64_t *>(ctx->h_res + ctx->res.ids_off);
+    float *h_sc = reinterpret_cast<float *>(ctx->h_res + ctx->res.scores_off);
+    float *h_thr = reinterpret_cast<float *>(ctx->h_res + ctx->res.thr_off);
+    uint32_t *h_status = reinterpret_cast<uint32_t *>(ctx->h_res + ctx->res.status_off);
+    int rc = pta_query(ctx, k, qq, weights, attr, FinalOut{h_ids, h_sc, h_thr, h_status});
     if (rc) return rc;
     Lane &L = ctx->lane[0];
-    TK_HIP(hipMemcpyAsync(ctx->h_res, ctx->d_res, ctx->res.bytes, hipMemcpyDeviceToHost, L.stream));
-    TK_HIP(hipMemcpyAsync(ctx->h_state, L.st, sizeof(FteState), hipMemcpyDeviceToHost, L.stream));
     TK_HIP(hipStreamSynchronize(L.stream));
-    uint64_t scored = 0, objs = 0;
-    for (uint32_t i = 0; i < ctx->pta.nscored; i++) {
-        scored += ctx->pta.h_scored[2 * i];
-        objs += ctx->pta.h_scored[2 * i + 1];
-    }
+    const uint32_t *rep = ctx->pta.h_report;
+    const uint64_t cand = rep[0], scored = rep[1], objs = (uint64_t)rep[2] | ((uint64_t)rep[3] << 32);
     const uint32_t status = *reinterpret_cast<const uint32_t *>(ctx->h_res + ctx->res.status_off);
     if (status) {
         // candidate storage overflowed (pathological data): the exact FTE path answers
@@ -285,7 +280,7 @@ int topk_pta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     if (out_ids) memcpy(out_ids, ids, sizeof(uint64_t) * k);
     if (out_scores) memcpy(out_scores, sc, sizeof(float) * k);
     if (out_threshold) *out_threshold = *reinterpret_cast<const float *>(ctx->h_res + ctx->res.thr_off);
-    ctx->stats.candidates = ctx->h_state->cand_last;
+    ctx->stats.candidates = cand;
     ctx->stats.partitions_scored = scored;
     ctx->stats.partitions_total = ctx->pta.nparts;
     ctx->stats.fallback = 0;

@@ -193,9 +193,15 @@ struct Timing {
 // Objects are grouped by a k-d tree into nparts partitions of about psize
 // objects; partition p occupies positions [poff[p], poff[p] + psz[p]) of the
 // partition-order columns, padded to a multiple of PART_ALIGN positions.
+// Batches keep a query's partition-group list and a group's partition
+// counters in LDS (4096 groups of up to 64 partitions): a layout has at most
+// PTA_MAX_PARTS partitions, so stores past 2^30 objects get partitions
+// larger than the requested size.
+constexpr uint32_t PTA_MAX_PARTS = 1u << 18;
+
 struct PtaLayout {
     bool built = false;
-    uint32_t psize = 4096;       // target objects per partition
+    uint32_t psize = 4096;       // target objects per partition (grown to keep PTA_MAX_PARTS)
     uint32_t nparts = 0;
     uint32_t nonempty = 0;       // partitions holding at least one object
     uint32_t maxsz = 0;          // largest partition
@@ -212,10 +218,11 @@ struct PtaLayout {
     float *pminsum = nullptr;    // nparts: sum of the d minima (rounded down)
     uint64_t *pbound = nullptr;  // query scratch: bound key per partition
     uint64_t *bmax = nullptr;    // query scratch: max bound key per 256 partitions
-    uint32_t *pscored = nullptr; // query scratch: per stream block, partitions then objects scored
-    uint32_t *pcut = nullptr;    // query scratch: chunks to score per partition (0 = ruled out)
-    uint32_t *h_scored = nullptr; // pinned copy of pscored (read after the query's one synchronisation)
-    uint32_t nscored = 0;        // stream blocks of the last query (2 words each in h_scored)
+    uint32_t *pcut = nullptr;    // query scratch: chunks to score per partition (0: ruled out)
+    uint32_t *wchunks = nullptr; // query scratch: chunks to score per cut workgroup
+    uint2 *wstat = nullptr;      // query scratch: partitions and objects to score per cut workgroup
+    int cus = 0;                 // compute units of the device (the scoring grid)
+    uint32_t *h_report = nullptr; // pinned, written by the query: candidates, partitions, objects (u64)
     // Bound tree: an implicit binary tree over partition ids padded to p2 (a
     // power of two); node v has children 2v, 2v+1 and leaf p is node p2 + p.
     // Leaves are numbered in k-d tree order, so a node is a contiguous,
@@ -298,6 +305,7 @@ struct FinalOut {
     float *scores;
     float *thr;
     uint32_t *status;
+    uint32_t *cand;  // optional: the candidate count the final select saw
 };
 
 // Launch helpers implemented in the .hip translation units.

@@ -21,10 +21,12 @@
 //    rounded up to binary32.  It is never below an object's score.
 //  * query: bound keys for all partitions; the exact k-th key among the best
 //    partitions' objects (seed) is a proven lower bound tau of the global
-//    k-th key; the stream pass scores only partitions whose bound reaches
-//    tau's score (skip iff bound < tau strictly), each up to its first
-//    suffix row whose bound falls below tau, and keeps keys >= tau; the FTE
-//    final select finishes.  Results equal topk_fte's.
+//    k-th key; the task pass lists the partitions whose bound reaches tau's
+//    score (skip iff bound < tau strictly), each up to its first suffix row
+//    whose bound falls below tau, as chunk ranges; the scoring pass spreads
+//    those chunks evenly over every wave of the GPU and keeps keys >= tau;
+//    the FTE final select writes the exact top k straight into pinned host
+//    memory.  Results equal topk_fte's.
 #include <string.h>
 
 #include <algorithm>
@@ -41,12 +43,14 @@
 namespace topk {
 
 constexpr int PTA_BS = 256;
-constexpr uint32_t PTA_TILE = 1024;          // objects per stream step (4 per lane)
+constexpr uint32_t PTA_CUT_GROUPS = 4096;  // cut workgroups at most (the scoring pass scans their totals in LDS)
 constexpr uint32_t PTA_SORT_MAX = 8192;      // partitions up to this size are sorted in LDS (larger: scatter order)
 constexpr uint32_t PTA_SEED_OBJS = 1024;     // leading objects (largest attribute sums) of a seed partition
 constexpr uint32_t PTA_MIN_SEED = 16;        // seed partitions at least
 constexpr uint32_t PTA_SEED_RANK = 1024;     // block maxima ranked in LDS by the seed kernel
 constexpr uint32_t PTA_MAX_SEED = 64;
+constexpr int PTA_SEED_REG = 16;             // seed winners per thread the merge reads at once
+constexpr uint32_t PTA_SEED_LDS = PTA_SEED_REG * PTA_BS;  // seed winners the merge ranks in LDS
 
 struct KdNode {
     float split;
@@ -411,9 +415,10 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.pminsum);
     hipFree(P.pbound);
     hipFree(P.bmax);
-    hipFree(P.pscored);
     hipFree(P.pcut);
-    if (P.h_scored) hipHostFree(P.h_scored);
+    hipFree(P.wchunks);
+    hipFree(P.wstat);
+    if (P.h_report) hipHostFree(P.h_report);
     hipFree(P.tmax);
     hipFree(P.tmin);
     hipFree(P.tsum);
@@ -439,7 +444,8 @@ int pta_build(topk_ctx *ctx) {
     const uint32_t d = ctx->d;
     hipStream_t s = ctx->stream;
     if (n >= ID_LIMIT) return TOPK_ERR_UNSUPPORTED;  // positions and partition ids are 32-bit
-    const uint32_t L = (uint32_t)((n + P.psize - 1) / P.psize);
+    const uint64_t want = (n + P.psize - 1) / P.psize;
+    const uint32_t L = (uint32_t)(want < PTA_MAX_PARTS ? want : PTA_MAX_PARTS);
 
     // 1. deterministic sample -> host k-d tree with L leaves
     uint64_t ns = (uint64_t)L * 64;
@@ -538,9 +544,12 @@ int pta_build(topk_ctx *ctx) {
                                (uint64_t)P.nchunks, P.cstat);
         TK_TRY(hipMalloc(&P.pbound, sizeof(uint64_t) * L));
         TK_TRY(hipMalloc(&P.bmax, sizeof(uint64_t) * ((L + PTA_BS - 1) / PTA_BS)));
-        TK_TRY(hipMalloc(&P.pscored, sizeof(uint32_t) * 2 * MAX_STREAM_GRID));
         TK_TRY(hipMalloc(&P.pcut, sizeof(uint32_t) * L));
-        TK_TRY(hipHostMalloc(&P.h_scored, sizeof(uint32_t) * 2 * MAX_STREAM_GRID, hipHostMallocDefault));
+        TK_TRY(hipMalloc(&P.wchunks, sizeof(uint32_t) * PTA_CUT_GROUPS));
+        TK_TRY(hipMalloc(&P.wstat, sizeof(uint2) * PTA_CUT_GROUPS));
+        TK_TRY(hipHostMalloc(&P.h_report, sizeof(uint32_t) * 4, hipHostMallocDefault));
+        TK_TRY(hipDeviceGetAttribute(&P.cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        if (P.cus < 1) P.cus = 1;
         TK_TRY(hipGetLastError());
         TK_TRY(hipStreamSynchronize(s));
         rc = build_bound_tree(P, d, s);
@@ -586,44 +595,24 @@ __global__ __launch_bounds__(PTA_BS) void pta_bound_kernel(BoundArgs b, uint32_t
     }
 }
 
-// Per partition whose bound reaches tau: the first chunk whose suffix row
-// bound falls below tau (every later object of the partition scores below
-// tau too), as a count of chunks to score; 0 for partitions ruled out.
-// One wave per partition, one chunk row per lane (64 bounds per round); tau
-// is the seed kernel's, read on the device.
-template <int NB>
-__global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L, const uint64_t *__restrict__ pbound,
-                                                         const uint32_t *__restrict__ poff,
-                                                         const uint32_t *__restrict__ psz,
-                                                         const float *__restrict__ cstat, uint64_t nchunks,
-                                                         const FteState *__restrict__ st, uint32_t *__restrict__ cut) {
-    const uint32_t p = blockIdx.x * (PTA_BS / 64) + threadIdx.x / 64;
-    if (p >= L) return;
-    const uint32_t lane = lane_id();
-    const uint32_t tau_hi = (uint32_t)(st->tau >> 32);
-    if ((uint32_t)(pbound[p] >> 32) < tau_hi) {
-        if (lane == 0) cut[p] = 0;
-        return;
-    }
-    const uint32_t c0 = poff[p] / PTA_CHUNK, nc = (psz[p] + PTA_CHUNK - 1) / PTA_CHUNK;
-    const uint32_t d = b.d;
-    uint32_t first = nc;
-    for (uint32_t cb = 0; cb < nc; cb += 64) {
-        const uint32_t c = cb + lane;
-        bool fail = false;
-        if (c < nc) {
-            const float *row = cstat + (c0 + c);  // element e at row[e * nchunks]
-            fail = okey(lp_bound<NB>(b, row, row + d * nchunks, row[2 * d * nchunks], row[(2 * d + 1) * nchunks],
-                                     nchunks)) < tau_hi;
+// Seed winners blk[0..cnt) as a key source: KPT loads per thread issued
+// together, the same trip count on every thread (ballots inside).
+template <int KPT>
+struct SeedKeys {
+    const uint64_t *p;
+    uint32_t cnt;
+    template <class F>
+    __device__ __forceinline__ void for_each(F f) const {
+        uint64_t v[KPT];
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+            const uint32_t i = j * PTA_BS + threadIdx.x;
+            v[j] = i < cnt ? p[i] : 0ull;
         }
-        const uint64_t m = __ballot(fail);
-        if (m) {
-            first = cb + (uint32_t)__builtin_ctzll(m);
-            break;
-        }
+#pragma unroll
+        for (int j = 0; j < KPT; j++) f(v[j]);
     }
-    if (lane == 0) cut[p] = first;
-}
+};
 
 // Block t scores the leading PTA_SEED_OBJS objects (largest attribute sums)
 // of the partition whose bound key is the (t+1)-th largest of bmax[0..nb)
@@ -635,11 +624,14 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
                                                           const uint32_t *__restrict__ poff,
                                                           const uint32_t *__restrict__ psz,
                                                           const uint32_t *__restrict__ pid, uint32_t k,
-                                                          uint64_t *__restrict__ blk, FteState *__restrict__ st) {
+                                                          uint64_t *__restrict__ blk, uint64_t *__restrict__ bthr,
+                                                          FteState *__restrict__ st) {
     __shared__ SelectScratch ss;
     __shared__ uint64_t pick;
     __shared__ uint32_t is_last, nvalid;
     __shared__ uint64_t bm[PTA_SEED_RANK];
+    __shared__ uint64_t cands[PTA_SEED_LDS];
+    __shared__ uint64_t kth;
     const uint32_t tid = threadIdx.x;
     if (tid == 0) pick = 0;
     if (nb <= PTA_SEED_RANK) {
@@ -649,10 +641,9 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
         __syncthreads();
         for (uint32_t i = tid; i < nb; i += PTA_BS) {
             const uint64_t v = bm[i];
-            if (!v) continue;
-            uint32_t r = 0;
-            for (uint32_t j = 0; j < nb; j++) r += bm[j] > v ? 1u : 0u;
-            if (r == blockIdx.x) pick = v;
+            uint32_t r = 0, unused = 0;
+            rank2_lds(bm, nb, v, v, r, unused);
+            if (v && r == blockIdx.x) pick = v;
         }
     } else if (tid < 64) {
         // (t+1)-th largest block maximum by wave-max rounds
@@ -696,10 +687,12 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
         for (int e = 0; e < 4; e++) rk.k[j * 4 + e] = lo + e < size ? rkey(sc[e], pid[start + lo + e]) : 0ull;
     }
     uint64_t thr = 1;
-    if (size > k) thr = block_kth_largest<PTA_BS>(rk, size, k, ss);
+    if (size > k) thr = block_kth_largest<PTA_BS, RegKeys<PTA_SEED_OBJS / PTA_BS>, 11>(rk, size, k, ss);
     uint64_t *out = blk + (uint64_t)blockIdx.x * k;
     const uint32_t got = block_gather_ge<PTA_BS>(rk, thr, out, ss);
     for (uint32_t i = got + tid; i < k; i += PTA_BS) out[i] = 0;
+    // this block's k-th key (k keys at or above it), or 0 with fewer than k
+    if (tid == 0) bthr[blockIdx.x] = got >= k ? thr : 0ull;
 
     __threadfence();
     __syncthreads();
@@ -710,12 +703,37 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
     const uint32_t nk = gridDim.x * k;
     if (tid == 0) nvalid = 0;
     __syncthreads();
-    uint32_t mine = 0;
-    for (uint32_t i = tid; i < nk; i += PTA_BS) mine += blk[i] != 0;
-    atomicAdd(&nvalid, mine);
-    __syncthreads();
     uint64_t tau = 1;
-    if (nvalid > k) tau = block_kth_largest<PTA_BS>(MemKeys<PTA_BS>{blk, nk}, nvalid, k, ss);
+    if (nk <= PTA_SEED_LDS) {
+        // The union's k-th key is at least the largest block k-th key (that
+        // block alone has k keys at or above it): only keys at or above it
+        // compete.  They are gathered into LDS and the one with exactly k-1
+        // larger keys is the k-th (keys are distinct).
+        uint64_t floor_key = 0;
+        for (uint32_t b = 0; b < gridDim.x; b++) floor_key = bthr[b] > floor_key ? bthr[b] : floor_key;
+        const SeedKeys<PTA_SEED_REG> wk{blk, nk};
+        const uint32_t nc = block_gather_ge<PTA_BS>(wk, floor_key ? floor_key : 1ull, cands, ss);
+        if (nc > k && nc <= 2 * PTA_BS) {
+            // two keys per thread, each ranked against all nc
+            const uint64_t x = tid < nc ? cands[tid] : 0ull, y = tid + PTA_BS < nc ? cands[tid + PTA_BS] : 0ull;
+            uint32_t rx = 0, ry = 0;
+            rank2_lds(cands, nc, x, y, rx, ry);
+            if (x && rx == k - 1) kth = x;
+            if (y && ry == k - 1) kth = y;
+            __syncthreads();
+            tau = kth;
+        } else if (nc > k) {
+            tau = block_kth_largest<PTA_BS, SeedKeys<PTA_SEED_REG>, 11>(SeedKeys<PTA_SEED_REG>{cands, nc}, nc, k, ss);
+        } else if (nc == k && floor_key) {
+            tau = floor_key;  // exactly the largest block's k keys: its k-th is the union's
+        }
+    } else {
+        uint32_t mine = 0;
+        for (uint32_t i = tid; i < nk; i += PTA_BS) mine += blk[i] != 0;
+        atomicAdd(&nvalid, mine);
+        __syncthreads();
+        if (nvalid > k) tau = block_kth_largest<PTA_BS>(MemKeys<PTA_BS>{blk, nk}, nvalid, k, ss);
+    }
     if (tid == 0) {
         st->tau = tau;
         st->pre_count = 0;
@@ -723,56 +741,230 @@ __global__ __launch_bounds__(PTA_BS) void pta_seed_kernel(QCols q, const uint64_
     }
 }
 
-// Grid-stride over partitions: skip a partition whose bound score is below
-// tau's score, stream the others in 1024-object steps and keep keys >= tau.
-template <int M>
-__global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L, const uint32_t *__restrict__ cut,
-                                                            const uint32_t *__restrict__ poff,
-                                                            const uint32_t *__restrict__ psz,
-                                                            const uint32_t *__restrict__ pid,
-                                                            FteState *__restrict__ st, CandSet cs,
-                                                            uint32_t *__restrict__ pscored) {
-    __shared__ uint32_t lcount;
-    if (threadIdx.x == 0) lcount = 0;
+// Cut per partition, as the number of its leading objects to score: 0 when
+// its bound is below tau's score (skip iff bound < tau strictly), else the
+// objects before its first chunk whose suffix-row bound falls below tau
+// (every later object of the partition scores below tau too).  Workgroup
+// g owns partitions [g * R, (g + 1) * R) (R a multiple of 16), four per
+// wave per round, 16 lanes each, one chunk row per lane (a 4096-object
+// partition in one round), and publishes its chunk total (wchunks[g]) and
+// its partitions and objects to score (wstat[g]).  No atomics: the
+// scoring pass scans the workgroup totals.
+template <int NB>
+__global__ __launch_bounds__(PTA_BS) void pta_cut_kernel(BoundArgs b, uint32_t L, uint32_t R,
+                                                         const uint64_t *__restrict__ pbound,
+                                                         const uint32_t *__restrict__ poff,
+                                                         const uint32_t *__restrict__ psz,
+                                                         const float *__restrict__ cstat, uint64_t nchunks,
+                                                         const FteState *__restrict__ st, uint32_t *__restrict__ cut,
+                                                         uint32_t *__restrict__ wchunks, uint2 *__restrict__ wstat) {
+    __shared__ uint32_t wsum[PTA_BS / 64][3];
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64, g = lane / 16, gl = lane % 16;
+    const uint32_t tau_hi = (uint32_t)(st->tau >> 32);
+    const uint32_t d = b.d;
+    uint32_t ch = 0, parts = 0, objs = 0;  // this lane's partitions (lanes gl == 0)
+    for (uint32_t r = 0; r < R; r += 16) {
+        const uint32_t p = blockIdx.x * R + r + wave * 4 + g;
+        const uint64_t pb = p < L ? pbound[p] : 0ull;  // key 0: an empty partition
+        const bool alive = pb != 0 && (uint32_t)(pb >> 32) >= tau_hi;
+        const uint32_t sz = alive ? psz[p] : 0u;
+        const uint32_t c0 = alive ? poff[p] / PTA_CHUNK : 0u;
+        const uint32_t nc = (sz + PTA_CHUNK - 1) / PTA_CHUNK;
+        uint32_t first = nc;  // this group's answer, settled once `done`
+        bool done = !alive;
+        for (uint32_t cb = 0; __ballot(!done); cb += 16) {
+            const uint32_t c = cb + gl;
+            bool fail = false;
+            if (!done && c < nc) {
+                const float *row = cstat + (c0 + c);  // element e at row[e * nchunks]
+                fail = okey(lp_bound<NB>(b, row, row + d * nchunks, row[2 * d * nchunks],
+                                         row[(2 * d + 1) * nchunks], nchunks)) < tau_hi;
+            }
+            const uint32_t m = (uint32_t)(__ballot(fail) >> (16 * g)) & 0xFFFFu;
+            if (!done) {
+                if (m) {
+                    first = cb + (uint32_t)__builtin_ctz(m);
+                    done = true;
+                } else if (cb + 16 >= nc) {
+                    done = true;
+                }
+            }
+        }
+        // objects to score: the chunks before the cut, the last one partial
+        const uint32_t o = alive ? min(sz, first * PTA_CHUNK) : 0u;
+        if (gl == 0 && p < L) {
+            cut[p] = o;
+            ch += (o + PTA_CHUNK - 1) / PTA_CHUNK;
+            parts += o ? 1u : 0u;
+            objs += o;
+        }
+    }
+    for (int x = 32; x > 0; x >>= 1) {
+        ch += __shfl_xor(ch, x, 64);
+        parts += __shfl_xor(parts, x, 64);
+        objs += __shfl_xor(objs, x, 64);
+    }
+    if (lane == 0) {
+        wsum[wave][0] = ch;
+        wsum[wave][1] = parts;
+        wsum[wave][2] = objs;
+    }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a = 0, bp = 0, bo = 0;
+        for (int w = 0; w < PTA_BS / 64; w++) {
+            a += wsum[w][0];
+            bp += wsum[w][1];
+            bo += wsum[w][2];
+        }
+        wchunks[blockIdx.x] = a;
+        wstat[blockIdx.x] = make_uint2(bp, bo);
+    }
+}
+
+// Scores the chunks of the partitions the cut pass kept.  Every workgroup
+// first scans the G cut-workgroup chunk totals into LDS (E[g]: chunks
+// before group g).  The waves split the chunks into equal contiguous
+// ranges; a wave finds its group by a binary search of E, holds the
+// group's cuts (lane j: partition g * R + j) and their chunk prefix in
+// registers, and walks on, one chunk per step: 4 objects per lane, the
+// queried columns loaded together.  Keys >= tau go to the workgroup's own
+// slot region of the candidate buffer (an LDS counter; past the slot, the
+// shared spill region), its count to cs.counts[workgroup].  Workgroup 0
+// also reports the partitions and objects scored.
+template <int M>
+__global__ __launch_bounds__(PTA_BS) void pta_score_kernel(QCols q, uint32_t L, uint32_t R, uint32_t G,
+                                                           const uint32_t *__restrict__ cut,
+                                                           const uint32_t *__restrict__ wchunks,
+                                                           const uint2 *__restrict__ wstat,
+                                                           const uint32_t *__restrict__ poff,
+                                                           const uint32_t *__restrict__ pid,
+                                                           FteState *__restrict__ st, CandSet cs,
+                                                           uint32_t *__restrict__ report) {
+    __shared__ uint32_t E[PTA_CUT_GROUPS + 1];
+    __shared__ uint32_t wtot[PTA_BS / 64];
+    __shared__ uint32_t wrep[PTA_BS / 64][2];
+    __shared__ uint32_t lcount;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
+    // exclusive scan of the group totals: thread t owns groups [16t, 16t + 16)
+    {
+        uint32_t v[16], sum = 0, pp = 0, oo = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t gi = tid * 16 + j;
+            v[j] = gi < G ? wchunks[gi] : 0u;
+            sum += v[j];
+        }
+        if (blockIdx.x == 0) {
+            // partitions and objects scored, for the report (fit 32 bits: positions do)
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const uint32_t gi = tid * 16 + j;
+                const uint2 w = gi < G ? wstat[gi] : make_uint2(0u, 0u);
+                pp += w.x;
+                oo += w.y;
+            }
+            for (int x = 32; x > 0; x >>= 1) {
+                pp += __shfl_xor(pp, x, 64);
+                oo += __shfl_xor(oo, x, 64);
+            }
+            if (lane == 0) {
+                wrep[wave][0] = pp;
+                wrep[wave][1] = oo;
+            }
+        }
+        const uint32_t incl = wave_incl_scan(sum);
+        if (lane == 63) wtot[wave] = incl;
+        if (tid == 0) lcount = 0;
+        __syncthreads();
+        uint32_t run = incl - sum, all = 0;
+        for (uint32_t w = 0; w < PTA_BS / 64; w++) {
+            run += w < wave ? wtot[w] : 0u;
+            all += wtot[w];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t gi = tid * 16 + j;
+            if (gi < G) E[gi] = run;
+            run += v[j];
+        }
+        if (tid == 0) E[G] = all;  // the total
+        if (blockIdx.x == 0 && tid == 0) {
+            uint32_t a = 0, o = 0;
+            for (int w = 0; w < PTA_BS / 64; w++) {
+                a += wrep[w][0];
+                o += wrep[w][1];
+            }
+            report[1] = a;
+            report[2] = o;
+            report[3] = 0;
+        }
+        __syncthreads();
+    }
+    const uint32_t total = E[G];
+    const uint32_t nw = gridDim.x * (PTA_BS / 64), gw = blockIdx.x * (PTA_BS / 64) + wave;
     const uint64_t tau = st->tau;
     const uint32_t tau_hi = (uint32_t)(tau >> 32);
-    const uint32_t lane = lane_id();
+    const uint32_t per = (total + nw - 1) / nw;
+    uint32_t c = gw * per;
+    const uint32_t c_end = c + per < total ? c + per : total;
     uint64_t *my = cs.keys + (uint64_t)blockIdx.x * cs.slot;
-    uint32_t scored = 0, objs = 0;
-    for (uint32_t p = blockIdx.x; p < L; p += gridDim.x) {
-        const uint32_t nc = cut[p];
-        if (!nc) continue;
-        scored++;
-        const uint32_t start = poff[p], end = start + min(psz[p], nc * PTA_CHUNK);
-        objs += end - start;
-        for (uint32_t t0 = start; t0 < end; t0 += PTA_TILE) {
-            const uint64_t v = t0 / 4 + threadIdx.x;
+    if (c < c_end) {
+        // group of chunk c: the last g with E[g] <= c (E non-decreasing, E[0] = 0)
+        uint32_t lo = 0, hi = G;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (E[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        uint32_t grp = lo;
+        uint32_t gbase = 0, lch = 0, lincl = 0;  // lane j: chunks of partition grp * R + j, inclusive prefix
+        auto load_group = [&]() {
+            gbase = E[grp];
+            const uint32_t p = grp * R + lane;
+            const uint32_t o = lane < R && p < L ? cut[p] : 0u;
+            lch = (o + PTA_CHUNK - 1) / PTA_CHUNK;
+            lincl = wave_incl_scan(lch);
+        };
+        load_group();
+        for (; c < c_end; c++) {
+            while (c >= E[grp + 1]) {
+                grp++;
+                if (c < E[grp + 1]) load_group();
+            }
+            // partition: the first lane whose inclusive prefix passes c
+            const uint64_t pm = __ballot(gbase + lincl > c);
+            const uint32_t j = (uint32_t)__builtin_ctzll(pm);
+            const uint32_t pincl = (uint32_t)__shfl(lincl, (int)j, 64), pch = (uint32_t)__shfl(lch, (int)j, 64);
+            const uint32_t p = grp * R + j, lc = c - (gbase + pincl - pch);
+            const uint32_t objs_p = cut[p];  // objects to score (uniform load)
+            const uint32_t first = lc * PTA_CHUNK + 4 * lane;
+            const uint64_t pos = (uint64_t)poff[p] + first;
             f4 sc = {0.0f, 0.0f, 0.0f, 0.0f};
             if constexpr (M > 0) {
                 f4 x[M];
 #pragma unroll
-                for (int c = 0; c < M; c++) x[c] = ld_stream(q.col[c] + v * 4);
+                for (int jj = 0; jj < M; jj++) x[jj] = ld_stream(q.col[jj] + pos);
 #pragma unroll
-                for (int c = 0; c < M; c++) {
-                    f4 pr = x[c] * q.w[c];
+                for (int jj = 0; jj < M; jj++) {
+                    f4 pr = x[jj] * q.w[jj];
                     sc = sc + pr;
                 }
             } else {
-                for (uint32_t c = 0; c < q.m; c++) {
-                    f4 pr = ld_stream(q.col[c] + v * 4) * q.w[c];
+                for (uint32_t jj = 0; jj < q.m; jj++) {
+                    f4 pr = ld_stream(q.col[jj] + pos) * q.w[jj];
                     sc = sc + pr;
                 }
             }
-            uint32_t npass = 0;
+            const uint32_t nvalid = first < objs_p ? (objs_p - first < 4u ? objs_p - first : 4u) : 0u;
             uint64_t kk[4];
+            uint32_t npass = 0;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const uint32_t pos = (uint32_t)(v * 4 + e);
                 const uint32_t ok = okey(sc[e]);
                 uint64_t key = 0;
-                if (pos < end && ok >= tau_hi) {
-                    key = rkey_from_okey(ok, pid[pos]);
+                if ((uint32_t)e < nvalid && ok >= tau_hi) {
+                    key = rkey_from_okey(ok, pid[pos + e]);
                     if (key < tau) key = 0;
                 }
                 kk[e] = key;
@@ -780,46 +972,43 @@ __global__ __launch_bounds__(PTA_BS) void pta_stream_kernel(QCols q, uint32_t L,
             }
             if (__ballot(npass != 0)) {
                 const uint32_t incl = wave_incl_scan(npass);
-                uint32_t base = 0;
-                if (lane == 63) base = atomicAdd(&lcount, incl);
-                base = __shfl(base, 63, 64);
-                const uint32_t wtotal = __shfl(incl, 63, 64);
-                const uint32_t first_spill = base > cs.slot ? base : cs.slot;
-                const uint32_t nspill = base + wtotal > first_spill ? base + wtotal - first_spill : 0u;
+                const uint32_t wt = __shfl(incl, 63, 64);
+                uint32_t cnt = 0;
+                if (lane == 63) cnt = atomicAdd(&lcount, wt);
+                cnt = __shfl(cnt, 63, 64);
+                const uint32_t first_spill = cnt > cs.slot ? cnt : cs.slot;
+                const uint32_t nspill = cnt + wt > first_spill ? cnt + wt - first_spill : 0u;
                 uint32_t sbase = 0;
                 if (nspill) {
                     if (lane == 63) sbase = atomicAdd(&st->spill, nspill);
                     sbase = __shfl(sbase, 63, 64);
                 }
-                uint32_t pos = base + incl - npass;
+                uint32_t pos_k = cnt + incl - npass;
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
+                for (int e = 0; e < 4; e++)
                     if (kk[e]) {
-                        if (pos < cs.slot) {
-                            my[pos] = kk[e];
+                        if (pos_k < cs.slot) {
+                            my[pos_k] = kk[e];
                         } else {
-                            const uint32_t sp = sbase + (pos - first_spill);
+                            const uint32_t sp = sbase + (pos_k - first_spill);
                             if (sp < cs.spill_cap) cs.spill[sp] = kk[e];
                         }
-                        pos++;
+                        pos_k++;
                     }
-                }
             }
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        cs.counts[blockIdx.x] = lcount < cs.slot ? lcount : cs.slot;
-        pscored[2 * blockIdx.x] = scored;
-        pscored[2 * blockIdx.x + 1] = objs;
-    }
+    if (tid == 0) cs.counts[blockIdx.x] = lcount < cs.slot ? lcount : cs.slot;
 }
 
 // ---------------------------------------------------------------------------
 // host: query
 // ---------------------------------------------------------------------------
-int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, uint64_t *d_ids,
-              float *d_scores, float *d_thr, uint32_t *d_status) {
+// bound -> seed -> chunk tasks -> chunk scoring -> final select, all on the
+// context's stream; fo points at pinned host memory, so the caller only
+// synchronises.
+int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const uint8_t *attr, const FinalOut &fo) {
     PtaLayout &P = ctx->pta;
     if (!P.built) return TOPK_ERR_STATE;
     QCols q;
@@ -845,42 +1034,47 @@ int pta_query(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const
     if (T > nr) T = nr;
 #define TK_SEED(MM)                                                                                              \
     hipLaunchKernelGGL(pta_seed_kernel<MM>, dim3(T), dim3(PTA_BS), 0, s, q, rkeys, nr, P.poff, P.psz, P.pid, k, \
-                       Ln.blk, Ln.st)
+                       Ln.blk, Ln.pre, Ln.st)
     TK_SWITCH_M(q.m, TK_SEED);
 #undef TK_SEED
-    const uint32_t g = L < MAX_STREAM_GRID / 2 ? L : MAX_STREAM_GRID / 2;
+    hipEvent_t ev0 = nullptr;
+    timing_begin(ctx, s, ev0);
+    // cut pass: R partitions per workgroup (four per wave per round), at
+    // most PTA_CUT_GROUPS workgroups
+    uint32_t R = 16;
+    while ((L + R - 1) / R > PTA_CUT_GROUPS) R += 16;
+    if (R > 64) return TOPK_ERR_UNSUPPORTED;  // a scoring wave holds one group's cuts
+    const uint32_t G = (L + R - 1) / R;
+    if (ctx->d <= 8)
+        hipLaunchKernelGGL(pta_cut_kernel<8>, dim3(G), dim3(PTA_BS), 0, s, b, L, R, P.pbound, P.poff, P.psz, P.cstat,
+                           (uint64_t)P.nchunks, Ln.st, P.pcut, P.wchunks, P.wstat);
+    else if (ctx->d <= 16)
+        hipLaunchKernelGGL(pta_cut_kernel<16>, dim3(G), dim3(PTA_BS), 0, s, b, L, R, P.pbound, P.poff, P.psz,
+                           P.cstat, (uint64_t)P.nchunks, Ln.st, P.pcut, P.wchunks, P.wstat);
+    else
+        hipLaunchKernelGGL(pta_cut_kernel<TOPK_MAX_DIMS>, dim3(G), dim3(PTA_BS), 0, s, b, L, R, P.pbound, P.poff,
+                           P.psz, P.cstat, (uint64_t)P.nchunks, Ln.st, P.pcut, P.wchunks, P.wstat);
+    // scoring: four resident 4-wave workgroups per CU, one slot region each
+    const uint32_t sg = std::min((uint32_t)P.cus * 4, MAX_STREAM_GRID);
     CandSet cs;
     cs.keys = Ln.cand;
     cs.counts = Ln.counts;
-    cs.nblk = g;
-    cs.slot = (uint32_t)(ctx->cand_cap / g);
+    cs.nblk = sg;
+    cs.slot = (uint32_t)(ctx->cand_cap / cs.nblk);
     cs.pre = nullptr;
     cs.spill = Ln.spill;
     cs.spill_cap = ctx->spill_cap;
-    hipEvent_t ev0 = nullptr;
-    timing_begin(ctx, s, ev0);
-    const uint32_t nw = (L + PTA_BS / 64 - 1) / (PTA_BS / 64);  // one wave per partition
-    if (ctx->d <= 8)
-        hipLaunchKernelGGL(pta_cut_kernel<8>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff, P.psz, P.cstat,
-                           (uint64_t)P.nchunks, Ln.st, P.pcut);
-    else if (ctx->d <= 16)
-        hipLaunchKernelGGL(pta_cut_kernel<16>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff, P.psz,
-                           P.cstat, (uint64_t)P.nchunks, Ln.st, P.pcut);
-    else
-        hipLaunchKernelGGL(pta_cut_kernel<TOPK_MAX_DIMS>, dim3(nw), dim3(PTA_BS), 0, s, b, L, P.pbound, P.poff,
-                           P.psz, P.cstat, (uint64_t)P.nchunks, Ln.st, P.pcut);
-#define TK_STREAM(MM)                                                                                      \
-    hipLaunchKernelGGL(pta_stream_kernel<MM>, dim3(g), dim3(PTA_BS), 0, s, q, L, P.pcut, P.poff, P.psz, \
-                       P.pid, Ln.st, cs, P.pscored)
-    TK_SWITCH_M(q.m, TK_STREAM);
-#undef TK_STREAM
+#define TK_SCORE(MM)                                                                                          \
+    hipLaunchKernelGGL(pta_score_kernel<MM>, dim3(sg), dim3(PTA_BS), 0, s, q, L, R, G, P.pcut, P.wchunks, P.wstat, \
+                       P.poff, P.pid, Ln.st, cs, P.h_report)
+    TK_SWITCH_M(q.m, TK_SCORE);
+#undef TK_SCORE
     timing_end(ctx, s, "pta_stream", ev0);
     const uint32_t kreq = (uint32_t)(ctx->n < k ? ctx->n : k);
-    final_enqueue(ctx, Ln, cs, k, kreq, FinalOut{d_ids, d_scores, d_thr, d_status}, false);
+    FinalOut f = fo;
+    f.cand = P.h_report;  // report[0]
+    final_enqueue(ctx, Ln, cs, k, kreq, f, false);
     TK_HIP(hipGetLastError());
-    // partitions and objects scored per stream block: read by the caller after its synchronisation
-    TK_HIP(hipMemcpyAsync(P.h_scored, P.pscored, sizeof(uint32_t) * 2 * g, hipMemcpyDeviceToHost, s));
-    P.nscored = g;
     return TOPK_OK;
 }
 
