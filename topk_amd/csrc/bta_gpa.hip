@@ -82,28 +82,56 @@ __global__ void dsel_init_kernel(DselState *st, const uint32_t *cnt, uint64_t ba
     }
 }
 
+// Digit histogram of one pass in LDS.  The keys of a wave are folded with
+// wave primitives before they reach LDS: the lanes sharing the first
+// pending lane's digit (a ballot of digit equality) add their count with
+// one atomic from that lane, twice per key slot, and the rest add one by
+// one.  Keys with concentrated digits -- score keys share their exponent
+// bits, and a late pass sees one bucket's keys -- then cost one or two LDS
+// atomics per wave instead of up to 64 on one address; spread digits cost
+// two extra ballots.  Every lane runs the same trip count (ballots inside).
+constexpr int DSEL_FOLD = 2;  // leader-digit folds per key slot
 template <class Src>
 __global__ __launch_bounds__(DSEL_BS) void dsel_hist_kernel(Src src, const uint32_t *cnt, uint64_t n_host,
                                                             DselState *st, int shift, int width, int quads) {
     __shared__ uint32_t h[2048];
-    const uint32_t nb = 1u << width;
+    const uint32_t nb = 1u << width, lane = lane_id();
     for (uint32_t i = threadIdx.x; i < nb; i += DSEL_BS) h[i] = 0;
     __syncthreads();
     const uint64_t n = cnt ? *cnt : n_host;
     const uint32_t prefix = st->prefix, mask = st->mask;
-    auto take = [&](uint32_t k) {
-        if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & (nb - 1)], 1u);
+    auto take = [&](uint32_t k, bool valid) {
+        const uint32_t d = (k >> shift) & (nb - 1);
+        bool pend = valid && (k & mask) == prefix;
+#pragma unroll
+        for (int it = 0; it < DSEL_FOLD; it++) {
+            const uint64_t act = __ballot(pend);
+            if (!act) break;
+            const uint32_t lead = (uint32_t)__builtin_ctzll(act);
+            const uint32_t ld = (uint32_t)__shfl((int)d, (int)lead, 64);
+            const bool same = pend && d == ld;
+            const uint64_t sm = __ballot(same);
+            if (lane == lead) atomicAdd(&h[ld], (uint32_t)__popcll(sm));
+            pend = pend && !same;
+        }
+        if (pend) atomicAdd(&h[d], 1u);
     };
     const uint64_t gid = (uint64_t)blockIdx.x * DSEL_BS + threadIdx.x, stride = (uint64_t)gridDim.x * DSEL_BS;
     const uint64_t n4 = quads ? n / 4 : 0;
-    for (uint64_t q = gid; q < n4; q += stride) {
-        const uint4 k = src.quad(q);
-        take(k.x);
-        take(k.y);
-        take(k.z);
-        take(k.w);
+    for (uint64_t qb = gid - lane; qb < n4; qb += stride) {  // wave-uniform trip count
+        const uint64_t q = qb + lane;
+        const bool valid = q < n4;
+        const uint4 k = valid ? src.quad(q) : make_uint4(0u, 0u, 0u, 0u);
+        take(k.x, valid);
+        take(k.y, valid);
+        take(k.z, valid);
+        take(k.w, valid);
     }
-    for (uint64_t i = n4 * 4 + gid; i < n; i += stride) take(src(i));
+    for (uint64_t ib = n4 * 4 + gid - lane; ib < n; ib += stride) {
+        const uint64_t i = ib + lane;
+        const bool valid = i < n;
+        take(valid ? src(i) : 0u, valid);
+    }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += DSEL_BS)
         if (h[i]) atomicAdd(&st->hist[i], h[i]);
