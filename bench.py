@@ -148,22 +148,13 @@ def learned_batched(eng, n, k, args):
 
     def timed(steps, warmup, mode):
         """Two batches in flight (topk_batch_submit / topk_batch_wait): the
-        host stages batch i+1 and copies out batch i-1 while batch i runs."""
+        host stages batch i+1 and completes batch i-1 while batch i runs.
+        The statistics (fallbacks, scored pairs) of the same sequence of
+        batches are read afterwards, one synchronous call per batch, so
+        their reads stay out of the timed loop (batches are deterministic)."""
         eng.set_tuning("batch_mode", mode)
         for i in range(warmup):
             eng.batch_run(held[i % len(held)])
-        fb = ff = pairs = objs = cand = items = 0
-
-        def account():
-            nonlocal fb, ff, pairs, objs, cand, items
-            st = eng.stats()
-            fb += st["fallback"]
-            ff += st["fte_fallback"]
-            pairs += st["partitions_scored"]
-            objs += st["tuple_count"]
-            cand += st["candidates"]
-            items += st["items"]
-
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         pending = []
@@ -171,12 +162,20 @@ def learned_batched(eng, n, k, args):
             pending.append(eng.batch_submit(held[i % len(held)]))
             if len(pending) == 2:
                 eng.batch_wait(pending.pop(0))
-                account()
         for t in pending:
             eng.batch_wait(t)
-            account()
         torch.cuda.synchronize()
         el = time.perf_counter() - t1
+        fb = ff = pairs = objs = cand = items = 0
+        for i in range(steps):
+            eng.batch_run(held[i % len(held)])
+            st = eng.stats()
+            fb += st["fallback"]
+            ff += st["fte_fallback"]
+            pairs += st["partitions_scored"]
+            objs += st["tuple_count"]
+            cand += st["candidates"]
+            items += st["items"]
         return el, fb, ff, pairs, objs, cand, items
 
     def timed_sync(steps):
@@ -341,27 +340,31 @@ def subset_queries(rng, count, d, ks):
 
 
 def pipelined(eng, pbs, steps):
-    """Batches per second with two in flight; returns (q/s, fallback, fte fallback)."""
+    """Batches per second with two in flight; returns (q/s, fallback, fte
+    fallback).  The fallback counts of the same batches are read afterwards,
+    one synchronous call per batch (batches are deterministic)."""
     import torch
     for pb in pbs[:2]:
         eng.batch_run(pb)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    pend, fb, ff, nq = [], 0, 0, 0
+    pend, nq = [], 0
     for i in range(steps):
         pend.append(eng.batch_submit(pbs[i % len(pbs)]))
         nq += pbs[i % len(pbs)].nq
         if len(pend) == 2:
             eng.batch_wait(pend.pop(0))
-            st = eng.stats()
-            fb, ff = fb + st["fallback"], ff + st["fte_fallback"]
     for t in pend:
         eng.batch_wait(t)
-        st = eng.stats()
-        fb, ff = fb + st["fallback"], ff + st["fte_fallback"]
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return nq / el, fb / nq, ff / nq
+    fb = ff = 0
+    for pb in pbs:
+        eng.batch_run(pb)
+        st = eng.stats()
+        fb, ff = fb + st["fallback"], ff + st["fte_fallback"]
+    nb = len(pbs) * pbs[0].nq
+    return nq / el, fb / nb, ff / nb
 
 
 def pta_bench(threads, cpu=True, queries=20):
@@ -800,8 +803,10 @@ def stream_bench(n, d, chunk, nbatch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed steps (batches; queries with --mode fte)")
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps: batches (default 200; 20 with --mode batched / learned16), queries with "
+                         "--mode fte (200), file loads with --mode ingest (3)")
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels", "c1",
                                        "batched", "learned16"],
                     default="learned",
@@ -823,6 +828,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=2)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"batched": 20, "learned16": 20, "ingest": 3}.get(args.mode, 200)
     if args.train_queries is None and args.mode != "learned16":
         args.train_queries = 16384
     if args.calibrate_copy:

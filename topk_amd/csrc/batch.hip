@@ -776,15 +776,6 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
 }
 #undef TK_TERM
 
-// Where a batch's final select writes ids / scores / thresholds: the
-// caller's own arrays when they are pinned host memory the device can write
-// (direct), else the device output block.  Staged with the queries, so a
-// captured graph reads it from memory.
-struct OutDesc {
-    uint64_t ids, scores, thr;
-    uint32_t direct, pad;
-};
-
 struct FinalArgs {
     BatchQuery *q;
     const uint64_t *cand;
@@ -805,8 +796,53 @@ struct FinalArgs {
     unsigned int *pcnt;      // first pass: cleared here for the second
     uint32_t L;
     unsigned int *flist;     // first pass: queries flagged for the second ([0] count, then ids); may be null
-    const OutDesc *od;       // result destination of this batch (null: the device output block)
 };
+
+// Where a batch's results go: the caller's own arrays when every query
+// fills its stride and they are pinned host memory (direct), else the
+// slot's pinned staging block.  Staged with the queries, so a captured
+// graph reads it from device memory.
+struct OutDesc {
+    uint64_t ids, scores, thr;
+    uint32_t direct, pad;
+};
+
+// Publishes a finished batch to pinned host memory, as the last step of
+// the batch on its slot's stream: direct, the ids, scores and thresholds
+// into the caller's arrays and the status words into the staging block;
+// otherwise the whole output block into the staging block.  Its time is
+// the host link's, and the other slot's batch runs meanwhile.
+constexpr uint32_t BPUB_GRID = 64;
+struct PublishArgs {
+    const unsigned char *d_out;
+    size_t ids_off, sc_off, thr_off, st_off, bytes;
+    uint64_t nres;
+    uint32_t nq;
+    const OutDesc *od;
+    unsigned char *h_out;
+};
+
+__global__ __launch_bounds__(256) void batch_publish_kernel(PublishArgs a) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
+    const uint32_t *dw = reinterpret_cast<const uint32_t *>(a.d_out);
+    uint32_t *hw = reinterpret_cast<uint32_t *>(a.h_out);
+    const OutDesc od = *a.od;
+    if (od.direct) {
+        const uint64_t *di = reinterpret_cast<const uint64_t *>(a.d_out + a.ids_off);
+        const float *ds = reinterpret_cast<const float *>(a.d_out + a.sc_off);
+        uint64_t *ids = reinterpret_cast<uint64_t *>(od.ids);
+        float *scores = reinterpret_cast<float *>(od.scores), *thr = reinterpret_cast<float *>(od.thr);
+        for (uint64_t i = t; i < a.nres; i += stride) {
+            ids[i] = di[i];
+            scores[i] = ds[i];
+        }
+        const float *dt = reinterpret_cast<const float *>(a.d_out + a.thr_off);
+        for (uint64_t i = t; i < a.nq; i += stride) thr[i] = dt[i];
+        for (uint64_t i = a.st_off / 4 + t; i < a.bytes / 4; i += stride) hw[i] = dw[i];
+    } else {
+        for (uint64_t i = t; i < a.bytes / 4; i += stride) hw[i] = dw[i];
+    }
+}
 
 template <int DB>
 __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
@@ -896,14 +932,7 @@ __global__ __launch_bounds__(BFIN_BS) void batch_final_kernel(FinalArgs a) {
         return;
     }
     if (tid == 0 && !a.second) a.qflag[q] = QF_DONE;
-    uint64_t *ids = a.ids;
-    float *scores = a.scores, *thr = a.thr;
-    if (a.od && a.od->direct) {
-        ids = reinterpret_cast<uint64_t *>(a.od->ids);
-        scores = reinterpret_cast<float *>(a.od->scores);
-        thr = reinterpret_cast<float *>(a.od->thr);
-    }
-    const FinalOut o{ids + (uint64_t)q * a.kstride, scores + (uint64_t)q * a.kstride, thr + q, a.status + q};
+    const FinalOut o{a.ids + (uint64_t)q * a.kstride, a.scores + (uint64_t)q * a.kstride, a.thr + q, a.status + q};
     select_and_emit<BFIN_BS, BFIN_KPT, FlatView, DB>(FlatView{a.cand + (uint64_t)q * BATCH_CAP}, c, k, o, top, sorted,
                                                      ss);
 }
@@ -1344,23 +1373,43 @@ struct BatchVariant {
 
 using GraphKey = std::array<uint64_t, 16>;
 
-// Pinned staging of one batch in flight, and where its results go.
-// One batch in flight: its pinned staging, its device inputs and outputs,
-// and the events that order its upload, kernels and result copy.
+// One batch in flight: its stream, pinned staging, device inputs and
+// outputs, the scratch its kernels use, and the event that marks its
+// results on the host.  The two slots share nothing a kernel writes, so
+// their batches run concurrently on their own streams.
 struct HostSlot {
+    hipStream_t stream = nullptr;    // pipeline lane `slot`'s stream (owned by the context)
     unsigned char *h_in = nullptr, *h_out = nullptr;
     size_t in_cap = 0, out_cap = 0;
-    unsigned char *d_in = nullptr;   // raw queries | taus | BatchQuery[nq] | BoundArgs[nq]
+    unsigned char *d_in = nullptr;   // raw queries | taus | output descriptor | BatchQuery[nq] | BoundArgs[nq]
     unsigned char *d_out = nullptr;  // ids | scores | thr | status | cnt | flags | ctr
     size_t din_cap = 0, dout_cap = 0;
-    hipEvent_t done = nullptr;       // recorded after the batch's status copy
+    hipEvent_t done = nullptr;       // recorded after the batch's results reached the host
     bool busy = false;
-    bool direct = false;             // ids / scores / thresholds were copied straight into the caller's arrays
+    bool direct = false;             // ids / scores / thresholds were written straight into the caller's arrays
     uint64_t ticket = 0;
     uint32_t nq = 0, ks = 0;
     uint64_t *out_ids = nullptr;
     float *out_scores = nullptr, *out_thr = nullptr;
     double t0 = 0;
+    // scratch of this slot's kernels
+    uint32_t nq_cap = 0;
+    uint64_t plist_cap = 0, pcnt_cap = 0, item_cap = 0;
+    uint16_t *d_plist = nullptr;     // per-partition query lists (nq entries each)
+    unsigned int *d_pcnt = nullptr;  // per-partition list length
+    BatchItem *d_items = nullptr;
+    unsigned int *d_ctr = nullptr;
+    unsigned long long *d_scored = nullptr;  // per scoring workgroup: pairs, then terms scored
+    // query-major pass: task lists per (query, slice) unit, their counts and
+    // the prefix of their chunk counts (null when over the memory budget)
+    uint2 *d_tasks = nullptr;
+    uint64_t tasks_cap = 0;
+    uint2 *d_ucnt = nullptr;
+    uint32_t *d_uoff = nullptr;
+    uint32_t unit_cap = 0;
+    unsigned int *d_flist = nullptr;  // queries the first pass flagged for the second: [0] count, then ids
+    unsigned int *d_ccount = nullptr;
+    uint64_t *d_cand = nullptr;
 };
 
 // True when p is pinned (page-locked) host memory the GPU can write.
@@ -1387,30 +1436,11 @@ static hipError_t wait_event(hipEvent_t e) {
 
 struct BatchState {
     uint64_t gen = ~0ull;   // layout generation the buffers and graphs belong to
-    uint32_t nq_cap = 0;
-    uint64_t plist_cap = 0, pcnt_cap = 0, item_cap = 0;
-    // Two batches may be in flight (topk_batch_submit): each has its own slot
-    // (staging, inputs, outputs); the scratch below is shared because the
-    // batches run one after the other on the context's stream.
+    // Two batches may be in flight (topk_batch_submit), each in its own slot.
     HostSlot slot[2];
     int cur = 0;                     // slot of the batch being enqueued
     int next = 0;                    // slot the next submit uses
     uint64_t next_ticket = 1;
-    uint16_t *d_plist = nullptr;     // per-partition query lists (nq entries each)
-    unsigned int *d_pcnt = nullptr;  // per-partition list length
-    BatchItem *d_items = nullptr;
-    unsigned int *d_ctr = nullptr;
-    unsigned long long *d_scored = nullptr;  // per scoring workgroup: pairs scored
-    // query-major pass: task lists per (query, slice) unit, their counts and
-    // the prefix of their chunk counts (null when over the memory budget)
-    uint2 *d_tasks = nullptr;
-    uint64_t tasks_cap = 0;
-    uint2 *d_ucnt = nullptr;
-    uint32_t *d_uoff = nullptr;
-    uint32_t unit_cap = 0;
-    unsigned int *d_flist = nullptr;  // queries the first pass flagged for the second: [0] count, then ids
-    unsigned int *d_ccount = nullptr;
-    uint64_t *d_cand = nullptr;
     int cus = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
     // pinned-memory answers for recently seen output arrays (callers reuse them)
@@ -1470,24 +1500,25 @@ static void drop_graphs(BatchState &B) {
 void batch_free(topk_ctx *ctx) {
     BatchState *B = static_cast<BatchState *>(ctx->batch);
     if (!B) return;
-    hipStreamSynchronize(ctx->lane[0].stream);  // batches still in flight
+    for (HostSlot &h : B->slot)  // batches still in flight
+        if (h.stream) hipStreamSynchronize(h.stream);
     drop_graphs(*B);
-    hipFree(B->d_plist);
-    hipFree(B->d_tasks);
-    hipFree(B->d_ucnt);
-    hipFree(B->d_uoff);
-    hipFree(B->d_flist);
-    hipFree(B->d_pcnt);
-    hipFree(B->d_items);
-    hipFree(B->d_ctr);
-    hipFree(B->d_scored);
-    hipFree(B->d_ccount);
-    hipFree(B->d_cand);
     for (HostSlot &h : B->slot) {
         if (h.h_in) hipHostFree(h.h_in);
         if (h.h_out) hipHostFree(h.h_out);
         hipFree(h.d_in);
         hipFree(h.d_out);
+        hipFree(h.d_plist);
+        hipFree(h.d_tasks);
+        hipFree(h.d_ucnt);
+        hipFree(h.d_uoff);
+        hipFree(h.d_flist);
+        hipFree(h.d_pcnt);
+        hipFree(h.d_items);
+        hipFree(h.d_ctr);
+        hipFree(h.d_scored);
+        hipFree(h.d_ccount);
+        hipFree(h.d_cand);
         if (h.done) hipEventDestroy(h.done);
     }
     delete B;
@@ -1526,10 +1557,10 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
     const uint64_t items_need =
         (uint64_t)L * (4 * nw + 1) * ((P.maxsz + ITEM_MIN_OBJECTS - 1) / ITEM_MIN_OBJECTS + 1);
     const uint64_t te = bq_task_entries(nq, P.p2, group_height(P.p2));
-    const bool grow = B.gen != P.generation || nq > B.nq_cap || in_need > H.din_cap || out_need > H.dout_cap ||
-                      in_need > H.in_cap || out_need > H.out_cap || (uint64_t)L * nq > B.plist_cap ||
-                      (uint64_t)L > B.pcnt_cap || items_need > B.item_cap || !B.d_ctr || !H.done ||
-                      te > B.tasks_cap || (te && nq * BQ_MAX_S > B.unit_cap) || (!te && B.d_tasks);
+    const bool grow = B.gen != P.generation || nq > H.nq_cap || in_need > H.din_cap || out_need > H.dout_cap ||
+                      in_need > H.in_cap || out_need > H.out_cap || (uint64_t)L * nq > H.plist_cap ||
+                      (uint64_t)L > H.pcnt_cap || items_need > H.item_cap || !H.d_ctr || !H.done ||
+                      te > H.tasks_cap || (te && nq * BQ_MAX_S > H.unit_cap) || (!te && H.d_tasks);
     if (!grow) return TOPK_OK;
     int rc = batch_drain(ctx, B);
     if (rc) return rc;
@@ -1538,21 +1569,21 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
         changed = true;
         B.gen = P.generation;
     }
-    if (nq > B.nq_cap) {
+    if (nq > H.nq_cap) {
         uint32_t cap = 64;
         while (cap < nq) cap *= 2;
-        hipFree(B.d_ccount);
-        hipFree(B.d_cand);
-        hipFree(B.d_flist);
-        B.d_ccount = nullptr;
-        B.d_cand = nullptr;
-        B.d_flist = nullptr;
-        B.nq_cap = 0;
+        hipFree(H.d_ccount);
+        hipFree(H.d_cand);
+        hipFree(H.d_flist);
+        H.d_ccount = nullptr;
+        H.d_cand = nullptr;
+        H.d_flist = nullptr;
+        H.nq_cap = 0;
         changed = true;
-        TK_HIP(hipMalloc(&B.d_flist, sizeof(unsigned int) * (cap + 1)));
-        TK_HIP(hipMalloc(&B.d_ccount, sizeof(unsigned int) * cap));
-        TK_HIP(hipMalloc(&B.d_cand, sizeof(uint64_t) * BATCH_CAP * cap));
-        B.nq_cap = cap;
+        TK_HIP(hipMalloc(&H.d_flist, sizeof(unsigned int) * (cap + 1)));
+        TK_HIP(hipMalloc(&H.d_ccount, sizeof(unsigned int) * cap));
+        TK_HIP(hipMalloc(&H.d_cand, sizeof(uint64_t) * BATCH_CAP * cap));
+        H.nq_cap = cap;
     }
     if (in_need > H.din_cap) {
         hipFree(H.d_in);
@@ -1587,39 +1618,42 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
         H.out_cap = out_need;
     }
     if (!H.done) TK_HIP(hipEventCreateWithFlags(&H.done, hipEventDisableTiming));
-    rc = grow_dev(B.d_plist, B.plist_cap, (uint64_t)L * nq, changed);
-    if (!rc) rc = grow_dev(B.d_pcnt, B.pcnt_cap, (uint64_t)L, changed);
+    // slot i runs on pipeline lane i's stream: no streams beyond the
+    // context's, so the two slots keep hardware queues of their own
+    H.stream = ctx->lane[si].stream;
+    rc = grow_dev(H.d_plist, H.plist_cap, (uint64_t)L * nq, changed);
+    if (!rc) rc = grow_dev(H.d_pcnt, H.pcnt_cap, (uint64_t)L, changed);
     // budget >= 16 queries, chunks >= ITEM_MIN_OBJECTS objects
-    if (!rc) rc = grow_dev(B.d_items, B.item_cap, items_need, changed);
+    if (!rc) rc = grow_dev(H.d_items, H.item_cap, items_need, changed);
     if (rc) return rc;
     {
         const uint32_t units = nq * BQ_MAX_S;
-        if (!te && B.d_tasks) {  // over the budget for this batch size: the item path runs
-            hipFree(B.d_tasks);
-            B.d_tasks = nullptr;
-            B.tasks_cap = 0;
+        if (!te && H.d_tasks) {  // over the budget for this batch size: the item path runs
+            hipFree(H.d_tasks);
+            H.d_tasks = nullptr;
+            H.tasks_cap = 0;
             changed = true;
         }
         if (te) {
-            rc = grow_dev(B.d_tasks, B.tasks_cap, te, changed);
-            if (!rc && units > B.unit_cap) {
-                hipFree(B.d_ucnt);
-                hipFree(B.d_uoff);
-                B.d_ucnt = nullptr;
-                B.d_uoff = nullptr;
-                B.unit_cap = 0;
+            rc = grow_dev(H.d_tasks, H.tasks_cap, te, changed);
+            if (!rc && units > H.unit_cap) {
+                hipFree(H.d_ucnt);
+                hipFree(H.d_uoff);
+                H.d_ucnt = nullptr;
+                H.d_uoff = nullptr;
+                H.unit_cap = 0;
                 changed = true;
-                TK_HIP(hipMalloc(&B.d_ucnt, sizeof(uint2) * units));
-                TK_HIP(hipMalloc(&B.d_uoff, sizeof(uint32_t) * (units + 1)));
-                B.unit_cap = units;
+                TK_HIP(hipMalloc(&H.d_ucnt, sizeof(uint2) * units));
+                TK_HIP(hipMalloc(&H.d_uoff, sizeof(uint32_t) * (units + 1)));
+                H.unit_cap = units;
             }
             if (rc) return rc;
         }
     }
-    if (!B.d_ctr) {
+    if (!H.d_ctr) {
         changed = true;
-        TK_HIP(hipMalloc(&B.d_ctr, sizeof(unsigned int) * BCTR_WORDS));
-        TK_HIP(hipMalloc(&B.d_scored, sizeof(unsigned long long) * 2 * BATCH_MAX_GRID));
+        TK_HIP(hipMalloc(&H.d_ctr, sizeof(unsigned int) * BCTR_WORDS));
+        TK_HIP(hipMalloc(&H.d_scored, sizeof(unsigned long long) * 2 * BATCH_MAX_GRID));
     }
     if (!B.cus) {
         TK_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -1718,9 +1752,9 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     if (G > BMASK_PAIRS) return TOPK_ERR_UNSUPPORTED;  // layouts stop at PTA_MAX_PARTS partitions
     const int NB = dims_class(ctx->d);
     FinalArgs fa{dq,
-                 B.d_cand,
-                 B.d_ccount,
-                 B.d_ctr,
+                 B.slot[B.cur].d_cand,
+                 B.slot[B.cur].d_ccount,
+                 B.slot[B.cur].d_ctr,
                  ks,
                  reinterpret_cast<uint64_t *>(d_out + ol.ids_off),
                  reinterpret_cast<float *>(d_out + ol.sc_off),
@@ -1730,29 +1764,28 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                  reinterpret_cast<uint32_t *>(d_out + (second ? ol.ctr2_off : ol.ctr_off)),
                  qflag,
                  second ? 1 : 0,
-                 B.d_scored,
+                 B.slot[B.cur].d_scored,
                  0u,
                  seed2 ? 1 : 0,
-                 B.d_pcnt,
+                 B.slot[B.cur].d_pcnt,
                  L,
-                 second ? nullptr : B.d_flist,
-                 reinterpret_cast<const OutDesc *>(d_in + il.od_off)};
-    if (v.fused && B.d_tasks) {
+                 second ? nullptr : B.slot[B.cur].d_flist};
+    if (v.fused && B.slot[B.cur].d_tasks) {
         // a second pass re-runs few queries: each is spread over S slices
         const uint32_t S = bq_slices(nq, second);
         const uint64_t ucap = (uint64_t)((G + S - 1) / S) << H;
         // the second pass works through the list of flagged queries the
         // first final select wrote: a few workgroups per slice, not one per query
-        const uint32_t *fl = second ? B.d_flist : nullptr;
+        const uint32_t *fl = second ? B.slot[B.cur].d_flist : nullptr;
         const uint32_t gx = second ? std::min(nq, 16u) : nq;
         const CutArgs ca{dq,       db,    ta,     G,       H,       L,         ctx->d,  P.poff, P.psz,
-                         P.cstat,  (uint64_t)P.nchunks, B.d_tasks, ucap, B.d_ucnt, B.d_ctr, pass_flags, S, fl};
+                         P.cstat,  (uint64_t)P.nchunks, B.slot[B.cur].d_tasks, ucap, B.slot[B.cur].d_ucnt, B.slot[B.cur].d_ctr, pass_flags, S, fl};
         auto ck = NB == 8 ? batch_cut_kernel<8> : (NB == 16 ? batch_cut_kernel<16> : batch_cut_kernel<32>);
         hipLaunchKernelGGL(ck, dim3(gx, S), dim3(BQ_BS), 0, s, ca);
-        hipLaunchKernelGGL(batch_unit_scan_kernel, dim3(1), dim3(1024), 0, s, B.d_ucnt, nq * S, fl, S, B.d_uoff);
+        hipLaunchKernelGGL(batch_unit_scan_kernel, dim3(1), dim3(1024), 0, s, B.slot[B.cur].d_ucnt, nq * S, fl, S, B.slot[B.cur].d_uoff);
         const uint32_t g2 = (uint32_t)B.cus * (second ? 2 : 7);  // 7 resident 4-wave workgroups per CU
-        const ChunkArgs xa{dq, P.pcols, P.ld, P.poff, P.psz, P.pid, B.d_tasks, ucap, B.d_ucnt, B.d_uoff, nq * S, S,
-                           fl, B.d_cand, B.d_ccount, B.d_scored};
+        const ChunkArgs xa{dq, P.pcols, P.ld, P.poff, P.psz, P.pid, B.slot[B.cur].d_tasks, ucap, B.slot[B.cur].d_ucnt, B.slot[B.cur].d_uoff, nq * S, S,
+                           fl, B.slot[B.cur].d_cand, B.slot[B.cur].d_ccount, B.slot[B.cur].d_scored};
         hipEvent_t ev0 = nullptr;
         timing_begin(ctx, s, ev0);
         hipLaunchKernelGGL(batch_chunk_kernel, dim3(g2), dim3(BC_BS), 0, s, xa);
@@ -1776,20 +1809,20 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
         S = std::max(S, (G + BMASK_PAIRS - 1) / BMASK_PAIRS);
         auto mask = NB == 8 ? batch_mask_query_kernel<8>
                             : (NB == 16 ? batch_mask_query_kernel<16> : batch_mask_query_kernel<32>);
-        hipLaunchKernelGGL(mask, dim3(nq, S), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.d_pcnt, B.d_plist,
+        hipLaunchKernelGGL(mask, dim3(nq, S), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, B.slot[B.cur].d_pcnt, B.slot[B.cur].d_plist,
                            pass_flags);
     } else {
         const uint32_t QS = BMASK_PAIRS >> H, Y = (nq + QS - 1) / QS;
         auto mask = NB == 8 ? batch_mask_kernel<8> : (NB == 16 ? batch_mask_kernel<16> : batch_mask_kernel<32>);
-        hipLaunchKernelGGL(mask, dim3(G, Y), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, QS, B.d_pcnt,
-                           B.d_plist, pass_flags);
+        hipLaunchKernelGGL(mask, dim3(G, Y), dim3(BMASK_BS), 0, s, dq, db, ta, G, H, L, nq, QS, B.slot[B.cur].d_pcnt,
+                           B.slot[B.cur].d_plist, pass_flags);
     }
     const uint32_t ochunk = item_objects(v);
-    hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.d_pcnt, L,
-                       (uint32_t)v.qpi, ochunk, B.item_cap, P.psz, B.d_items, B.d_ctr);
-    ScoreArgs sa{P.pcols, P.ld, ctx->d,    P.poff,     P.psz,  P.pid,  B.d_items, B.item_cap,
-                 ochunk,  B.d_plist, nq, B.d_ctr, dq, B.d_cand, B.d_ccount, P.cstat,  P.crow, (uint64_t)P.nchunks,
-                 B.d_scored};
+    hipLaunchKernelGGL(batch_compact_kernel, dim3((L + BAUX_BS - 1) / BAUX_BS), dim3(BAUX_BS), 0, s, B.slot[B.cur].d_pcnt, L,
+                       (uint32_t)v.qpi, ochunk, B.slot[B.cur].item_cap, P.psz, B.slot[B.cur].d_items, B.slot[B.cur].d_ctr);
+    ScoreArgs sa{P.pcols, P.ld, ctx->d,    P.poff,     P.psz,  P.pid,  B.slot[B.cur].d_items, B.slot[B.cur].item_cap,
+                 ochunk,  B.slot[B.cur].d_plist, nq, B.slot[B.cur].d_ctr, dq, B.slot[B.cur].d_cand, B.slot[B.cur].d_ccount, P.cstat,  P.crow, (uint64_t)P.nchunks,
+                 B.slot[B.cur].d_scored};
     BatchVariant vs = v;
     if (second) vs.grid = v.grid / 8 > 32 ? v.grid / 8 : 32;  // the second pass has few items
     hipEvent_t ev0 = nullptr;
@@ -1810,20 +1843,29 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
 static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
                          const BatchVariant &v) {
     const InLayout il = InLayout::make(nq);
-    unsigned char *d_in = B.slot[B.cur].d_in;
+    const OutLayout ol = OutLayout::make(nq, ks);
+    HostSlot &H = B.slot[B.cur];
+    unsigned char *d_in = H.d_in;
     BatchQuery *dq = reinterpret_cast<BatchQuery *>(d_in + il.q_off);
     BoundArgs *db = reinterpret_cast<BoundArgs *>(d_in + il.b_off);
     const uint32_t L = ctx->pta.nparts;
     uint32_t pg = (L + 63) / 64;
     if (pg > 1024) pg = 1024;
     if (pg < nq) pg = nq;
+    // the staged queries, taus and output descriptor (pinned) -> device
+    TK_HIP(hipMemcpyAsync(H.d_in, H.h_in, il.host, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(batch_prep_kernel, dim3(pg), dim3(64), 0, s,
                        reinterpret_cast<const topk_query *>(d_in + il.raw_off),
                        mode == BMODE_GIVEN ? reinterpret_cast<const float *>(d_in + il.tau_off) : nullptr, nq, ctx->n,
-                       ctx->d, dq, db, B.d_pcnt, L, B.d_ccount, B.d_ctr, B.d_flist);
+                       ctx->d, dq, db, H.d_pcnt, L, H.d_ccount, H.d_ctr, H.d_flist);
     int rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, false);
     if (!rc) rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, true);
-    return rc;
+    if (rc) return rc;
+    const PublishArgs pa{H.d_out,  ol.ids_off, ol.sc_off, ol.thr_off, ol.st_off,
+                         ol.bytes, (uint64_t)nq * ks, nq, reinterpret_cast<const OutDesc *>(d_in + il.od_off), H.h_out};
+    hipLaunchKernelGGL(batch_publish_kernel, dim3(BPUB_GRID), dim3(256), 0, s, pa);
+    TK_HIP(hipGetLastError());
+    return TOPK_OK;
 }
 
 static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq, uint32_t ks, int mode,
@@ -2071,13 +2113,11 @@ static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     B.cur = si;
     rc = stage_queries(ctx, B, nq, qs, ks);
     if (rc) return rc;
-    hipStream_t s = ctx->lane[0].stream;
     HostSlot &H = B.slot[si];
-    const InLayout il = InLayout::make(nq);
-    const OutLayout ol = OutLayout::make(nq, ks);
+    hipStream_t s = H.stream;
     // Every query filling its stride (k = k_stride <= n) into pinned caller
-    // arrays: the final select writes them directly (the device reaches
-    // pinned host memory) and only the status words are copied back.
+    // arrays: the batch's publish kernel writes them directly; otherwise it
+    // fills the staging block and the wait copies out.
     bool full = ctx->n >= ks;
     for (uint32_t i = 0; full && i < nq; i++) full = qs[i].k == ks;
     H.direct = full && pinned_host(B, out_ids) && pinned_host(B, out_scores) && pinned_host(B, out_thr);
@@ -2088,17 +2128,10 @@ static int batch_submit(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
         od.thr = reinterpret_cast<uint64_t>(out_thr);
         od.direct = 1;
     }
-    memcpy(H.h_in + il.od_off, &od, sizeof(od));
-    // upload -> kernels -> status copy, in order on the context's stream:
-    // three calls per batch keep the host ahead of the device
-    TK_HIP(hipMemcpyAsync(H.d_in, H.h_in, il.host, hipMemcpyHostToDevice, s));
+    memcpy(H.h_in + InLayout::make(nq).od_off, &od, sizeof(od));
+    // upload, kernels and publish on the slot's stream (one graph launch)
     rc = batch_launch(ctx, B, s, nq, ks, mode, v);
     if (rc) return rc;
-    if (H.direct)
-        TK_HIP(hipMemcpyAsync(H.h_out + ol.st_off, H.d_out + ol.st_off, ol.bytes - ol.st_off, hipMemcpyDeviceToHost,
-                              s));
-    else
-        TK_HIP(hipMemcpyAsync(H.h_out, H.d_out, ol.bytes, hipMemcpyDeviceToHost, s));
     TK_HIP(hipEventRecord(H.done, s));
     H.busy = true;
     H.ticket = B.next_ticket++;
@@ -2168,12 +2201,11 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
     B.cur = 0;
     rc = stage_queries(ctx, B, nq, qs, k_stride);
     if (rc) return rc;
-    hipStream_t s = ctx->lane[0].stream;
+    hipStream_t s = B.slot[0].stream;
     {
-        const OutDesc od{};  // results to the device output block
+        const OutDesc od{};  // results to the staging block
         memcpy(B.slot[0].h_in + InLayout::make(nq).od_off, &od, sizeof(od));
     }
-    TK_HIP(hipMemcpyAsync(B.slot[0].d_in, B.slot[0].h_in, InLayout::make(nq).host, hipMemcpyHostToDevice, s));
     const int D = dims_class(ctx->d);
     const BatchVariant def = pick_variant(ctx, B);
     std::vector<BatchVariant> cand;

@@ -294,27 +294,42 @@ __global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict
                                                          uint32_t nq, Sketch S, float margin,
                                                          float *__restrict__ pred, float retry_width,
                                                          int saturated) {
-    // W1 / W2 as stored (row j = hidden unit j), rows padded by one word: the
-    // copy writes consecutive words and lane j's reads of row j hit a
-    // distinct bank per lane
-    __shared__ float W1s[MLP_H][MLP_IN + 1];
-    __shared__ float W2s[MLP_H][MLP_H + 1];
+    // Lane j owns hidden unit j of both layers: its rows of W1 and W2 (stored
+    // row-major, row j = unit j) come straight from memory as 16-byte loads,
+    // all in flight at once -- no LDS staging, no barrier -- and the second
+    // layer reads the first through lane shuffles.
     const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid / 64;
-    for (uint32_t o = tid; o < MLP_H * MLP_IN; o += TAU_BS) W1s[o / MLP_IN][o % MLP_IN] = P[OFF_W1 + o];
-    for (uint32_t o = tid; o < MLP_H * MLP_H; o += TAU_BS) W2s[o / MLP_H][o % MLP_H] = P[OFF_W2 + o];
-    const float b1 = P[OFF_B1 + lane], b2 = P[OFF_B2 + lane], w3 = P[OFF_W3 + lane], b3 = P[OFF_B3];
-    __syncthreads();
     const uint32_t nw = gridDim.x * (TAU_BS / 64);
-    for (uint32_t q = blockIdx.x * (TAU_BS / 64) + wv; q < nq; q += nw) {
+    const uint32_t q0 = blockIdx.x * (TAU_BS / 64) + wv;
+    if (q0 >= nq) return;
+    float w1[MLP_IN], w2[MLP_H];
+#pragma unroll
+    for (int i = 0; i < MLP_IN / 4; i++) {
+        const float4 v = reinterpret_cast<const float4 *>(P + OFF_W1 + lane * MLP_IN)[i];
+        w1[4 * i] = v.x;
+        w1[4 * i + 1] = v.y;
+        w1[4 * i + 2] = v.z;
+        w1[4 * i + 3] = v.w;
+    }
+#pragma unroll
+    for (int i = 0; i < MLP_H / 4; i++) {
+        const float4 v = reinterpret_cast<const float4 *>(P + OFF_W2 + lane * MLP_H)[i];
+        w2[4 * i] = v.x;
+        w2[4 * i + 1] = v.y;
+        w2[4 * i + 2] = v.z;
+        w2[4 * i + 3] = v.w;
+    }
+    const float b1 = P[OFF_B1 + lane], b2 = P[OFF_B2 + lane], w3 = P[OFF_W3 + lane], b3 = P[OFF_B3];
+    for (uint32_t q = q0; q < nq; q += nw) {
         float f[MLP_IN], sc;
         mlp_features(qs[q], S, f, &sc);
         float s = b1;
 #pragma unroll
-        for (int i = 0; i < MLP_IN; i++) s += f[i] * W1s[lane][i];
+        for (int i = 0; i < MLP_IN; i++) s += f[i] * w1[i];
         const float h1 = s > 0.0f ? s : 0.0f;
         float t = b2;
-#pragma unroll 8
-        for (int i = 0; i < MLP_H; i++) t += __shfl(h1, i, 64) * W2s[lane][i];
+#pragma unroll
+        for (int i = 0; i < MLP_H; i++) t += __shfl(h1, i, 64) * w2[i];
         t = t > 0.0f ? t : 0.0f;
         float o = w3 * t;
         for (int x = 32; x > 0; x >>= 1) o += __shfl_xor(o, x, 64);

@@ -136,6 +136,9 @@ class PreparedBatch:
         self.ids, self._tids = _host_array(self.nq * self.kmax, np.uint64)
         self.scores, self._tsc = _host_array(self.nq * self.kmax, np.float32)
         self.thr, self._tthr = _host_array(self.nq, np.float32)
+        # the submit arguments, converted once (a serving loop resubmits them)
+        self._args = (self.nq, _c.cast(self.qs, _c.c_void_p), self.kmax, _ptr(self.ids), _ptr(self.scores),
+                      _ptr(self.thr))
 
     def result(self, i, n):
         c = min(int(self.queries[i][0]), n)
@@ -287,8 +290,7 @@ class Engine:
         """Launches a PreparedBatch without waiting; returns its ticket.  The
         results are in pb.ids / pb.scores / pb.thr after batch_wait(ticket)."""
         t = _c.c_uint64()
-        check(lib().topk_batch_submit(self._h, pb.nq, _c.cast(pb.qs, _c.c_void_p), pb.kmax, _ptr(pb.ids),
-                                      _ptr(pb.scores), _ptr(pb.thr), _c.byref(t)), "topk_batch_submit")
+        check(lib().topk_batch_submit(self._h, *pb._args, _c.byref(t)), "topk_batch_submit")
         return t.value
 
     def batch_wait(self, ticket):
