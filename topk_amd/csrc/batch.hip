@@ -1348,8 +1348,9 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
 // ---------------------------------------------------------------------------
 // Slices per query of the query-major pass: 1 in a first pass (every
 // query its own workgroup), up to BQ_MAX_S in a second (few queries).
-static uint32_t bq_slices(uint32_t nq, bool second) {
-    return second ? std::max(1u, std::min(BQ_MAX_S, BATCH_MAX_GRID / nq)) : 1u;
+static uint32_t bq_slices(uint32_t nq, bool second, int first = 0) {
+    if (second) return std::max(1u, std::min(BQ_MAX_S, BATCH_MAX_GRID / nq));
+    return std::max(1u, std::min((uint32_t)(first ? first : 1), std::min(BQ_MAX_S, BATCH_MAX_GRID / nq)));
 }
 
 // Task-list entries both passes need (a unit lists at most its groups'
@@ -1358,8 +1359,8 @@ static uint64_t bq_task_entries(uint32_t nq, uint32_t p2, uint32_t H) {
     const uint32_t G = p2 >> H;
     uint64_t need = 0;
     for (int second = 0; second < 2; second++) {
-        const uint32_t S = bq_slices(nq, second != 0);
-        need = std::max(need, (uint64_t)nq * S * ((uint64_t)((G + S - 1) / S) << H));
+        for (uint32_t S : {bq_slices(nq, second != 0), bq_slices(nq, second != 0, (int)BQ_MAX_S)})
+            need = std::max(need, (uint64_t)nq * S * ((uint64_t)((G + S - 1) / S) << H));
     }
     constexpr uint64_t BUDGET = 4ull << 30;  // bytes
     return need * sizeof(uint2) <= BUDGET ? need : 0;
@@ -1371,7 +1372,7 @@ struct BatchVariant {
     int fused;  // 1: the query-major fused pass replaces mask, compaction and item scoring
 };
 
-using GraphKey = std::array<uint64_t, 16>;
+using GraphKey = std::array<uint64_t, 17>;
 
 // One batch in flight: its stream, pinned staging, device inputs and
 // outputs, the scratch its kernels use, and the event that marks its
@@ -1772,7 +1773,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                  second ? nullptr : B.slot[B.cur].d_flist};
     if (v.fused && B.slot[B.cur].d_tasks) {
         // a second pass re-runs few queries: each is spread over S slices
-        const uint32_t S = bq_slices(nq, second);
+        const uint32_t S = bq_slices(nq, second, ctx->tune.batch_slices);
         const uint64_t ucap = (uint64_t)((G + S - 1) / S) << H;
         // the second pass works through the list of flagged queries the
         // first final select wrote: a few workgroups per slice, not one per query
@@ -1874,7 +1875,8 @@ static int batch_launch(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     const GraphKey key{nq, ks, (uint64_t)mode, (uint64_t)v.vpt, (uint64_t)v.bs, (uint64_t)v.qpi, (uint64_t)v.digit,
                        (uint64_t)v.grid, ctx->d, mode == BMODE_LEARNED ? learned_version(ctx) : 0,
                        (uint64_t)ctx->tune.batch_mask, (uint64_t)B.cur, (uint64_t)v.item,
-                       (uint64_t)ctx->tune.batch_retry, (uint64_t)ctx->tune.batch_retry_width, (uint64_t)v.fused};
+                       (uint64_t)ctx->tune.batch_retry, (uint64_t)ctx->tune.batch_retry_width, (uint64_t)v.fused,
+                       (uint64_t)ctx->tune.batch_slices};
     auto it = B.graphs.find(key);
     if (it == B.graphs.end()) {
         TK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

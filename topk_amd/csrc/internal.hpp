@@ -93,6 +93,7 @@ struct Tuning {
     int batch_mode = 0;         // 0 auto (learned when trained), 1 seed threshold, 2 learned threshold
     int batch_mask = 0;         // mask layout: 0 auto, 1 query-major, 2 group-major
     int batch_fused = 0;        // 0 default (work items), 1 work items, 2 query-major fused pass
+    int batch_slices = 0;       // query-major first pass: workgroups per query (0: 1)
 };
 
 // One query of a batch as the device kernels read it (prepared on the host;
