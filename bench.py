@@ -225,6 +225,8 @@ def learned_batched(eng, n, k, args):
                              and gt == rt))
     eng.set_tuning("batch_mode", 0)
     total_pairs = n * nb
+    # which scoring kernel the installed launch shape runs (the autotuner's choice)
+    query_major = bool(tuning and tuning.get("fused"))
     return {
         "qps": qps, "ms_per_batch": el * 1e3 / steps, "steps": steps, "queries_per_batch": nb,
         "pipelining": "two batches in flight (topk_batch_submit / topk_batch_wait)",
@@ -247,8 +249,10 @@ def learned_batched(eng, n, k, args):
                      "calib_fallback": rep["calib_fallback"], "label_s": rep["ms_label"] / 1e3,
                      "train_s": rep["ms_train"] / 1e3},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / FP32_PEAK_TFLOPS, "traffic": load_profile_traffic("batch_score_pmc.md"),
-                     "kernel": "batch_score_kernel", "avg_launch_ms": avg_ms, "flops_per_launch": flops,
+                     "frac": tflops / FP32_PEAK_TFLOPS,
+                     "traffic": load_profile_traffic("batch_chunk_pmc.md" if query_major else "batch_score_pmc.md"),
+                     "kernel": "batch_chunk_kernel" if query_major else "batch_score_kernel",
+                     "avg_launch_ms": avg_ms, "flops_per_launch": flops,
                      "issue_peak": FP32_ISSUE_TFLOPS, "issue_frac": tflops / FP32_ISSUE_TFLOPS},
         "held": held,
     }
