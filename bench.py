@@ -28,7 +28,8 @@ the command the FTE rocprofv3 summaries under profiles/ were taken with.
 ``--mode stream`` a store in host memory streamed through HBM, ``--mode
 kernels`` BTA, GPA, radix select, normalisation and reorder_attr, and
 ``--mode configs`` the other workloads (PTA at N = 2^28, batched and learned
-d = 16, ingest, host-memory store).
+d = 16, ingest, host-memory store); ``--mode train`` the predictor's training
+alone on the headline store (the command of the training kernel summary).
 """
 import argparse
 import json
@@ -119,6 +120,15 @@ def plain_fte(eng, n, m, k, steps, warmup, lanes):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_profile_traffic("fte_stream_pmc.md"),
                      "kernel": "fte_stream_kernel", "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes},
     }
+
+
+def train_step_stats(rep):
+    """Mini-batch steps of the predictor's training: count, device time per
+    step (HIP events around all launches, gaps included) and the rate over
+    6 x 10433 flops per sample."""
+    ms = rep["ms_steps"]
+    return {"steps": rep["steps"], "step_us": ms * 1e3 / max(1, rep["steps"]),
+            "step_tflops": rep["flops"] / (ms * 1e-3) / 1e12 if ms > 0 else 0.0}
 
 
 def random_queries(rng, count, k):
@@ -247,7 +257,7 @@ def learned_batched(eng, n, k, args):
         "training": {"queries": len(train), "n_train": rep["n_train"], "n_calib": rep["n_calib"],
                      "train_mse": rep["train_mse"], "calib_mse": rep["calib_mse"], "margin": rep["margin"],
                      "calib_fallback": rep["calib_fallback"], "label_s": rep["ms_label"] / 1e3,
-                     "train_s": rep["ms_train"] / 1e3},
+                     "train_s": rep["ms_train"] / 1e3, **train_step_stats(rep)},
         "roofline": {"bound": "valu", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / FP32_PEAK_TFLOPS,
                      "traffic": load_profile_traffic("batch_chunk_pmc.md" if query_major else "batch_score_pmc.md"),
@@ -586,7 +596,8 @@ def learned16_bench(steps=20, train_queries=100_000):
         "kernel_view": {"learned": lview, "seed": sview},
         "roofline": valu_roofline(lview, "batch scoring kernel (batch_score timing), learned thresholds"),
         "training": {"queries": len(train), "margin": rep["margin"], "calib_fallback": rep["calib_fallback"],
-                     "saturated": rep.get("saturated"), "train_mse": rep["train_mse"], "wall_s": train_s},
+                     "saturated": rep.get("saturated"), "train_mse": rep["train_mse"], "wall_s": train_s,
+                     **train_step_stats(rep)},
         "autotune": tune}
 
 
@@ -811,7 +822,7 @@ def main():
                     help="timed steps: batches (default 200; 20 with --mode batched / learned16), queries with "
                          "--mode fte (200), file loads with --mode ingest (3)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels", "c1",
+    ap.add_argument("--mode", choices=["learned", "fte", "configs", "ingest", "stream", "pta", "kernels", "c1", "train",
                                        "batched", "learned16"],
                     default="learned",
                     help="learned: the headline line; fte: plain FTE alone; configs: the other workloads; "
@@ -879,6 +890,13 @@ def main():
 
     n, k = args.n, args.k
     eng = topk_amd.Engine.generated(n, D, topk_amd.INDEPENDENT, 0)
+    if args.mode == "train":
+        eng.init(topk_amd.TOPK_INIT_PTA)
+        rep = eng.learned_train(random_queries(np.random.default_rng(2026), args.train_queries, k))
+        print(json.dumps({"train_n%d_d8_k%d" % (n, k): dict(rep, queries=args.train_queries,
+                                                             **train_step_stats(rep))}), flush=True)
+        eng.close()
+        return
     base = {"metric": METRIC, "unit": "queries/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (independent U[0,1), seed 0, generated in HBM)"}
 

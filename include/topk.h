@@ -270,7 +270,7 @@ int topk_batch_autotune(topk_ctx *ctx, uint32_t nq, const topk_query *qs, uint32
  * is answered by topk_fte).  Stores with d <= 16 only. */
 typedef struct topk_learned_config {
     uint32_t epochs;          /* passes over the training queries (default 60) */
-    uint32_t batch;           /* mini-batch, a multiple of 64 (default 512) */
+    uint32_t batch;           /* mini-batch, a multiple of 64, at most 4096 (default 512) */
     float lr;                 /* Adam step size (default 2e-3, decayed x0.3 / x0.1 late) */
     float target_fallback;    /* fraction of calibration queries allowed to fall back (0.0005) */
     float calib_fraction;     /* held-out fraction used to calibrate the margin (0.2) */
@@ -285,6 +285,9 @@ typedef struct topk_learned_report {
     double ms_label, ms_train;   /* wall time of labelling (exact batch path) and of training */
     uint32_t saturated;          /* 1: labels reach the largest score the data allows (ties there);
                                     queries predicted near it take exact seed thresholds */
+    uint32_t steps;              /* mini-batch steps run */
+    double flops;                /* their arithmetic: 6 x 10433 flops per sample (forward + backward) */
+    double ms_steps;             /* device time of the steps (HIP events around their launches) */
 } topk_learned_report;
 
 int topk_learned_default_config(topk_learned_config *cfg);
@@ -393,6 +396,9 @@ void *topk_stream(topk_ctx *ctx);
  *                     partitions and chunks and scores the chunks that
  *                     reach the threshold, with no work list (0 = 1, or the
  *                     autotuner's choice)
+ *   "mlp_train"       predictor training: 0 = two launches per mini-batch
+ *                     step (default), 1 = one cooperative launch for all
+ *                     steps (same parameters)
  *   "score_divide"    1 = real-data scores: the running score is divided by
  *                     qq after every term, each step rounded to binary32
  *                     (the reference's LD == 2 build, cpu/TPAc.h:143-146,

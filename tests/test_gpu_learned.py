@@ -198,6 +198,35 @@ def test_forced_fallback_stays_exact(gpu, trained_unsaturated):
         assert np.array_equal(gi, ri) and np.array_equal(gs, rs)
 
 
+def test_learned_training_deterministic(gpu):
+    """Training (fixed-order gradient sums) gives bit-identical parameters
+    for the same seed, with two launches per step and with one cooperative
+    launch for all steps; the mini-batch cap is 4096 rows."""
+    n, d = 1 << 18, 8
+    x = topk_amd.generate(n, d, topk_amd.INDEPENDENT, 5)
+    rng = np.random.default_rng(55)
+    train = make_queries(rng, 1500, d, [16, 64])
+    with topk_amd.Engine(x) as eng:
+        r1 = eng.learned_train(train, epochs=6)
+        p1, m1 = eng.learned_get()
+        r2 = eng.learned_train(train, epochs=6)
+        p2, m2 = eng.learned_get()
+        assert np.array_equal(p1, p2) and m1 == m2
+        assert r1["train_mse"] == r2["train_mse"] and r1["calib_mse"] == r2["calib_mse"]
+        eng.set_tuning("mlp_train", 1)
+        eng.learned_train(train, epochs=6)
+        p4, m4 = eng.learned_get()
+        eng.set_tuning("mlp_train", 0)
+        assert np.array_equal(p1, p4) and m1 == m4
+        r3 = eng.learned_train(train, epochs=3, batch=1024, seed=9)  # last batch of 176 rows dropped
+        p3, _ = eng.learned_get()
+        assert np.all(np.isfinite(p3)) and not np.array_equal(p3, p1)
+        assert np.isfinite(r3["train_mse"])
+        with pytest.raises(topk_amd.TopkError) as e:
+            eng.learned_train(train, epochs=1, batch=4160)
+        assert e.value.status == 1
+
+
 def test_learned_state_errors(gpu):
     x = topk_amd.generate(10_000, 20, topk_amd.INDEPENDENT, 1)
     with topk_amd.Engine(x) as eng:

@@ -63,7 +63,8 @@ class TopkLearnedConfig(ctypes.Structure):
 class TopkLearnedReport(ctypes.Structure):
     _fields_ = [("n_train", _c.c_uint32), ("n_calib", _c.c_uint32), ("train_mse", _c.c_float),
                 ("calib_mse", _c.c_float), ("margin", _c.c_float), ("calib_fallback", _c.c_float),
-                ("ms_label", _c.c_double), ("ms_train", _c.c_double), ("saturated", _c.c_uint32)]
+                ("ms_label", _c.c_double), ("ms_train", _c.c_double), ("saturated", _c.c_uint32),
+                ("steps", _c.c_uint32), ("flops", _c.c_double), ("ms_steps", _c.c_double)]
 
 
 class TopkIngestReport(ctypes.Structure):

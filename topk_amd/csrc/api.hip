@@ -524,6 +524,9 @@ int topk_set_tuning(topk_ctx *ctx, const char *name, int64_t value) {
         if (value < 0 || value > 64) return TOPK_ERR_INVALID;
         batch_quiesce(ctx);
         ctx->tune.batch_slices = (int)value;
+    } else if (key == "mlp_train") {
+        if (value < 0 || value > 1) return TOPK_ERR_INVALID;
+        ctx->tune.mlp_train = (int)value;
     } else if (key == "batch_fused") {
         if (value < 0 || value > 2) return TOPK_ERR_INVALID;
         ctx->tune.batch_fused = (int)value;

@@ -94,6 +94,7 @@ struct Tuning {
     int batch_mask = 0;         // mask layout: 0 auto, 1 query-major, 2 group-major
     int batch_fused = 0;        // 0 default (work items), 1 work items, 2 query-major fused pass
     int batch_slices = 0;       // query-major first pass: workgroups per query (0: 1)
+    int mlp_train = 0;          // predictor training: 0 two launches per step, 1 one cooperative launch
 };
 
 // One query of a batch as the device kernels read it (prepared on the host;
