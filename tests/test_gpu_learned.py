@@ -125,6 +125,33 @@ def test_learned_batch_is_exact(gpu, trained):
     eng.set_tuning("batch_mode", 0)
 
 
+def test_learned_saturated_tie_thresholds(gpu, trained):
+    """On a store whose scores tie at the top (the correlated third clips at
+    1.0), queries the predictor marks as saturated take the exact key from
+    the tie scan (the k-th object at the largest score in id order), and
+    partitions and chunks whose smallest id is past that key's id are
+    skipped.  Same answers as seed thresholds and no more scored pairs."""
+    eng, x, n, rep, test = trained
+    assert rep["saturated"] == 1
+    res, stats, tau_ms = {}, {}, {}
+    for mode, name in ((1, "seed"), (2, "learned")):
+        eng.set_tuning("batch_mode", mode)
+        eng.timing(True)
+        eng.timing_reset()
+        res[mode] = eng.batch(test)
+        stats[mode] = eng.stats()
+        tau_ms[mode] = eng.timing_get(f"batch_tau_{name}")[0]
+        eng.timing(False)
+    eng.set_tuning("batch_mode", 0)
+    for i, ((a, b, c), (d, e, f)) in enumerate(zip(res[1], res[2])):
+        assert np.array_equal(a, d) and np.array_equal(b.view(np.uint32), e.view(np.uint32)) and c == f, i
+    s_seed, s_lrn = stats[1], stats[2]
+    print("scored pairs seed/learned", s_seed["tuple_count"], s_lrn["tuple_count"],
+          "partitions", s_seed["partitions_scored"], s_lrn["partitions_scored"],
+          "threshold stage ms seed/learned", tau_ms[1], tau_ms[2])
+    assert s_lrn["tuple_count"] <= s_seed["tuple_count"]
+
+
 @pytest.fixture(scope="module")
 def trained_unsaturated():
     """N=2^20, d=16 independent store (no ties at the top) with a predictor

@@ -97,10 +97,14 @@ struct BatchItem {
 // j): terms in query order, the ascending form, and the bound arguments --
 // the distinct attributes with aggregated FP64 weights, heaviest first (ties
 // by attribute id).  taus (optional) gives the thresholds of a second pass.
-// Every thread also clears its share of the per-batch counters.
+// rmax / rmin: the store's per-attribute maxima / minima (attribute a at
+// [a * rstride]).  Every thread also clears its share of the per-batch
+// counters.
 __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__restrict__ raw,
                                                         const float *__restrict__ taus, uint32_t nq, uint64_t n,
-                                                        uint32_t d, BatchQuery *__restrict__ out,
+                                                        uint32_t d, const float *__restrict__ rmax,
+                                                        const float *__restrict__ rmin, uint64_t rstride,
+                                                        BatchQuery *__restrict__ out,
                                                         BoundArgs *__restrict__ outb, unsigned int *__restrict__ pcnt,
                                                         uint32_t L, unsigned int *__restrict__ ccount,
                                                         unsigned int *__restrict__ ctr, unsigned int *__restrict__ flist) {
@@ -168,6 +172,16 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
     }
     const uint32_t nd = (uint32_t)__popcll(__ballot(used && lane < d));
     if (lane == 0) {
+        // largest score: each term at the end of its attribute's range that
+        // its weight's sign prefers, summed in term order as the scores are
+        float sm = 0.0f;
+        for (uint32_t t = 0; t < qq; t++) {
+            const uint32_t at = Q.attr[t];
+            const float w = Q.weights[at];
+            const float pr = (w >= 0.0f ? rmax[at * rstride] : rmin[at * rstride]) * w;
+            sm = sm + pr;
+        }
+        b.smax = sm;
         b.tau = taus ? taus[q] : 0.0f;
         b.tau_retry = -INFINITY;
         b.tau_id = 0xFFFFFFFFu;
@@ -184,8 +198,16 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
 // Bound tree of the partition layout as the batch kernels read it.
 struct TreeArgs {
     const float *tmax, *tmin, *tsum, *tminsum;  // tmax / tmin attribute-major: [a * 2*p2 + v]
+    const uint32_t *tminid;                     // smallest object id under node v
     uint32_t p2, d;
 };
+
+// Query Q's threshold sits at the largest score the store allows, so an
+// object is a candidate only if its id is at most tau_id: a node, partition
+// or chunk whose smallest id exceeds tau_id holds no candidate.
+__device__ __forceinline__ bool at_smax(const BatchQuery &Q) {
+    return Q.tau == Q.smax && Q.tau_id != 0xFFFFFFFFu;
+}
 
 // Upper bound of query b's score over tree node v (-inf for an empty node);
 // NB >= b.nd.  The node's loads are issued together, not behind the
@@ -320,6 +342,93 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
     }
 }
 
+// Tie scan, learned first pass: one workgroup per query the predictor
+// marked as near its largest score (seed = 1).  When at least k objects
+// reach smax, the query's top k are the k smallest ids among them, so its
+// k-th key is (smax, id of the k-th such object in id order).  Each step
+// scores BTIE_STEP objects in id order straight from the id-order columns
+// (terms in query order, as every scoring kernel): wave w takes ids
+// [i0 + w * 64 U, i0 + (w + 1) * 64 U), lane l the ids u * 64 + l of that
+// range.  Ballots count the objects at smax per 64-id block; once the
+// running count reaches k, the block holding the k-th gives its id: tau =
+// (smax, that id), exact, and the mark is cleared.  The scan gives up past
+// `budget` ids, or once the rate of ties so far projects more than `budget`
+// ids to reach k; such queries keep the mark and take a seed threshold.
+constexpr uint32_t BTIE_U = 8;                     // 64-id blocks per wave per step
+constexpr uint32_t BTIE_STEP = 4 * 64 * BTIE_U;    // ids per workgroup step
+constexpr uint32_t BTIE_BUDGET = 1u << 16;         // ids a tie scan may read
+__global__ __launch_bounds__(256) void batch_tie_kernel(BatchQuery *__restrict__ qs,
+                                                        const float *__restrict__ cols, uint64_t ld, uint64_t n,
+                                                        uint32_t budget) {
+    __shared__ uint32_t cnt[4][BTIE_U];
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64, q = blockIdx.x;
+    if (!qs[q].seed) return;  // uniform over the workgroup
+    const BatchQuery &Q = qs[q];
+    const float smax = Q.smax;
+    const uint32_t k = Q.k, m = Q.m;
+    const uint32_t la = lane < m ? Q.attr[lane] : 0u;
+    const float lw = lane < m ? Q.w[lane] : 0.0f;
+    const uint64_t lim = n < budget ? n : budget;
+    uint32_t found = 0;
+    for (uint64_t i0 = 0; i0 < lim; i0 += BTIE_STEP) {
+        const uint64_t wb = i0 + (uint64_t)w * 64 * BTIE_U;
+        float sc[BTIE_U];
+#pragma unroll
+        for (uint32_t u = 0; u < BTIE_U; u++) sc[u] = 0.0f;
+        for (uint32_t t = 0; t < m; t++) {
+            const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)la, (int)t);
+            const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lw), (int)t));
+            const float *col = cols + (uint64_t)at * ld;
+#pragma unroll
+            for (uint32_t u = 0; u < BTIE_U; u++) {
+                const uint64_t id = wb + u * 64 + lane;
+                const float pr = col[id < n ? id : 0] * wt;
+                sc[u] = sc[u] + pr;
+            }
+        }
+        uint64_t mask[BTIE_U];
+#pragma unroll
+        for (uint32_t u = 0; u < BTIE_U; u++) {
+            mask[u] = __ballot(wb + u * 64 + lane < lim && sc[u] == smax);
+            if (lane == 0) cnt[w][u] = (uint32_t)__popcll(mask[u]);
+        }
+        __syncthreads();
+        // blocks in id order: wave-major, then u
+        uint32_t before = found, hit_w = 4, hit_u = 0;
+        for (uint32_t ww = 0; ww < 4 && hit_w == 4; ww++)
+            for (uint32_t u = 0; u < BTIE_U; u++) {
+                const uint32_t c = cnt[ww][u];
+                if (before + c >= k) {
+                    hit_w = ww;
+                    hit_u = u;
+                    break;
+                }
+                before += c;
+            }
+        if (hit_w < 4) {  // uniform: every thread read the same counts
+            if (w == hit_w) {
+                uint64_t mk = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < BTIE_U; u++) mk = u == hit_u ? mask[u] : mk;
+                const uint32_t r = k - before - 1;  // rank of the k-th tie within its block
+                const uint32_t below = (uint32_t)__popcll(mk & ((1ull << lane) - 1ull));
+                const uint64_t sel = __ballot(((mk >> lane) & 1ull) && below == r);
+                if (lane == 0) {
+                    BatchQuery &W = qs[q];
+                    W.tau = smax;
+                    W.tau_id = (uint32_t)(wb + hit_u * 64) + (uint32_t)(__ffsll((long long)sel) - 1);
+                    W.seed = 0u;
+                }
+            }
+            return;
+        }
+        found = before;
+        __syncthreads();  // cnt is rewritten by the next step
+        const uint64_t done = i0 + BTIE_STEP;
+        if (done >= 8192 && (uint64_t)found * budget < (uint64_t)k * done) return;  // projected past the budget
+    }
+}
+
 // Query-major mask: workgroup (q, y) of S per query walks the bound tree
 // two levels deep.  Groups y, y + S, ... of 2^H partitions (tree nodes
 // G + g) whose bound reaches tau_q go to an LDS list; every partition of a
@@ -343,9 +452,11 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_query_kernel(const BatchQ
     __syncthreads();
     const BoundArgs &b = qb[q];
     const float tau = qs[q].tau;
+    const bool sat = at_smax(qs[q]);
+    const uint32_t tau_id = qs[q].tau_id;
     for (uint32_t g = y + S * tid; g < G; g += S * BMASK_BS) {
         const float ub = node_bound<NB>(b, t, G + g);
-        if (ub > -INFINITY && ub >= tau) list[atomicAdd(&nlist, 1u)] = g;
+        if (ub > -INFINITY && ub >= tau && !(sat && t.tminid[G + g] > tau_id)) list[atomicAdd(&nlist, 1u)] = g;
     }
     __syncthreads();
     const uint32_t nl = nlist, per = 1u << H, total = nl * per;
@@ -357,7 +468,7 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_query_kernel(const BatchQ
             const uint32_t i = i0 + u * BMASK_BS;
             p[u] = i < total ? (list[i >> H] << H) + (i & (per - 1)) : L;
             const float ub = p[u] < L ? node_bound<NB>(b, t, t.p2 + p[u]) : -INFINITY;
-            in[u] = ub > -INFINITY && ub >= tau;
+            in[u] = ub > -INFINITY && ub >= tau && !(sat && t.tminid[t.p2 + p[u]] > tau_id);
         }
         uint32_t pos[U];
 #pragma unroll
@@ -396,7 +507,8 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *
     for (uint32_t q = qa + tid; q < qe; q += BMASK_BS) {
         if (qflag && qflag[q] == QF_DONE) continue;  // second pass: settled queries take no part
         const float ub = node_bound<NB>(qb[q], t, G + g);
-        if (ub > -INFINITY && ub >= qs[q].tau) lq[atomicAdd(&nlist, 1u)] = (uint16_t)q;
+        if (ub > -INFINITY && ub >= qs[q].tau && !(at_smax(qs[q]) && t.tminid[G + g] > qs[q].tau_id))
+            lq[atomicAdd(&nlist, 1u)] = (uint16_t)q;
     }
     __syncthreads();
     const uint32_t nl = nlist;
@@ -411,7 +523,7 @@ __global__ __launch_bounds__(BMASK_BS) void batch_mask_kernel(const BatchQuery *
             if (i < total && p < L) {
                 const uint32_t q = lq[i >> H];
                 const float ub = node_bound<NB>(qb[q], t, t.p2 + p);
-                in[u] = ub > -INFINITY && ub >= qs[q].tau;
+                in[u] = ub > -INFINITY && ub >= qs[q].tau && !(at_smax(qs[q]) && t.tminid[t.p2 + p] > qs[q].tau_id);
             }
         }
 #pragma unroll
@@ -526,6 +638,7 @@ struct ScoreArgs {
     uint32_t crow;               // elements per chunk
     uint64_t nchunks;
     unsigned long long *scored;  // per workgroup: (object, query) pairs scored
+    const uint32_t *cminid;      // smallest object id per chunk
 };
 
 template <int VPT>
@@ -601,6 +714,7 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
             // lane j: query q0 + j of the item (lanes past nqb hold query 0)
             const uint32_t lq = lane < nqb ? (uint32_t)ql[q0 + lane] : 0u;
             const f4 lt = *reinterpret_cast<const f4 *>(&a.q[lq].tau);  // tau, tau_id, ascending, m
+            const bool lsat = at_smax(a.q[lq]);  // lane j's query sits at its largest score
             float lw[LANE_W ? D : 1];
             f4 lws[LANE_W ? D / 4 : 1];
             uint4 las = {0u, 0u, 0u, 0u};
@@ -661,6 +775,19 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                     const bool asc_l = __float_as_int(lt[2]) != 0;
                     pass = pass && (!asc_l || ub >= lt[0]);
                     alive = pass;  // a failing suffix bound covers every later object of the partition
+                }
+                // queries at their largest score skip tiles whose ids all
+                // exceed tau_id (not a suffix property: alive is unchanged)
+                if (__ballot(pass && lsat)) {
+                    constexpr uint32_t TILE = 64 * VPT;
+                    const uint32_t c0 = (start + wb) / PTA_CHUNK;
+                    const uint32_t cn = (TILE + PTA_CHUNK - 1) / PTA_CHUNK;
+                    uint32_t wmin = 0xFFFFFFFFu;
+                    for (uint32_t i = 0; i < cn && c0 + i < a.nchunks; i++) {  // may reach the next partition: still a bound
+                        const uint32_t mi = a.cminid[c0 + i];
+                        wmin = mi < wmin ? mi : wmin;
+                    }
+                    pass = pass && !(lsat && wmin > __float_as_uint(lt[1]));
                 }
                 const uint64_t qmask = __ballot(pass);
                 if (!qmask) continue;
@@ -1086,6 +1213,8 @@ __global__ __launch_bounds__(BQ_BS, bq_waves_per_simd(NB)) void batch_cut_kernel
     }
     const BoundArgs &b = a.qb[q];
     const float tau = a.q[q].tau;
+    const bool sat = at_smax(a.q[q]);
+    const uint32_t tau_id = a.q[q].tau_id;
     if (tid == 0) {
         ng = 0;
         np = 0;
@@ -1094,7 +1223,8 @@ __global__ __launch_bounds__(BQ_BS, bq_waves_per_simd(NB)) void batch_cut_kernel
     // groups of this slice
     for (uint32_t g = sl + a.S * tid; g < a.G; g += a.S * BQ_BS) {
         const float ub = node_bound<NB>(b, a.t, a.G + g);
-        if (ub > -INFINITY && ub >= tau) glist[atomicAdd(&ng, 1u)] = (uint16_t)g;
+        if (ub > -INFINITY && ub >= tau && !(sat && a.t.tminid[a.G + g] > tau_id))
+            glist[atomicAdd(&ng, 1u)] = (uint16_t)g;
     }
     __syncthreads();
     const uint32_t H = a.H, per = 1u << H, total = ng << H;
@@ -1124,7 +1254,7 @@ __global__ __launch_bounds__(BQ_BS, bq_waves_per_simd(NB)) void batch_cut_kernel
             const uint32_t i = i0 + v * BQ_BS + tid;
             p[v] = i < total ? ((uint32_t)glist[i >> H] << H) + (i & (per - 1)) : a.L;
             const float ub = p[v] < a.L ? node_bound<NB>(b, a.t, a.t.p2 + p[v]) : -INFINITY;
-            in[v] = ub > -INFINITY && ub >= tau;
+            in[v] = ub > -INFINITY && ub >= tau && !(sat && a.t.tminid[a.t.p2 + p[v]] > tau_id);
         }
 #pragma unroll
         for (int v = 0; v < U; v++)
@@ -1187,6 +1317,7 @@ struct ChunkArgs {
     uint64_t *cand;
     unsigned int *ccount;
     unsigned long long *scored;  // per workgroup
+    const uint32_t *cminid;      // smallest object id per chunk
 };
 
 constexpr uint32_t BC_STAGE = 256;  // candidate keys a wave stages in LDS before one append
@@ -1216,6 +1347,7 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
         // the query's terms, lane j holding term j
         float lw = 0.0f, tau = 0.0f;
         uint32_t la = 0, tau_id = 0, m = 0;
+        bool sat = false;
         auto load_query = [&]() {
             const BatchQuery &Q = a.q[q];
             lw = lane < TOPK_MAX_DIMS ? Q.w[lane] : 0.0f;
@@ -1223,6 +1355,7 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
             tau = Q.tau;
             tau_id = Q.tau_id;
             m = Q.m;
+            sat = at_smax(Q);
         };
         load_query();
         // staged candidates of query q (wave-uniform count); one append each flush
@@ -1271,6 +1404,8 @@ __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
                 p = tl[j].x;
             }
             const uint32_t c = lt - jb, sz = a.psz[p];
+            // at the largest score, a chunk whose ids all exceed tau_id holds no candidate
+            if (sat && a.cminid[a.poff[p] / PTA_CHUNK + c] > tau_id) continue;
             const uint32_t first = c * PTA_CHUNK + 4 * lane;
             const uint64_t pos0 = (uint64_t)a.poff[p] + first;
             f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -1725,7 +1860,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     const uint32_t *pass_flags = second ? qflag : nullptr;
     // counters: cleared by the prep kernel for the first pass, by the first
     // pass's final kernel for the second
-    const TreeArgs ta{P.tmax, P.tmin, P.tsum, P.tminsum, P.p2, ctx->d};
+    const TreeArgs ta{P.tmax, P.tmin, P.tsum, P.tminsum, P.tminid, P.p2, ctx->d};
     // a second-pass seed only serves learned misses that rerun with seeds
     const bool seed2 = mode == BMODE_LEARNED && ctx->tune.batch_retry == 2;
     hipEvent_t evt = nullptr;
@@ -1738,7 +1873,11 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
     } else if (mode == BMODE_LEARNED && !second) {
         const int rc = learned_tau_enqueue(ctx, s, dq, nq);
         if (rc) return rc;
-        if (learned_saturated(ctx)) {  // queries predicted at their largest score: a seed (key) threshold
+        if (learned_saturated(ctx)) {
+            // queries predicted at their largest score: the exact key from the
+            // tie scan, else a seed (key) threshold
+            hipLaunchKernelGGL(batch_tie_kernel, dim3(nq), dim3(256), 0, s, dq, ctx->cols, ctx->ld, ctx->n,
+                               BTIE_BUDGET);
             const int NB = dims_class(ctx->d);
             auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
             hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz,
@@ -1786,7 +1925,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
         hipLaunchKernelGGL(batch_unit_scan_kernel, dim3(1), dim3(1024), 0, s, B.slot[B.cur].d_ucnt, nq * S, fl, S, B.slot[B.cur].d_uoff);
         const uint32_t g2 = (uint32_t)B.cus * (second ? 2 : 7);  // 7 resident 4-wave workgroups per CU
         const ChunkArgs xa{dq, P.pcols, P.ld, P.poff, P.psz, P.pid, B.slot[B.cur].d_tasks, ucap, B.slot[B.cur].d_ucnt, B.slot[B.cur].d_uoff, nq * S, S,
-                           fl, B.slot[B.cur].d_cand, B.slot[B.cur].d_ccount, B.slot[B.cur].d_scored};
+                           fl, B.slot[B.cur].d_cand, B.slot[B.cur].d_ccount, B.slot[B.cur].d_scored, P.cminid};
         hipEvent_t ev0 = nullptr;
         timing_begin(ctx, s, ev0);
         hipLaunchKernelGGL(batch_chunk_kernel, dim3(g2), dim3(BC_BS), 0, s, xa);
@@ -1823,7 +1962,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
                        (uint32_t)v.qpi, ochunk, B.slot[B.cur].item_cap, P.psz, B.slot[B.cur].d_items, B.slot[B.cur].d_ctr);
     ScoreArgs sa{P.pcols, P.ld, ctx->d,    P.poff,     P.psz,  P.pid,  B.slot[B.cur].d_items, B.slot[B.cur].item_cap,
                  ochunk,  B.slot[B.cur].d_plist, nq, B.slot[B.cur].d_ctr, dq, B.slot[B.cur].d_cand, B.slot[B.cur].d_ccount, P.cstat,  P.crow, (uint64_t)P.nchunks,
-                 B.slot[B.cur].d_scored};
+                 B.slot[B.cur].d_scored, P.cminid};
     BatchVariant vs = v;
     if (second) vs.grid = v.grid / 8 > 32 ? v.grid / 8 : 32;  // the second pass has few items
     hipEvent_t ev0 = nullptr;
@@ -1858,7 +1997,8 @@ static int batch_enqueue(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t n
     hipLaunchKernelGGL(batch_prep_kernel, dim3(pg), dim3(64), 0, s,
                        reinterpret_cast<const topk_query *>(d_in + il.raw_off),
                        mode == BMODE_GIVEN ? reinterpret_cast<const float *>(d_in + il.tau_off) : nullptr, nq, ctx->n,
-                       ctx->d, dq, db, H.d_pcnt, L, H.d_ccount, H.d_ctr, H.d_flist);
+                       ctx->d, ctx->pta.tmax + 1, ctx->pta.tmin + 1, 2 * (uint64_t)ctx->pta.p2, dq, db, H.d_pcnt, L,
+                       H.d_ccount, H.d_ctr, H.d_flist);
     int rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, false);
     if (!rc) rc = enqueue_pass(ctx, B, s, nq, ks, mode, v, true);
     if (rc) return rc;
@@ -1921,6 +2061,7 @@ int topk::make_batch_query(const topk_ctx *ctx, const topk_query &Q, BatchQuery 
     if (b.ascending)
         for (uint32_t j = 0; j < Q.qq; j++) b.wa[Q.attr[j]] = b.w[j];
     b.tau_id = 0xFFFFFFFFu;
+    b.smax = INFINITY;  // not known here: never taken as the largest score
     const BoundArgs g = make_bound_args(ctx->d, Q.qq, Q.weights, Q.attr);
     for (uint32_t i = 0; i < g.nd && i < 16; i++) {
         b.ws[i] = (float)g.w[i];

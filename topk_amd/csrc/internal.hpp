@@ -126,6 +126,13 @@ struct alignas(16) BatchQuery {
     float ws[16];
     uint8_t as[16];
     uint8_t attr[TOPK_MAX_DIMS];    // term j reads attribute attr[j]
+    // Largest score the store allows: the binary32 score, in the query's term
+    // order, of the corner of the store's box that each weight's sign prefers.
+    // Rounding is monotone, so no object scores above it.  While tau ==
+    // smax, an object is a candidate only if its id <= tau_id, so a
+    // partition or chunk whose smallest id exceeds tau_id holds none.
+    float smax;
+    uint32_t pad_[3];
 };
 static_assert(sizeof(BatchQuery) % 16 == 0, "BatchQuery records stay 16-byte aligned");
 static_assert(offsetof(BatchQuery, tau) % 16 == 0 && offsetof(BatchQuery, tau_id) == offsetof(BatchQuery, tau) + 4 &&
@@ -212,6 +219,7 @@ struct PtaLayout {
     uint64_t ld = 0;             // column stride of pcols
     float *pcols = nullptr;      // d columns in partition order, + one stream tile of padding
     uint32_t *pid = nullptr;     // global object id per position (0xFFFFFFFF: padding)
+    uint32_t *cminid = nullptr;  // nchunks: smallest object id of each chunk (0xFFFFFFFF: none)
     uint32_t *poff = nullptr;    // nparts + 1 start positions
     uint32_t *psz = nullptr;     // real objects per partition
     float *pmax = nullptr;       // nparts x d per-attribute maxima
@@ -239,6 +247,7 @@ struct PtaLayout {
     float *tmin = nullptr;       // d x 2*p2
     float *tsum = nullptr;       // 2*p2
     float *tminsum = nullptr;    // 2*p2
+    uint32_t *tminid = nullptr;  // 2*p2: smallest object id under each node (0xFFFFFFFF: none)
     // Chunk statistics: positions [c * PTA_CHUNK, (c + 1) * PTA_CHUNK) form
     // chunk c (partitions start on chunk boundaries); the crow = 2d + 2
     // statistics of chunk c -- maxima, minima, largest attribute sum (rounded

@@ -202,10 +202,11 @@ __global__ __launch_bounds__(PTA_BS) void pta_bounds_kernel(const float *__restr
 // cstat[e * nchunks + c]) holds the chunk's per-attribute maxima (e in
 // [0, d)) and minima ([d, 2d)), the largest attribute sum of its objects
 // (FP64, rounded up) at 2d and the sum of its minima (FP64, rounded down)
-// at 2d + 1.  A chunk without objects has sum -inf.
+// at 2d + 1.  A chunk without objects has sum -inf.  cminid[c] gets the
+// chunk's smallest object id (0xFFFFFFFF: none).
 __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict__ pcols, uint64_t ldp, uint32_t d,
                                                         const uint32_t *__restrict__ pid, uint64_t nchunks,
-                                                        float *__restrict__ cstat) {
+                                                        float *__restrict__ cstat, uint32_t *__restrict__ cminid) {
     const uint64_t c = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
     if (c >= nchunks) return;
     const uint32_t lane = lane_id();
@@ -213,12 +214,20 @@ __global__ __launch_bounds__(256) void pta_chunk_kernel(const float *__restrict_
     const uint64_t pos0 = c * PTA_CHUNK + lane * PER;
     bool ok[PER];
     bool any = false;
+    uint32_t idmin = 0xFFFFFFFFu;  // padding positions hold id 0xFFFFFFFF
 #pragma unroll
     for (uint32_t j = 0; j < PER; j++) {
-        ok[j] = pid[pos0 + j] != 0xFFFFFFFFu;
+        const uint32_t id = pid[pos0 + j];
+        ok[j] = id != 0xFFFFFFFFu;
         any |= ok[j];
+        idmin = id < idmin ? id : idmin;
     }
     const bool nonempty = __ballot(any) != 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t t = (uint32_t)__shfl_xor((int)idmin, o, 64);
+        idmin = t < idmin ? t : idmin;
+    }
+    if (lane == 0) cminid[c] = idmin;
     float *row = cstat + c;  // element e at row[e * nchunks]
     double best = -INFINITY, minsum = 0.0;
     double sums[PER];
@@ -356,13 +365,25 @@ static int build_bound_tree(PtaLayout &P, uint32_t d, hipStream_t s) {
     while (p2 < L) p2 <<= 1;
     const size_t nn = 2 * (size_t)p2;
     std::vector<float> mx(nn * d, -INFINITY), mn(nn * d, INFINITY), sm(nn, -INFINITY), ms(nn, 0.0f);
-    std::vector<uint32_t> sz(L);
+    std::vector<uint32_t> sz(L), off(L + 1), cmin(P.nchunks), idm(nn, 0xFFFFFFFFu);
     TK_HIP(hipMemcpyAsync(mx.data() + (size_t)p2 * d, P.pmax, sizeof(float) * L * d, hipMemcpyDeviceToHost, s));
     TK_HIP(hipMemcpyAsync(mn.data() + (size_t)p2 * d, P.pmin, sizeof(float) * L * d, hipMemcpyDeviceToHost, s));
     TK_HIP(hipMemcpyAsync(sm.data() + p2, P.psum, sizeof(float) * L, hipMemcpyDeviceToHost, s));
     TK_HIP(hipMemcpyAsync(ms.data() + p2, P.pminsum, sizeof(float) * L, hipMemcpyDeviceToHost, s));
     TK_HIP(hipMemcpyAsync(sz.data(), P.psz, sizeof(uint32_t) * L, hipMemcpyDeviceToHost, s));
+    TK_HIP(hipMemcpyAsync(off.data(), P.poff, sizeof(uint32_t) * (L + 1), hipMemcpyDeviceToHost, s));
+    if (P.nchunks)
+        TK_HIP(hipMemcpyAsync(cmin.data(), P.cminid, sizeof(uint32_t) * P.nchunks, hipMemcpyDeviceToHost, s));
     TK_HIP(hipStreamSynchronize(s));
+    // smallest id per partition (its chunks), then per node
+    for (uint32_t p = 0; p < L; p++) {
+        uint32_t m = 0xFFFFFFFFu;
+        if (sz[p])
+            for (uint64_t c = off[p] / PTA_CHUNK; c < ((uint64_t)off[p] + sz[p] + PTA_CHUNK - 1) / PTA_CHUNK; c++)
+                m = std::min(m, cmin[c]);
+        idm[p2 + p] = m;
+    }
+    for (size_t v = p2 - 1; v >= 1; v--) idm[v] = std::min(idm[2 * v], idm[2 * v + 1]);
     for (uint32_t p = 0; p < L; p++)
         if (!sz[p]) {  // empty partition: identity box
             for (uint32_t m = 0; m < d; m++) {
@@ -396,6 +417,8 @@ static int build_bound_tree(PtaLayout &P, uint32_t d, hipStream_t s) {
     TK_HIP(hipMalloc(&P.tmin, sizeof(float) * nn * d));
     TK_HIP(hipMalloc(&P.tsum, sizeof(float) * nn));
     TK_HIP(hipMalloc(&P.tminsum, sizeof(float) * nn));
+    TK_HIP(hipMalloc(&P.tminid, sizeof(uint32_t) * nn));
+    TK_HIP(hipMemcpyAsync(P.tminid, idm.data(), sizeof(uint32_t) * nn, hipMemcpyHostToDevice, s));
     TK_HIP(hipMemcpyAsync(P.tmax, mxt.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
     TK_HIP(hipMemcpyAsync(P.tmin, mnt.data(), sizeof(float) * nn * d, hipMemcpyHostToDevice, s));
     TK_HIP(hipMemcpyAsync(P.tsum, sm.data(), sizeof(float) * nn, hipMemcpyHostToDevice, s));
@@ -424,6 +447,8 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.tsum);
     hipFree(P.tminsum);
     hipFree(P.cstat);
+    hipFree(P.cminid);
+    hipFree(P.tminid);
     const uint32_t psize = P.psize;
     const uint64_t gen = P.generation;
     P = PtaLayout();
@@ -536,9 +561,10 @@ int pta_build(topk_ctx *ctx) {
         P.nchunks = P.npos / PTA_CHUNK;
         P.crow = 2 * d + 2;
         TK_TRY(hipMalloc(&P.cstat, sizeof(float) * (P.nchunks ? P.nchunks : 1) * P.crow));
+        TK_TRY(hipMalloc(&P.cminid, sizeof(uint32_t) * (P.nchunks ? P.nchunks : 1)));
         if (P.nchunks)
             hipLaunchKernelGGL(pta_chunk_kernel, dim3((uint32_t)((P.nchunks + 3) / 4)), dim3(256), 0, s, P.pcols, P.ld,
-                               d, P.pid, (uint64_t)P.nchunks, P.cstat);
+                               d, P.pid, (uint64_t)P.nchunks, P.cstat, P.cminid);
         if (P.nchunks)
             hipLaunchKernelGGL(pta_chunk_suffix_kernel, dim3((L + 255) / 256), dim3(256), 0, s, L, d, P.poff, P.psz,
                                (uint64_t)P.nchunks, P.cstat);
