@@ -343,90 +343,99 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 }
 
 // Tie scan, learned first pass: one workgroup per query the predictor
-// marked as near its largest score (seed = 1).  When at least k objects
-// reach smax, the query's top k are the k smallest ids among them, so its
-// k-th key is (smax, id of the k-th such object in id order).  Each step
-// scores BTIE_STEP objects in id order straight from the id-order columns
-// (terms in query order, as every scoring kernel): wave w takes ids
-// [i0 + w * 64 U, i0 + (w + 1) * 64 U), lane l the ids u * 64 + l of that
-// range.  Ballots count the objects at smax per 64-id block; once the
-// running count reaches k, the block holding the k-th gives its id: tau =
-// (smax, that id), exact, and the mark is cleared.  The scan gives up past
-// `budget` ids, or once the rate of ties so far projects more than `budget`
-// ids to reach k; such queries keep the mark and take a seed threshold.
-constexpr uint32_t BTIE_U = 8;                     // 64-id blocks per wave per step
-constexpr uint32_t BTIE_STEP = 4 * 64 * BTIE_U;    // ids per workgroup step
-constexpr uint32_t BTIE_BUDGET = 1u << 16;         // ids a tie scan may read
-__global__ __launch_bounds__(256) void batch_tie_kernel(BatchQuery *__restrict__ qs,
-                                                        const float *__restrict__ cols, uint64_t ld, uint64_t n,
-                                                        uint32_t budget) {
-    __shared__ uint32_t cnt[4][BTIE_U];
-    const uint32_t lane = lane_id(), w = threadIdx.x / 64, q = blockIdx.x;
+// marked as near its largest score (seed = 1).  An object whose every
+// queried attribute of nonzero weight holds the store's extreme value on
+// the side its weight prefers (largest for w > 0, smallest for w < 0) scores
+// exactly smax: its terms are smax's terms.  Such objects are found by ANDing
+// the extreme-value bitmaps of the query's attributes, in id order; if at
+// least k exist, the k-th of them in id order has an id no smaller than the
+// k-th smallest id among all objects scoring smax (they are a subset), so
+// tau = (smax, that id) is a key no higher than the query's k-th key -- and
+// equal to it when no other object rounds to smax.  The mark is then
+// cleared.  Each step ANDs W consecutive words per thread (256 x 64 x W
+// ids per step) and a block scan of the per-thread counts places the k-th.
+// The scan gives up past `budget` ids, or once the ties counted so far
+// project more than `budget` ids to reach k; such queries keep the mark
+// and take a seed threshold if it saw any tie, else their learned score
+// threshold.
+constexpr uint64_t BXB_BUDGET = 1ull << 22;  // ids a scan may cover
+// MAXR bounds the query's rows (terms of nonzero weight; 16 or 32); a
+// thread takes W words of every row per step, all loads in flight at once.
+template <int MAXR, int W>
+__global__ __launch_bounds__(256) void batch_tiebits_kernel(BatchQuery *__restrict__ qs,
+                                                            const uint64_t *__restrict__ xbits, uint32_t d,
+                                                            uint64_t nw, uint64_t n, uint64_t budget) {
+    constexpr uint64_t STEP = 256ull * 64 * W;  // ids per step
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t total, nrows;
+    __shared__ uint32_t rrow[MAXR];
+    const uint32_t tid = threadIdx.x, q = blockIdx.x;
     if (!qs[q].seed) return;  // uniform over the workgroup
     const BatchQuery &Q = qs[q];
-    const float smax = Q.smax;
     const uint32_t k = Q.k, m = Q.m;
-    const uint32_t la = lane < m ? Q.attr[lane] : 0u;
-    const float lw = lane < m ? Q.w[lane] : 0.0f;
-    const uint64_t lim = n < budget ? n : budget;
+    if (tid == 0) {
+        uint32_t c = 0;
+        for (uint32_t t = 0; t < m && c < MAXR; t++) {
+            const float w = Q.w[t];
+            if (w != 0.0f) rrow[c++] = w > 0.0f ? Q.attr[t] : d + Q.attr[t];
+        }
+        nrows = c;
+        for (uint32_t r = c; r < MAXR; r++) rrow[r] = 0u;  // unused rows: row 0, never ANDed
+    }
+    __syncthreads();
+    const uint32_t nr = nrows;
+    const uint64_t lim = n < budget ? n : budget, wlim = (lim + 63) / 64;
     uint32_t found = 0;
-    for (uint64_t i0 = 0; i0 < lim; i0 += BTIE_STEP) {
-        const uint64_t wb = i0 + (uint64_t)w * 64 * BTIE_U;
-        float sc[BTIE_U];
+    for (uint64_t w0 = 0; w0 < wlim; w0 += 256 * W) {
+        const uint64_t wb = w0 + (uint64_t)tid * W;
+        uint64_t x[MAXR][W];
 #pragma unroll
-        for (uint32_t u = 0; u < BTIE_U; u++) sc[u] = 0.0f;
-        for (uint32_t t = 0; t < m; t++) {
-            const uint32_t at = (uint32_t)__builtin_amdgcn_readlane((int)la, (int)t);
-            const float wt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lw), (int)t));
-            const float *col = cols + (uint64_t)at * ld;
+        for (int r = 0; r < MAXR; r++) {
+            const uint64_t *row = xbits + (uint64_t)rrow[r] * nw;
 #pragma unroll
-            for (uint32_t u = 0; u < BTIE_U; u++) {
-                const uint64_t id = wb + u * 64 + lane;
-                const float pr = col[id < n ? id : 0] * wt;
-                sc[u] = sc[u] + pr;
-            }
+            for (int i = 0; i < W; i++) x[r][i] = r < (int)nr && wb + i < wlim ? row[wb + i] : ~0ull;
         }
-        uint64_t mask[BTIE_U];
+        uint64_t acc[W];
+        uint32_t c = 0;
 #pragma unroll
-        for (uint32_t u = 0; u < BTIE_U; u++) {
-            mask[u] = __ballot(wb + u * 64 + lane < lim && sc[u] == smax);
-            if (lane == 0) cnt[w][u] = (uint32_t)__popcll(mask[u]);
+        for (int i = 0; i < W; i++) {
+            acc[i] = wb + i < wlim ? ~0ull : 0ull;
+#pragma unroll
+            for (int r = 0; r < MAXR; r++) acc[i] &= x[r][i];
+            if (wb + i == wlim - 1 && lim % 64) acc[i] &= (1ull << (lim % 64)) - 1ull;  // the last word may run past lim
+            c += (uint32_t)__popcll(acc[i]);
         }
-        __syncthreads();
-        // blocks in id order: wave-major, then u
-        uint32_t before = found, hit_w = 4, hit_u = 0;
-        for (uint32_t ww = 0; ww < 4 && hit_w == 4; ww++)
-            for (uint32_t u = 0; u < BTIE_U; u++) {
-                const uint32_t c = cnt[ww][u];
-                if (before + c >= k) {
-                    hit_w = ww;
-                    hit_u = u;
-                    break;
-                }
-                before += c;
-            }
-        if (hit_w < 4) {  // uniform: every thread read the same counts
-            if (w == hit_w) {
-                uint64_t mk = 0;
+        const uint32_t excl = block_excl_scan<256>(c, wsum, &total);
+        const uint32_t tot = total;
+        if (found + tot >= k) {  // uniform
+            if (found + excl < k && k <= found + excl + c) {  // this thread holds the k-th
+                uint32_t r = k - found - excl - 1;
 #pragma unroll
-                for (uint32_t u = 0; u < BTIE_U; u++) mk = u == hit_u ? mask[u] : mk;
-                const uint32_t r = k - before - 1;  // rank of the k-th tie within its block
-                const uint32_t below = (uint32_t)__popcll(mk & ((1ull << lane) - 1ull));
-                const uint64_t sel = __ballot(((mk >> lane) & 1ull) && below == r);
-                if (lane == 0) {
-                    BatchQuery &W = qs[q];
-                    W.tau = smax;
-                    W.tau_id = (uint32_t)(wb + hit_u * 64) + (uint32_t)(__ffsll((long long)sel) - 1);
-                    W.seed = 0u;
+                for (int i = 0; i < W; i++) {
+                    const uint32_t pc = (uint32_t)__popcll(acc[i]);
+                    if (r < pc) {
+                        uint64_t y = acc[i];
+                        for (uint32_t j = 0; j < r; j++) y &= y - 1ull;
+                        BatchQuery &Wq = qs[q];
+                        Wq.tau = Q.smax;
+                        Wq.tau_id = (uint32_t)(64 * (wb + i)) + (uint32_t)(__ffsll((long long)y) - 1);
+                        Wq.seed = 0u;
+                        break;
+                    }
+                    r -= pc;
                 }
             }
             return;
         }
-        found = before;
-        __syncthreads();  // cnt is rewritten by the next step
-        const uint64_t done = i0 + BTIE_STEP;
-        if (done >= 8192 && (uint64_t)found * budget < (uint64_t)k * done) return;  // projected past the budget
+        found += tot;
+        const uint64_t done = (w0 + 256 * W) * 64;
+        if ((uint64_t)found * budget < (uint64_t)k * done) break;  // projected past the budget
     }
+    // No object at smax among the ids read: the predictor's mark was most
+    // likely wrong (fewer than k objects reach smax), so the query keeps its
+    // learned score threshold (exact either way: fewer than k candidates
+    // send it to the second pass).  Ties seen but too sparse: a seed.
+    if (found == 0 && tid == 0) qs[q].seed = 0u;
 }
 
 // Query-major mask: workgroup (q, y) of S per query walks the bound tree
@@ -1874,10 +1883,12 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
         const int rc = learned_tau_enqueue(ctx, s, dq, nq);
         if (rc) return rc;
         if (learned_saturated(ctx)) {
-            // queries predicted at their largest score: the exact key from the
-            // tie scan, else a seed (key) threshold
-            hipLaunchKernelGGL(batch_tie_kernel, dim3(nq), dim3(256), 0, s, dq, ctx->cols, ctx->ld, ctx->n,
-                               BTIE_BUDGET);
+            // queries predicted at their largest score: a key from the
+            // extreme-value bitmaps, else a seed (key) threshold
+            if (P.xbits) {
+                auto tk = ctx->d <= 16 ? batch_tiebits_kernel<16, 4> : batch_tiebits_kernel<32, 2>;
+                hipLaunchKernelGGL(tk, dim3(nq), dim3(256), 0, s, dq, P.xbits, ctx->d, P.nw, ctx->n, BXB_BUDGET);
+            }
             const int NB = dims_class(ctx->d);
             auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
             hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz,

@@ -248,6 +248,11 @@ struct PtaLayout {
     float *tsum = nullptr;       // 2*p2
     float *tminsum = nullptr;    // 2*p2
     uint32_t *tminid = nullptr;  // 2*p2: smallest object id under each node (0xFFFFFFFF: none)
+    // Extreme-value bitmaps (built on demand, pta_extreme_bits): bit i of
+    // word w of row a says whether object 64 w + i holds the store's largest
+    // value of attribute a (rows 0..d-1) or its smallest (rows d..2d-1).
+    uint64_t *xbits = nullptr;
+    uint64_t nw = 0;             // words per row: ceil(n / 64)
     // Chunk statistics: positions [c * PTA_CHUNK, (c + 1) * PTA_CHUNK) form
     // chunk c (partitions start on chunk boundaries); the crow = 2d + 2
     // statistics of chunk c -- maxima, minima, largest attribute sum (rounded
@@ -355,6 +360,7 @@ void batch_free(topk_ctx *ctx);
 void aux_free(topk_ctx *ctx);
 int pta_build(topk_ctx *ctx);
 void pta_release(topk_ctx *ctx);
+int pta_extreme_bits(topk_ctx *ctx);
 
 // Event bracketing for topk_timing_*.
 void timing_begin(topk_ctx *ctx, hipStream_t s, hipEvent_t &ev0);

@@ -830,6 +830,10 @@ int topk_learned_train(topk_ctx *ctx, uint32_t nq, const topk_query *qs, const t
         }
         Ls->saturated = sat >= usable.size() / 200 + 1;  // at least 0.5% of the labels
     }
+    if (Ls->saturated) {  // saturated queries find their threshold in the extreme-value bitmaps
+        const int brc = pta_extreme_bits(ctx);
+        if (brc) return brc;
+    }
     std::mt19937_64 rng(cfg.seed);
     std::shuffle(usable.begin(), usable.end(), rng);
     uint32_t ncal = (uint32_t)(cfg.calib_fraction * usable.size());

@@ -449,11 +449,49 @@ static void pta_free(PtaLayout &P) {
     hipFree(P.cstat);
     hipFree(P.cminid);
     hipFree(P.tminid);
+    hipFree(P.xbits);
     const uint32_t psize = P.psize;
     const uint64_t gen = P.generation;
     P = PtaLayout();
     P.psize = psize;
     P.generation = gen + 1;
+}
+
+// One wave per (row, word): lane i reads object 64 w + i of the attribute
+// and the ballot of "equals the store's largest / smallest value" is the word.
+__global__ __launch_bounds__(256) void extreme_bits_kernel(const float *__restrict__ cols, uint64_t ld, uint64_t n,
+                                                           uint32_t d, const float *__restrict__ rmax,
+                                                           const float *__restrict__ rmin, uint64_t rstride,
+                                                           uint64_t nw, uint64_t *__restrict__ bits) {
+    const uint32_t lane = lane_id();
+    const uint64_t nwaves = (uint64_t)gridDim.x * 4;
+    for (uint64_t v = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64; v < (uint64_t)d * nw; v += nwaves) {
+        const uint32_t a = (uint32_t)(v / nw);
+        const uint64_t w = v - (uint64_t)a * nw, id = 64 * w + lane;
+        const float x = id < n ? cols[(uint64_t)a * ld + id] : 0.0f;
+        const uint64_t hi = __ballot(id < n && x == rmax[a * rstride]);
+        const uint64_t lo = __ballot(id < n && x == rmin[a * rstride]);
+        if (lane == 0) {
+            bits[(uint64_t)a * nw + w] = hi;
+            bits[(uint64_t)(d + a) * nw + w] = lo;
+        }
+    }
+}
+
+int pta_extreme_bits(topk_ctx *ctx) {
+    PtaLayout &P = ctx->pta;
+    if (!P.built) return TOPK_ERR_STATE;
+    if (P.xbits) return TOPK_OK;
+    const uint64_t nw = (ctx->n + 63) / 64;
+    TK_HIP(hipMalloc(&P.xbits, sizeof(uint64_t) * 2 * ctx->d * nw));
+    P.nw = nw;
+    const uint64_t waves = (uint64_t)ctx->d * nw;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 65536);
+    hipLaunchKernelGGL(extreme_bits_kernel, dim3(grid), dim3(256), 0, ctx->stream, ctx->cols, ctx->ld, ctx->n, ctx->d,
+                       P.tmax + 1, P.tmin + 1, 2 * (uint64_t)P.p2, nw, P.xbits);
+    TK_HIP(hipGetLastError());
+    TK_HIP(hipStreamSynchronize(ctx->stream));
+    return TOPK_OK;
 }
 
 void pta_release(topk_ctx *ctx) {
