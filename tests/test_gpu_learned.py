@@ -152,6 +152,43 @@ def test_learned_saturated_tie_thresholds(gpu, trained):
     assert s_lrn["tuple_count"] <= s_seed["tuple_count"]
 
 
+def test_learned_beats_seed_on_saturated_mixed_store(gpu):
+    """The learned d = 16 workload at N = 2^24 (mixed store, seed 5; random
+    attribute subsets, k 16..256): 20k training queries, one batch of 1024
+    held-out queries in learned and in seed mode.  Same answers in both
+    modes (and as topk_fte on a sample), fewer scored pairs and a cheaper
+    threshold stage with learned thresholds."""
+    import bench
+    ks = [16, 32, 64, 128, 256]
+    n, d = 1 << 24, 16
+    with topk_amd.Engine.generated(n, d, topk_amd.MIXED, 5) as eng:
+        rep = eng.learned_train(bench.subset_queries(np.random.default_rng(300), 20000, d, ks))
+        assert rep["saturated"] == 1
+        held = bench.subset_queries(np.random.default_rng(301), 1024, d, ks)
+        pb = eng.prepare_batch(held)
+        res, pairs, tau_ms = {}, {}, {}
+        for mode, name in ((1, "seed"), (2, "learned")):
+            eng.set_tuning("batch_mode", mode)
+            eng.batch_run(pb)  # warm: graph capture
+            eng.timing(True)
+            eng.timing_reset()
+            eng.batch_run(pb)
+            pairs[mode] = eng.stats()["tuple_count"]
+            tau_ms[mode] = eng.timing_get(f"batch_tau_{name}")[0]
+            eng.timing(False)
+            res[mode] = [pb.result(i, n) for i in range(len(held))]
+        eng.set_tuning("batch_mode", 0)
+        for i, ((a, b, c), (e, f, g)) in enumerate(zip(res[1], res[2])):
+            assert np.array_equal(a, e) and np.array_equal(b.view(np.uint32), f.view(np.uint32)) and c == g, i
+        for i in range(0, len(held), 16):
+            ri, rs, rt = eng.fte(*held[i])
+            gi, gs, gt = res[2][i]
+            assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)) and gt == rt, i
+    print("scored pairs seed/learned", pairs[1], pairs[2], "threshold stage ms seed/learned", tau_ms[1], tau_ms[2])
+    assert pairs[2] <= pairs[1]
+    assert tau_ms[2] < tau_ms[1]
+
+
 @pytest.fixture(scope="module")
 def trained_unsaturated():
     """N=2^20, d=16 independent store (no ties at the top) with a predictor

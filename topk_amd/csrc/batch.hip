@@ -352,21 +352,21 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 // k-th smallest id among all objects scoring smax (they are a subset), so
 // tau = (smax, that id) is a key no higher than the query's k-th key -- and
 // equal to it when no other object rounds to smax.  The mark is then
-// cleared.  Each step ANDs W consecutive words per thread (256 x 64 x W
+// cleared.  Each step ANDs W consecutive words per thread (BXB_BS x 64 x W
 // ids per step) and a block scan of the per-thread counts places the k-th.
 // The scan gives up past `budget` ids, or once the ties counted so far
-// project more than `budget` ids to reach k; such queries keep the mark
-// and take a seed threshold if it saw any tie, else their learned score
-// threshold.
-constexpr uint64_t BXB_BUDGET = 1ull << 22;  // ids a scan may cover
+// project more than `budget` ids to reach k; such queries then take all
+// ties as candidates, their learned score threshold or a seed, by the
+// projected number of ties (end of the kernel).
+constexpr uint64_t BXB_BUDGET = 1ull << 21;  // ids a scan may cover
+constexpr int BXB_BS = 512;                  // threads per scan workgroup
 // MAXR bounds the query's rows (terms of nonzero weight; 16 or 32); a
 // thread takes W words of every row per step, all loads in flight at once.
 template <int MAXR, int W>
-__global__ __launch_bounds__(256) void batch_tiebits_kernel(BatchQuery *__restrict__ qs,
+__global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__restrict__ qs,
                                                             const uint64_t *__restrict__ xbits, uint32_t d,
                                                             uint64_t nw, uint64_t n, uint64_t budget) {
-    constexpr uint64_t STEP = 256ull * 64 * W;  // ids per step
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[BXB_BS / 64];
     __shared__ uint32_t total, nrows;
     __shared__ uint32_t rrow[MAXR];
     const uint32_t tid = threadIdx.x, q = blockIdx.x;
@@ -386,7 +386,8 @@ __global__ __launch_bounds__(256) void batch_tiebits_kernel(BatchQuery *__restri
     const uint32_t nr = nrows;
     const uint64_t lim = n < budget ? n : budget, wlim = (lim + 63) / 64;
     uint32_t found = 0;
-    for (uint64_t w0 = 0; w0 < wlim; w0 += 256 * W) {
+    uint64_t scanned = lim;  // ids read when the scan stops
+    for (uint64_t w0 = 0; w0 < wlim; w0 += BXB_BS * W) {
         const uint64_t wb = w0 + (uint64_t)tid * W;
         uint64_t x[MAXR][W];
 #pragma unroll
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(256) void batch_tiebits_kernel(BatchQuery *__restri
             if (wb + i == wlim - 1 && lim % 64) acc[i] &= (1ull << (lim % 64)) - 1ull;  // the last word may run past lim
             c += (uint32_t)__popcll(acc[i]);
         }
-        const uint32_t excl = block_excl_scan<256>(c, wsum, &total);
+        const uint32_t excl = block_excl_scan<BXB_BS>(c, wsum, &total);
         const uint32_t tot = total;
         if (found + tot >= k) {  // uniform
             if (found + excl < k && k <= found + excl + c) {  // this thread holds the k-th
@@ -428,14 +429,31 @@ __global__ __launch_bounds__(256) void batch_tiebits_kernel(BatchQuery *__restri
             return;
         }
         found += tot;
-        const uint64_t done = (w0 + 256 * W) * 64;
-        if ((uint64_t)found * budget < (uint64_t)k * done) break;  // projected past the budget
+        const uint64_t done = (w0 + BXB_BS * W) * 64;
+        if ((uint64_t)found * budget < (uint64_t)k * done) {  // projected past the budget
+            scanned = done;
+            break;
+        }
     }
-    // No object at smax among the ids read: the predictor's mark was most
-    // likely wrong (fewer than k objects reach smax), so the query keeps its
-    // learned score threshold (exact either way: fewer than k candidates
-    // send it to the second pass).  Ties seen but too sparse: a seed.
-    if (found == 0 && tid == 0) qs[q].seed = 0u;
+    // The scan stopped short of k ties; the rate read projects the store's.
+    // Between 2k and BATCH_CAP / 2 of them: tau = (smax, every id) takes
+    // them all as candidates (a key no higher than the k-th when at least k
+    // exist).  Fewer: the predictor's mark was most likely wrong (about k
+    // objects or fewer reach smax), so the query keeps its learned score
+    // threshold.  More: a seed.  Every choice is exact: a threshold that
+    // leaves fewer than k candidates, or more than the list holds, sends
+    // the query to the second pass.
+    if (tid == 0) {
+        const uint64_t proj = (uint64_t)found * n;  // ties in the store x ids read
+        if (proj < 2ull * k * scanned) {
+            qs[q].seed = 0u;
+        } else if (proj <= (uint64_t)(BATCH_CAP / 2) * scanned) {
+            BatchQuery &Wq = qs[q];
+            Wq.tau = Q.smax;
+            Wq.tau_id = 0xFFFFFFFFu;
+            Wq.seed = 0u;
+        }
+    }
 }
 
 // Query-major mask: workgroup (q, y) of S per query walks the bound tree
@@ -1887,7 +1905,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
             // extreme-value bitmaps, else a seed (key) threshold
             if (P.xbits) {
                 auto tk = ctx->d <= 16 ? batch_tiebits_kernel<16, 4> : batch_tiebits_kernel<32, 2>;
-                hipLaunchKernelGGL(tk, dim3(nq), dim3(256), 0, s, dq, P.xbits, ctx->d, P.nw, ctx->n, BXB_BUDGET);
+                hipLaunchKernelGGL(tk, dim3(nq), dim3(BXB_BS), 0, s, dq, P.xbits, ctx->d, P.nw, ctx->n, BXB_BUDGET);
             }
             const int NB = dims_class(ctx->d);
             auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
