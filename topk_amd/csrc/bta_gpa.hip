@@ -261,6 +261,121 @@ __global__ __launch_bounds__(BTA_BS) void bta_block_kernel(QCols q, uint64_t n, 
     for (uint32_t i = threadIdx.x; i < K2; i += BTA_BS) runs[(uint64_t)blockIdx.x * K2 + i] = r[i];
 }
 
+// ---- register-resident block kernel for runs of 64 keys (k <= 64) ----
+// The key of lane l's partner lane l ^ S, each 32-bit half moved without
+// LDS where a lane pattern allows: quad permutes (S = 1, 2), a row rotation
+// by 8 (S = 8: lane l ^ 8 either way round), swizzles inside 32-lane halves
+// (S = 4, 16) and a permute for S = 32.
+template <int S>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (S == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (S == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (S == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+    else if constexpr (S == 4 || S == 16) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (S << 10));
+    else return (uint32_t)__shfl_xor((int)v, S, 64);
+}
+
+template <int S>
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v) {
+    const uint32_t lo = xor_lane<S>((uint32_t)v), hi = xor_lane<S>((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Compare-exchange with lane l ^ S: the lane keeps the smaller key when
+// keep_min, else the larger.
+template <int S>
+__device__ __forceinline__ void cx_lane(uint64_t &v, bool keep_min) {
+    const uint64_t p = xor_lane64<S>(v);
+    v = (p < v) == keep_min ? p : v;
+}
+
+// Bitonic merge of a 64-lane bitonic sequence into descending (desc) or
+// ascending lane order.
+__device__ __forceinline__ void clean64(uint64_t &v, uint32_t l, bool desc) {
+    cx_lane<32>(v, ((l & 32) == 0) != desc);
+    cx_lane<16>(v, ((l & 16) == 0) != desc);
+    cx_lane<8>(v, ((l & 8) == 0) != desc);
+    cx_lane<4>(v, ((l & 4) == 0) != desc);
+    cx_lane<2>(v, ((l & 2) == 0) != desc);
+    cx_lane<1>(v, ((l & 1) == 0) != desc);
+}
+
+// Bitonic sort of one key per lane into descending or ascending lane order.
+__device__ __forceinline__ void sort64(uint64_t &v, uint32_t l, bool desc) {
+    // stage (size, stride): the lane keeps the minimum iff its block of
+    // `size` ascends and it is the lower lane of its pair (or neither)
+    cx_lane<1>(v, (((l & 2) == 0) == ((l & 1) == 0)) != desc);
+    cx_lane<2>(v, (((l & 4) == 0) == ((l & 2) == 0)) != desc);
+    cx_lane<1>(v, (((l & 4) == 0) == ((l & 1) == 0)) != desc);
+    cx_lane<4>(v, (((l & 8) == 0) == ((l & 4) == 0)) != desc);
+    cx_lane<2>(v, (((l & 8) == 0) == ((l & 2) == 0)) != desc);
+    cx_lane<1>(v, (((l & 8) == 0) == ((l & 1) == 0)) != desc);
+    cx_lane<8>(v, (((l & 16) == 0) == ((l & 8) == 0)) != desc);
+    cx_lane<4>(v, (((l & 16) == 0) == ((l & 4) == 0)) != desc);
+    cx_lane<2>(v, (((l & 16) == 0) == ((l & 2) == 0)) != desc);
+    cx_lane<1>(v, (((l & 16) == 0) == ((l & 1) == 0)) != desc);
+    cx_lane<16>(v, (((l & 32) == 0) == ((l & 16) == 0)) != desc);
+    cx_lane<8>(v, (((l & 32) == 0) == ((l & 8) == 0)) != desc);
+    cx_lane<4>(v, (((l & 32) == 0) == ((l & 4) == 0)) != desc);
+    cx_lane<2>(v, (((l & 32) == 0) == ((l & 2) == 0)) != desc);
+    cx_lane<1>(v, (((l & 32) == 0) == ((l & 1) == 0)) != desc);
+    clean64(v, l, desc);  // size 64: one block, ascending unless desc
+}
+
+// One workgroup per BTA_OBJS objects, K2 = 64: wave w scores its 512
+// objects into 8 keys per lane, sorts each register across the 64 lanes
+// into a run (even registers descending, odd ascending), halves the runs in
+// registers (lane-wise max of a descending and an ascending run is a
+// bitonic sequence holding their 64 largest keys, then a clean) down to
+// one run per wave, and the four waves' runs are halved twice through LDS.
+__global__ __launch_bounds__(BTA_BS) void bta_block64_kernel(QCols q, uint64_t n, uint64_t *__restrict__ runs) {
+    __shared__ uint64_t x[BTA_BS / 64][64];
+    const uint32_t l = lane_id(), w = threadIdx.x / 64;
+    const uint64_t wbase = (uint64_t)blockIdx.x * BTA_OBJS + (uint64_t)w * 512;
+    uint64_t key[8];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const uint64_t o = wbase + (uint64_t)h * 256 + (uint64_t)l * 4;
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t t = 0; t < q.m; t++) {
+            const f4 xv = *reinterpret_cast<const f4 *>(q.col[t] + o);
+            const f4 p = xv * q.w[t];
+            acc = acc + p;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) key[4 * h + e] = o + e < n ? rkey(acc[e], (uint32_t)(o + e)) : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) sort64(key[j], l, (j & 1) == 0);
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        key[p] = key[2 * p] > key[2 * p + 1] ? key[2 * p] : key[2 * p + 1];
+        clean64(key[p], l, (p & 1) == 0);
+    }
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        key[p] = key[2 * p] > key[2 * p + 1] ? key[2 * p] : key[2 * p + 1];
+        clean64(key[p], l, (p & 1) == 0);
+    }
+    key[0] = key[0] > key[1] ? key[0] : key[1];
+    clean64(key[0], l, (w & 1) == 0);  // odd waves ascend for the next halving
+    x[w][l] = key[0];
+    __syncthreads();
+    if ((w & 1) == 0) {
+        uint64_t v = x[w][l] > x[w + 1][l] ? x[w][l] : x[w + 1][l];
+        clean64(v, l, w == 0);  // wave 0's run descends, wave 2's ascends
+        key[0] = v;
+    }
+    __syncthreads();
+    if ((w & 1) == 0) x[w][l] = key[0];
+    __syncthreads();
+    if (w == 0) {
+        uint64_t v = x[0][l] > x[2][l] ? x[0][l] : x[2][l];
+        clean64(v, l, true);
+        runs[(uint64_t)blockIdx.x * 64 + l] = v;
+    }
+}
+
 // Workgroup g merges runs [g*G, g*G + G) of `in` (descending, K2 keys each;
 // missing runs are empty) into one descending run of `out`.
 __global__ __launch_bounds__(BTA_BS) void bta_merge_kernel(const uint64_t *__restrict__ in, uint32_t nruns,
@@ -575,7 +690,8 @@ int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     if (rc) return rc;
     const double t0 = now_ms();
     AuxState &A = aux(ctx);
-    const uint32_t K2 = next_pow2(k < 2 ? 2 : k);
+    // runs of 64 keys for k <= 64: the register-resident block kernel
+    const uint32_t K2 = k <= 64 ? 64u : next_pow2(k);
     const uint32_t nblk = (uint32_t)((ctx->n + BTA_OBJS - 1) / BTA_OBJS);
     const uint64_t need = (uint64_t)nblk * K2;
     if (need > A.bta_cap) {
@@ -590,7 +706,10 @@ int topk_bta(topk_ctx *ctx, uint32_t k, uint8_t qq, const float *weights, const 
     hipStream_t s = ctx->stream;
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
-    hipLaunchKernelGGL(bta_block_kernel, dim3(nblk), dim3(BTA_BS), 0, s, q, ctx->n, K2, A.bta[0]);
+    if (K2 == 64)
+        hipLaunchKernelGGL(bta_block64_kernel, dim3(nblk), dim3(BTA_BS), 0, s, q, ctx->n, A.bta[0]);
+    else
+        hipLaunchKernelGGL(bta_block_kernel, dim3(nblk), dim3(BTA_BS), 0, s, q, ctx->n, K2, A.bta[0]);
     timing_end(ctx, s, "bta_block", ev0);
     uint32_t nruns = nblk, cur = 0;
     const uint32_t G = BTA_OBJS / K2;
