@@ -186,7 +186,7 @@ def test_learned_beats_seed_on_saturated_mixed_store(gpu):
             assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)) and gt == rt, i
     print("scored pairs seed/learned", pairs[1], pairs[2], "threshold stage ms seed/learned", tau_ms[1], tau_ms[2])
     assert pairs[2] <= pairs[1]
-    assert tau_ms[2] < tau_ms[1]
+    assert 1.5 * tau_ms[2] < tau_ms[1]  # measured 2.6x
 
 
 @pytest.fixture(scope="module")

@@ -356,9 +356,9 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 // ids per step) and a block scan of the per-thread counts places the k-th.
 // The scan gives up past `budget` ids, or once the ties counted so far
 // project more than `budget` ids to reach k; such queries then take all
-// ties as candidates, their learned score threshold or a seed, by the
-// projected number of ties (end of the kernel).
-constexpr uint64_t BXB_BUDGET = 1ull << 21;  // ids a scan may cover
+// ties as candidates or their learned score threshold, by the projected
+// number of ties (end of the kernel).  No query leaves it marked.
+constexpr uint64_t BXB_BUDGET = 1ull << 20;  // ids a scan may cover
 constexpr int BXB_BS = 512;                  // threads per scan workgroup
 // MAXR bounds the query's rows (terms of nonzero weight; 16 or 32); a
 // thread takes W words of every row per step, all loads in flight at once.
@@ -366,7 +366,7 @@ template <int MAXR, int W>
 __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__restrict__ qs,
                                                             const uint64_t *__restrict__ xbits, uint32_t d,
                                                             uint64_t nw, uint64_t n, uint64_t budget) {
-    __shared__ uint32_t wsum[BXB_BS / 64];
+    __shared__ uint32_t wsum[BXB_BS / 64], wtot[2][BXB_BS / 64];
     __shared__ uint32_t total, nrows;
     __shared__ uint32_t rrow[MAXR];
     const uint32_t tid = threadIdx.x, q = blockIdx.x;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__res
     __syncthreads();
     const uint32_t nr = nrows;
     const uint64_t lim = n < budget ? n : budget, wlim = (lim + 63) / 64;
-    uint32_t found = 0;
+    uint32_t found = 0, step = 0;
     uint64_t scanned = lim;  // ids read when the scan stops
     for (uint64_t w0 = 0; w0 < wlim; w0 += BXB_BS * W) {
         const uint64_t wb = w0 + (uint64_t)tid * W;
@@ -406,9 +406,17 @@ __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__res
             if (wb + i == wlim - 1 && lim % 64) acc[i] &= (1ull << (lim % 64)) - 1ull;  // the last word may run past lim
             c += (uint32_t)__popcll(acc[i]);
         }
-        const uint32_t excl = block_excl_scan<BXB_BS>(c, wsum, &total);
-        const uint32_t tot = total;
+        // the step's count: wave sums through a double-buffered LDS row, one
+        // barrier; the prefix scan runs only in the step that reaches k
+        const uint32_t cw = wave_sum_u32(c);
+        if (lane_id() == 0) wtot[step & 1][tid / 64] = cw;
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int i = 0; i < BXB_BS / 64; i++) tot += wtot[step & 1][i];
+        step++;
         if (found + tot >= k) {  // uniform
+            const uint32_t excl = block_excl_scan<BXB_BS>(c, wsum, &total);
             if (found + excl < k && k <= found + excl + c) {  // this thread holds the k-th
                 uint32_t r = k - found - excl - 1;
 #pragma unroll
@@ -436,23 +444,20 @@ __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__res
         }
     }
     // The scan stopped short of k ties; the rate read projects the store's.
-    // Between 2k and BATCH_CAP / 2 of them: tau = (smax, every id) takes
-    // them all as candidates (a key no higher than the k-th when at least k
-    // exist).  Fewer: the predictor's mark was most likely wrong (about k
-    // objects or fewer reach smax), so the query keeps its learned score
-    // threshold.  More: a seed.  Every choice is exact: a threshold that
-    // leaves fewer than k candidates, or more than the list holds, sends
-    // the query to the second pass.
+    // At least 2k of them: tau = (smax, every id) takes them all as
+    // candidates (a key no higher than the k-th when at least k exist).
+    // Fewer: the predictor's mark was most likely wrong (about k objects or
+    // fewer reach smax), so the query keeps its learned score threshold.
+    // Either is exact: a threshold that leaves fewer than k candidates, or
+    // more than the list holds, sends the query to the second pass.
     if (tid == 0) {
         const uint64_t proj = (uint64_t)found * n;  // ties in the store x ids read
-        if (proj < 2ull * k * scanned) {
-            qs[q].seed = 0u;
-        } else if (proj <= (uint64_t)(BATCH_CAP / 2) * scanned) {
+        if (proj >= 2ull * k * scanned) {
             BatchQuery &Wq = qs[q];
             Wq.tau = Q.smax;
             Wq.tau_id = 0xFFFFFFFFu;
-            Wq.seed = 0u;
         }
+        qs[q].seed = 0u;
     }
 }
 
@@ -1906,11 +1911,12 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
             if (P.xbits) {
                 auto tk = ctx->d <= 16 ? batch_tiebits_kernel<16, 4> : batch_tiebits_kernel<32, 2>;
                 hipLaunchKernelGGL(tk, dim3(nq), dim3(BXB_BS), 0, s, dq, P.xbits, ctx->d, P.nw, ctx->n, BXB_BUDGET);
+            } else {  // no bitmaps (the layout was rebuilt after training): seeds
+                const int NB = dims_class(ctx->d);
+                auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
+                hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz,
+                                   P.pid, (const uint32_t *)nullptr, 1);
             }
-            const int NB = dims_class(ctx->d);
-            auto seed = NB == 8 ? batch_seed_kernel<8> : (NB == 16 ? batch_seed_kernel<16> : batch_seed_kernel<32>);
-            hipLaunchKernelGGL(seed, dim3(nq), dim3(BSEED_BS), 0, s, dq, db, ta, L, P.pcols, P.ld, P.poff, P.psz,
-                               P.pid, (const uint32_t *)nullptr, 1);
         }
     }
     timing_end(ctx, s,
