@@ -81,15 +81,38 @@ __global__ __launch_bounds__(256) void colminmax_kernel(const float *__restrict_
     }
 }
 
-// Per column minimum and range (max - min), computed on the host.
-struct NormArgs {
+// Per column minimum and range (max - min), and whether every column can
+// be normalised (computed on the device: no host round trip between the
+// statistics and the writes).
+struct NormParams {
     float mn[TOPK_MAX_DIMS];
     float mm[TOPK_MAX_DIMS];
+    uint32_t ok;
 };
 
-__global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t ld, NormArgs a) {
+// One wave; lane m < d takes column m.  A minimum of zero is read back from
+// the column's first zero, so its sign is the stored one.
+__global__ void norm_params_kernel(const float *__restrict__ cols, uint64_t ld, const uint32_t *__restrict__ k,
+                                   const unsigned long long *__restrict__ z, uint32_t d, NormParams *__restrict__ p) {
+    const uint32_t m = lane_id();
+    bool good = true;
+    if (m < d) {
+        float mn = okey_inv(k[d + m]);
+        if (mn == 0.0f && z[m] != ~0ull) mn = cols[(uint64_t)m * ld + z[m]];
+        const float mm = okey_inv(k[m]) - mn;
+        p->mn[m] = mn;
+        p->mm[m] = mm;
+        good = mm > 0.0f && mm <= 3.40282347e+38f;
+    }
+    const uint64_t bad = __ballot(!good);
+    if (m == 0) p->ok = bad == 0ull ? 1u : 0u;
+}
+
+// Writes nothing unless every column can be normalised (p->ok).
+__global__ void normalize_kernel(float *__restrict__ cols, uint64_t n, uint64_t ld, const NormParams *__restrict__ p) {
+    if (!p->ok) return;
     const uint32_t m = blockIdx.y;
-    const float mn = a.mn[m], mm = a.mm[m];
+    const float mn = p->mn[m], mm = p->mm[m];
     float *c = cols + (uint64_t)m * ld;
     const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x, stride = (uint64_t)gridDim.x * 256;
     const uint64_t n4 = n / 4;
@@ -146,15 +169,17 @@ __global__ void minmax_init_kernel(uint32_t *k, unsigned long long *z, uint32_t 
 // The store holds only finite values (checked when it is built), and
 // mx >= every value >= mn, so 0 <= x - mn <= mx - mn = mm for every x of a
 // column (binary32 subtraction is monotone): every (x - mn) / mm is finite
-// exactly when mm is finite and > 0.  The ranges are checked on the host
+// exactly when mm is finite and > 0.  The ranges are checked on the device
 // before anything is written, so a column that cannot be normalised
 // (constant, or a range past FLT_MAX) leaves the whole store untouched.
 static int normalize_store(topk_ctx *ctx) {
     hipStream_t s = ctx->stream;
     const uint32_t d = ctx->d;
     unsigned long long *z = nullptr;
-    TK_HIP(hipMalloc(&z, sizeof(unsigned long long) * d + sizeof(uint32_t) * 2 * d));
-    uint32_t *k = reinterpret_cast<uint32_t *>(z + d);
+    TK_HIP(hipMalloc(&z, sizeof(unsigned long long) * TOPK_MAX_DIMS + sizeof(uint32_t) * 2 * TOPK_MAX_DIMS +
+                             sizeof(NormParams)));
+    uint32_t *k = reinterpret_cast<uint32_t *>(z + TOPK_MAX_DIMS);
+    NormParams *np = reinterpret_cast<NormParams *>(k + 2 * TOPK_MAX_DIMS);
     hipEvent_t ev0 = nullptr;
     timing_begin(ctx, s, ev0);
     hipLaunchKernelGGL(minmax_init_kernel, dim3(1), dim3(TOPK_MAX_DIMS), 0, s, k, z, d);
@@ -163,28 +188,18 @@ static int normalize_store(topk_ctx *ctx) {
         hipLaunchKernelGGL(colminmax_kernel, dim3(512, d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, k, k + d, z);
         e = hipGetLastError();
     }
-    std::vector<uint32_t> hk(2 * d);
-    std::vector<unsigned long long> hz(d);
-    if (e == hipSuccess) e = hipMemcpyAsync(hk.data(), k, sizeof(uint32_t) * 2 * d, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hz.data(), z, sizeof(unsigned long long) * d, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    NormArgs a;
-    bool ok = true;
-    for (uint32_t m = 0; e == hipSuccess && m < d; m++) {
-        float mn = okey_inv(hk[d + m]);
-        if (mn == 0.0f && hz[m] != ~0ull)  // the first zero of the column, with its sign
-            e = hipMemcpy(&mn, ctx->cols + (uint64_t)m * ctx->ld + hz[m], sizeof(float), hipMemcpyDeviceToHost);
-        const float mm = okey_inv(hk[m]) - mn;
-        a.mn[m] = mn;
-        a.mm[m] = mm;
-        ok = ok && mm > 0.0f && mm <= 3.40282347e+38f;
-    }
-    if (e == hipSuccess && ok) {
-        hipLaunchKernelGGL(normalize_kernel, dim3(2048, d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, a);
-        timing_end(ctx, s, "normalize", ev0);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(norm_params_kernel, dim3(1), dim3(64), 0, s, ctx->cols, ctx->ld, k, z, d, np);
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
     }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(normalize_kernel, dim3(2048, d), dim3(256), 0, s, ctx->cols, ctx->n, ctx->ld, np);
+        e = hipGetLastError();
+    }
+    timing_end(ctx, s, "normalize", ev0);
+    uint32_t ok = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&ok, &np->ok, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     hipFree(z);
     TK_HIP(e);
     return ok ? TOPK_OK : TOPK_ERR_NONFINITE;
