@@ -9,7 +9,7 @@ import oracle
 import topk_amd
 from topk_amd._lib import MLP_NPARAMS
 
-from test_gpu_batch import make_queries, oracle_batch
+from test_gpu_batch import check_batch, make_queries, oracle_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -304,3 +304,29 @@ def test_learned_state_errors(gpu):
         with pytest.raises(topk_amd.TopkError) as e:
             eng.learned_train([(4, np.ones(20, np.float32), [0, 1])] * 20)
         assert e.value.status == 4  # d > 16
+
+
+def test_extreme_ties_mixed_signs_all_modes(gpu):
+    """Values in {0, 1/2, 1}: many objects sit at both ends of every column,
+    so keys at the largest score (the corner each weight's sign prefers:
+    maxima for w > 0, minima for w < 0) prune by id and the learned mode's
+    extreme-value bitmaps use both sides.  Seed, learned and query-major
+    batches against the oracle, with mixed-sign weights."""
+    n, d = 300_007, 8
+    rng = np.random.default_rng(808)
+    x = (rng.integers(0, 3, size=(d, n)) / 2.0).astype(np.float32)
+    with topk_amd.Engine(x) as eng:
+        qs = make_queries(rng, 192, d, [1, 16, 64, 256], mmin=1, mmax=4, wlo=-1.0, whi=1.0)
+        eng.set_tuning("batch_mode", 1)
+        check_batch(eng, x, n, qs)
+        rep = eng.learned_train(make_queries(rng, 3000, d, [1, 16, 64, 256], mmin=1, mmax=4, wlo=-1.0, whi=1.0),
+                                epochs=8)
+        assert rep["saturated"] == 1  # the all-positive queries tie at their maximum
+        eng.set_tuning("batch_mode", 2)
+        check_batch(eng, x, n, qs)
+        eng.set_tuning("batch_fused", 2)
+        check_batch(eng, x, n, qs)
+        eng.set_tuning("batch_mode", 1)
+        check_batch(eng, x, n, qs)
+        eng.set_tuning("batch_fused", 0)
+        eng.set_tuning("batch_mode", 0)
