@@ -29,7 +29,8 @@ the command the FTE rocprofv3 summaries under profiles/ were taken with.
 kernels`` BTA, GPA, radix select, normalisation and reorder_attr, and
 ``--mode configs`` the other workloads (PTA at N = 2^28, batched and learned
 d = 16, ingest, host-memory store); ``--mode train`` the predictor's training
-alone on the headline store (the command of the training kernel summary).
+alone on the headline store (the command of the training kernel summary),
+then the same training at mini-batches of 1024, 2048 and 4096 samples.
 """
 import argparse
 import json
@@ -895,9 +896,18 @@ def main():
     eng = topk_amd.Engine.generated(n, D, topk_amd.INDEPENDENT, 0)
     if args.mode == "train":
         eng.init(topk_amd.TOPK_INIT_PTA)
-        rep = eng.learned_train(random_queries(np.random.default_rng(2026), args.train_queries, k))
+        train = random_queries(np.random.default_rng(2026), args.train_queries, k)
+        rep = eng.learned_train(train)
+        # the same training at other mini-batch sizes: step rate against
+        # the fit (calibration error and margin) it buys
+        sweep = {}
+        for b in (1024, 2048, 4096):
+            r = eng.learned_train(train, batch=b)
+            sweep[str(b)] = {"margin": r["margin"], "calib_mse": r["calib_mse"], "train_s": r["ms_train"] / 1e3,
+                             **train_step_stats(r)}
         print(json.dumps({"train_n%d_d8_k%d" % (n, k): dict(rep, queries=args.train_queries,
-                                                             **train_step_stats(rep))}), flush=True)
+                                                             **train_step_stats(rep), batch_sweep=sweep)}),
+              flush=True)
         eng.close()
         return
     base = {"metric": METRIC, "unit": "queries/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",

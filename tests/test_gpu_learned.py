@@ -171,10 +171,13 @@ def test_learned_beats_seed_on_saturated_mixed_store(gpu):
             eng.set_tuning("batch_mode", mode)
             eng.batch_run(pb)  # warm: graph capture
             eng.timing(True)
-            eng.timing_reset()
-            eng.batch_run(pb)
+            best = float("inf")
+            for _ in range(3):  # best of three event-timed runs
+                eng.timing_reset()
+                eng.batch_run(pb)
+                best = min(best, eng.timing_get(f"batch_tau_{name}")[0])
             pairs[mode] = eng.stats()["tuple_count"]
-            tau_ms[mode] = eng.timing_get(f"batch_tau_{name}")[0]
+            tau_ms[mode] = best
             eng.timing(False)
             res[mode] = [pb.result(i, n) for i in range(len(held))]
         eng.set_tuning("batch_mode", 0)
@@ -186,7 +189,7 @@ def test_learned_beats_seed_on_saturated_mixed_store(gpu):
             assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)) and gt == rt, i
     print("scored pairs seed/learned", pairs[1], pairs[2], "threshold stage ms seed/learned", tau_ms[1], tau_ms[2])
     assert pairs[2] <= pairs[1]
-    assert 1.5 * tau_ms[2] < tau_ms[1]  # measured 2.6x
+    assert 3.0 * tau_ms[2] < tau_ms[1]  # measured 3.3x
 
 
 @pytest.fixture(scope="module")

@@ -359,9 +359,25 @@ __global__ __launch_bounds__(BSEED_BS) void batch_seed_kernel(BatchQuery *__rest
 // ties as candidates or their learned score threshold, by the projected
 // number of ties (end of the kernel).  No query leaves it marked.
 constexpr uint64_t BXB_BUDGET = 1ull << 20;  // ids a scan may cover
-constexpr int BXB_BS = 512;                  // threads per scan workgroup
-// MAXR bounds the query's rows (terms of nonzero weight; 16 or 32); a
-// thread takes W words of every row per step, all loads in flight at once.
+constexpr int BXB_BS = 256;                  // threads per scan workgroup
+constexpr int BXB_R0 = 4;                    // rows ANDed for every word
+// W words of one row from 16-byte loads (rows are 128-byte aligned and w
+// is a multiple of W).
+template <int W>
+__device__ __forceinline__ void load_words(const uint64_t *__restrict__ p, uint64_t (&x)[W]) {
+    static_assert(W % 2 == 0, "words per thread");
+#pragma unroll
+    for (int v = 0; v < W / 2; v++) {
+        const ulonglong2 t = reinterpret_cast<const ulonglong2 *>(p)[v];
+        x[2 * v] = t.x;
+        x[2 * v + 1] = t.y;
+    }
+}
+// MAXR bounds the query's rows (terms of nonzero weight; 16 or 32).  Per
+// step a thread ANDs the first BXB_R0 rows of its W words, then reads the
+// other rows only if that partial AND is nonzero -- past a few attributes
+// most words hold no common tie, and their remaining rows are never
+// fetched.  The rows of each phase are loaded all at once.
 template <int MAXR, int W>
 __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__restrict__ qs,
                                                             const uint64_t *__restrict__ xbits, uint32_t d,
@@ -380,29 +396,62 @@ __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__res
             if (w != 0.0f) rrow[c++] = w > 0.0f ? Q.attr[t] : d + Q.attr[t];
         }
         nrows = c;
-        for (uint32_t r = c; r < MAXR; r++) rrow[r] = 0u;  // unused rows: row 0, never ANDed
+        for (uint32_t r = c; r < MAXR; r++) rrow[r] = 0u;  // unused rows: never loaded
     }
     __syncthreads();
-    const uint32_t nr = nrows;
+    const uint32_t nr = __builtin_amdgcn_readfirstlane(nrows);
+    const uint64_t *rowp[MAXR];  // uniform: scalar registers
+#pragma unroll
+    for (int r = 0; r < MAXR; r++) rowp[r] = xbits + (uint64_t)__builtin_amdgcn_readfirstlane(rrow[r]) * nw;
     const uint64_t lim = n < budget ? n : budget, wlim = (lim + 63) / 64;
     uint32_t found = 0, step = 0;
     uint64_t scanned = lim;  // ids read when the scan stops
     for (uint64_t w0 = 0; w0 < wlim; w0 += BXB_BS * W) {
         const uint64_t wb = w0 + (uint64_t)tid * W;
-        uint64_t x[MAXR][W];
-#pragma unroll
-        for (int r = 0; r < MAXR; r++) {
-            const uint64_t *row = xbits + (uint64_t)rrow[r] * nw;
-#pragma unroll
-            for (int i = 0; i < W; i++) x[r][i] = r < (int)nr && wb + i < wlim ? row[wb + i] : ~0ull;
-        }
+        // a thread past wlim reads words 0.. (in bounds) and masks them out;
+        // words in [wlim, wb + W) lie inside the padded row
+        const uint64_t wr = wb < wlim ? wb : 0;
         uint64_t acc[W];
+#pragma unroll
+        for (int i = 0; i < W; i++) acc[i] = wb + i < wlim ? ~0ull : 0ull;
+        {
+            uint64_t x[BXB_R0][W];
+#pragma unroll
+            for (int r = 0; r < BXB_R0; r++) {
+                if ((uint32_t)r < nr) {
+                    load_words<W>(rowp[r] + wr, x[r]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < W; i++) x[r][i] = ~0ull;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < BXB_R0; r++)
+#pragma unroll
+                for (int i = 0; i < W; i++) acc[i] &= x[r][i];
+        }
+        uint64_t any = 0;
+#pragma unroll
+        for (int i = 0; i < W; i++) any |= acc[i];
+        if (nr > (uint32_t)BXB_R0 && any) {
+            uint64_t x[MAXR - BXB_R0][W];
+#pragma unroll
+            for (int r = BXB_R0; r < MAXR; r++) {
+                if ((uint32_t)r < nr) {
+                    load_words<W>(rowp[r] + wr, x[r - BXB_R0]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < W; i++) x[r - BXB_R0][i] = ~0ull;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < MAXR - BXB_R0; r++)
+#pragma unroll
+                for (int i = 0; i < W; i++) acc[i] &= x[r][i];
+        }
         uint32_t c = 0;
 #pragma unroll
         for (int i = 0; i < W; i++) {
-            acc[i] = wb + i < wlim ? ~0ull : 0ull;
-#pragma unroll
-            for (int r = 0; r < MAXR; r++) acc[i] &= x[r][i];
             if (wb + i == wlim - 1 && lim % 64) acc[i] &= (1ull << (lim % 64)) - 1ull;  // the last word may run past lim
             c += (uint32_t)__popcll(acc[i]);
         }

@@ -252,7 +252,7 @@ struct PtaLayout {
     // word w of row a says whether object 64 w + i holds the store's largest
     // value of attribute a (rows 0..d-1) or its smallest (rows d..2d-1).
     uint64_t *xbits = nullptr;
-    uint64_t nw = 0;             // words per row: ceil(n / 64)
+    uint64_t nw = 0;             // words per row: ceil(n / 64) rounded up to 16 (128-byte rows; the padding is 0)
     // Chunk statistics: positions [c * PTA_CHUNK, (c + 1) * PTA_CHUNK) form
     // chunk c (partitions start on chunk boundaries); the crow = 2d + 2
     // statistics of chunk c -- maxima, minima, largest attribute sum (rounded

@@ -482,7 +482,7 @@ int pta_extreme_bits(topk_ctx *ctx) {
     PtaLayout &P = ctx->pta;
     if (!P.built) return TOPK_ERR_STATE;
     if (P.xbits) return TOPK_OK;
-    const uint64_t nw = (ctx->n + 63) / 64;
+    const uint64_t nw = ((ctx->n + 63) / 64 + 15) & ~15ull;  // padding words hold ids >= n: zero
     TK_HIP(hipMalloc(&P.xbits, sizeof(uint64_t) * 2 * ctx->d * nw));
     P.nw = nw;
     const uint64_t waves = (uint64_t)ctx->d * nw;
