@@ -829,6 +829,14 @@ __global__ __launch_bounds__(BS) void batch_score_kernel(ScoreArgs a) {
                 // lanes)
                 bool pass = alive;
                 if constexpr (LANE_W) {
+                    // the lane's weights and attribute bytes stay in this
+                    // compact form across tiles: the bound's FP64 weights and
+                    // shuffle addresses are rebuilt per tile, not kept live
+                    // (at D = 16: 154 registers instead of 199, three waves
+                    // per SIMD instead of two)
+#pragma unroll
+                    for (int i = 0; i < D / 4; i++) asm volatile("" : "+v"(lws[i]));
+                    asm volatile("" : "+v"(las.x), "+v"(las.y), "+v"(las.z), "+v"(las.w));
                     const uint32_t dd = a.d;
                     const uint32_t c0 = (start + wb) / PTA_CHUNK;
                     const float cs = lane < a.crow ? a.cstat[(uint64_t)lane * a.nchunks + c0] : 0.0f;  // lane e: element e

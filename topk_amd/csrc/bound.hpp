@@ -72,22 +72,21 @@ inline BoundArgs make_bound_args(uint32_t d, uint32_t qq, const float *weights, 
 template <int NMAX>
 __device__ __forceinline__ float lp_bound_regs(const double (&wv)[NMAX], const float (&mxv)[NMAX],
                                                const float (&mnv)[NMAX], float psum, float pminsum) {
-    double base = 0, box = 0, absw = 0;
+    // one pass: every attribute adds w * min to the knapsack value, and a
+    // positive weight then takes its room above the minimum while the sum
+    // budget lasts (the weights are sorted heaviest first, so the positive
+    // ones form a prefix and the budget only shrinks); an attribute's
+    // operands are dead once it is added
+    double base = 0, box = 0, absw = 0, budget = (double)psum - (double)pminsum;
 #pragma unroll
     for (int i = 0; i < NMAX; i++) {
         const double mx = mxv[i], mn = mnv[i], w = wv[i];
         base += w * mn;
         box += w * (w >= 0 ? mx : mn);
         absw += fabs(w) * fmax(fabs(mx), fabs(mn));
-    }
-    // fractional knapsack: the weights are sorted heaviest first, so the
-    // positive ones form a prefix; the budget only shrinks
-    double budget = (double)psum - (double)pminsum;
-#pragma unroll
-    for (int i = 0; i < NMAX; i++) {
-        if (wv[i] > 0 && budget > 0) {
-            const double room = fmin((double)mxv[i] - (double)mnv[i], budget);
-            base += wv[i] * room;
+        if (w > 0 && budget > 0) {
+            const double room = fmin(mx - mn, budget);
+            base += w * room;
             budget -= room;
         }
     }
