@@ -1667,6 +1667,7 @@ struct BatchState {
     int next = 0;                    // slot the next submit uses
     uint64_t next_ticket = 1;
     int cus = 0;
+    int chunk_wgs = 0;               // batch_chunk_kernel workgroups resident per CU (occupancy query)
     std::map<GraphKey, hipGraphExec_t> graphs;
     // pinned-memory answers for recently seen output arrays (callers reuse them)
     std::map<const void *, bool> pinned;
@@ -1883,6 +1884,11 @@ static int ensure(topk_ctx *ctx, BatchState &B, uint32_t nq, uint32_t ks, int si
     if (!B.cus) {
         TK_HIP(hipDeviceGetAttribute(&B.cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
         if (B.cus < 1) B.cus = 1;
+        // the chunk kernel splits its chunks evenly over a grid that must be
+        // resident at once: as many workgroups per CU as its registers allow
+        int nb = 0;
+        TK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_chunk_kernel, BC_BS, 0));
+        B.chunk_wgs = nb < 1 ? 1 : (nb > 8 ? 8 : nb);
     }
     if (changed) drop_graphs(B);
     return TOPK_OK;
@@ -2015,7 +2021,7 @@ static int enqueue_pass(topk_ctx *ctx, BatchState &B, hipStream_t s, uint32_t nq
         auto ck = NB == 8 ? batch_cut_kernel<8> : (NB == 16 ? batch_cut_kernel<16> : batch_cut_kernel<32>);
         hipLaunchKernelGGL(ck, dim3(gx, S), dim3(BQ_BS), 0, s, ca);
         hipLaunchKernelGGL(batch_unit_scan_kernel, dim3(1), dim3(1024), 0, s, B.slot[B.cur].d_ucnt, nq * S, fl, S, B.slot[B.cur].d_uoff);
-        const uint32_t g2 = (uint32_t)B.cus * (second ? 2 : 7);  // 7 resident 4-wave workgroups per CU
+        const uint32_t g2 = (uint32_t)B.cus * (second ? 2 : (uint32_t)B.chunk_wgs);  // all resident at once
         const ChunkArgs xa{dq, P.pcols, P.ld, P.poff, P.psz, P.pid, B.slot[B.cur].d_tasks, ucap, B.slot[B.cur].d_ucnt, B.slot[B.cur].d_uoff, nq * S, S,
                            fl, B.slot[B.cur].d_cand, B.slot[B.cur].d_ccount, B.slot[B.cur].d_scored, P.cminid};
         hipEvent_t ev0 = nullptr;
