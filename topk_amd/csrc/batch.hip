@@ -1414,7 +1414,10 @@ constexpr uint32_t BC_STAGE = 256;  // candidate keys a wave stages in LDS befor
 __global__ __launch_bounds__(BC_BS) void batch_chunk_kernel(ChunkArgs a) {
     __shared__ unsigned long long wpairs[BC_BS / 64][2];
     __shared__ uint64_t stage[BC_BS / 64][BC_STAGE];
-    const uint32_t lane = lane_id(), wave = threadIdx.x / 64;
+    // the wave index made visibly uniform: the whole chunk cursor (unit,
+    // entry, partition, query) then lives in scalar registers and its loads
+    // are scalar (62 vector registers instead of 76: eight waves per SIMD)
+    const uint32_t lane = lane_id(), wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const uint32_t nwaves = gridDim.x * (BC_BS / 64), gw = blockIdx.x * (BC_BS / 64) + wave;
     const uint32_t nu = a.flist ? a.flist[0] * a.S : a.nu;
     const uint32_t total = a.uoff[nu];
