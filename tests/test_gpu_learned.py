@@ -235,6 +235,23 @@ def test_learned_gain_over_seed(gpu, trained_unsaturated):
     assert s_lrn["partitions_scored"] <= 1.15 * s_seed["partitions_scored"]
 
 
+def test_inference_matches_calibration_forward(gpu, trained):
+    """Batch inference (mlp_tau_kernel) runs the matrix-core forward of the
+    training and calibration pass: its predictions equal topk_mlp_eval on
+    the device-built features of the same queries, bit for bit (so the
+    calibrated margin describes exactly the predictions batches use)."""
+    eng, _, _, _, test = trained
+    params, _ = eng.learned_get()
+    x, sc = eng.learned_features(test)
+    y, _, _ = eng.mlp_eval(params, x)
+    pred, _ = eng.learned_predict(test)
+    want = (y * sc).astype(np.float32)
+    assert np.array_equal(pred.view(np.uint32), want.view(np.uint32))
+    # a partial last tile (nq not a multiple of 16) takes the same values
+    pred13, _ = eng.learned_predict(test[:13])
+    assert np.array_equal(pred13.view(np.uint32), want[:13].view(np.uint32))
+
+
 def test_learned_thresholds_below_kth(gpu, trained):
     eng, _, _, _, test = trained
     pred, tau = eng.learned_predict(test)

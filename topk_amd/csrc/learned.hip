@@ -50,7 +50,6 @@ constexpr int MLP_BS = 256;
 constexpr int MLP_TR = 16;     // samples per forward / backward workgroup
 constexpr uint32_t MLP_MAX_BATCH = 4096;       // training mini-batch cap: batch / 16 co-resident workgroups
 constexpr uint32_t MLP_EPOCH_ROWS = 1u << 24;  // row-order entries per training launch
-constexpr int TAU_BS = 256;    // inference workgroup: 4 waves, one query per wave at a time
 
 struct Sketch {
     float q[LRN_MAX_D][4];     // quantiles 0.5, 0.9, 0.99, 0.999 of a 1% sample, per attribute
@@ -541,65 +540,91 @@ __global__ __launch_bounds__(MLP_BS) void mlp_train_kernel(MlpTrainArgs a) {
 
 // Threshold per batch query: tau = (prediction - margin) * sum |W| (or
 // -inf when the weights are all zero).  pred (optional) gets the unlowered
-// prediction in score units.  One wave per query: every lane builds the
-// feature vector in registers, lane j owns hidden unit j of both layers and
-// the second layer reads the first through lane shuffles (no barrier after
-// the parameter load, so waves may take different numbers of queries).
-__global__ __launch_bounds__(TAU_BS) void mlp_tau_kernel(const float *__restrict__ P, BatchQuery *__restrict__ qs,
+// prediction in score units.  One workgroup per 16 queries: lanes 0..15 of
+// the first wave build the queries' feature rows in the LDS tile while
+// every wave loads its B operands, then the forward pass runs on FP32
+// matrix cores with exactly the operation sequence of mlp_tile's forward
+// part -- the predictions equal those the calibration computed (mlp_pass
+// over the same features), bit for bit, and the weights are read once per
+// 16 queries instead of once per query.
+__global__ __launch_bounds__(MLP_BS) void mlp_tau_kernel(const float *__restrict__ P, BatchQuery *__restrict__ qs,
                                                          uint32_t nq, Sketch S, float margin,
                                                          float *__restrict__ pred, float retry_width,
                                                          int saturated) {
-    // Lane j owns hidden unit j of both layers: its rows of W1 and W2 (stored
-    // row-major, row j = unit j) come straight from memory as 16-byte loads,
-    // all in flight at once -- no LDS staging, no barrier -- and the second
-    // layer reads the first through lane shuffles.
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid / 64;
-    const uint32_t nw = gridDim.x * (TAU_BS / 64);
-    const uint32_t q0 = blockIdx.x * (TAU_BS / 64) + wv;
-    if (q0 >= nq) return;
-    float w1[MLP_IN], w2[MLP_H];
+    __shared__ MlpTile T;
+    __shared__ float fsc[MLP_TR], fmx[MLP_TR];  // per query: sum |W| and feature 91 (largest score / sum |W|)
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid / 64;
+    const uint32_t c = lane & 15, g = lane >> 4;
+    const uint32_t unit = 16 * w + c;
+    const uint32_t base = blockIdx.x * MLP_TR;
+    const uint32_t nr = nq - base < MLP_TR ? nq - base : MLP_TR;
+    f4 w1[6], w2[4];
+    {
+        const f4 *p1 = reinterpret_cast<const f4 *>(P + OFF_W1 + unit * MLP_IN + 24 * g);
+        const f4 *p2 = reinterpret_cast<const f4 *>(P + OFF_W2 + unit * MLP_H + 16 * g);
 #pragma unroll
-    for (int i = 0; i < MLP_IN / 4; i++) {
-        const float4 v = reinterpret_cast<const float4 *>(P + OFF_W1 + lane * MLP_IN)[i];
-        w1[4 * i] = v.x;
-        w1[4 * i + 1] = v.y;
-        w1[4 * i + 2] = v.z;
-        w1[4 * i + 3] = v.w;
+        for (int t = 0; t < 6; t++) w1[t] = p1[t];
+#pragma unroll
+        for (int t = 0; t < 4; t++) w2[t] = p2[t];
     }
+    const float b1 = P[OFF_B1 + unit], b2 = P[OFF_B2 + unit], w3 = P[OFF_W3 + unit], b3 = P[OFF_B3];
+    if (tid < MLP_TR) {
+        float f[MLP_IN], sc = 0.0f;
+        if (tid < nr) {
+            mlp_features(qs[base + tid], S, f, &sc);
+        } else {
 #pragma unroll
-    for (int i = 0; i < MLP_H / 4; i++) {
-        const float4 v = reinterpret_cast<const float4 *>(P + OFF_W2 + lane * MLP_H)[i];
-        w2[4 * i] = v.x;
-        w2[4 * i + 1] = v.y;
-        w2[4 * i + 2] = v.z;
-        w2[4 * i + 3] = v.w;
-    }
-    const float b1 = P[OFF_B1 + lane], b2 = P[OFF_B2 + lane], w3 = P[OFF_W3 + lane], b3 = P[OFF_B3];
-    for (uint32_t q = q0; q < nq; q += nw) {
-        float f[MLP_IN], sc;
-        mlp_features(qs[q], S, f, &sc);
-        float s = b1;
-#pragma unroll
-        for (int i = 0; i < MLP_IN; i++) s += f[i] * w1[i];
-        const float h1 = s > 0.0f ? s : 0.0f;
-        float t = b2;
-#pragma unroll
-        for (int i = 0; i < MLP_H; i++) t += __shfl(h1, i, 64) * w2[i];
-        t = t > 0.0f ? t : 0.0f;
-        float o = w3 * t;
-        for (int x = 32; x > 0; x >>= 1) o += __shfl_xor(o, x, 64);
-        if (lane == 0) {
-            const float y = o + b3;
-            qs[q].tau = sc > 0.0f ? (y - margin) * sc : -INFINITY;
-            // a miss reruns retry_width margins below the prediction, or
-            // (retry_width < 0) with a seed threshold
-            const float wide = retry_width * fmaxf(fabsf(margin), 1e-3f);  // strictly below tau
-            qs[q].tau_retry = retry_width < 0.0f ? INFINITY : (sc > 0.0f ? (y - wide) * sc : -INFINITY);
-            // within two margins of the query's largest score (feature 91):
-            // ties likely, seed this query
-            qs[q].seed = saturated && sc > 0.0f && y + 2.0f * fabsf(margin) >= f[91] ? 1u : 0u;
-            if (pred) pred[q] = y * sc;
+            for (int i = 0; i < MLP_IN; i++) f[i] = 0.0f;  // rows past nr: zero, as mlp_tile pads them
         }
+#pragma unroll
+        for (int i = 0; i < MLP_IN; i += 4) *reinterpret_cast<f4 *>(&T.X[tid][i]) = f4{f[i], f[i + 1], f[i + 2], f[i + 3]};
+        fsc[tid] = sc;
+        fmx[tid] = f[91];
+    }
+    __syncthreads();
+    // layer 1 (K = 96), as in mlp_tile
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+        const f4 a = *reinterpret_cast<const f4 *>(&T.X[c][24 * g + 4 * t]);
+#pragma unroll
+        for (int e = 0; e < 4; e++) acc = mfma4(a[e], w1[t][e], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const float v = acc[r] + b1;
+        T.H1[4 * g + r][unit] = v > 0.0f ? v : 0.0f;
+    }
+    __syncthreads();
+    // layer 2 (K = 64) and the output's partial sums, as in mlp_tile
+    acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const f4 a = *reinterpret_cast<const f4 *>(&T.H1[c][16 * g + 4 * t]);
+#pragma unroll
+        for (int e = 0; e < 4; e++) acc = mfma4(a[e], w2[t][e], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const float v = acc[r] + b2;
+        const float h2 = v > 0.0f ? v : 0.0f;
+        const float p = row_sum16(w3 * h2);
+        if (c == 0) T.part[w][4 * g + r] = p;
+    }
+    __syncthreads();
+    if (tid < nr) {
+        const uint32_t q = base + tid;
+        const float y = ((T.part[0][tid] + T.part[1][tid]) + (T.part[2][tid] + T.part[3][tid])) + b3;
+        const float sc = fsc[tid];
+        qs[q].tau = sc > 0.0f ? (y - margin) * sc : -INFINITY;
+        // a miss reruns retry_width margins below the prediction, or
+        // (retry_width < 0) with a seed threshold
+        const float wide = retry_width * fmaxf(fabsf(margin), 1e-3f);  // strictly below tau
+        qs[q].tau_retry = retry_width < 0.0f ? INFINITY : (sc > 0.0f ? (y - wide) * sc : -INFINITY);
+        // within two margins of the query's largest score (feature 91):
+        // ties likely, seed this query
+        qs[q].seed = saturated && sc > 0.0f && y + 2.0f * fabsf(margin) >= fmx[tid] ? 1u : 0u;
+        if (pred) pred[q] = y * sc;
     }
 }
 
@@ -711,11 +736,10 @@ bool learned_saturated(const topk_ctx *ctx) {
 int learned_tau_enqueue(topk_ctx *ctx, hipStream_t s, BatchQuery *q, uint32_t nq) {
     LearnedState *L = static_cast<LearnedState *>(ctx->learned);
     if (!L || !L->ready) return TOPK_ERR_STATE;
-    uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
-    if (g > 1024) g = 1024;
+    const uint32_t g = (nq + MLP_TR - 1) / MLP_TR;
     const float width =
         ctx->tune.batch_retry == 2 ? -1.0f : (ctx->tune.batch_retry_width ? (float)ctx->tune.batch_retry_width : 3.0f);
-    hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, s, L->d_params, q, nq, L->sk, L->margin,
+    hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(MLP_BS), 0, s, L->d_params, q, nq, L->sk, L->margin,
                        (float *)nullptr, width, L->saturated ? 1 : 0);
     TK_HIP(hipGetLastError());
     return TOPK_OK;
@@ -1119,9 +1143,8 @@ int topk_learned_predict(topk_ctx *ctx, uint32_t nq, const topk_query *qs, float
     TK_HIP(hipMalloc(&B.q, sizeof(BatchQuery) * nq));
     TK_HIP(hipMalloc(&B.out, sizeof(float) * nq));
     TK_HIP(hipMemcpy(B.q, hq.data(), sizeof(BatchQuery) * nq, hipMemcpyHostToDevice));
-    uint32_t g = (nq + TAU_BS / 64 - 1) / (TAU_BS / 64);
-    if (g > 1024) g = 1024;
-    hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(TAU_BS), 0, ctx->stream, L->d_params, B.q, nq, L->sk, L->margin,
+    const uint32_t g = (nq + MLP_TR - 1) / MLP_TR;
+    hipLaunchKernelGGL(mlp_tau_kernel, dim3(g), dim3(MLP_BS), 0, ctx->stream, L->d_params, B.q, nq, L->sk, L->margin,
                        B.out, 3.0f, 0);
     TK_HIP(hipGetLastError());
     TK_HIP(hipStreamSynchronize(ctx->stream));

@@ -909,7 +909,9 @@ __global__ __launch_bounds__(PTA_BS) void pta_score_kernel(QCols q, uint32_t L, 
     __shared__ uint32_t wtot[PTA_BS / 64];
     __shared__ uint32_t wrep[PTA_BS / 64][2];
     __shared__ uint32_t lcount;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
+    // the wave index made visibly uniform: the chunk cursor and its loads
+    // (group offsets, cuts, partition offsets) are then scalar
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = __builtin_amdgcn_readfirstlane(tid / 64);
     // exclusive scan of the group totals: thread t owns groups [16t, 16t + 16)
     {
         uint32_t v[16], sum = 0, pp = 0, oo = 0;
