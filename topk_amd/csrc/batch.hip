@@ -61,7 +61,11 @@ namespace topk {
 constexpr uint32_t BATCH_MAX_Q = 16384;   // queries per call
 constexpr uint32_t BATCH_CAP = 32768;     // candidate keys per query
 constexpr uint32_t BATCH_MAX_GRID = 65536;  // persistent scoring workgroups
-constexpr int BFIN_BS = 512, BFIN_KPT = 16;  // lists up to 8192 keys are selected in registers, longer ones in memory
+// lists up to 8192 keys are selected in registers, longer ones in memory;
+// 256 threads x 32 keys (127 registers) keep a 1024-query batch's
+// workgroups resident at once -- 512 x 16 needed eight waves per SIMD where
+// five fit, and ran in two rounds
+constexpr int BFIN_BS = 256, BFIN_KPT = 32;
 constexpr int BSEED_BS = 512, BSEED_KPT = 8;
 constexpr uint32_t BSEED_T = 4;                          // seed partitions per query
 // objects a seed scores (4096): the leading objects of each seed partition,
