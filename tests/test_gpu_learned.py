@@ -189,7 +189,9 @@ def test_learned_beats_seed_on_saturated_mixed_store(gpu):
             assert np.array_equal(gi, ri) and np.array_equal(gs.view(np.uint32), rs.view(np.uint32)) and gt == rt, i
     print("scored pairs seed/learned", pairs[1], pairs[2], "threshold stage ms seed/learned", tau_ms[1], tau_ms[2])
     assert pairs[2] <= pairs[1]
-    assert 3.0 * tau_ms[2] < tau_ms[1]  # measured 3.3x
+    # measured 2.0x against the 256-thread seed kernel (3.3x against the
+    # 512-thread one it replaced, whose own stage was 1.6x longer)
+    assert 1.6 * tau_ms[2] < tau_ms[1]
 
 
 @pytest.fixture(scope="module")

@@ -66,7 +66,9 @@ constexpr uint32_t BATCH_MAX_GRID = 65536;  // persistent scoring workgroups
 // workgroups resident at once -- 512 x 16 needed eight waves per SIMD where
 // five fit, and ran in two rounds
 constexpr int BFIN_BS = 256, BFIN_KPT = 32;
-constexpr int BSEED_BS = 512, BSEED_KPT = 8;
+// 256 threads x 16 objects: a batch's 1024 seed workgroups are resident at
+// once at every d (512 x 8 needed eight waves per SIMD, five fit at d = 16)
+constexpr int BSEED_BS = 256, BSEED_KPT = 16;
 constexpr uint32_t BSEED_T = 4;                          // seed partitions per query
 // objects a seed scores (4096): the leading objects of each seed partition,
 // which hold its largest attribute sums
