@@ -140,6 +140,8 @@ static_assert(offsetof(BatchQuery, tau) % 16 == 0 && offsetof(BatchQuery, tau_id
                   offsetof(BatchQuery, m) == offsetof(BatchQuery, tau) + 12,
               "tau, tau_id, ascending, m form one 16-byte word");
 static_assert(offsetof(BatchQuery, ws) % 16 == 0 && offsetof(BatchQuery, as) % 16 == 0, "16-byte loads");
+static_assert(offsetof(BatchQuery, w) % 16 == 0 && offsetof(BatchQuery, attr) % 16 == 0 && TOPK_MAX_DIMS % 16 == 0,
+              "the term weights and attributes are read with 16-byte loads");
 
 // Expands CASE(M) for the query width m: M = 1..16 unrolled, 0 = runtime width.
 #define TK_SWITCH_M(m, CASE)          \

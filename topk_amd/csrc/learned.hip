@@ -84,10 +84,34 @@ __device__ void mlp_features(const BatchQuery &Q, const Sketch &S, float *f, flo
     float wa[LRN_MAX_D];
 #pragma unroll
     for (int a = 0; a < LRN_MAX_D; a++) wa[a] = 0.0f;
-    for (uint32_t j = 0; j < Q.m; j++) {
-        const uint32_t at = Q.attr[j];
+    // every term slot's weight and attribute read at once (16-byte loads; a
+    // loop bounded by m would wait on memory once per term), then the
+    // weights accumulated per attribute in term order
+    const uint32_t m = Q.m;
+    float wt[TOPK_MAX_DIMS];
+    uint32_t ap[TOPK_MAX_DIMS / 4];  // four attribute bytes per word
 #pragma unroll
-        for (int a = 0; a < LRN_MAX_D; a++) wa[a] = (uint32_t)a == at ? wa[a] + Q.w[j] : wa[a];
+    for (int i = 0; i < TOPK_MAX_DIMS; i += 4) {
+        const f4 v = *reinterpret_cast<const f4 *>(&Q.w[i]);
+        wt[i] = v[0];
+        wt[i + 1] = v[1];
+        wt[i + 2] = v[2];
+        wt[i + 3] = v[3];
+    }
+#pragma unroll
+    for (int i = 0; i < TOPK_MAX_DIMS / 16; i++) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(&Q.attr[16 * i]);
+        ap[4 * i] = v.x;
+        ap[4 * i + 1] = v.y;
+        ap[4 * i + 2] = v.z;
+        ap[4 * i + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < TOPK_MAX_DIMS; j++) {
+        const uint32_t at = (ap[j / 4] >> (8 * (j % 4))) & 0xFFu;
+        const bool on = (uint32_t)j < m;
+#pragma unroll
+        for (int a = 0; a < LRN_MAX_D; a++) wa[a] = on && (uint32_t)a == at ? wa[a] + wt[j] : wa[a];
     }
     float sabs = 0.0f;
 #pragma unroll
