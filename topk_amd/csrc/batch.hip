@@ -177,16 +177,13 @@ __global__ __launch_bounds__(64) void batch_prep_kernel(const topk_query *__rest
         b.as[lane] = (uint8_t)sas[lane];
     }
     const uint32_t nd = (uint32_t)__popcll(__ballot(used && lane < d));
+    // largest score: each term at the end of its attribute's range that its
+    // weight's sign prefers (lane j: term j's product, one load), summed in
+    // term order as the scores are
+    const float prj = term ? (wj >= 0.0f ? rmax[aj * rstride] : rmin[aj * rstride]) * wj : 0.0f;
+    float sm = 0.0f;
+    for (uint32_t t = 0; t < qq; t++) sm = sm + __shfl(prj, t, 64);
     if (lane == 0) {
-        // largest score: each term at the end of its attribute's range that
-        // its weight's sign prefers, summed in term order as the scores are
-        float sm = 0.0f;
-        for (uint32_t t = 0; t < qq; t++) {
-            const uint32_t at = Q.attr[t];
-            const float w = Q.weights[at];
-            const float pr = (w >= 0.0f ? rmax[at * rstride] : rmin[at * rstride]) * w;
-            sm = sm + pr;
-        }
         b.smax = sm;
         b.tau = taus ? taus[q] : 0.0f;
         b.tau_retry = -INFINITY;
@@ -395,14 +392,18 @@ __global__ __launch_bounds__(BXB_BS) void batch_tiebits_kernel(BatchQuery *__res
     if (!qs[q].seed) return;  // uniform over the workgroup
     const BatchQuery &Q = qs[q];
     const uint32_t k = Q.k, m = Q.m;
-    if (tid == 0) {
-        uint32_t c = 0;
-        for (uint32_t t = 0; t < m && c < MAXR; t++) {
-            const float w = Q.w[t];
-            if (w != 0.0f) rrow[c++] = w > 0.0f ? Q.attr[t] : d + Q.attr[t];
-        }
-        nrows = c;
-        for (uint32_t r = c; r < MAXR; r++) rrow[r] = 0u;  // unused rows: never loaded
+    if (tid < 64) {
+        // lane t: term t; the terms of nonzero weight, in term order, name
+        // the rows (the weight's sign picks the maximum or the minimum row)
+        const float w = tid < m ? Q.w[tid] : 0.0f;
+        const uint32_t at = tid < m ? Q.attr[tid] : 0u;
+        const uint64_t nz = __ballot(w != 0.0f);
+        const uint32_t c = (uint32_t)__popcll(nz & ((1ull << tid) - 1ull));
+        if (w != 0.0f && c < (uint32_t)MAXR) rrow[c] = w > 0.0f ? at : d + at;
+        const uint32_t cnt = (uint32_t)__popcll(nz);
+        const uint32_t nrw = cnt < (uint32_t)MAXR ? cnt : (uint32_t)MAXR;
+        if (tid >= nrw && tid < (uint32_t)MAXR) rrow[tid] = 0u;  // unused rows: never loaded
+        if (tid == 0) nrows = nrw;
     }
     __syncthreads();
     const uint32_t nr = __builtin_amdgcn_readfirstlane(nrows);
